@@ -1,0 +1,242 @@
+/*
+ * vtkgpu.h -- C ABI of the MI355X (gfx950) library for the vtkmakeCompare
+ * time-step hot path.
+ *
+ * Each reference routine on the path has one entry point with the
+ * reference's argument list and order.  Every argument is passed by
+ * pointer, as a Fortran caller passes it (`bind(C)` interfaces in
+ * vtkmakecompare_amd/host/vk_capi.f90), so a Fortran host keeps its call
+ * sites.  Array arguments may be
+ *   - host arrays registered with vk_register (the call then works on the
+ *     device mirror; vk_upload / vk_download move data), or
+ *   - device pointers (vk_malloc or any HIP allocation on this device).
+ * Arrays keep the reference layout: halo fields (0:nx+1,0:ny+1,0:nz+1)
+ * column-major doubles, cellvalue (1:nx,1:ny,1:nz,1:4), spectra
+ * (0:nx/2,0:ny+1,0:nz+1) complex double (re,im interleaved).
+ *
+ * All entry points return a status (VK_OK = 0); vk_last_error() gives the
+ * message.  Work is queued on the library's stream; vk_download and
+ * vk_synchronize wait for it.  Calls come from one host thread.
+ */
+#ifndef VTKGPU_H
+#define VTKGPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  VK_OK = 0,
+  VK_ERR_NOT_INITIALISED = 1,
+  VK_ERR_NOT_RESIDENT = 2, /* array neither registered nor device memory */
+  VK_ERR_BAD_SIZE = 3,     /* grid size the kernels do not support */
+  VK_ERR_HIP = 4,          /* HIP runtime error */
+  VK_ERR_ARGUMENT = 5,     /* invalid argument */
+  VK_ERR_NONFINITE = 6     /* a field check found NaN or Inf */
+};
+
+/* ------------------------------------------------------------ runtime */
+/* No reference counterpart: the reference runs in host memory. */
+int vk_init(const int *device);
+int vk_finalize(void);
+const char *vk_last_error(void);
+int vk_register(const void *host, const size_t *bytes);
+int vk_unregister(const void *host);
+int vk_upload(const void *host);
+int vk_download(void *host);
+int vk_synchronize(void);
+int vk_malloc(void **dptr, const size_t *bytes);
+int vk_free(void *dptr);
+int vk_memcpy_htod(void *dst, const void *src, const size_t *bytes);
+int vk_memcpy_dtoh(void *dst, const void *src, const size_t *bytes);
+int vk_memset(void *dst, const int *value, const size_t *bytes);
+
+/* ---------------------------------------------------- boundary halos */
+/* SET_BC(U,V,W,P,NX,NY,NZ,T), nse_dominik.f:611-753 */
+int vk_set_bc(double *u, double *v, double *w, double *p, const int *nx,
+              const int *ny, const int *nz, double *t);
+
+/* ---------------------------------------------------------- stencils */
+/* SET_CONVECT, nse_dominik.f:757-831 */
+int vk_set_convect(const double *u, double *u_v, const double *v,
+                   double *v_v, const double *w, double *w_v,
+                   const double *dt, const int *nx, const int *ny,
+                   const int *nz, const double *dx, const double *dy,
+                   const double *dz, const double *dx12, const double *dy12,
+                   const double *dz12);
+/* SET_CONVECTION_FOR_SCALAR, nse_dominik.f:834-884 */
+int vk_set_convection_for_scalar(const double *u, const double *v,
+                                 const double *w, const double *dt,
+                                 const int *nx, const int *ny, const int *nz,
+                                 const double *dx, const double *dy,
+                                 const double *dz, const double *dx12,
+                                 const double *dy12, const double *dz12,
+                                 const double *t, double *u_vt,
+                                 const double *rhocp);
+/* SET_CONDUCTION, nse_dominik.f:888-945 */
+int vk_set_conduction(const double *u, double *u_d, const double *v,
+                      double *v_d, const double *w, double *w_d,
+                      const double *re, const double *dt, const int *nx,
+                      const int *ny, const int *nz, const double *dx,
+                      const double *dy, const double *dz, const double *dx12,
+                      const double *dy12, const double *dz12);
+/* SET_DIFFUSION_FOR_SCALAR, nse_dominik.f:950-1004 */
+int vk_set_diffusion_for_scalar(const double *re, const double *dt,
+                                const int *nx, const int *ny, const int *nz,
+                                const double *dx, const double *dy,
+                                const double *dz, const double *dx12,
+                                const double *dy12, const double *dz12,
+                                const double *t, double *u_dt,
+                                const double *sc, const double *thc,
+                                const double *rhocp);
+/* SET_IMMERSED_BOUNDARY, nse_dominik.f:1011-1129 */
+int vk_set_immersed_boundary(const double *eta, const int *nx, const int *ny,
+                             const int *nz, const double *dt, double *u_l,
+                             double *v_l, double *w_l, const double *dx,
+                             const double *dy, const double *dz, double *t_l,
+                             double *cellvalue, const double *u_ta,
+                             const double *v_ta, const double *w_ta,
+                             const int *l, double *cons_cell);
+/* SET_EXPLICIT, nse_dominik.f:1132-1286 */
+int vk_set_explicit(const double *u_v, double *u_pre, const double *u_d,
+                    double *u_a, const double *v_v, double *v_pre,
+                    const double *v_d, double *v_a, const double *w_v,
+                    double *w_pre, const double *w_d, double *w_a,
+                    const double *u_l, const double *v_l, const double *w_l,
+                    const int *nx, const int *ny, const int *nz,
+                    const int *l);
+/* set_explicit_for_tempreture, nse_dominik.f:1291-1341 */
+int vk_set_explicit_for_tempreture(const double *u_vt, const double *u_dt,
+                                   double *u_t, double *t_pre,
+                                   const double *t_l, const int *nx,
+                                   const int *ny, const int *nz,
+                                   const int *l);
+/* main-loop temperature divide u_T = (T+u_T)/(1+eta cv1 dt),
+   nse_dominik.f:387-396 (inline in program main, no routine name) */
+int vk_t_divide(double *u_t, const double *t, const double *cellvalue,
+                const double *eta, const double *dt, const int *nx,
+                const int *ny, const int *nz);
+
+/* ----------------------------------------------------- implicit sweeps */
+/* solve_u_temp, nse_dominik.f:1344-1461 (with implicit_uy/vy,
+   implicit_solver/cyclic/TDMA and the parallel_* gathers) */
+int vk_solve_u_temp(double *u_a, double *v_a, double *w_a, const int *nx,
+                    const int *ny, const int *nz, const double *dx,
+                    const double *dy, const double *dz, const double *dx12,
+                    const double *dy12, const double *dz12, const double *re,
+                    const double *dt, const double *eta,
+                    const double *cellvalue, const double *u, const double *v,
+                    const double *w);
+/* solve_t, nse_dominik.f:1465-1501 (implicit_Tx/Ty, parallel_Tx/Ty/Tz).
+   The conductivity ghosts the reference leaves undefined
+   (nse_dominik.f:1755,1760,1814,1819) are x/z periodic and y wall copies. */
+int vk_solve_t(double *u_t, const int *nx, const int *ny, const int *nz,
+               const double *dx, const double *dy, const double *dz,
+               const double *dx12, const double *dy12, const double *dz12,
+               const double *re, const double *sc, const double *dt,
+               const double *rhocp, const double *thc,
+               const double *cellvalue, const double *eta);
+
+/* ------------------------------------------------- pressure projection */
+/* MKP, mkp.f:9-220 (ompnthreads is accepted and ignored, as there) */
+int vk_mkp(double *p, const double *u_temp, const double *v_temp,
+           const double *w_temp, const double *dt, const double *dx,
+           const double *dy, const double *dz, const double *dx12,
+           const double *dy12, const double *dz12, const double *re,
+           const int *nx, const int *ny, const int *nz,
+           const int *ompnthreads, double *u_ta, double *v_ta, double *w_ta,
+           double *u_ep2, double *v_ep2, double *w_ep2, double *divu,
+           double *u, double *v, double *w, const double *xl,
+           const double *zl);
+/* P2S(FP,FPT,NX,NY,NZ), p2s_fftw2d.f:5-114 */
+int vk_p2s(const double *fp, double *fpt, const int *nx, const int *ny,
+           const int *nz);
+/* S2P(FPT,FP,NX,NY,NZ), p2s_fftw2d.f:116-223 */
+int vk_s2p(const double *fpt, double *fp, const int *nx, const int *ny,
+           const int *nz);
+/* SPN(NX,NY,NZ,AC,KXZ,PHIT), mkp.f:223-260; AC(1:ny,3), KXZ(0:nx/2,0:nz) */
+int vk_spn(const int *nx, const int *ny, const int *nz, const double *ac,
+           const double *kxz, double *phit);
+/* SP0B(NX,NY,NZ,AC,PHIT), mkp.f:263-290 */
+int vk_sp0b(const int *nx, const int *ny, const int *nz, const double *ac,
+            double *phit);
+/* PHIBC(NX,NY,NZ,PHI), mkp.f:293-320 */
+int vk_phibc(const int *nx, const int *ny, const int *nz, double *phi);
+
+/* ---------------------------------------------------------- level set */
+/* set_levelset, set_levelset.f:1-199 (2001 reinitialisation sweeps; the
+   VTK dumps and the printed residual are host I/O and not performed) */
+int vk_set_levelset(const int *nx, const int *ny, const int *nz,
+                    const double *dx, const double *dx12, const double *dy12,
+                    const double *dz12, const double *phi0, double *phi,
+                    const double *x, const double *y, const double *z,
+                    double *cellvalue);
+
+/* ------------------------------------------------------ adjoint sweeps */
+/* set_adj_convect, adjoint.f:1-126 */
+int vk_set_adj_convect(const double *u, double *u_v, const double *v,
+                       double *v_v, const double *w, double *w_v,
+                       const double *re, const double *dt, const int *nx,
+                       const int *ny, const int *nz, const double *dx,
+                       const double *dy, const double *dz, const double *dx12,
+                       const double *dy12, const double *dz12,
+                       const double *u_final, const double *v_final,
+                       const double *w_final);
+/* set_adj_convection_for_scalar, adjoint.f:128-181 */
+int vk_set_adj_convection_for_scalar(
+    const double *u, const double *v, const double *w, const double *re,
+    const double *dt, const int *nx, const int *ny, const int *nz,
+    const double *dx, const double *dy, const double *dz, const double *dx12,
+    const double *dy12, const double *dz12, const double *t, double *u_vt,
+    const double *rhocp, const double *u_final, const double *v_final,
+    const double *w_final);
+/* set_adj_NS_scalar, adjoint.f:185-211, plus a trailing T_final argument
+   (the reference reads an uninitialised local of that name) */
+int vk_set_adj_ns_scalar(const double *dt, const int *nx, const int *ny,
+                         const int *nz, const double *dx12,
+                         const double *dy12, const double *dz12,
+                         const double *t, double *u_s, double *v_s,
+                         double *w_s, const double *u_final,
+                         const double *v_final, const double *w_final,
+                         const double *t_final);
+
+/* ------------------------------------------------------ the time step */
+/* One pass of the main-loop body nse_dominik.f:290-448 (SET_BC through
+   MKP) with the phases fused: no reference routine of this name exists;
+   it is what program main's loop does per step.  Reads and writes the
+   same arrays the loop does.  u_ep2/v_ep2/w_ep2 (MKP's diagnostic
+   splitting error) are skipped when passed as NULL. */
+int vk_time_step(double *u, double *v, double *w, double *p, double *t,
+                 double *u_pre, double *v_pre, double *w_pre, double *t_pre,
+                 double *u_ta, double *v_ta, double *w_ta, double *u_temp,
+                 double *v_temp, double *w_temp, double *divu,
+                 double *cellvalue, const double *rhocp, const double *thc,
+                 double *u_ep2, double *v_ep2, double *w_ep2,
+                 const double *dx, const double *dy, const double *dz,
+                 const double *dx12, const double *dy12, const double *dz12,
+                 const int *nx, const int *ny, const int *nz,
+                 const double *dt, const double *re, const double *sc,
+                 const double *eta, const double *xl, const double *zl,
+                 const int *l);
+
+/* dst := src over a whole halo field, or over its interior when
+   *interior_only is non-zero: the inline array copies of program main
+   (u_pre := u at nse_dominik.f:246-257, u_temp := u_a at :407-417,
+   T := u_T at :429-437), which a host with device-resident arrays cannot
+   do itself.  No reference routine of this name. */
+int vk_field_copy(const double *src, double *dst, const int *nx,
+                  const int *ny, const int *nz, const int *interior_only);
+
+/* Max |x| over the interior of a halo field and a NaN/Inf flag
+   (returns VK_ERR_NONFINITE when one is found).  No reference
+   counterpart: a device reduction for run checks and the divergence
+   residual. */
+int vk_field_absmax(const double *f, const int *nx, const int *ny,
+                    const int *nz, double *result);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

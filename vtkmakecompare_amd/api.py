@@ -1,0 +1,200 @@
+"""Python host mirror of the reference's routine interface.
+
+Each function has the name and the argument list of the Fortran routine it
+replaces (see the citations in include/vtkgpu.h); arrays are
+:class:`~vtkmakecompare_amd.device.DeviceArray` objects in the reference
+layout, scalars are Python numbers.  Every call runs the gfx950 library;
+a non-zero status raises :class:`VKError`.
+
+:class:`ChannelRun` holds a whole configuration resident in HBM and
+advances it with the fused time step (the loop body of program main).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import grid as G
+from ._lib import D, I, VKError, check, lib  # noqa: F401
+from .device import DeviceArray, init, synchronize  # noqa: F401
+
+NULL = None
+
+
+def set_bc(u, v, w, p, nx, ny, nz, t):
+    check(lib().vk_set_bc(u, v, w, p, I(nx), I(ny), I(nz), t), "SET_BC")
+
+
+def set_convect(u, u_v, v, v_v, w, w_v, dt, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12):
+    check(lib().vk_set_convect(u, u_v, v, v_v, w, w_v, D(dt), I(nx), I(ny), I(nz),
+                               dx, dy, dz, dx12, dy12, dz12), "SET_CONVECT")
+
+
+def set_convection_for_scalar(u, v, w, dt, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12,
+                              t, u_vt, rhocp):
+    check(lib().vk_set_convection_for_scalar(u, v, w, D(dt), I(nx), I(ny), I(nz), dx, dy, dz,
+                                             dx12, dy12, dz12, t, u_vt, rhocp),
+          "SET_CONVECTION_FOR_SCALAR")
+
+
+def set_conduction(u, u_d, v, v_d, w, w_d, re, dt, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12):
+    check(lib().vk_set_conduction(u, u_d, v, v_d, w, w_d, D(re), D(dt), I(nx), I(ny), I(nz),
+                                  dx, dy, dz, dx12, dy12, dz12), "SET_CONDUCTION")
+
+
+def set_diffusion_for_scalar(re, dt, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12, t, u_dt, sc,
+                             thc, rhocp):
+    check(lib().vk_set_diffusion_for_scalar(D(re), D(dt), I(nx), I(ny), I(nz), dx, dy, dz,
+                                            dx12, dy12, dz12, t, u_dt, D(sc), thc, rhocp),
+          "SET_DIFFUSION_FOR_SCALAR")
+
+
+def set_immersed_boundary(eta, nx, ny, nz, dt, u_l, v_l, w_l, dx, dy, dz, t_l, cellvalue,
+                          u_ta, v_ta, w_ta, l):
+    import ctypes
+    cons = ctypes.c_double(0.0)
+    check(lib().vk_set_immersed_boundary(D(eta), I(nx), I(ny), I(nz), D(dt), u_l, v_l, w_l,
+                                         dx, dy, dz, t_l, cellvalue, u_ta, v_ta, w_ta, I(l),
+                                         ctypes.byref(cons)), "SET_IMMERSED_BOUNDARY")
+    return cons.value
+
+
+def set_explicit(u_v, u_pre, u_d, u_a, v_v, v_pre, v_d, v_a, w_v, w_pre, w_d, w_a,
+                 u_l, v_l, w_l, nx, ny, nz, l):
+    check(lib().vk_set_explicit(u_v, u_pre, u_d, u_a, v_v, v_pre, v_d, v_a, w_v, w_pre, w_d,
+                                w_a, u_l, v_l, w_l, I(nx), I(ny), I(nz), I(l)), "SET_EXPLICIT")
+
+
+def set_explicit_for_tempreture(u_vt, u_dt, u_t, t_pre, t_l, nx, ny, nz, l):
+    check(lib().vk_set_explicit_for_tempreture(u_vt, u_dt, u_t, t_pre, t_l, I(nx), I(ny),
+                                               I(nz), I(l)), "set_explicit_for_tempreture")
+
+
+def t_divide(u_t, t, cellvalue, eta, dt, nx, ny, nz):
+    check(lib().vk_t_divide(u_t, t, cellvalue, D(eta), D(dt), I(nx), I(ny), I(nz)),
+          "T divide")
+
+
+def solve_u_temp(u_a, v_a, w_a, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12, re, dt, eta,
+                 cellvalue, u, v, w):
+    check(lib().vk_solve_u_temp(u_a, v_a, w_a, I(nx), I(ny), I(nz), dx, dy, dz, dx12, dy12,
+                                dz12, D(re), D(dt), D(eta), cellvalue, u, v, w),
+          "solve_u_temp")
+
+
+def solve_t(u_t, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12, re, sc, dt, rhocp, thc,
+            cellvalue, eta):
+    check(lib().vk_solve_t(u_t, I(nx), I(ny), I(nz), dx, dy, dz, dx12, dy12, dz12, D(re),
+                           D(sc), D(dt), rhocp, thc, cellvalue, D(eta)), "solve_t")
+
+
+def mkp(p, u_temp, v_temp, w_temp, dt, dx, dy, dz, dx12, dy12, dz12, re, nx, ny, nz,
+        ompnthreads, u_ta, v_ta, w_ta, u_ep2, v_ep2, w_ep2, divu, u, v, w, xl, zl):
+    check(lib().vk_mkp(p, u_temp, v_temp, w_temp, D(dt), dx, dy, dz, dx12, dy12, dz12, D(re),
+                       I(nx), I(ny), I(nz), I(ompnthreads), u_ta, v_ta, w_ta, u_ep2, v_ep2,
+                       w_ep2, divu, u, v, w, D(xl), D(zl)), "MKP")
+
+
+def p2s(fp, fpt, nx, ny, nz):
+    check(lib().vk_p2s(fp, fpt, I(nx), I(ny), I(nz)), "P2S")
+
+
+def s2p(fpt, fp, nx, ny, nz):
+    check(lib().vk_s2p(fpt, fp, I(nx), I(ny), I(nz)), "S2P")
+
+
+def spn(nx, ny, nz, ac, kxz, phit):
+    check(lib().vk_spn(I(nx), I(ny), I(nz), ac, kxz, phit), "SPN")
+
+
+def sp0b(nx, ny, nz, ac, phit):
+    check(lib().vk_sp0b(I(nx), I(ny), I(nz), ac, phit), "SP0B")
+
+
+def phibc(nx, ny, nz, phi):
+    check(lib().vk_phibc(I(nx), I(ny), I(nz), phi), "PHIBC")
+
+
+def set_levelset(nx, ny, nz, dx, dx12, dy12, dz12, phi0, phi, x, y, z, cellvalue):
+    check(lib().vk_set_levelset(I(nx), I(ny), I(nz), dx, dx12, dy12, dz12, phi0, phi, x, y, z,
+                                cellvalue), "set_levelset")
+
+
+def set_adj_convect(u, u_v, v, v_v, w, w_v, re, dt, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12,
+                    u_final, v_final, w_final):
+    check(lib().vk_set_adj_convect(u, u_v, v, v_v, w, w_v, D(re), D(dt), I(nx), I(ny), I(nz),
+                                   dx, dy, dz, dx12, dy12, dz12, u_final, v_final, w_final),
+          "set_adj_convect")
+
+
+def set_adj_convection_for_scalar(u, v, w, re, dt, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12,
+                                  t, u_vt, rhocp, u_final, v_final, w_final):
+    check(lib().vk_set_adj_convection_for_scalar(u, v, w, D(re), D(dt), I(nx), I(ny), I(nz),
+                                                 dx, dy, dz, dx12, dy12, dz12, t, u_vt, rhocp,
+                                                 u_final, v_final, w_final),
+          "set_adj_convection_for_scalar")
+
+
+def set_adj_ns_scalar(dt, nx, ny, nz, dx12, dy12, dz12, t, u_s, v_s, w_s, u_final, v_final,
+                      w_final, t_final):
+    check(lib().vk_set_adj_ns_scalar(D(dt), I(nx), I(ny), I(nz), dx12, dy12, dz12, t, u_s, v_s,
+                                     w_s, u_final, v_final, w_final, t_final),
+          "set_adj_NS_scalar")
+
+
+def field_absmax(f, nx, ny, nz) -> float:
+    import ctypes
+    r = ctypes.c_double(0.0)
+    check(lib().vk_field_absmax(f, I(nx), I(ny), I(nz), ctypes.byref(r)), "field_absmax")
+    return r.value
+
+
+# ------------------------------------------------------------------ runs
+
+STEP_FIELDS = ("u", "v", "w", "p", "t", "u_pre", "v_pre", "w_pre", "t_pre",
+               "u_ta", "v_ta", "w_ta", "u_temp", "v_temp", "w_temp", "divu",
+               "rhocp", "thc", "u_ep2", "v_ep2", "w_ep2")
+
+
+class ChannelRun:
+    """A configuration resident in HBM, advanced by vk_time_step.
+
+    ``state`` is the dict of initial_state(); the level set is either
+    given (``cellvalue``) or computed on the device by set_levelset.
+    """
+
+    def __init__(self, cfg: G.Config, g: G.Grid, state: dict, cellvalue=None,
+                 diagnostics=True):
+        init()
+        self.cfg, self.g = cfg, g
+        self.d = {}
+        for n in STEP_FIELDS:
+            self.d[n] = DeviceArray.from_host(state[n])
+        for n in ("dx", "dy", "dz", "dx12", "dy12", "dz12", "x", "y", "z"):
+            self.d[n] = DeviceArray.from_host(getattr(g, n))
+        if cellvalue is None:
+            self.d["phi0"] = DeviceArray.from_host(state["phi0"])
+            self.d["phi"] = DeviceArray(g.shape)
+            self.d["cellvalue"] = DeviceArray((4, g.nz, g.ny, g.nx))
+            d = self.d
+            set_levelset(g.nx, g.ny, g.nz, d["dx"], d["dx12"], d["dy12"], d["dz12"], d["phi0"],
+                         d["phi"], d["x"], d["y"], d["z"], d["cellvalue"])
+        else:
+            self.d["cellvalue"] = DeviceArray.from_host(cellvalue)
+        self.diagnostics = diagnostics
+        self.l = 0
+
+    def step(self, n: int = 1) -> None:
+        d, g, c = self.d, self.g, self.cfg
+        ep = (d["u_ep2"], d["v_ep2"], d["w_ep2"]) if self.diagnostics else (None, None, None)
+        for _ in range(n):
+            self.l += 1
+            check(lib().vk_time_step(
+                d["u"], d["v"], d["w"], d["p"], d["t"], d["u_pre"], d["v_pre"], d["w_pre"],
+                d["t_pre"], d["u_ta"], d["v_ta"], d["w_ta"], d["u_temp"], d["v_temp"],
+                d["w_temp"], d["divu"], d["cellvalue"], d["rhocp"], d["thc"], ep[0], ep[1],
+                ep[2], d["dx"], d["dy"], d["dz"], d["dx12"], d["dy12"], d["dz12"], I(g.nx),
+                I(g.ny), I(g.nz), D(c.dt), D(c.re), D(c.sc), D(c.eta), D(g.xl), D(g.zl),
+                I(self.l)), f"time step {self.l}")
+
+    def get(self, name: str) -> np.ndarray:
+        return self.d[name].download()

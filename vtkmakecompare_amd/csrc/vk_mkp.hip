@@ -1,0 +1,189 @@
+// vk_mkp.hip -- the pressure projection MKP (mkp.f:9-220): right-hand side
+// div(u_temp)/dt fused into the forward FFT, spectral y solves (SPN,
+// SP0B), inverse FFT, halo fill, then gradient, velocity correction,
+// pressure, splitting-error diagnostic, u_ta and divergence in one pass.
+#include "vk_kernels.h"
+#include "vk_runtime.h"
+
+namespace vk {
+
+// mkp.f:54-84: KXZ(0:nx/2,0:nz) and AC(1:ny,3), PI = 3.14159265358979312
+__global__ void k_mkp_coefficients(Geom g, Metrics m, double xl, double zl, double* kxz,
+                                   double* ac) {
+  const double pi = 3.14159265358979312;
+  const int nh = g.nx / 2;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int nkxz = (nh + 1) * (g.nz + 1);
+  if (t < nkxz) {
+    int I = t % (nh + 1), K = t / (nh + 1);
+    double kx = (double)I / xl * pi * 2.0;
+    double kz = (K <= g.nz / 2) ? (double)K : (double)(K - g.nz);
+    if (K <= g.nz - 1) kz = kz / zl * pi * 2.0;
+    double dxI = m.dx[I], dzK = m.dz[K];
+    kxz[t] = 2.0 * (1.0 - cos(kx * dxI)) / dxI / dxI + 2.0 * (1.0 - cos(kz * dzK)) / dzK / dzK;
+  } else if (t < nkxz + g.ny) {
+    int J = t - nkxz + 1;
+    double a1 = 1.0 / m.dy12[J - 1] / m.dy[J];
+    double a3 = 1.0 / m.dy12[J] / m.dy[J];
+    ac[J - 1] = a1;
+    ac[J - 1 + 2 * g.ny] = a3;
+    ac[J - 1 + g.ny] = -a1 - a3;
+  }
+}
+
+// mkp.f:141-217 over i,j,k = 0..n: gradient of PHI at the faces, the
+// corrected velocity, and on the interior p = PHI, the diagnostic
+// u_ep2 = -dt grad(p) + u_ta (read before u_ta is replaced; p's halo at
+// nx+1, ny+1, nz+1 is the caller's, as in the reference), u_ta = dt grad,
+// and divu of the corrected velocity.
+__global__ void k_project(Geom g, Metrics m, const double* phi, const double* ut,
+                          const double* vt, const double* wt, double dt, MkpOut o) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (i > g.nx || j > g.ny) return;
+  long long c = g.at(i, j, k);
+  const long long sy = g.sy, sz = g.sz;
+  double ph = phi[c];
+  double gx = (phi[c + 1] - ph) / m.dx12[i];
+  double gy = (phi[c + sy] - ph) / m.dy12[j];
+  double gz = (phi[c + sz] - ph) / m.dz12[k];
+  double un = ut[c] - dt * gx, vn = vt[c] - dt * gy, wn = wt[c] - dt * gz;
+  o.u[c] = un;
+  o.v[c] = vn;
+  o.w[c] = wn;
+  bool interior = i >= 1 && j >= 1 && k >= 1;
+  if (interior) {
+    o.p[c] = ph;
+    if (o.u_ep2) {
+      double px = (i < g.nx) ? phi[c + 1] : o.p[c + 1];
+      double py = (j < g.ny) ? phi[c + sy] : o.p[c + sy];
+      double pz = (k < g.nz) ? phi[c + sz] : o.p[c + sz];
+      o.u_ep2[c] = -dt * (px - ph) / m.dx12[i] + o.u_ta[c];
+      o.v_ep2[c] = -dt * (py - ph) / m.dy12[j] + o.v_ta[c];
+      o.w_ep2[c] = -dt * (pz - ph) / m.dz12[k] + o.w_ta[c];
+    }
+    double um = ut[c - 1] - dt * ((ph - phi[c - 1]) / m.dx12[i - 1]);
+    double vm = vt[c - sy] - dt * ((ph - phi[c - sy]) / m.dy12[j - 1]);
+    double wm = wt[c - sz] - dt * ((ph - phi[c - sz]) / m.dz12[k - 1]);
+    o.divu[c] = (un - um) / m.dx[i] + (vn - vm) / m.dy[j] + (wn - wm) / m.dz[k];
+  }
+  o.u_ta[c] = dt * gx;
+  o.v_ta[c] = dt * gy;
+  o.w_ta[c] = dt * gz;
+}
+
+int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const double* v_temp,
+               const double* w_temp, double dt, double xl, double zl, const MkpOut& o,
+               hipStream_t s) {
+  const int nh = g.nx / 2;
+  size_t nk = (size_t)(nh + 1) * (g.nz + 1) + 3 * (size_t)g.ny;
+  double* coef = static_cast<double*>(workspace(kWorkMisc, (nk + 8) * sizeof(double)));
+  double* phit = static_cast<double*>(workspace(kWorkSpec, 2 * g.spec_cells() * sizeof(double)));
+  double* phi = static_cast<double*>(workspace(kWorkPhi, g.cells() * sizeof(double)));
+  if (!coef || !phit || !phi) return VK_ERR_HIP;
+  double* kxz = coef;
+  double* ac = coef + (size_t)(nh + 1) * (g.nz + 1);
+  int nthreads = (int)nk;
+  k_mkp_coefficients<<<(nthreads + 255) / 256, 256, 0, s>>>(g, m, xl, zl, kxz, ac);
+  int rc = launch_p2s_div(g, m, u_temp, v_temp, w_temp, dt, phit, s);
+  if (rc) return rc;
+  if (launch_spn(g, phit, ac, kxz, s)) return fail(VK_ERR_BAD_SIZE, "mkp: ny not supported");
+  launch_sp0b(g, phit, ac, s);
+  rc = launch_s2p_inplace(g, phit, phi, s);
+  if (rc) return rc;
+  launch_halo_neumann(g, phi, s);
+  dim3 blk(64, 4, 1), grd((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
+  k_project<<<grd, blk, 0, s>>>(g, m, phi, u_temp, v_temp, w_temp, dt, o);
+  return VK_OK;
+}
+
+}  // namespace vk
+
+using namespace vk;
+
+extern "C" {
+
+int vk_mkp(double* p, const double* u_temp, const double* v_temp, const double* w_temp,
+           const double* dt, const double* dx, const double* dy, const double* dz,
+           const double* dx12, const double* dy12, const double* dz12, const double* re,
+           const int* nx, const int* ny, const int* nz, const int* ompnthreads, double* u_ta,
+           double* v_ta, double* w_ta, double* u_ep2, double* v_ep2, double* w_ep2,
+           double* divu, double* u, double* v, double* w, const double* xl, const double* zl) {
+  VK_REQUIRE_INIT();
+  (void)re;
+  (void)ompnthreads;
+  Geom g(*nx, *ny, *nz);
+  if (g.nx < 4 || g.ny < 2 || g.nz < 2) return fail(VK_ERR_BAD_SIZE, "mkp: grid too small");
+  size_t fb = g.cells() * sizeof(double);
+  VK_RESOLVE(dut, const double, u_temp, fb);
+  VK_RESOLVE(dvt, const double, v_temp, fb);
+  VK_RESOLVE(dwt, const double, w_temp, fb);
+  MkpOut o;
+  {
+    VK_RESOLVE(a, double, p, fb);
+    o.p = a;
+  }
+  {
+    VK_RESOLVE(a, double, u_ta, fb);
+    o.u_ta = a;
+  }
+  {
+    VK_RESOLVE(a, double, v_ta, fb);
+    o.v_ta = a;
+  }
+  {
+    VK_RESOLVE(a, double, w_ta, fb);
+    o.w_ta = a;
+  }
+  {
+    VK_RESOLVE(a, double, u_ep2, fb);
+    o.u_ep2 = a;
+  }
+  {
+    VK_RESOLVE(a, double, v_ep2, fb);
+    o.v_ep2 = a;
+  }
+  {
+    VK_RESOLVE(a, double, w_ep2, fb);
+    o.w_ep2 = a;
+  }
+  {
+    VK_RESOLVE(a, double, divu, fb);
+    o.divu = a;
+  }
+  {
+    VK_RESOLVE(a, double, u, fb);
+    o.u = a;
+  }
+  {
+    VK_RESOLVE(a, double, v, fb);
+    o.v = a;
+  }
+  {
+    VK_RESOLVE(a, double, w, fb);
+    o.w = a;
+  }
+  VK_RESOLVE(ddx, const double, dx, (g.nx + 2) * sizeof(double));
+  VK_RESOLVE(ddy, const double, dy, (g.ny + 2) * sizeof(double));
+  VK_RESOLVE(ddz, const double, dz, (g.nz + 2) * sizeof(double));
+  VK_RESOLVE(ddx12, const double, dx12, (g.nx + 2) * sizeof(double));
+  VK_RESOLVE(ddy12, const double, dy12, (g.ny + 2) * sizeof(double));
+  VK_RESOLVE(ddz12, const double, dz12, (g.nz + 2) * sizeof(double));
+  Metrics m{ddx, ddy, ddz, ddx12, ddy12, ddz12};
+  int rc = launch_mkp(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, rt().stream);
+  if (rc) return rc;
+  VK_LAUNCH_CHECK("mkp");
+  return VK_OK;
+}
+
+int vk_phibc(const int* nx, const int* ny, const int* nz, double* phi) {
+  VK_REQUIRE_INIT();
+  Geom g(*nx, *ny, *nz);
+  VK_RESOLVE(dp, double, phi, g.cells() * sizeof(double));
+  launch_halo_neumann(g, dp, rt().stream);
+  VK_LAUNCH_CHECK("phibc");
+  return VK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,245 @@
+// vk_runtime.cpp -- device selection, the library stream, the residency
+// registry (host array -> device mirror), workspace and error reporting.
+#include "vk_runtime.h"
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+
+namespace vk {
+
+namespace {
+
+struct Mirror {
+  size_t bytes;
+  void* dev;
+};
+
+std::map<uintptr_t, Mirror>& registry() {
+  static std::map<uintptr_t, Mirror> m;
+  return m;
+}
+
+std::string& last_error() {
+  static std::string s;
+  return s;
+}
+
+struct Work {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+Work g_work[kWorkSlots];
+
+}  // namespace
+
+Runtime& rt() {
+  static Runtime r;
+  return r;
+}
+
+int fail(int code, const std::string& msg) {
+  last_error() = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return VK_OK;
+  return fail(VK_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void* resolve(const void* p, size_t bytes, const char* what) {
+  if (p == nullptr) {
+    fail(VK_ERR_NOT_RESIDENT, std::string(what) + ": null array");
+    return nullptr;
+  }
+  auto& reg = registry();
+  uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  auto it = reg.upper_bound(a);
+  if (it != reg.begin()) {
+    --it;
+    uintptr_t base = it->first;
+    if (a >= base && a < base + it->second.bytes) {
+      size_t off = a - base;
+      if (off + bytes > it->second.bytes) {
+        fail(VK_ERR_NOT_RESIDENT, std::string(what) + ": registered array is " +
+                                      std::to_string(it->second.bytes) + " bytes, need " +
+                                      std::to_string(off + bytes));
+        return nullptr;
+      }
+      return static_cast<char*>(it->second.dev) + off;
+    }
+  }
+  hipPointerAttribute_t attr;
+  std::memset(&attr, 0, sizeof(attr));
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e == hipSuccess && attr.type == hipMemoryTypeDevice) return const_cast<void*>(p);
+  (void)hipGetLastError();  // clear the sticky error of the query
+  fail(VK_ERR_NOT_RESIDENT,
+       std::string(what) + ": neither a registered host array nor device memory");
+  return nullptr;
+}
+
+void* workspace(WorkSlot slot, size_t bytes) {
+  Work& w = g_work[slot];
+  if (w.bytes >= bytes && w.ptr) return w.ptr;
+  if (w.ptr) {
+    (void)hipStreamSynchronize(rt().stream);
+    (void)hipFree(w.ptr);
+    w.ptr = nullptr;
+    w.bytes = 0;
+  }
+  if (hipMalloc(&w.ptr, bytes) != hipSuccess) {
+    w.ptr = nullptr;
+    fail(VK_ERR_HIP, "workspace allocation of " + std::to_string(bytes) + " bytes failed");
+    return nullptr;
+  }
+  w.bytes = bytes;
+  return w.ptr;
+}
+
+}  // namespace vk
+
+using namespace vk;
+
+extern "C" {
+
+int vk_init(const int* device) {
+  int dev = device ? *device : 0;
+  Runtime& r = rt();
+  if (r.initialised) {
+    if (r.device == dev) return VK_OK;
+    return fail(VK_ERR_ARGUMENT, "vk_init: already initialised on another device");
+  }
+  int count = 0;
+  int rc = hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+  if (rc) return rc;
+  if (dev < 0 || dev >= count)
+    return fail(VK_ERR_ARGUMENT, "vk_init: device " + std::to_string(dev) + " of " +
+                                     std::to_string(count));
+  rc = hip_check(hipSetDevice(dev), "hipSetDevice");
+  if (rc) return rc;
+  rc = hip_check(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking), "hipStreamCreate");
+  if (rc) return rc;
+  r.device = dev;
+  r.initialised = true;
+  return VK_OK;
+}
+
+int vk_finalize(void) {
+  Runtime& r = rt();
+  if (!r.initialised) return VK_OK;
+  (void)hipStreamSynchronize(r.stream);
+  for (auto& kv : registry()) (void)hipFree(kv.second.dev);
+  registry().clear();
+  for (auto& w : g_work)
+    if (w.ptr) {
+      (void)hipFree(w.ptr);
+      w.ptr = nullptr;
+      w.bytes = 0;
+    }
+  (void)hipStreamDestroy(r.stream);
+  r.stream = nullptr;
+  r.initialised = false;
+  return VK_OK;
+}
+
+const char* vk_last_error(void) { return last_error().c_str(); }
+
+int vk_register(const void* host, const size_t* bytes) {
+  VK_REQUIRE_INIT();
+  if (!host || !bytes || *bytes == 0) return fail(VK_ERR_ARGUMENT, "vk_register: empty array");
+  auto& reg = registry();
+  uintptr_t a = reinterpret_cast<uintptr_t>(host);
+  auto it = reg.find(a);
+  if (it != reg.end()) {
+    if (it->second.bytes >= *bytes) return VK_OK;
+    return fail(VK_ERR_ARGUMENT, "vk_register: address already registered with a smaller size");
+  }
+  void* d = nullptr;
+  int rc = hip_check(hipMalloc(&d, *bytes), "vk_register hipMalloc");
+  if (rc) return rc;
+  rc = hip_check(hipMemsetAsync(d, 0, *bytes, rt().stream), "vk_register memset");
+  if (rc) return rc;
+  reg[a] = Mirror{*bytes, d};
+  return VK_OK;
+}
+
+int vk_unregister(const void* host) {
+  VK_REQUIRE_INIT();
+  auto& reg = registry();
+  auto it = reg.find(reinterpret_cast<uintptr_t>(host));
+  if (it == reg.end()) return fail(VK_ERR_ARGUMENT, "vk_unregister: not registered");
+  (void)hipStreamSynchronize(rt().stream);
+  (void)hipFree(it->second.dev);
+  reg.erase(it);
+  return VK_OK;
+}
+
+int vk_upload(const void* host) {
+  VK_REQUIRE_INIT();
+  auto& reg = registry();
+  auto it = reg.find(reinterpret_cast<uintptr_t>(host));
+  if (it == reg.end()) return fail(VK_ERR_NOT_RESIDENT, "vk_upload: array not registered");
+  int rc = hip_check(hipMemcpyAsync(it->second.dev, host, it->second.bytes,
+                                    hipMemcpyHostToDevice, rt().stream),
+                     "vk_upload");
+  if (rc) return rc;
+  return hip_check(hipStreamSynchronize(rt().stream), "vk_upload sync");
+}
+
+int vk_download(void* host) {
+  VK_REQUIRE_INIT();
+  auto& reg = registry();
+  auto it = reg.find(reinterpret_cast<uintptr_t>(host));
+  if (it == reg.end()) return fail(VK_ERR_NOT_RESIDENT, "vk_download: array not registered");
+  int rc = hip_check(hipMemcpyAsync(host, it->second.dev, it->second.bytes,
+                                    hipMemcpyDeviceToHost, rt().stream),
+                     "vk_download");
+  if (rc) return rc;
+  return hip_check(hipStreamSynchronize(rt().stream), "vk_download sync");
+}
+
+int vk_synchronize(void) {
+  VK_REQUIRE_INIT();
+  return hip_check(hipStreamSynchronize(rt().stream), "vk_synchronize");
+}
+
+int vk_malloc(void** dptr, const size_t* bytes) {
+  VK_REQUIRE_INIT();
+  if (!dptr || !bytes) return fail(VK_ERR_ARGUMENT, "vk_malloc: null argument");
+  int rc = hip_check(hipMalloc(dptr, *bytes ? *bytes : 1), "vk_malloc");
+  if (rc) return rc;
+  return hip_check(hipMemsetAsync(*dptr, 0, *bytes ? *bytes : 1, rt().stream), "vk_malloc memset");
+}
+
+int vk_free(void* dptr) {
+  VK_REQUIRE_INIT();
+  (void)hipStreamSynchronize(rt().stream);
+  return hip_check(hipFree(dptr), "vk_free");
+}
+
+int vk_memcpy_htod(void* dst, const void* src, const size_t* bytes) {
+  VK_REQUIRE_INIT();
+  int rc = hip_check(hipMemcpyAsync(dst, src, *bytes, hipMemcpyHostToDevice, rt().stream),
+                     "vk_memcpy_htod");
+  if (rc) return rc;
+  return hip_check(hipStreamSynchronize(rt().stream), "vk_memcpy_htod sync");
+}
+
+int vk_memcpy_dtoh(void* dst, const void* src, const size_t* bytes) {
+  VK_REQUIRE_INIT();
+  int rc = hip_check(hipMemcpyAsync(dst, src, *bytes, hipMemcpyDeviceToHost, rt().stream),
+                     "vk_memcpy_dtoh");
+  if (rc) return rc;
+  return hip_check(hipStreamSynchronize(rt().stream), "vk_memcpy_dtoh sync");
+}
+
+int vk_memset(void* dst, const int* value, const size_t* bytes) {
+  VK_REQUIRE_INIT();
+  return hip_check(hipMemsetAsync(dst, *value, *bytes, rt().stream), "vk_memset");
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+// vk_runtime.h -- host-side runtime shared by the entry points: device
+// selection, the library stream, the residency registry that maps a
+// caller's host array to its device mirror, workspace and error state.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/vtkgpu.h"
+#include "vk_common.h"
+
+namespace vk {
+
+struct Runtime {
+  bool initialised = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+Runtime& rt();
+
+// Records an error message and returns the code.
+int fail(int code, const std::string& msg);
+// Checks a HIP status; records and returns VK_ERR_HIP on failure.
+int hip_check(hipError_t e, const char* what);
+
+// Resolves a caller pointer to device memory holding at least `bytes`:
+// a registered host array (or an address inside one) maps to its mirror,
+// device memory is used as is.  Returns nullptr and records the error
+// otherwise.
+void* resolve(const void* p, size_t bytes, const char* what);
+
+// Grow-only device workspace, one per slot.
+enum WorkSlot { kWorkPhi = 0, kWorkSpec, kWorkSpec2, kWorkGrad, kWorkMisc, kWorkSlots };
+void* workspace(WorkSlot slot, size_t bytes);
+
+}  // namespace vk
+
+#define VK_REQUIRE_INIT()                                                   \
+  do {                                                                      \
+    if (!vk::rt().initialised)                                              \
+      return vk::fail(VK_ERR_NOT_INITIALISED, "vk_init has not been called"); \
+  } while (0)
+
+#define VK_RESOLVE(var, type, ptr, bytes)                                   \
+  type* var = static_cast<type*>(vk::resolve((const void*)(ptr), (bytes), #ptr)); \
+  if (!var) return VK_ERR_NOT_RESIDENT
+
+#define VK_LAUNCH_CHECK(what)                                               \
+  do {                                                                      \
+    int _rc = vk::hip_check(hipGetLastError(), what);                       \
+    if (_rc) return _rc;                                                    \
+  } while (0)

@@ -1,0 +1,162 @@
+// vk_stencil_ops.h -- point formulas of the explicit terms, one device
+// function per reference expression.  They keep the reference's operation
+// order (the library is built with -ffp-contract=off), so on identical
+// inputs they return the reference's bits; divisions are IEEE-correct.
+#pragma once
+
+#include "vk_common.h"
+
+namespace vk {
+
+template <typename R>
+struct F3 {  // read-only view of a halo field
+  const R* __restrict__ p;
+  Geom g;
+  __device__ R operator()(int i, int j, int k) const { return p[g.at(i, j, k)]; }
+};
+
+// SET_CONVECT u (nse_dominik.f:774-785)
+__device__ inline double conv_u(const F3<double>& U, const F3<double>& V, const F3<double>& W,
+                                const Metrics& m, int i, int j, int k, double dt) {
+  double c = U(i, j, k);
+  double up = U(i + 1, j, k), um = U(i - 1, j, k);
+  double u1 = ((c + up) * (c + up) - (c + um) * (c + um)) / m.dx12[i];
+  double u2 = ((V(i + 1, j, k) * m.dx[i + 1] + V(i, j, k) * m.dx[i]) * (U(i, j + 1, k) + c) -
+               (V(i + 1, j - 1, k) * m.dx[i + 1] + V(i, j - 1, k) * m.dx[i]) *
+                   (c + U(i, j - 1, k))) /
+              m.dx12[i] / m.dy[j];
+  double u3 = ((W(i + 1, j, k) * m.dx[i + 1] + W(i, j, k) * m.dx[i]) * (U(i, j, k + 1) + c) -
+               (W(i + 1, j, k - 1) * m.dx[i + 1] + W(i, j, k - 1) * m.dx[i]) *
+                   (c + U(i, j, k - 1))) /
+              m.dx12[i] / m.dz[k];
+  return 0.25 * (-u1 - u2 - u3) * dt + 1.0 * dt;
+}
+
+// SET_CONVECT v (nse_dominik.f:794-805)
+__device__ inline double conv_v(const F3<double>& U, const F3<double>& V, const F3<double>& W,
+                                const Metrics& m, int i, int j, int k, double dt) {
+  double c = V(i, j, k);
+  double v1 = ((U(i, j + 1, k) * m.dy[j + 1] + U(i, j, k) * m.dy[j]) * (V(i + 1, j, k) + c) -
+               (U(i - 1, j + 1, k) * m.dy[j + 1] + U(i - 1, j, k) * m.dy[j]) *
+                   (c + V(i - 1, j, k))) /
+              m.dy12[j] / m.dx[i];
+  double vp = V(i, j + 1, k), vm = V(i, j - 1, k);
+  double v2 = ((c + vp) * (c + vp) - (c + vm) * (c + vm)) / m.dy12[j];
+  double v3 = ((W(i, j + 1, k) * m.dy[j + 1] + W(i, j, k) * m.dy[j]) * (V(i, j, k + 1) + c) -
+               (W(i, j + 1, k - 1) * m.dy[j + 1] + W(i, j, k - 1) * m.dy[j]) *
+                   (c + V(i, j, k - 1))) /
+              m.dy12[j] / m.dz[k];
+  return 0.25 * (-v1 - v2 - v3) * dt;
+}
+
+// SET_CONVECT w (nse_dominik.f:814-825)
+__device__ inline double conv_w(const F3<double>& U, const F3<double>& V, const F3<double>& W,
+                                const Metrics& m, int i, int j, int k, double dt) {
+  double c = W(i, j, k);
+  double w1 = ((U(i, j, k + 1) * m.dz[k + 1] + U(i, j, k) * m.dz[k]) * (W(i + 1, j, k) + c) -
+               (U(i - 1, j, k + 1) * m.dz[k + 1] + U(i - 1, j, k) * m.dz[k]) *
+                   (c + W(i - 1, j, k))) /
+              m.dz12[k] / m.dx[i];
+  double w2 = ((V(i, j, k + 1) * m.dz[k + 1] + V(i, j, k) * m.dz[k]) * (W(i, j + 1, k) + c) -
+               (V(i, j - 1, k + 1) * m.dz[k + 1] + V(i, j - 1, k) * m.dz[k]) *
+                   (c + W(i, j - 1, k))) /
+              m.dz12[k] / m.dy[j];
+  double wp = W(i, j, k + 1), wm = W(i, j, k - 1);
+  double w3 = ((c + wp) * (c + wp) - (c + wm) * (c + wm)) / m.dz12[k];
+  return 0.25 * (-w1 - w2 - w3) * dt;
+}
+
+// SET_CONVECTION_FOR_SCALAR (nse_dominik.f:852-879); `sgn` = -1 gives the
+// forward form (-UT-VT-WT), +1 the adjoint.f:149-176 form (UT+VT+WT) with
+// the adjoint velocity in place of u.
+__device__ inline double conv_scalar(const F3<double>& U, const F3<double>& V,
+                                     const F3<double>& W, const F3<double>& T,
+                                     const F3<double>& RC, const Metrics& m, int i, int j, int k,
+                                     double dt, bool adjoint_form) {
+  double rc = RC(i, j, k), tc = T(i, j, k);
+  double ut = (-((RC(i - 1, j, k) * T(i - 1, j, k) * m.dx[i] + rc * tc * m.dx[i - 1]) *
+                 U(i - 1, j, k)) /
+                   2.0 / m.dx12[i - 1] +
+               ((rc * tc * m.dx[i + 1] + RC(i + 1, j, k) * T(i + 1, j, k) * m.dx[i]) *
+                U(i, j, k)) /
+                   2.0 / m.dx12[i]) /
+              m.dx[i];
+  double vt = (-((RC(i, j - 1, k) * T(i, j - 1, k) * m.dy[j] + rc * tc * m.dy[j - 1]) *
+                 V(i, j - 1, k)) /
+                   2.0 / m.dy12[j - 1] +
+               ((rc * tc * m.dy[j + 1] + RC(i, j + 1, k) * T(i, j + 1, k) * m.dy[j]) *
+                V(i, j, k)) /
+                   2.0 / m.dy12[j]) /
+              m.dy[j];
+  double wt = (-((RC(i, j, k - 1) * T(i, j, k - 1) * m.dz[k] + rc * tc * m.dz[k - 1]) *
+                 W(i, j, k - 1)) /
+                   2.0 / m.dz12[k - 1] +
+               ((rc * tc * m.dz[k + 1] + RC(i, j, k + 1) * T(i, j, k + 1) * m.dz[k]) *
+                W(i, j, k)) /
+                   2.0 / m.dz12[k]) /
+              m.dz[k];
+  if (adjoint_form) return (ut + vt + wt) * dt / rc + 1.0 * dt;
+  return (-ut - vt - wt) * dt / rc + 1.0 * dt;
+}
+
+// SET_CONDUCTION (nse_dominik.f:903-939); c0 = DT/RE/2.0
+__device__ inline double cond_u(const F3<double>& U, const Metrics& m, int i, int j, int k,
+                                double c0) {
+  double c = U(i, j, k);
+  double u1 = ((U(i + 1, j, k) - c) / m.dx[i + 1] - (c - U(i - 1, j, k)) / m.dx[i]) / m.dx12[i];
+  double u2 =
+      ((U(i, j + 1, k) - c) / m.dy12[j] - (c - U(i, j - 1, k)) / m.dy12[j - 1]) / m.dy[j];
+  double u3 =
+      ((U(i, j, k + 1) - c) / m.dz12[k] - (c - U(i, j, k - 1)) / m.dz12[k - 1]) / m.dz[k];
+  return c0 * (u1 + u2 + u3);
+}
+__device__ inline double cond_v(const F3<double>& V, const Metrics& m, int i, int j, int k,
+                                double c0) {
+  double c = V(i, j, k);
+  double v1 =
+      ((V(i + 1, j, k) - c) / m.dx12[i] - (c - V(i - 1, j, k)) / m.dx12[i - 1]) / m.dx[i];
+  double v2 = ((V(i, j + 1, k) - c) / m.dy[j + 1] - (c - V(i, j - 1, k)) / m.dy[j]) / m.dy12[j];
+  double v3 =
+      ((V(i, j, k + 1) - c) / m.dz12[k] - (c - V(i, j, k - 1)) / m.dz12[k - 1]) / m.dz[k];
+  return c0 * (v1 + v2 + v3);
+}
+__device__ inline double cond_w(const F3<double>& W, const Metrics& m, int i, int j, int k,
+                                double c0) {
+  double c = W(i, j, k);
+  double w1 =
+      ((W(i + 1, j, k) - c) / m.dx12[i] - (c - W(i - 1, j, k)) / m.dx12[i - 1]) / m.dx[i];
+  double w2 =
+      ((W(i, j + 1, k) - c) / m.dy12[j] - (c - W(i, j - 1, k)) / m.dy12[j - 1]) / m.dy[j];
+  double w3 = ((W(i, j, k + 1) - c) / m.dz[k + 1] - (c - W(i, j, k - 1)) / m.dz[k]) / m.dz12[k];
+  return c0 * (w1 + w2 + w3);
+}
+
+// (d1+d2)/(d1/k1+d2/k2), the harmonic face conductance
+__device__ inline double hmean(double d1, double k1, double d2, double k2) {
+  return (d1 + d2) / (d1 / k1 + d2 / k2);
+}
+
+// SET_DIFFUSION_FOR_SCALAR (nse_dominik.f:969-999)
+__device__ inline double diff_scalar(const F3<double>& T, const F3<double>& K,
+                                     const F3<double>& RC, const Metrics& m, int i, int j,
+                                     int k, double re, double sc, double dt) {
+  double tc = T(i, j, k), kc = K(i, j, k);
+  double ut1 = (hmean(m.dx[i + 1], K(i + 1, j, k), m.dx[i], kc) * (T(i + 1, j, k) - tc) /
+                    m.dx12[i] -
+                hmean(m.dx[i - 1], K(i - 1, j, k), m.dx[i], kc) * (tc - T(i - 1, j, k)) /
+                    m.dx12[i - 1]) /
+               m.dx[i];
+  double ut2 = (hmean(m.dy[j + 1], K(i, j + 1, k), m.dy[j], kc) * (T(i, j + 1, k) - tc) /
+                    m.dy12[j] -
+                hmean(m.dy[j - 1], K(i, j - 1, k), m.dy[j], kc) * (tc - T(i, j - 1, k)) /
+                    m.dy12[j - 1]) /
+               m.dy[j];
+  double ut3 = (hmean(m.dz[k + 1], K(i, j, k + 1), m.dz[k], kc) * (T(i, j, k + 1) - tc) /
+                    m.dz12[k] -
+                hmean(m.dz[k - 1], K(i, j, k - 1), m.dz[k], kc) * (tc - T(i, j, k - 1)) /
+                    m.dz12[k - 1]) /
+               m.dz[k];
+  return (ut1 + ut2 + ut3) / 2.0 / re / sc / RC(i, j, k) * dt;
+}
+
+}  // namespace vk

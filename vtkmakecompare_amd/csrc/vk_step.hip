@@ -1,0 +1,175 @@
+// vk_step.hip -- one pass of the main-loop body (nse_dominik.f:290-448)
+// with its pointwise phases fused:
+//
+//   SET_BC(u,v,w,p,T)                                     halo kernel
+//   A: SET_CONVECT, SET_CONVECTION_FOR_SCALAR, SET_CONDUCTION,
+//      SET_DIFFUSION_FOR_SCALAR, SET_IMMERSED_BOUNDARY, SET_EXPLICIT,
+//      set_explicit_for_tempreture, the T divide (:387-396) and the
+//      penalisation divides of solve_u_temp              one kernel
+//   B-D: ADI sweeps of solve_u_temp (y,z,x) and solve_t (y,x,z)
+//   SET_BC(u_temp,v_temp,w_temp,p,T)                      halo kernel
+//   MKP                                                   FFT/solve/project
+//
+// Intermediate arrays of the reference (u_v, u_d, u_l, u_a, u_vT, u_dT,
+// T_l, u_T) never reach HBM: phase A writes the divided right-hand sides
+// straight into u_temp/v_temp/w_temp and a temperature work field, and
+// the sweeps solve in place.  Every value written is computed with the
+// reference's operation order.
+#include "vk_kernels.h"
+#include "vk_runtime.h"
+#include "vk_stencil_ops.h"
+
+namespace vk {
+
+struct ExplicitArgs {
+  const double *u, *v, *w, *t, *rc, *thc, *u_ta, *v_ta, *w_ta, *cv;
+  double *u_pre, *v_pre, *w_pre, *t_pre;
+  double *ua, *va, *wa, *ut;
+  double dt, re, sc, eta, a1, a2, b1, c0;
+};
+
+__global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, ExplicitArgs x) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  int k = 1 + blockIdx.z;
+  if (i > g.nx || j > g.ny) return;
+  F3<double> U{x.u, g}, V{x.v, g}, W{x.w, g}, T{x.t, g}, RC{x.rc, g}, K{x.thc, g};
+  const long long o = g.at(i, j, k);
+  const double dt = x.dt, eta = x.eta;
+  // u: SET_CONVECT, SET_CONDUCTION, u_l, SET_EXPLICIT, solve_u_temp divide
+  {
+    double uv = conv_u(U, V, W, m, i, j, k, dt);
+    double ud = cond_u(U, m, i, j, k, x.c0);
+    double cv2 = x.cv[g.cv(i, j, k, 2)];
+    double ul = eta * cv2 * x.u_ta[o] * dt;
+    double ua = x.a1 * uv + x.a2 * x.u_pre[o] + x.b1 * ud + x.b1 * ul;
+    x.u_pre[o] = uv;
+    x.ua[o] = (x.u[o] + ua) / (1.0 + eta * cv2 * dt);
+  }
+  {
+    double vv = conv_v(U, V, W, m, i, j, k, dt);
+    double vd = cond_v(V, m, i, j, k, x.c0);
+    double cv3 = x.cv[g.cv(i, j, k, 3)];
+    double vl = eta * cv3 * x.v_ta[o] * dt;
+    double va = x.a1 * vv + x.a2 * x.v_pre[o] + x.b1 * vd + x.b1 * vl;
+    x.v_pre[o] = vv;
+    x.va[o] = (x.v[o] + va) / (1.0 + eta * cv3 * dt);
+    // SET_EXPLICIT's reflection v_a(i,ny+1,k) = -v_a(i,ny,k) survives into
+    // v_temp (no later routine writes that plane of v)
+    if (j == g.ny) x.va[g.at(i, g.ny + 1, k)] = -1.0 * va;
+  }
+  {
+    double wv = conv_w(U, V, W, m, i, j, k, dt);
+    double wd = cond_w(W, m, i, j, k, x.c0);
+    double cv4 = x.cv[g.cv(i, j, k, 4)];
+    double wl = eta * cv4 * x.w_ta[o] * dt;
+    double wa = x.a1 * wv + x.a2 * x.w_pre[o] + x.b1 * wd + x.b1 * wl;
+    x.w_pre[o] = wv;
+    x.wa[o] = (x.w[o] + wa) / (1.0 + eta * cv4 * dt);
+  }
+  // T: convection, diffusion, T_l = 0, combine, history, main-loop divide
+  {
+    double vt = conv_scalar(U, V, W, T, RC, m, i, j, k, dt, false);
+    double dtt = diff_scalar(T, K, RC, m, i, j, k, x.re, x.sc, dt);
+    double ut = x.a1 * vt + x.a2 * x.t_pre[o] + x.b1 * dtt + x.b1 * 0.0;
+    x.t_pre[o] = vt;
+    x.ut[o] = (x.t[o] + ut) / (1.0 + eta * x.cv[g.cv(i, j, k, 1)] * dt);
+  }
+}
+
+__global__ void k_copy_interior_step(Geom g, const double* src, double* dst) {
+  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  int k = 1 + blockIdx.z;
+  if (i > g.nx || j > g.ny) return;
+  long long o = g.at(i, j, k);
+  dst[o] = src[o];
+}
+
+}  // namespace vk
+
+using namespace vk;
+
+extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* t,
+                            double* u_pre, double* v_pre, double* w_pre, double* t_pre,
+                            double* u_ta, double* v_ta, double* w_ta, double* u_temp,
+                            double* v_temp, double* w_temp, double* divu, double* cellvalue,
+                            const double* rhocp, const double* thc, double* u_ep2,
+                            double* v_ep2, double* w_ep2, const double* dx, const double* dy,
+                            const double* dz, const double* dx12, const double* dy12,
+                            const double* dz12, const int* nx, const int* ny, const int* nz,
+                            const double* dt, const double* re, const double* sc,
+                            const double* eta, const double* xl, const double* zl,
+                            const int* l) {
+  VK_REQUIRE_INIT();
+  Geom g(*nx, *ny, *nz);
+  if (g.nx < 4 || g.ny < 4 || g.nz < 4) return fail(VK_ERR_BAD_SIZE, "time_step: n >= 4");
+  const size_t fb = g.cells() * sizeof(double);
+  VK_RESOLVE(du, double, u, fb);
+  VK_RESOLVE(dv, double, v, fb);
+  VK_RESOLVE(dw, double, w, fb);
+  VK_RESOLVE(dp, double, p, fb);
+  VK_RESOLVE(dtt, double, t, fb);
+  VK_RESOLVE(dup, double, u_pre, fb);
+  VK_RESOLVE(dvp, double, v_pre, fb);
+  VK_RESOLVE(dwp, double, w_pre, fb);
+  VK_RESOLVE(dtp, double, t_pre, fb);
+  VK_RESOLVE(duta, double, u_ta, fb);
+  VK_RESOLVE(dvta, double, v_ta, fb);
+  VK_RESOLVE(dwta, double, w_ta, fb);
+  VK_RESOLVE(dut, double, u_temp, fb);
+  VK_RESOLVE(dvt, double, v_temp, fb);
+  VK_RESOLVE(dwt, double, w_temp, fb);
+  VK_RESOLVE(ddiv, double, divu, fb);
+  VK_RESOLVE(dcv, double, cellvalue, g.interior() * 4 * sizeof(double));
+  VK_RESOLVE(drc, const double, rhocp, fb);
+  VK_RESOLVE(dk, const double, thc, fb);
+  double *de1 = nullptr, *de2 = nullptr, *de3 = nullptr;
+  if (u_ep2 && v_ep2 && w_ep2) {
+    de1 = static_cast<double*>(resolve(u_ep2, fb, "u_ep2"));
+    de2 = static_cast<double*>(resolve(v_ep2, fb, "v_ep2"));
+    de3 = static_cast<double*>(resolve(w_ep2, fb, "w_ep2"));
+    if (!de1 || !de2 || !de3) return VK_ERR_NOT_RESIDENT;
+  }
+  VK_RESOLVE(ddx, const double, dx, (g.nx + 2) * sizeof(double));
+  VK_RESOLVE(ddy, const double, dy, (g.ny + 2) * sizeof(double));
+  VK_RESOLVE(ddz, const double, dz, (g.nz + 2) * sizeof(double));
+  VK_RESOLVE(ddx12, const double, dx12, (g.nx + 2) * sizeof(double));
+  VK_RESOLVE(ddy12, const double, dy12, (g.ny + 2) * sizeof(double));
+  VK_RESOLVE(ddz12, const double, dz12, (g.nz + 2) * sizeof(double));
+  Metrics m{ddx, ddy, ddz, ddx12, ddy12, ddz12};
+  double* twork = static_cast<double*>(workspace(kWorkGrad, fb));
+  if (!twork) return VK_ERR_HIP;
+  hipStream_t s = rt().stream;
+
+  launch_set_bc(g, du, dv, dw, dp, dtt, s);
+  if (*l == 1) launch_cv_faces(g, dcv, s);
+  ExplicitArgs x;
+  x.u = du; x.v = dv; x.w = dw; x.t = dtt; x.rc = drc; x.thc = dk;
+  x.u_ta = duta; x.v_ta = dvta; x.w_ta = dwta; x.cv = dcv;
+  x.u_pre = dup; x.v_pre = dvp; x.w_pre = dwp; x.t_pre = dtp;
+  x.ua = dut; x.va = dvt; x.wa = dwt; x.ut = twork;
+  x.dt = *dt; x.re = *re; x.sc = *sc; x.eta = *eta;
+  if (*l == 1) {
+    x.a1 = 1.0; x.a2 = 0.0; x.b1 = 1.0;
+  } else {
+    x.a1 = 1.5; x.a2 = -0.5; x.b1 = 1.0;
+  }
+  x.c0 = *dt / *re / 2.0;
+  k_explicit_fused<<<interior_grid(g), interior_block(), 0, s>>>(g, m, x);
+
+  const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
+  double* comps[3] = {dut, dvt, dwt};
+  for (int c = 0; c < 3; c++)
+    if (launch_adi_velocity(g, m, comps[c], dcv, c, beta_u, dteta, s))
+      return fail(VK_ERR_BAD_SIZE, "time_step: line length not supported");
+  if (launch_adi_temperature(g, m, twork, dcv, drc, dk, beta_t, dteta, s))
+    return fail(VK_ERR_BAD_SIZE, "time_step: line length not supported");
+  k_copy_interior_step<<<interior_grid(g), interior_block(), 0, s>>>(g, twork, dtt);
+  launch_set_bc(g, dut, dvt, dwt, dp, dtt, s);
+  MkpOut o{dp, duta, dvta, dwta, de1, de2, de3, ddiv, du, dv, dw};
+  int rc = launch_mkp(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s);
+  if (rc) return rc;
+  VK_LAUNCH_CHECK("time_step");
+  return VK_OK;
+}

@@ -1,0 +1,413 @@
+// vk_tridiag.hip -- the implicit line solves: solve_u_temp and solve_t
+// (ADI sweeps y -> z -> x for u, v, w; y -> x -> z for T), SPN and SP0B of
+// the Poisson solve.  Each reference line system is a row functor for the
+// partition solver of vk_partition.h; the coefficient expressions keep the
+// reference's operation order.
+#include "vk_kernels.h"
+#include "vk_partition.h"
+#include "vk_runtime.h"
+
+namespace vk {
+
+// ------------------------------------------------------ y line systems
+
+// implicit_uy (nse_dominik.f:1505-1577) on line (i,k): rows j = 0..ny+1,
+// ghost rows x0 + x1 = 0 and x_ny + x_{ny+1} = 0 (wall reflection); all
+// rows are written back as parallel_uy does (nse_dominik.f:2907-2909).
+struct SysYFace {
+  using V = double;
+  static constexpr bool kCyclic = false;
+  Geom g;
+  double* f;
+  const double* cv;
+  const double *dy, *dy12;
+  double beta, dteta;
+  int ncv;
+  __device__ int n() const { return g.ny + 2; }
+  __device__ bool valid(int la, int lb) const { return la < g.nx; }
+  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, double& d) const {
+    int i = la + 1, k = lb + 1;
+    if (r == 0) {
+      a = 0.0; b = 1.0; c = 1.0; d = 0.0;
+    } else if (r == g.ny + 1) {
+      a = 1.0; b = 1.0; c = 0.0; d = 0.0;
+    } else {
+      int j = r;
+      double alpha = dteta * cv[g.cv(i, j, k, ncv)] + 1.0;
+      a = -beta / alpha / dy12[j - 1] / dy[j];
+      b = 1.0 + beta / alpha * (1.0 / dy12[j] + 1.0 / dy12[j - 1]) / dy[j];
+      c = -beta / alpha / dy12[j] / dy[j];
+      d = f[g.at(i, j, k)];
+    }
+  }
+  __device__ void store(int la, int lb, int r, double x) const {
+    f[g.at(la + 1, r, lb + 1)] = x;
+  }
+};
+
+// implicit_vy (nse_dominik.f:1583-1648): rows j = 0..ny with x0 = x_ny = 0
+struct SysYV {
+  using V = double;
+  static constexpr bool kCyclic = false;
+  Geom g;
+  double* f;
+  const double* cv;
+  const double *dy, *dy12;
+  double beta, dteta;
+  __device__ int n() const { return g.ny + 1; }
+  __device__ bool valid(int la, int lb) const { return la < g.nx; }
+  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, double& d) const {
+    int i = la + 1, k = lb + 1;
+    if (r == 0 || r == g.ny) {
+      a = 0.0; b = 1.0; c = 0.0; d = 0.0;
+    } else {
+      int j = r;
+      double alpha = dteta * cv[g.cv(i, j, k, 3)] + 1.0;
+      a = -beta / alpha / dy[j] / dy12[j];
+      b = 1.0 + beta / alpha * (1.0 / dy[j + 1] + 1.0 / dy[j]) / dy12[j];
+      c = -beta / alpha / dy[j + 1] / dy12[j];
+      d = f[g.at(i, j, k)];
+    }
+  }
+  __device__ void store(int la, int lb, int r, double x) const {
+    f[g.at(la + 1, r, lb + 1)] = x;
+  }
+};
+
+__device__ inline double hm(double d1, double k1, double d2, double k2) {
+  return (d1 + d2) / (d1 / k1 + d2 / k2);
+}
+
+// implicit_Ty (nse_dominik.f:1784-1870) with lly(0) = lly(1) and
+// lly(ny+1) = lly(ny); only rows 1..ny are written back (parallel_Ty).
+struct SysYT {
+  using V = double;
+  static constexpr bool kCyclic = false;
+  Geom g;
+  double* f;
+  const double *cv, *rc, *thc;
+  const double *dy, *dy12;
+  double beta, dteta;
+  __device__ int n() const { return g.ny + 2; }
+  __device__ bool valid(int la, int lb) const { return la < g.nx; }
+  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, double& d) const {
+    int i = la + 1, k = lb + 1;
+    if (r == 0) {
+      a = 0.0; b = 1.0; c = 1.0; d = 0.0;
+    } else if (r == g.ny + 1) {
+      a = 1.0; b = 1.0; c = 0.0; d = 0.0;
+    } else {
+      int j = r;
+      double alpha = dteta * cv[g.cv(i, j, k, 1)] + 1.0;
+      double l0 = thc[g.at(i, j, k)];
+      double lm = (j == 1) ? l0 : thc[g.at(i, j - 1, k)];
+      double lp = (j == g.ny) ? l0 : thc[g.at(i, j + 1, k)];
+      double rr = rc[g.at(i, j, k)];
+      double h1 = hm(dy[j], l0, dy[j - 1], lm);
+      double h2 = hm(dy[j + 1], lp, dy[j], l0);
+      a = -beta / alpha * h1 / dy12[j - 1] / dy[j] / rr;
+      b = 1.0 + beta / alpha * (h2 / dy12[j] / dy[j] + h1 / dy12[j - 1] / dy[j]) / rr;
+      c = -beta / alpha * h2 / dy12[j] / dy[j] / rr;
+      d = f[g.at(i, j, k)];
+    }
+  }
+  __device__ void store(int la, int lb, int r, double x) const {
+    if (r >= 1 && r <= g.ny) f[g.at(la + 1, r, lb + 1)] = x;
+  }
+};
+
+// --------------------------------------------------- periodic systems
+
+// A periodic line along axis `dir` (0 = x, 2 = z).  Lines are indexed
+// (la, lb): for z lines (i, j) = (la+1, lb+1); for x lines (j, k) =
+// (la+1, lb+1).  Row r is grid index q = r + 1.
+struct LineAxis {
+  Geom g;
+  int dir;
+  __device__ long long at(int la, int lb, int q) const {
+    return dir == 2 ? g.at(la + 1, lb + 1, q) : g.at(q, la + 1, lb + 1);
+  }
+  __device__ long long cvat(int la, int lb, int q, int n) const {
+    return dir == 2 ? g.cv(la + 1, lb + 1, q, n) : g.cv(q, la + 1, lb + 1, n);
+  }
+  __device__ int len() const { return dir == 2 ? g.nz : g.nx; }
+};
+
+// implicit_solver + cyclic (periodic_implicit.f:1-93) on rows q = 1..n;
+// the ghost values the reference writes at q = 0 and n+1 (x0 = xn,
+// x_{n+1} = x1) are written too.
+struct SysPeriodicFace {
+  using V = double;
+  static constexpr bool kCyclic = true;
+  LineAxis ax;
+  double* f;
+  const double* cv;
+  const double *d, *d12;
+  double beta, dteta;
+  int ncv, nla, nlb;
+  __device__ int n() const { return ax.len(); }
+  __device__ bool valid(int la, int lb) const { return la < nla && lb < nlb; }
+  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, double& rhs) const {
+    int q = r + 1;
+    double alpha = dteta * cv[ax.cvat(la, lb, q, ncv)] + 1.0;
+    a = -beta / alpha / d[q] / d12[q];
+    b = 1.0 + beta / alpha * (1.0 / d[q + 1] + 1.0 / d[q]) / d12[q];
+    c = -beta / alpha / d[q + 1] / d12[q];
+    rhs = f[ax.at(la, lb, q)];
+  }
+  __device__ void store(int la, int lb, int r, double x) const {
+    int q = r + 1, n = ax.len();
+    f[ax.at(la, lb, q)] = x;
+    if (q == n) f[ax.at(la, lb, 0)] = x;
+    if (q == 1) f[ax.at(la, lb, n + 1)] = x;
+  }
+};
+
+// implicit_Tx (nse_dominik.f:1725-1781), used along x and z by solve_t,
+// with the conductivity ghosts wrapped periodically; rows 1..n written.
+struct SysPeriodicT {
+  using V = double;
+  static constexpr bool kCyclic = true;
+  LineAxis ax;
+  double* f;
+  const double *cv, *rc, *thc;
+  const double *d, *d12;
+  double beta, dteta;
+  int nla, nlb;
+  __device__ int n() const { return ax.len(); }
+  __device__ bool valid(int la, int lb) const { return la < nla && lb < nlb; }
+  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, double& rhs) const {
+    int q = r + 1, n = ax.len();
+    double alpha = dteta * cv[ax.cvat(la, lb, q, 1)] + 1.0;
+    double l0 = thc[ax.at(la, lb, q)];
+    double lm = thc[ax.at(la, lb, q == 1 ? n : q - 1)];
+    double lp = thc[ax.at(la, lb, q == n ? 1 : q + 1)];
+    double rr = rc[ax.at(la, lb, q)];
+    double h1 = hm(d[q], l0, d[q - 1], lm);
+    double h2 = hm(d[q + 1], lp, d[q], l0);
+    a = -beta / alpha * h1 / d12[q - 1] / d[q] / rr;
+    b = 1.0 + beta / alpha * (h2 / d12[q] / d[q] + h1 / d12[q - 1] / d[q]) / rr;
+    c = -beta / alpha * h2 / d12[q] / d[q] / rr;
+    rhs = f[ax.at(la, lb, q)];
+  }
+  __device__ void store(int la, int lb, int r, double x) const { f[ax.at(la, lb, r + 1)] = x; }
+};
+
+// -------------------------------------------------- spectral y solves
+
+// SPN + CTRIDIAG (mkp.f:235-258, tridiag.f:43-58) for mode (I,K) =
+// (la, lb), I = 0..nx/2, K = 0..nz-1, rows J = 1..ny with the Neumann
+// fold of the first and last rows.  The zero mode is left to SP0B.
+struct SysSpectral {
+  using V = dcx;
+  static constexpr bool kCyclic = false;
+  Geom g;
+  double* phit;  // interleaved complex
+  const double *ac, *kxz;
+  __device__ int n() const { return g.ny; }
+  __device__ bool valid(int la, int lb) const { return la <= g.nx / 2 && lb < g.nz; }
+  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, dcx& d) const {
+    int J = r + 1, ny = g.ny;
+    double a1 = ac[J - 1], a2 = ac[J - 1 + ny] - kxz[la + (g.nx / 2 + 1) * lb], a3 = ac[J - 1 + 2 * ny];
+    if (J == 1) {
+      a2 = a2 + a1;
+      a1 = 0.0;
+    }
+    if (J == ny) {
+      a2 = a2 + a3;
+      a3 = 0.0;
+    }
+    a = a1;
+    b = a2;
+    c = a3;
+    long long o = 2 * g.spec_at(la, J, lb);
+    d = {phit[o], phit[o + 1]};
+  }
+  __device__ void store(int la, int lb, int r, dcx x) const {
+    if (la == 0 && lb == 0) return;
+    long long o = 2 * g.spec_at(la, r + 1, lb);
+    phit[o] = x.x;
+    phit[o + 1] = x.y;
+  }
+};
+
+// SP0B (mkp.f:263-290): the zero mode by the forward march with
+// phi(0) = phi(1) = 0; one block loads the mode in parallel, one lane
+// marches, the block stores.
+__global__ void k_sp0b(Geom g, const double* ac, double* phit) {
+  extern __shared__ double sh[];  // 2*(ny+2) doubles
+  int ny = g.ny;
+  for (int J = threadIdx.x; J <= ny + 1; J += blockDim.x) {
+    long long o = 2 * g.spec_at(0, J, 0);
+    sh[2 * J] = phit[o];
+    sh[2 * J + 1] = phit[o + 1];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double d0r = 0.0, d0i = 0.0;
+    double prevr = 0.0, previ = 0.0, curr = 0.0, curi = 0.0;  // phi(J-1), phi(J)
+    (void)d0r; (void)d0i;
+    for (int J = 1; J <= ny - 1; J++) {
+      double a1 = ac[J - 1], a2 = ac[J - 1 + ny], a3 = ac[J - 1 + 2 * ny];
+      double dr = sh[2 * J], di = sh[2 * J + 1];
+      double nr = (dr - a1 * prevr - a2 * curr) / a3;
+      double ni = (di - a1 * previ - a2 * curi) / a3;
+      sh[2 * J] = curr;  // reuse: slot J now holds phi(J)
+      sh[2 * J + 1] = curi;
+      prevr = curr; previ = curi;
+      curr = nr; curi = ni;
+    }
+    sh[2 * ny] = curr;
+    sh[2 * ny + 1] = curi;
+    sh[0] = 0.0;
+    sh[1] = 0.0;
+    sh[2 * (ny + 1)] = curr;
+    sh[2 * (ny + 1) + 1] = curi;
+  }
+  __syncthreads();
+  for (int J = threadIdx.x; J <= ny + 1; J += blockDim.x) {
+    long long o = 2 * g.spec_at(0, J, 0);
+    phit[o] = sh[2 * J];
+    phit[o + 1] = sh[2 * J + 1];
+  }
+}
+
+int launch_spn(const Geom& g, double* phit, const double* ac, const double* kxz, hipStream_t s) {
+  SysSpectral sys{g, phit, ac, kxz};
+  return launch_line_solve<true>(sys, g.ny, g.nx / 2 + 1, g.nz, s);
+}
+
+void launch_sp0b(const Geom& g, double* phit, const double* ac, hipStream_t s) {
+  k_sp0b<<<1, 256, 2 * (g.ny + 2) * sizeof(double), s>>>(g, ac, phit);
+}
+
+// solve_u_temp body after the penalisation divide: y, z, x sweeps of one
+// component.  comp 0,1,2 = u, v, w (cellvalue slot comp+2); v's z and x
+// sweeps stop at j = ny-1 (nse_dominik.f:1411,1418).
+int launch_adi_velocity(const Geom& g, const Metrics& m, double* f, const double* cv, int comp,
+                        double beta, double dteta, hipStream_t s) {
+  int rc;
+  if (comp == 1) {
+    SysYV sy{g, f, cv, m.dy, m.dy12, beta, dteta};
+    rc = launch_line_solve<true>(sy, g.ny + 1, g.nx, g.nz, s);
+  } else {
+    SysYFace sy{g, f, cv, m.dy, m.dy12, beta, dteta, comp + 2};
+    rc = launch_line_solve<true>(sy, g.ny + 2, g.nx, g.nz, s);
+  }
+  if (rc) return rc;
+  int jmax = (comp == 1) ? g.ny - 1 : g.ny;
+  SysPeriodicFace sz{LineAxis{g, 2}, f, cv, m.dz, m.dz12, beta, dteta, comp + 2, g.nx, jmax};
+  rc = launch_line_solve<true>(sz, g.nz, g.nx, jmax, s);
+  if (rc) return rc;
+  SysPeriodicFace sx{LineAxis{g, 0}, f, cv, m.dx, m.dx12, beta, dteta, comp + 2, jmax, g.nz};
+  return launch_line_solve<false>(sx, g.nx, jmax, g.nz, s);
+}
+
+// solve_t: y, x, z sweeps (nse_dominik.f:1477-1500)
+int launch_adi_temperature(const Geom& g, const Metrics& m, double* f, const double* cv,
+                           const double* rc_, const double* thc, double beta, double dteta,
+                           hipStream_t s) {
+  SysYT sy{g, f, cv, rc_, thc, m.dy, m.dy12, beta, dteta};
+  int rc = launch_line_solve<true>(sy, g.ny + 2, g.nx, g.nz, s);
+  if (rc) return rc;
+  SysPeriodicT sx{LineAxis{g, 0}, f, cv, rc_, thc, m.dx, m.dx12, beta, dteta, g.ny, g.nz};
+  rc = launch_line_solve<false>(sx, g.nx, g.ny, g.nz, s);
+  if (rc) return rc;
+  SysPeriodicT sz{LineAxis{g, 2}, f, cv, rc_, thc, m.dz, m.dz12, beta, dteta, g.nx, g.ny};
+  return launch_line_solve<true>(sz, g.nz, g.nx, g.ny, s);
+}
+
+}  // namespace vk
+
+using namespace vk;
+
+extern "C" {
+
+int vk_solve_u_temp(double* u_a, double* v_a, double* w_a, const int* nx, const int* ny,
+                    const int* nz, const double* dx, const double* dy, const double* dz,
+                    const double* dx12, const double* dy12, const double* dz12, const double* re,
+                    const double* dt, const double* eta, const double* cellvalue,
+                    const double* u, const double* v, const double* w) {
+  VK_REQUIRE_INIT();
+  Geom g(*nx, *ny, *nz);
+  if (g.nx < 4 || g.ny < 4 || g.nz < 4) return fail(VK_ERR_BAD_SIZE, "solve_u_temp: n >= 4");
+  size_t fb = g.cells() * sizeof(double);
+  VK_RESOLVE(dua, double, u_a, fb);
+  VK_RESOLVE(dva, double, v_a, fb);
+  VK_RESOLVE(dwa, double, w_a, fb);
+  VK_RESOLVE(du, const double, u, fb);
+  VK_RESOLVE(dv, const double, v, fb);
+  VK_RESOLVE(dw, const double, w, fb);
+  VK_RESOLVE(dcv, const double, cellvalue, g.interior() * 4 * sizeof(double));
+  VK_RESOLVE(ddx, const double, dx, (g.nx + 2) * sizeof(double));
+  VK_RESOLVE(ddy, const double, dy, (g.ny + 2) * sizeof(double));
+  VK_RESOLVE(ddz, const double, dz, (g.nz + 2) * sizeof(double));
+  VK_RESOLVE(ddx12, const double, dx12, (g.nx + 2) * sizeof(double));
+  VK_RESOLVE(ddy12, const double, dy12, (g.ny + 2) * sizeof(double));
+  VK_RESOLVE(ddz12, const double, dz12, (g.nz + 2) * sizeof(double));
+  Metrics m{ddx, ddy, ddz, ddx12, ddy12, ddz12};
+  hipStream_t s = rt().stream;
+  double beta = *dt / *re / 2.0, dteta = *dt * *eta;
+  double* comps[3] = {dua, dva, dwa};
+  const double* olds[3] = {du, dv, dw};
+  for (int c = 0; c < 3; c++) {
+    launch_pen_divide(g, comps[c], olds[c], dcv, c + 2, *eta, *dt, g.ny, s);
+    if (launch_adi_velocity(g, m, comps[c], dcv, c, beta, dteta, s))
+      return fail(VK_ERR_BAD_SIZE, "solve_u_temp: line length not supported");
+  }
+  VK_LAUNCH_CHECK("solve_u_temp");
+  return VK_OK;
+}
+
+int vk_solve_t(double* u_t, const int* nx, const int* ny, const int* nz, const double* dx,
+               const double* dy, const double* dz, const double* dx12, const double* dy12,
+               const double* dz12, const double* re, const double* sc, const double* dt,
+               const double* rhocp, const double* thc, const double* cellvalue,
+               const double* eta) {
+  VK_REQUIRE_INIT();
+  Geom g(*nx, *ny, *nz);
+  if (g.nx < 4 || g.ny < 4 || g.nz < 4) return fail(VK_ERR_BAD_SIZE, "solve_t: n >= 4");
+  size_t fb = g.cells() * sizeof(double);
+  VK_RESOLVE(dut, double, u_t, fb);
+  VK_RESOLVE(drc, const double, rhocp, fb);
+  VK_RESOLVE(dk, const double, thc, fb);
+  VK_RESOLVE(dcv, const double, cellvalue, g.interior() * 4 * sizeof(double));
+  VK_RESOLVE(ddx, const double, dx, (g.nx + 2) * sizeof(double));
+  VK_RESOLVE(ddy, const double, dy, (g.ny + 2) * sizeof(double));
+  VK_RESOLVE(ddz, const double, dz, (g.nz + 2) * sizeof(double));
+  VK_RESOLVE(ddx12, const double, dx12, (g.nx + 2) * sizeof(double));
+  VK_RESOLVE(ddy12, const double, dy12, (g.ny + 2) * sizeof(double));
+  VK_RESOLVE(ddz12, const double, dz12, (g.nz + 2) * sizeof(double));
+  Metrics m{ddx, ddy, ddz, ddx12, ddy12, ddz12};
+  double beta = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
+  if (launch_adi_temperature(g, m, dut, dcv, drc, dk, beta, dteta, rt().stream))
+    return fail(VK_ERR_BAD_SIZE, "solve_t: line length not supported");
+  VK_LAUNCH_CHECK("solve_t");
+  return VK_OK;
+}
+
+int vk_spn(const int* nx, const int* ny, const int* nz, const double* ac, const double* kxz,
+           double* phit) {
+  VK_REQUIRE_INIT();
+  Geom g(*nx, *ny, *nz);
+  if (g.nx % 2 || g.ny < 2) return fail(VK_ERR_BAD_SIZE, "spn: nx even, ny >= 2");
+  VK_RESOLVE(dac, const double, ac, 3 * g.ny * sizeof(double));
+  VK_RESOLVE(dk, const double, kxz, (g.nx / 2 + 1) * (g.nz + 1) * sizeof(double));
+  VK_RESOLVE(dp, double, phit, 2 * g.spec_cells() * sizeof(double));
+  if (launch_spn(g, dp, dac, dk, rt().stream))
+    return fail(VK_ERR_BAD_SIZE, "spn: ny not supported");
+  VK_LAUNCH_CHECK("spn");
+  return VK_OK;
+}
+
+int vk_sp0b(const int* nx, const int* ny, const int* nz, const double* ac, double* phit) {
+  VK_REQUIRE_INIT();
+  Geom g(*nx, *ny, *nz);
+  VK_RESOLVE(dac, const double, ac, 3 * g.ny * sizeof(double));
+  VK_RESOLVE(dp, double, phit, 2 * g.spec_cells() * sizeof(double));
+  launch_sp0b(g, dp, dac, rt().stream);
+  VK_LAUNCH_CHECK("sp0b");
+  return VK_OK;
+}
+
+}  // extern "C"

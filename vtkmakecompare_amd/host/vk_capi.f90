@@ -1,0 +1,178 @@
+! vk_capi.f90 -- ISO_C_BINDING interfaces to the C ABI of include/vtkgpu.h.
+!
+! Every argument is passed by reference, exactly as the reference's
+! Fortran call sites pass it; arrays are assumed-size so any of the
+! reference's explicit-shape arrays (halo fields, cellvalue, 1-D metrics)
+! can be handed over unchanged.
+module vk_capi
+  use, intrinsic :: iso_c_binding
+  implicit none
+
+  interface
+    integer(c_int) function vk_init(device) bind(C, name="vk_init")
+      import :: c_int
+      integer(c_int), intent(in) :: device
+    end function
+    integer(c_int) function vk_finalize() bind(C, name="vk_finalize")
+      import :: c_int
+    end function
+    type(c_ptr) function vk_last_error() bind(C, name="vk_last_error")
+      import :: c_ptr
+    end function
+    integer(c_int) function vk_register(host, bytes) bind(C, name="vk_register")
+      import :: c_int, c_double, c_size_t
+      real(c_double) :: host(*)
+      integer(c_size_t), intent(in) :: bytes
+    end function
+    integer(c_int) function vk_upload(host) bind(C, name="vk_upload")
+      import :: c_int, c_double
+      real(c_double) :: host(*)
+    end function
+    integer(c_int) function vk_download(host) bind(C, name="vk_download")
+      import :: c_int, c_double
+      real(c_double) :: host(*)
+    end function
+    integer(c_int) function vk_synchronize() bind(C, name="vk_synchronize")
+      import :: c_int
+    end function
+
+    integer(c_int) function vk_set_bc(u, v, w, p, nx, ny, nz, t) bind(C, name="vk_set_bc")
+      import :: c_int, c_double
+      real(c_double) :: u(*), v(*), w(*), p(*), t(*)
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_set_convect(u, u_v, v, v_v, w, w_v, dt, nx, ny, nz, &
+        dx, dy, dz, dx12, dy12, dz12) bind(C, name="vk_set_convect")
+      import :: c_int, c_double
+      real(c_double) :: u(*), u_v(*), v(*), v_v(*), w(*), w_v(*)
+      real(c_double) :: dt, dx(*), dy(*), dz(*), dx12(*), dy12(*), dz12(*)
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_set_convection_for_scalar(u, v, w, dt, nx, ny, nz, &
+        dx, dy, dz, dx12, dy12, dz12, t, u_vt, rhocp) &
+        bind(C, name="vk_set_convection_for_scalar")
+      import :: c_int, c_double
+      real(c_double) :: u(*), v(*), w(*), dt, dx(*), dy(*), dz(*), dx12(*), dy12(*), dz12(*)
+      real(c_double) :: t(*), u_vt(*), rhocp(*)
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_set_conduction(u, u_d, v, v_d, w, w_d, re, dt, nx, ny, nz, &
+        dx, dy, dz, dx12, dy12, dz12) bind(C, name="vk_set_conduction")
+      import :: c_int, c_double
+      real(c_double) :: u(*), u_d(*), v(*), v_d(*), w(*), w_d(*), re, dt
+      real(c_double) :: dx(*), dy(*), dz(*), dx12(*), dy12(*), dz12(*)
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_set_diffusion_for_scalar(re, dt, nx, ny, nz, dx, dy, dz, &
+        dx12, dy12, dz12, t, u_dt, sc, thc, rhocp) bind(C, name="vk_set_diffusion_for_scalar")
+      import :: c_int, c_double
+      real(c_double) :: re, dt, dx(*), dy(*), dz(*), dx12(*), dy12(*), dz12(*)
+      real(c_double) :: t(*), u_dt(*), sc, thc(*), rhocp(*)
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_set_immersed_boundary(eta, nx, ny, nz, dt, u_l, v_l, w_l, &
+        dx, dy, dz, t_l, cellvalue, u_ta, v_ta, w_ta, l, cons_cell) &
+        bind(C, name="vk_set_immersed_boundary")
+      import :: c_int, c_double
+      real(c_double) :: eta, dt, u_l(*), v_l(*), w_l(*), dx(*), dy(*), dz(*), t_l(*)
+      real(c_double) :: cellvalue(*), u_ta(*), v_ta(*), w_ta(*), cons_cell
+      integer(c_int) :: nx, ny, nz, l
+    end function
+    integer(c_int) function vk_set_explicit(u_v, u_pre, u_d, u_a, v_v, v_pre, v_d, v_a, &
+        w_v, w_pre, w_d, w_a, u_l, v_l, w_l, nx, ny, nz, l) bind(C, name="vk_set_explicit")
+      import :: c_int, c_double
+      real(c_double) :: u_v(*), u_pre(*), u_d(*), u_a(*), v_v(*), v_pre(*), v_d(*), v_a(*)
+      real(c_double) :: w_v(*), w_pre(*), w_d(*), w_a(*), u_l(*), v_l(*), w_l(*)
+      integer(c_int) :: nx, ny, nz, l
+    end function
+    integer(c_int) function vk_set_explicit_for_tempreture(u_vt, u_dt, u_t, t_pre, t_l, &
+        nx, ny, nz, l) bind(C, name="vk_set_explicit_for_tempreture")
+      import :: c_int, c_double
+      real(c_double) :: u_vt(*), u_dt(*), u_t(*), t_pre(*), t_l(*)
+      integer(c_int) :: nx, ny, nz, l
+    end function
+    integer(c_int) function vk_t_divide(u_t, t, cellvalue, eta, dt, nx, ny, nz) &
+        bind(C, name="vk_t_divide")
+      import :: c_int, c_double
+      real(c_double) :: u_t(*), t(*), cellvalue(*), eta, dt
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_solve_u_temp(u_a, v_a, w_a, nx, ny, nz, dx, dy, dz, &
+        dx12, dy12, dz12, re, dt, eta, cellvalue, u, v, w) bind(C, name="vk_solve_u_temp")
+      import :: c_int, c_double
+      real(c_double) :: u_a(*), v_a(*), w_a(*), dx(*), dy(*), dz(*), dx12(*), dy12(*)
+      real(c_double) :: dz12(*), re, dt, eta, cellvalue(*), u(*), v(*), w(*)
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_solve_t(u_t, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12, &
+        re, sc, dt, rhocp, thc, cellvalue, eta) bind(C, name="vk_solve_t")
+      import :: c_int, c_double
+      real(c_double) :: u_t(*), dx(*), dy(*), dz(*), dx12(*), dy12(*), dz12(*)
+      real(c_double) :: re, sc, dt, rhocp(*), thc(*), cellvalue(*), eta
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_mkp(p, u_temp, v_temp, w_temp, dt, dx, dy, dz, dx12, dy12, &
+        dz12, re, nx, ny, nz, ompnthreads, u_ta, v_ta, w_ta, u_ep2, v_ep2, w_ep2, divu, &
+        u, v, w, xl, zl) bind(C, name="vk_mkp")
+      import :: c_int, c_double
+      real(c_double) :: p(*), u_temp(*), v_temp(*), w_temp(*), dt, dx(*), dy(*), dz(*)
+      real(c_double) :: dx12(*), dy12(*), dz12(*), re, u_ta(*), v_ta(*), w_ta(*)
+      real(c_double) :: u_ep2(*), v_ep2(*), w_ep2(*), divu(*), u(*), v(*), w(*), xl, zl
+      integer(c_int) :: nx, ny, nz, ompnthreads
+    end function
+    integer(c_int) function vk_set_levelset(nx, ny, nz, dx, dx12, dy12, dz12, phi0, phi, &
+        x, y, z, cellvalue) bind(C, name="vk_set_levelset")
+      import :: c_int, c_double
+      real(c_double) :: dx(*), dx12(*), dy12(*), dz12(*), phi0(*), phi(*), x(*), y(*), z(*)
+      real(c_double) :: cellvalue(*)
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_field_copy(src, dst, nx, ny, nz, interior_only) &
+        bind(C, name="vk_field_copy")
+      import :: c_int, c_double
+      real(c_double) :: src(*), dst(*)
+      integer(c_int) :: nx, ny, nz, interior_only
+    end function
+    integer(c_int) function vk_time_step(u, v, w, p, t, u_pre, v_pre, w_pre, t_pre, &
+        u_ta, v_ta, w_ta, u_temp, v_temp, w_temp, divu, cellvalue, rhocp, thc, &
+        u_ep2, v_ep2, w_ep2, dx, dy, dz, dx12, dy12, dz12, nx, ny, nz, dt, re, sc, eta, &
+        xl, zl, l) bind(C, name="vk_time_step")
+      import :: c_int, c_double
+      real(c_double) :: u(*), v(*), w(*), p(*), t(*), u_pre(*), v_pre(*), w_pre(*), t_pre(*)
+      real(c_double) :: u_ta(*), v_ta(*), w_ta(*), u_temp(*), v_temp(*), w_temp(*), divu(*)
+      real(c_double) :: cellvalue(*), rhocp(*), thc(*), u_ep2(*), v_ep2(*), w_ep2(*)
+      real(c_double) :: dx(*), dy(*), dz(*), dx12(*), dy12(*), dz12(*)
+      real(c_double) :: dt, re, sc, eta, xl, zl
+      integer(c_int) :: nx, ny, nz, l
+    end function
+  end interface
+
+contains
+
+  ! Stops with the library's message on a non-zero status (the reference
+  ! routines have no status argument, so a failure ends the run).
+  subroutine vk_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    character(len=512) :: text
+    integer :: n
+    if (rc == 0) return
+    call c_f_pointer(vk_last_error(), msg, [512])
+    text = ' '
+    do n = 1, 512
+      if (msg(n) == c_null_char) exit
+      text(n:n) = msg(n)
+    end do
+    write (0, '(a,a,i0,a,a)') what, ': vtkgpu status ', rc, ': ', trim(text)
+    error stop 2
+  end subroutine
+
+  ! Registers a host array and copies it to its device mirror.
+  subroutine vk_resident(a, n)
+    real(c_double) :: a(*)
+    integer(c_size_t), intent(in) :: n
+    call vk_check(vk_register(a, n * 8_c_size_t), 'vk_register')
+    call vk_check(vk_upload(a), 'vk_upload')
+  end subroutine
+end module vk_capi

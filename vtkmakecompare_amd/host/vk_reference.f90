@@ -1,0 +1,180 @@
+! vk_reference.f90 -- drop-in replacements for the reference routines on
+! the time-step path.  Each subroutine has the reference's name, argument
+! list and array shapes, so program main's call sites compile unchanged;
+! the body forwards to the gfx950 library through vk_capi.  The arrays
+! must be device-resident (vk_resident) before the first call.
+!
+!   SET_BC                      nse_dominik.f:611
+!   SET_CONVECT                 nse_dominik.f:757
+!   SET_CONVECTION_FOR_SCALAR   nse_dominik.f:834
+!   SET_CONDUCTION              nse_dominik.f:888
+!   SET_DIFFUSION_FOR_SCALAR    nse_dominik.f:950
+!   SET_IMMERSED_BOUNDARY       nse_dominik.f:1011
+!   SET_EXPLICIT                nse_dominik.f:1132
+!   set_explicit_for_tempreture nse_dominik.f:1291
+!   solve_u_temp                nse_dominik.f:1344
+!   solve_t                     nse_dominik.f:1465
+!   MKP                         mkp.f:9
+!   set_levelset                set_levelset.f:1
+
+subroutine set_bc(u, v, w, p, nx, ny, nz, t)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: u(0:nx+1,0:ny+1,0:nz+1), v(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: w(0:nx+1,0:ny+1,0:nz+1), p(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: t(0:nx+1,0:ny+1,0:nz+1)
+  call vk_check(vk_set_bc(u, v, w, p, nx, ny, nz, t), 'SET_BC')
+end subroutine
+
+subroutine set_convect(u, u_v, v, v_v, w, w_v, dt, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: dt
+  double precision :: u(0:nx+1,0:ny+1,0:nz+1), u_v(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: v(0:nx+1,0:ny+1,0:nz+1), v_v(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: w(0:nx+1,0:ny+1,0:nz+1), w_v(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: dx(0:nx+1), dy(0:ny+1), dz(0:nz+1)
+  double precision :: dx12(0:nx+1), dy12(0:ny+1), dz12(0:nz+1)
+  call vk_check(vk_set_convect(u, u_v, v, v_v, w, w_v, dt, nx, ny, nz, dx, dy, dz, &
+                               dx12, dy12, dz12), 'SET_CONVECT')
+end subroutine
+
+subroutine set_convection_for_scalar(u, v, w, dt, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12, &
+                                     t, u_vt, rhocp)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: dt
+  double precision :: u(0:nx+1,0:ny+1,0:nz+1), v(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: w(0:nx+1,0:ny+1,0:nz+1), t(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: u_vt(0:nx+1,0:ny+1,0:nz+1), rhocp(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: dx(0:nx+1), dy(0:ny+1), dz(0:nz+1)
+  double precision :: dx12(0:nx+1), dy12(0:ny+1), dz12(0:nz+1)
+  call vk_check(vk_set_convection_for_scalar(u, v, w, dt, nx, ny, nz, dx, dy, dz, dx12, &
+                                             dy12, dz12, t, u_vt, rhocp), &
+                'SET_CONVECTION_FOR_SCALAR')
+end subroutine
+
+subroutine set_conduction(u, u_d, v, v_d, w, w_d, re, dt, nx, ny, nz, dx, dy, dz, dx12, &
+                          dy12, dz12)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: re, dt
+  double precision :: u(0:nx+1,0:ny+1,0:nz+1), u_d(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: v(0:nx+1,0:ny+1,0:nz+1), v_d(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: w(0:nx+1,0:ny+1,0:nz+1), w_d(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: dx(0:nx+1), dy(0:ny+1), dz(0:nz+1)
+  double precision :: dx12(0:nx+1), dy12(0:ny+1), dz12(0:nz+1)
+  call vk_check(vk_set_conduction(u, u_d, v, v_d, w, w_d, re, dt, nx, ny, nz, dx, dy, dz, &
+                                  dx12, dy12, dz12), 'SET_CONDUCTION')
+end subroutine
+
+subroutine set_diffusion_for_scalar(re, dt, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12, t, &
+                                    u_dt, sc, thc, rhocp)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: re, dt, sc
+  double precision :: t(0:nx+1,0:ny+1,0:nz+1), u_dt(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: thc(0:nx+1,0:ny+1,0:nz+1), rhocp(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: dx(0:nx+1), dy(0:ny+1), dz(0:nz+1)
+  double precision :: dx12(0:nx+1), dy12(0:ny+1), dz12(0:nz+1)
+  call vk_check(vk_set_diffusion_for_scalar(re, dt, nx, ny, nz, dx, dy, dz, dx12, dy12, &
+                                            dz12, t, u_dt, sc, thc, rhocp), &
+                'SET_DIFFUSION_FOR_SCALAR')
+end subroutine
+
+subroutine set_immersed_boundary(eta, nx, ny, nz, dt, u_l, v_l, w_l, dx, dy, dz, t_l, &
+                                 cellvalue, u_ta, v_ta, w_ta, l, cons_cell)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz, l
+  double precision :: eta, dt, cons_cell
+  double precision :: cellvalue(1:nx,1:ny,1:nz,1:4)
+  double precision :: u_l(0:nx+1,0:ny+1,0:nz+1), v_l(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: w_l(0:nx+1,0:ny+1,0:nz+1), t_l(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: u_ta(0:nx+1,0:ny+1,0:nz+1), v_ta(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: w_ta(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: dx(0:nx+1), dy(0:ny+1), dz(0:nz+1)
+  call vk_check(vk_set_immersed_boundary(eta, nx, ny, nz, dt, u_l, v_l, w_l, dx, dy, dz, &
+                                         t_l, cellvalue, u_ta, v_ta, w_ta, l, cons_cell), &
+                'SET_IMMERSED_BOUNDARY')
+end subroutine
+
+subroutine set_explicit(u_v, u_pre, u_d, u_a, v_v, v_pre, v_d, v_a, w_v, w_pre, w_d, w_a, &
+                        u_l, v_l, w_l, nx, ny, nz, l)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz, l
+  double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: u_v, u_pre, u_d, u_a, v_v, v_pre, &
+      v_d, v_a, w_v, w_pre, w_d, w_a, u_l, v_l, w_l
+  call vk_check(vk_set_explicit(u_v, u_pre, u_d, u_a, v_v, v_pre, v_d, v_a, w_v, w_pre, w_d, &
+                                w_a, u_l, v_l, w_l, nx, ny, nz, l), 'SET_EXPLICIT')
+end subroutine
+
+subroutine set_explicit_for_tempreture(u_vt, u_dt, u_t, t_pre, t_l, nx, ny, nz, l)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz, l
+  double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: u_vt, u_dt, u_t, t_pre, t_l
+  call vk_check(vk_set_explicit_for_tempreture(u_vt, u_dt, u_t, t_pre, t_l, nx, ny, nz, l), &
+                'set_explicit_for_tempreture')
+end subroutine
+
+subroutine solve_u_temp(u_a, v_a, w_a, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12, re, dt, &
+                        eta, cellvalue, u, v, w)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: re, dt, eta
+  double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: u_a, v_a, w_a, u, v, w
+  double precision :: cellvalue(1:nx,1:ny,1:nz,1:4)
+  double precision :: dx(0:nx+1), dy(0:ny+1), dz(0:nz+1)
+  double precision :: dx12(0:nx+1), dy12(0:ny+1), dz12(0:nz+1)
+  call vk_check(vk_solve_u_temp(u_a, v_a, w_a, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12, &
+                                re, dt, eta, cellvalue, u, v, w), 'solve_u_temp')
+end subroutine
+
+subroutine solve_t(u_t, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12, re, sc, dt, rhocp, thc, &
+                   cellvalue, eta)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: re, sc, dt, eta
+  double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: u_t, rhocp, thc
+  double precision :: cellvalue(1:nx,1:ny,1:nz,1:4)
+  double precision :: dx(0:nx+1), dy(0:ny+1), dz(0:nz+1)
+  double precision :: dx12(0:nx+1), dy12(0:ny+1), dz12(0:nz+1)
+  call vk_check(vk_solve_t(u_t, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12, re, sc, dt, &
+                           rhocp, thc, cellvalue, eta), 'solve_t')
+end subroutine
+
+subroutine mkp(p, u_temp, v_temp, w_temp, dt, dx, dy, dz, dx12, dy12, dz12, re, nx, ny, nz, &
+               ompnthreads, u_ta, v_ta, w_ta, u_ep2, v_ep2, w_ep2, divu, u, v, w, xl, zl)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz, ompnthreads
+  double precision :: dt, re, xl, zl
+  double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: p, u_temp, v_temp, w_temp, u_ta, &
+      v_ta, w_ta, u_ep2, v_ep2, w_ep2, divu, u, v, w
+  double precision :: dx(0:nx+1), dy(0:ny+1), dz(0:nz+1)
+  double precision :: dx12(0:nx+1), dy12(0:ny+1), dz12(0:nz+1)
+  call vk_check(vk_mkp(p, u_temp, v_temp, w_temp, dt, dx, dy, dz, dx12, dy12, dz12, re, &
+                       nx, ny, nz, ompnthreads, u_ta, v_ta, w_ta, u_ep2, v_ep2, w_ep2, &
+                       divu, u, v, w, xl, zl), 'MKP')
+end subroutine
+
+subroutine set_levelset(nx, ny, nz, dx, dx12, dy12, dz12, phi0, phi, x, y, z, cellvalue)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: phi0(0:nx+1,0:ny+1,0:nz+1), phi(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: x(0:nx+1), y(0:ny+1), z(0:nz+1)
+  double precision :: dx(0:nx+1), dx12(0:nx+1), dy12(0:ny+1), dz12(0:nz+1)
+  double precision :: cellvalue(1:nx,1:ny,1:nz,1:4)
+  call vk_check(vk_set_levelset(nx, ny, nz, dx, dx12, dy12, dz12, phi0, phi, x, y, z, &
+                                cellvalue), 'set_levelset')
+end subroutine
