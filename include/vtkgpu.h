@@ -53,6 +53,17 @@ int vk_memcpy_htod(void *dst, const void *src, const size_t *bytes);
 int vk_memcpy_dtoh(void *dst, const void *src, const size_t *bytes);
 int vk_memset(void *dst, const int *value, const size_t *bytes);
 
+/* Kernel timing on the library stream with HIP events, for benchmarks:
+   between vk_profile_begin and vk_profile_end every kernel launch is
+   bracketed by an event pair under its kernel's name; vk_profile_entry
+   returns the summed duration and the launch count per name.  No
+   reference counterpart (the reference prints system_clock deltas). */
+int vk_profile_begin(void);
+int vk_profile_end(void);
+int vk_profile_count(int *n);
+int vk_profile_entry(const int *index, char *name, const int *name_len,
+                     double *total_ms, int *launches);
+
 /* ---------------------------------------------------- boundary halos */
 /* SET_BC(U,V,W,P,NX,NY,NZ,T), nse_dominik.f:611-753 */
 int vk_set_bc(double *u, double *v, double *w, double *p, const int *nx,

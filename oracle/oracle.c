@@ -839,23 +839,24 @@ void o_solve_t(double *u_t, int nx, int ny, int nz, const double *dx,
 /* -------------------------------------------------------------- FFT */
 
 /* Recursive mixed-radix (2, 3, generic) decimation-in-time complex DFT,
-   out of place into `out` (contiguous), sign -1 forward.  Twiddles from
-   cos/sin of 2*pi*m/n with the libm constant; accuracy is checked against
-   numpy.fft in tests/. */
+   out of place into `out` (contiguous), sign -1 forward.  Twiddles come
+   from one table tw[m] = exp(sign 2 pi i m / n0) of the top-level size n0
+   (libm cos/sin); accuracy is checked against numpy.fft in tests/. */
 static void fft_rec(const double *in, int n, int stride, double *out,
-                    int sign, double *work) {
+                    double *work, const double *tw, int n0) {
   if (n == 1) {
     out[0] = in[0];
     out[1] = in[1];
     return;
   }
   int p = (n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : n;
+  const int step = n0 / n;
   if (p == n) { /* direct DFT */
     for (int k = 0; k < n; k++) {
       double sr = 0.0, si = 0.0;
       for (int m = 0; m < n; m++) {
-        double ang = sign * 2.0 * M_PI * (double)(((long)k * m) % n) / (double)n;
-        double c = cos(ang), s = sin(ang);
+        long e = ((long)k * m) % n * step;
+        double c = tw[2 * e], s = tw[2 * e + 1];
         double xr = in[2 * (size_t)m * stride], xi = in[2 * (size_t)m * stride + 1];
         sr += xr * c - xi * s;
         si += xr * s + xi * c;
@@ -866,15 +867,14 @@ static void fft_rec(const double *in, int n, int stride, double *out,
     return;
   }
   int m = n / p;
-  /* sub-transforms of the p decimated sequences into work */
   for (int q = 0; q < p; q++)
-    fft_rec(in + 2 * (size_t)q * stride, m, stride * p, work + 2 * (size_t)q * m,
-            sign, out);
+    fft_rec(in + 2 * (size_t)q * stride, m, stride * p, work + 2 * (size_t)q * m, out, tw,
+            n0);
   for (int k = 0; k < m; k++) {
     double tr[3], ti[3];
     for (int q = 0; q < p; q++) {
-      double ang = sign * 2.0 * M_PI * (double)((long)q * k) / (double)n;
-      double c = cos(ang), s = sin(ang);
+      long e = (long)q * k * step;
+      double c = tw[2 * e], s = tw[2 * e + 1];
       double xr = work[2 * ((size_t)q * m + k)], xi = work[2 * ((size_t)q * m + k) + 1];
       tr[q] = xr * c - xi * s;
       ti[q] = xr * s + xi * c;
@@ -882,8 +882,8 @@ static void fft_rec(const double *in, int n, int stride, double *out,
     for (int r = 0; r < p; r++) {
       double sr = 0.0, si = 0.0;
       for (int q = 0; q < p; q++) {
-        double ang = sign * 2.0 * M_PI * (double)((q * r) % p) / (double)p;
-        double c = cos(ang), s = sin(ang);
+        long e = (long)((q * r) % p) * (n0 / p);
+        double c = tw[2 * e], s = tw[2 * e + 1];
         sr += tr[q] * c - ti[q] * s;
         si += tr[q] * s + ti[q] * c;
       }
@@ -894,23 +894,22 @@ static void fft_rec(const double *in, int n, int stride, double *out,
 }
 
 void o_fft(double *data, int n, int stride, int sign) {
-  double *out = malloc(sizeof(double) * 2 * (size_t)n);
-  double *work = malloc(sizeof(double) * 2 * (size_t)n);
-  double *in = malloc(sizeof(double) * 2 * (size_t)n);
+  double *buf = malloc(sizeof(double) * 8 * (size_t)n);
+  double *out = buf, *work = buf + 2 * (size_t)n, *in = work + 2 * (size_t)n;
+  double *tw = in + 2 * (size_t)n;
   for (int m = 0; m < n; m++) {
+    double ang = sign * 2.0 * M_PI * (double)m / (double)n;
+    tw[2 * m] = cos(ang);
+    tw[2 * m + 1] = sin(ang);
     in[2 * m] = data[2 * (size_t)m * stride];
     in[2 * m + 1] = data[2 * (size_t)m * stride + 1];
   }
-  /* fft_rec needs scratch of the same size as out; the recursion swaps
-     the roles of out and work, so both are full-size */
-  fft_rec(in, n, 1, out, sign, work);
+  fft_rec(in, n, 1, out, work, tw, n);
   for (int m = 0; m < n; m++) {
     data[2 * (size_t)m * stride] = out[2 * m];
     data[2 * (size_t)m * stride + 1] = out[2 * m + 1];
   }
-  free(out);
-  free(work);
-  free(in);
+  free(buf);
 }
 
 /* p2s_fftw2d.f:5-114.  Per plane J: FI(I+1,K+1) = FP(I,J,K) for

@@ -129,11 +129,21 @@ def test_immersed_explicit_bit_exact(api, l):
     np.testing.assert_array_equal(dut.download(), ut)
 
 
+def sparse_cellvalue(rng, g):
+    """cellvalue zero except a blob in one corner: most lines are fluid-only
+    (their matrices equal the reference line's), a few are not."""
+    cv = np.zeros((4, g.nz, g.ny, g.nx))
+    kz, jy, ix = max(1, g.nz // 3), max(1, g.ny // 3), max(1, g.nx // 5)
+    cv[:, :kz, :jy, :ix] = rng.uniform(0, 1, (4, kz, jy, ix))
+    return cv
+
+
+@pytest.mark.parametrize("sparse", [False, True])
 @pytest.mark.parametrize("n", SIZES + [(64, 65, 64)])
-def test_solve_u_temp(api, n):
+def test_solve_u_temp(api, n, sparse):
     g = G.channel_grid(*n)
     rng = np.random.default_rng(3)
-    cv = rng.uniform(0, 1, (4, g.nz, g.ny, g.nx))
+    cv = sparse_cellvalue(rng, g) if sparse else rng.uniform(0, 1, (4, g.nz, g.ny, g.nx))
     a = [rfield(rng, g) for _ in range(3)]
     old = [rfield(rng, g) for _ in range(3)]
     re, dt, eta = 2800.0, 1e-3, 1e4
@@ -151,15 +161,19 @@ def test_solve_u_temp(api, n):
         assert nrel(got[1:-1, :, 1:-1], y[1:-1, :, 1:-1]) < TOL
 
 
+@pytest.mark.parametrize("sparse", [False, True])
 @pytest.mark.parametrize("n", SIZES + [(64, 65, 64)])
-def test_solve_t(api, n):
+def test_solve_t(api, n, sparse):
     g = G.channel_grid(*n)
     rng = np.random.default_rng(4)
-    cv = rng.uniform(0, 1, (4, g.nz, g.ny, g.nx))
     ut = rfield(rng, g)
     rc, th = np.ones(g.shape), np.ones(g.shape)
-    rc[1:-1, 1:-1, 1:-1] = rng.uniform(0.5, 2, (g.nz, g.ny, g.nx))
-    th[1:-1, 1:-1, 1:-1] = rng.uniform(0.5, 2, (g.nz, g.ny, g.nx))
+    if sparse:
+        cv = sparse_cellvalue(rng, g)
+    else:
+        cv = rng.uniform(0, 1, (4, g.nz, g.ny, g.nx))
+        rc[1:-1, 1:-1, 1:-1] = rng.uniform(0.5, 2, (g.nz, g.ny, g.nx))
+        th[1:-1, 1:-1, 1:-1] = rng.uniform(0.5, 2, (g.nz, g.ny, g.nx))
     re, sc, dt, eta = 2800.0, 0.71, 1e-3, 1e4
     d = dev(ut)
     M = metrics(g)

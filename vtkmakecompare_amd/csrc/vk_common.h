@@ -43,6 +43,76 @@ struct Metrics {
   const double *dx, *dy, *dz, *dx12, *dy12, *dz12;
 };
 
+// Reciprocals 1/dx, 1/dy, ... of the metric arrays, for the kernels of the
+// fused step that multiply where the reference divides.
+struct RMetrics {
+  const double *rdx, *rdy, *rdz, *rdx12, *rdy12, *rdz12;
+};
+
+// XCD-aware block order.  The dispatcher deals workgroups to the eight
+// XCDs round-robin in linear block order; this renumbers them so each XCD
+// works on one contiguous range of tiles and the neighbouring planes a
+// stencil re-reads stay in that XCD's L2.  Returns the logical block
+// (x, y, z) for the current workgroup.
+__device__ inline void xcd_block(int& bx, int& by, int& bz) {
+  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned p = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const unsigned X = 8;
+  const unsigned per = nb / X, rem = nb % X;
+  const unsigned x = p % X, r = p / X;
+  const unsigned q = x * per + (x < rem ? x : rem) + r;
+  bx = (int)(q % gridDim.x);
+  by = (int)((q / gridDim.x) % gridDim.y);
+  bz = (int)(q / (gridDim.x * gridDim.y));
+}
+
+// decode halo index q into (i,j,k); returns false past the end.
+// segments: z faces (all i,j), y faces (k interior, all i), x faces
+// (j,k interior), and -- when `vplane` -- the interior of the j = ny plane.
+__device__ inline bool halo_decode(const Geom& g, long long q, bool vplane, int& i, int& j,
+                                   int& k) {
+  const long long NX2 = g.nx + 2, NY2 = g.ny + 2;
+  long long nzf = 2 * NX2 * NY2;
+  if (q < nzf) {
+    k = (q < NX2 * NY2) ? 0 : g.nz + 1;
+    long long r = q % (NX2 * NY2);
+    i = (int)(r % NX2);
+    j = (int)(r / NX2);
+    return true;
+  }
+  q -= nzf;
+  long long nyf = 2 * NX2 * g.nz;
+  if (q < nyf) {
+    j = (q < NX2 * g.nz) ? 0 : g.ny + 1;
+    long long r = q % (NX2 * g.nz);
+    i = (int)(r % NX2);
+    k = 1 + (int)(r / NX2);
+    return true;
+  }
+  q -= nyf;
+  long long nxf = 2LL * g.ny * g.nz;
+  if (q < nxf) {
+    long long r = q >> 1;
+    i = (q & 1) ? g.nx + 1 : 0;
+    j = 1 + (int)(r % g.ny);
+    k = 1 + (int)(r / g.ny);
+    return true;
+  }
+  q -= nxf;
+  if (vplane && q < (long long)g.nx * g.nz) {
+    j = g.ny;
+    i = 1 + (int)(q % g.nx);
+    k = 1 + (int)(q / g.nx);
+    return true;
+  }
+  return false;
+}
+
+__host__ __device__ inline long long halo_count(const Geom& g, bool vplane) {
+  long long n = 2LL * (g.nx + 2) * (g.ny + 2) + 2LL * (g.nx + 2) * g.nz + 2LL * g.ny * g.nz;
+  return vplane ? n + (long long)g.nx * g.nz : n;
+}
+
 // A launch over the interior points (i,j,k) in 1..n, 64 x 4 threads per
 // block along (i, j), one z plane per grid row.
 constexpr int kBX = 64, kBY = 4;

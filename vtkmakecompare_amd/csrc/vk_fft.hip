@@ -301,26 +301,31 @@ static int plans_for(const Geom& g, FftPlan& px, FftPlan& pz, const double2*& tw
 }
 
 template <typename Sample>
-static int launch_forward(const Geom& g, Sample smp, double* phit, hipStream_t s) {
+static int launch_forward(const Geom& g, Sample smp, double* phit, const char* xname,
+                          hipStream_t s) {
   FftPlan px, pz;
   const double2* twl;
   int rc = plans_for(g, px, pz, twl);
   if (rc) return rc;
   size_t shx = sizeof(double2) * 2 * (g.nx / 2) * kRowsPerBlock;
-  k_rfft_x<Sample><<<dim3((g.ny + kRowsPerBlock - 1) / kRowsPerBlock, g.nz),
-                     kRowsPerBlock * kRowThreads, shx, s>>>(g, smp, phit, px, twl);
+  {
+    ProfScope ps(xname, s);
+    k_rfft_x<Sample><<<dim3((g.ny + kRowsPerBlock - 1) / kRowsPerBlock, g.nz),
+                       kRowsPerBlock * kRowThreads, shx, s>>>(g, smp, phit, px, twl);
+  }
   size_t shz = sizeof(double2) * 2 * g.nz * kCols;
+  ProfScope ps("cfft_z_fwd", s);
   k_cfft_z<<<dim3((g.nx / 2 + 1 + kCols - 1) / kCols, g.ny), 256, shz, s>>>(g, phit, pz, -1, 1);
   return VK_OK;
 }
 
 int launch_p2s(const Geom& g, const double* fp, double* phit, hipStream_t s) {
-  return launch_forward(g, SampleField{fp}, phit, s);
+  return launch_forward(g, SampleField{fp}, phit, "rfft_x", s);
 }
 
 int launch_p2s_div(const Geom& g, const Metrics& m, const double* u, const double* v,
                    const double* w, double dt, double* phit, hipStream_t s) {
-  return launch_forward(g, SampleDiv{u, v, w, m.dx, m.dy, m.dz, dt}, phit, s);
+  return launch_forward(g, SampleDiv{u, v, w, m.dx, m.dy, m.dz, dt}, phit, "rfft_x_div", s);
 }
 
 // S2P from a spectrum the call may overwrite (the z pass runs in place)
@@ -330,8 +335,12 @@ int launch_s2p_inplace(const Geom& g, double* phit, double* fp, hipStream_t s) {
   int rc = plans_for(g, px, pz, twl);
   if (rc) return rc;
   size_t shz = sizeof(double2) * 2 * g.nz * kCols;
-  k_cfft_z<<<dim3((g.nx / 2 + 1 + kCols - 1) / kCols, g.ny), 256, shz, s>>>(g, phit, pz, +1, 0);
+  {
+    ProfScope ps("cfft_z_inv", s);
+    k_cfft_z<<<dim3((g.nx / 2 + 1 + kCols - 1) / kCols, g.ny), 256, shz, s>>>(g, phit, pz, +1, 0);
+  }
   size_t shx = sizeof(double2) * 2 * (g.nx / 2) * kRowsPerBlock;
+  ProfScope ps("irfft_x", s);
   k_irfft_x<<<dim3((g.ny + kRowsPerBlock - 1) / kRowsPerBlock, g.nz),
               kRowsPerBlock * kRowThreads, shx, s>>>(g, phit, fp, px, twl);
   return VK_OK;

@@ -24,11 +24,13 @@ void launch_cv_faces(const Geom& g, double* cv, hipStream_t s);
 // line solves (vk_tridiag.hip); return non-zero for unsupported lengths
 int launch_spn(const Geom& g, double* phit, const double* ac, const double* kxz, hipStream_t s);
 void launch_sp0b(const Geom& g, double* phit, const double* ac, hipStream_t s);
-int launch_adi_velocity(const Geom& g, const Metrics& m, double* f, const double* cv, int comp,
-                        double beta, double dteta, hipStream_t s);
-int launch_adi_temperature(const Geom& g, const Metrics& m, double* f, const double* cv,
-                           const double* rc, const double* thc, double beta, double dteta,
-                           hipStream_t s);
+int launch_adi_velocity(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
+                        const double* cv, int comp, double beta, double dteta, hipStream_t s);
+int launch_adi_temperature(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
+                           const double* cv, const double* rc, const double* thc, double beta,
+                           double dteta, hipStream_t s);
+// reciprocal metric arrays in workspace (recomputed each call, 6(n+2) doubles)
+int make_rmetrics(const Geom& g, const Metrics& m, RMetrics& r, hipStream_t s);
 
 // FFT (vk_fft.hip); return a VK_ status
 int launch_p2s(const Geom& g, const double* fp, double* phit, hipStream_t s);

@@ -42,21 +42,16 @@ __global__ void k_reinit(Geom g, const double* __restrict__ a, double* __restric
 }
 
 // zero-gradient halo in all three directions (set_levelset.f:107-124);
-// every copy reads one interior cell, so one pass gives the same values
+// every copy reads one interior cell, so one pass over the halo cells
+// gives the same values
 __global__ void k_halo_copy_all(Geom g, double* f) {
   long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long tot = (long long)(g.nx + 2) * (g.ny + 2) * (g.nz + 2);
-  if (q >= tot) return;
-  int i = (int)(q % (g.nx + 2));
-  long long r = q / (g.nx + 2);
-  int j = (int)(r % (g.ny + 2));
-  int k = (int)(r / (g.ny + 2));
-  bool halo = i == 0 || i == g.nx + 1 || j == 0 || j == g.ny + 1 || k == 0 || k == g.nz + 1;
-  if (!halo) return;
+  int i, j, k;
+  if (!halo_decode(g, q, false, i, j, k)) return;
   int im = i == 0 ? 1 : (i == g.nx + 1 ? g.nx : i);
   int jm = j == 0 ? 1 : (j == g.ny + 1 ? g.ny : j);
   int km = k == 0 ? 1 : (k == g.nz + 1 ? g.nz : k);
-  f[q] = f[g.at(im, jm, km)];
+  f[g.at(i, j, k)] = f[g.at(im, jm, km)];
 }
 
 // set_levelset.f:157-170, pi = 3.1415926535
@@ -105,7 +100,7 @@ extern "C" int vk_set_levelset(const int* nx, const int* ny, const int* nz, cons
   rc = hip_check(hipMemcpyAsync(work, dp0, fb, hipMemcpyDeviceToDevice, s), "levelset copy");
   if (rc) return rc;
   const int iterations = 2001;  // it = 0..nnn with nnn = 2000 (set_levelset.f:4,30)
-  long long tot = (long long)g.cells();
+  long long tot = halo_count(g, false);
   double* a = dphi;
   double* b = work;
   for (int it = 0; it < iterations; it++) {

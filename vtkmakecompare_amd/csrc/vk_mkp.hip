@@ -36,18 +36,20 @@ __global__ void k_mkp_coefficients(Geom g, Metrics m, double xl, double zl, doub
 // u_ep2 = -dt grad(p) + u_ta (read before u_ta is replaced; p's halo at
 // nx+1, ny+1, nz+1 is the caller's, as in the reference), u_ta = dt grad,
 // and divu of the corrected velocity.
-__global__ void k_project(Geom g, Metrics m, const double* phi, const double* ut,
+__global__ void k_project(Geom g, RMetrics r, const double* phi, const double* ut,
                           const double* vt, const double* wt, double dt, MkpOut o) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y * blockDim.y + threadIdx.y;
-  int k = blockIdx.z;
+  int bx, by, bz;
+  xcd_block(bx, by, bz);
+  int i = bx * blockDim.x + threadIdx.x;
+  int j = by * blockDim.y + threadIdx.y;
+  int k = bz;
   if (i > g.nx || j > g.ny) return;
   long long c = g.at(i, j, k);
   const long long sy = g.sy, sz = g.sz;
   double ph = phi[c];
-  double gx = (phi[c + 1] - ph) / m.dx12[i];
-  double gy = (phi[c + sy] - ph) / m.dy12[j];
-  double gz = (phi[c + sz] - ph) / m.dz12[k];
+  double gx = (phi[c + 1] - ph) * r.rdx12[i];
+  double gy = (phi[c + sy] - ph) * r.rdy12[j];
+  double gz = (phi[c + sz] - ph) * r.rdz12[k];
   double un = ut[c] - dt * gx, vn = vt[c] - dt * gy, wn = wt[c] - dt * gz;
   o.u[c] = un;
   o.v[c] = vn;
@@ -59,14 +61,14 @@ __global__ void k_project(Geom g, Metrics m, const double* phi, const double* ut
       double px = (i < g.nx) ? phi[c + 1] : o.p[c + 1];
       double py = (j < g.ny) ? phi[c + sy] : o.p[c + sy];
       double pz = (k < g.nz) ? phi[c + sz] : o.p[c + sz];
-      o.u_ep2[c] = -dt * (px - ph) / m.dx12[i] + o.u_ta[c];
-      o.v_ep2[c] = -dt * (py - ph) / m.dy12[j] + o.v_ta[c];
-      o.w_ep2[c] = -dt * (pz - ph) / m.dz12[k] + o.w_ta[c];
+      o.u_ep2[c] = -dt * (px - ph) * r.rdx12[i] + o.u_ta[c];
+      o.v_ep2[c] = -dt * (py - ph) * r.rdy12[j] + o.v_ta[c];
+      o.w_ep2[c] = -dt * (pz - ph) * r.rdz12[k] + o.w_ta[c];
     }
-    double um = ut[c - 1] - dt * ((ph - phi[c - 1]) / m.dx12[i - 1]);
-    double vm = vt[c - sy] - dt * ((ph - phi[c - sy]) / m.dy12[j - 1]);
-    double wm = wt[c - sz] - dt * ((ph - phi[c - sz]) / m.dz12[k - 1]);
-    o.divu[c] = (un - um) / m.dx[i] + (vn - vm) / m.dy[j] + (wn - wm) / m.dz[k];
+    double um = ut[c - 1] - dt * ((ph - phi[c - 1]) * r.rdx12[i - 1]);
+    double vm = vt[c - sy] - dt * ((ph - phi[c - sy]) * r.rdy12[j - 1]);
+    double wm = wt[c - sz] - dt * ((ph - phi[c - sz]) * r.rdz12[k - 1]);
+    o.divu[c] = (un - um) * r.rdx[i] + (vn - vm) * r.rdy[j] + (wn - wm) * r.rdz[k];
   }
   o.u_ta[c] = dt * gx;
   o.v_ta[c] = dt * gy;
@@ -85,7 +87,10 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
   double* kxz = coef;
   double* ac = coef + (size_t)(nh + 1) * (g.nz + 1);
   int nthreads = (int)nk;
-  k_mkp_coefficients<<<(nthreads + 255) / 256, 256, 0, s>>>(g, m, xl, zl, kxz, ac);
+  {
+    ProfScope ps("mkp_coef", s);
+    k_mkp_coefficients<<<(nthreads + 255) / 256, 256, 0, s>>>(g, m, xl, zl, kxz, ac);
+  }
   int rc = launch_p2s_div(g, m, u_temp, v_temp, w_temp, dt, phit, s);
   if (rc) return rc;
   if (launch_spn(g, phit, ac, kxz, s)) return fail(VK_ERR_BAD_SIZE, "mkp: ny not supported");
@@ -93,8 +98,12 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
   rc = launch_s2p_inplace(g, phit, phi, s);
   if (rc) return rc;
   launch_halo_neumann(g, phi, s);
+  RMetrics r;
+  rc = make_rmetrics(g, m, r, s);
+  if (rc) return rc;
   dim3 blk(64, 4, 1), grd((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
-  k_project<<<grd, blk, 0, s>>>(g, m, phi, u_temp, v_temp, w_temp, dt, o);
+  ProfScope ps("project", s);
+  k_project<<<grd, blk, 0, s>>>(g, r, phi, u_temp, v_temp, w_temp, dt, o);
   return VK_OK;
 }
 

@@ -7,6 +7,8 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <utility>
+#include <vector>
 
 namespace vk {
 
@@ -98,6 +100,45 @@ void* workspace(WorkSlot slot, size_t bytes) {
   }
   w.bytes = bytes;
   return w.ptr;
+}
+
+namespace {
+struct ProfSlot {
+  std::string name;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  size_t used = 0;
+};
+bool g_prof_on = false;
+std::vector<ProfSlot>& prof_slots() {
+  static std::vector<ProfSlot> v;
+  return v;
+}
+}  // namespace
+
+ProfScope::ProfScope(const char* name, hipStream_t stream) : s(stream) {
+  if (!g_prof_on) return;
+  auto& v = prof_slots();
+  int idx = -1;
+  for (size_t q = 0; q < v.size(); q++)
+    if (v[q].name == name) idx = (int)q;
+  if (idx < 0) {
+    v.push_back(ProfSlot{name, {}, 0});
+    idx = (int)v.size() - 1;
+  }
+  ProfSlot& ps = v[idx];
+  if (ps.used == ps.ev.size()) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    ps.ev.emplace_back(a, b);
+  }
+  if (hipEventRecord(ps.ev[ps.used].first, s) != hipSuccess) return;
+  slot = idx;
+}
+
+ProfScope::~ProfScope() {
+  if (slot < 0) return;
+  ProfSlot& ps = prof_slots()[slot];
+  if (hipEventRecord(ps.ev[ps.used].second, s) == hipSuccess) ps.used++;
 }
 
 }  // namespace vk
@@ -235,6 +276,45 @@ int vk_memcpy_dtoh(void* dst, const void* src, const size_t* bytes) {
                      "vk_memcpy_dtoh");
   if (rc) return rc;
   return hip_check(hipStreamSynchronize(rt().stream), "vk_memcpy_dtoh sync");
+}
+
+int vk_profile_begin(void) {
+  VK_REQUIRE_INIT();
+  int rc = hip_check(hipStreamSynchronize(rt().stream), "vk_profile_begin");
+  if (rc) return rc;
+  for (auto& ps : prof_slots()) ps.used = 0;
+  g_prof_on = true;
+  return VK_OK;
+}
+
+int vk_profile_end(void) {
+  VK_REQUIRE_INIT();
+  g_prof_on = false;
+  return hip_check(hipStreamSynchronize(rt().stream), "vk_profile_end");
+}
+
+int vk_profile_count(int* n) {
+  *n = (int)prof_slots().size();
+  return VK_OK;
+}
+
+int vk_profile_entry(const int* index, char* name, const int* name_len, double* total_ms,
+                     int* launches) {
+  auto& v = prof_slots();
+  if (*index < 0 || *index >= (int)v.size()) return fail(VK_ERR_ARGUMENT, "profile index");
+  ProfSlot& ps = v[*index];
+  double tot = 0.0;
+  for (size_t q = 0; q < ps.used; q++) {
+    float ms = 0.0f;
+    int rc = hip_check(hipEventElapsedTime(&ms, ps.ev[q].first, ps.ev[q].second),
+                       "hipEventElapsedTime");
+    if (rc) return rc;
+    tot += ms;
+  }
+  std::snprintf(name, (size_t)*name_len, "%s", ps.name.c_str());
+  *total_ms = tot;
+  *launches = (int)ps.used;
+  return VK_OK;
 }
 
 int vk_memset(void* dst, const int* value, const size_t* bytes) {

@@ -31,8 +31,27 @@ int hip_check(hipError_t e, const char* what);
 void* resolve(const void* p, size_t bytes, const char* what);
 
 // Grow-only device workspace, one per slot.
-enum WorkSlot { kWorkPhi = 0, kWorkSpec, kWorkSpec2, kWorkGrad, kWorkMisc, kWorkSlots };
+enum WorkSlot {
+  kWorkPhi = 0,
+  kWorkSpec,
+  kWorkSpec2,
+  kWorkGrad,
+  kWorkMisc,
+  kWorkRecip,
+  kWorkSlots
+};
 void* workspace(WorkSlot slot, size_t bytes);
+
+// Per-kernel timing with HIP events on the library stream, active between
+// vk_profile_begin and vk_profile_end (a no-op otherwise).  A ProfScope
+// brackets one kernel launch (or one launch group) with a named pair of
+// events; vk_profile_entry reports the summed duration per name.
+struct ProfScope {
+  int slot = -1;
+  hipStream_t s = nullptr;
+  ProfScope(const char* name, hipStream_t stream);
+  ~ProfScope();
+};
 
 }  // namespace vk
 

@@ -41,53 +41,6 @@ __device__ inline void halo_cell(double* f, const Geom& g, int i, int j, int k, 
   f[g.at(i, j, k)] = nz ? s * f[g.at(im, jm, km)] : 0.0;
 }
 
-// decode halo index q into (i,j,k); returns false past the end.
-// segments: z faces (all i,j), y faces (k interior, all i), x faces
-// (j,k interior), and -- when `vplane` -- the interior of the j = ny plane.
-__device__ inline bool halo_decode(const Geom& g, long long q, bool vplane, int& i, int& j,
-                                   int& k) {
-  const long long NX2 = g.nx + 2, NY2 = g.ny + 2;
-  long long nzf = 2 * NX2 * NY2;
-  if (q < nzf) {
-    k = (q < NX2 * NY2) ? 0 : g.nz + 1;
-    long long r = q % (NX2 * NY2);
-    i = (int)(r % NX2);
-    j = (int)(r / NX2);
-    return true;
-  }
-  q -= nzf;
-  long long nyf = 2 * NX2 * g.nz;
-  if (q < nyf) {
-    j = (q < NX2 * g.nz) ? 0 : g.ny + 1;
-    long long r = q % (NX2 * g.nz);
-    i = (int)(r % NX2);
-    k = 1 + (int)(r / NX2);
-    return true;
-  }
-  q -= nyf;
-  long long nxf = 2LL * g.ny * g.nz;
-  if (q < nxf) {
-    long long r = q >> 1;
-    i = (q & 1) ? g.nx + 1 : 0;
-    j = 1 + (int)(r % g.ny);
-    k = 1 + (int)(r / g.ny);
-    return true;
-  }
-  q -= nxf;
-  if (vplane && q < (long long)g.nx * g.nz) {
-    j = g.ny;
-    i = 1 + (int)(q % g.nx);
-    k = 1 + (int)(q / g.nx);
-    return true;
-  }
-  return false;
-}
-
-__host__ __device__ inline long long halo_count(const Geom& g, bool vplane) {
-  long long n = 2LL * (g.nx + 2) * (g.ny + 2) + 2LL * (g.nx + 2) * g.nz + 2LL * g.ny * g.nz;
-  return vplane ? n + (long long)g.nx * g.nz : n;
-}
-
 __global__ void k_set_bc(Geom g, double* u, double* v, double* w, double* p, double* t) {
   long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
@@ -113,11 +66,13 @@ __global__ void k_halo_one(Geom g, double* f, int mode) {
 void launch_set_bc(const Geom& g, double* u, double* v, double* w, double* p, double* t,
                    hipStream_t s) {
   long long n = halo_count(g, true);
+  ProfScope ps("set_bc", s);
   k_set_bc<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, u, v, w, p, t);
 }
 
 void launch_halo_neumann(const Geom& g, double* f, hipStream_t s) {
   long long n = halo_count(g, false);
+  ProfScope ps("halo_neumann", s);
   k_halo_one<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, f, kNeumann);
 }
 
@@ -350,23 +305,35 @@ __global__ void k_adj_ns_scalar(Geom g, Metrics m, const double* t, const double
   ws[o] = dt * f * (t[g.at(i, j, k + 1)] - t[g.at(i, j, k - 1)]) / m.dz12[k] / 2.0;
 }
 
-// max |f| over the interior, NaN/Inf flagged
-__global__ void k_absmax(Geom g, const double* f, unsigned long long* out) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  int k = 1 + blockIdx.z;
-  double a = 0.0;
-  if (i <= g.nx && j <= g.ny) {
+// max |f| over the interior, NaN/Inf flagged: grid-stride loop, wave and
+// block reduction, one atomic per block (bit patterns of non-negative
+// doubles order like the doubles)
+__global__ __launch_bounds__(256) void k_absmax(Geom g, const double* f,
+                                                unsigned long long* out) {
+  __shared__ unsigned long long part[4];
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  unsigned long long best = 0;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    int i = 1 + (int)(q % g.nx);
+    long long r = q / g.nx;
+    int j = 1 + (int)(r % g.ny), k = 1 + (int)(r / g.ny);
     double x = f[g.at(i, j, k)];
-    a = isfinite(x) ? fabs(x) : __longlong_as_double(0x7ff0000000000000LL);
+    double a = isfinite(x) ? fabs(x) : __longlong_as_double(0x7ff0000000000000LL);
+    unsigned long long bits = (unsigned long long)__double_as_longlong(a);
+    best = bits > best ? bits : best;
   }
-  // bit patterns of non-negative doubles order like the doubles
-  unsigned long long bits = (unsigned long long)__double_as_longlong(a);
   for (int off = 32; off > 0; off >>= 1) {
-    unsigned long long o2 = __shfl_xor(bits, off);
-    bits = o2 > bits ? o2 : bits;
+    unsigned long long o2 = __shfl_xor(best, off);
+    best = o2 > best ? o2 : best;
   }
-  if ((threadIdx.x + threadIdx.y * blockDim.x) % warpSize == 0) atomicMax(out, bits);
+  if (threadIdx.x % 64 == 0) part[threadIdx.x / 64] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = part[0];
+    for (int w = 1; w < 4; w++) m = part[w] > m ? part[w] : m;
+    atomicMax(out, m);
+  }
 }
 
 void launch_convect(const Geom& g, const Metrics& m, const double* u, const double* v,
@@ -377,10 +344,51 @@ void launch_convect(const Geom& g, const Metrics& m, const double* u, const doub
 
 void launch_pen_divide(const Geom& g, double* x, const double* old, const double* cv, int n,
                        double eta, double dt, int jmax, hipStream_t s) {
+  ProfScope ps("pen_divide", s);
   k_pen_divide<<<interior_grid(g), interior_block(), 0, s>>>(g, x, old, cv, n, eta, dt, jmax);
 }
 
+__global__ void k_rmetrics(Metrics m, int nx, int ny, int nz, double* out) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int ax = nx + 2, ay = ny + 2, az = nz + 2;
+  double* rdx = out;
+  double* rdx12 = rdx + ax;
+  double* rdy = rdx12 + ax;
+  double* rdy12 = rdy + ay;
+  double* rdz = rdy12 + ay;
+  double* rdz12 = rdz + az;
+  if (t < ax) {
+    rdx[t] = 1.0 / m.dx[t];
+    rdx12[t] = 1.0 / m.dx12[t];
+  }
+  if (t < ay) {
+    rdy[t] = 1.0 / m.dy[t];
+    rdy12[t] = 1.0 / m.dy12[t];
+  }
+  if (t < az) {
+    rdz[t] = 1.0 / m.dz[t];
+    rdz12[t] = 1.0 / m.dz12[t];
+  }
+}
+
+int make_rmetrics(const Geom& g, const Metrics& m, RMetrics& r, hipStream_t s) {
+  int ax = g.nx + 2, ay = g.ny + 2, az = g.nz + 2;
+  double* w = static_cast<double*>(workspace(kWorkRecip, sizeof(double) * 2 * (ax + ay + az)));
+  if (!w) return VK_ERR_HIP;
+  int nmax = ax > ay ? ax : ay;
+  nmax = nmax > az ? nmax : az;
+  k_rmetrics<<<(nmax + 255) / 256, 256, 0, s>>>(m, g.nx, g.ny, g.nz, w);
+  r.rdx = w;
+  r.rdx12 = w + ax;
+  r.rdy = w + 2 * ax;
+  r.rdy12 = w + 2 * ax + ay;
+  r.rdz = w + 2 * ax + 2 * ay;
+  r.rdz12 = w + 2 * ax + 2 * ay + az;
+  return VK_OK;
+}
+
 void launch_cv_faces(const Geom& g, double* cv, hipStream_t s) {
+  ProfScope ps("cv_faces", s);
   k_cv_faces<<<interior_grid(g), interior_block(), 0, s>>>(g, cv);
 }
 
@@ -717,7 +725,7 @@ int vk_field_absmax(const double* f, const int* nx, const int* ny, const int* nz
   hipStream_t s = rt().stream;
   int rc = hip_check(hipMemsetAsync(dres, 0, sizeof(unsigned long long), s), "absmax memset");
   if (rc) return rc;
-  k_absmax<<<interior_grid(g), interior_block(), 0, s>>>(g, df, dres);
+  k_absmax<<<1024, 256, 0, s>>>(g, df, dres);
   VK_LAUNCH_CHECK("field_absmax");
   unsigned long long bits = 0;
   rc = hip_check(hipMemcpyAsync(&bits, dres, sizeof(bits), hipMemcpyDeviceToHost, s),

@@ -25,55 +25,64 @@ struct ExplicitArgs {
   const double *u, *v, *w, *t, *rc, *thc, *u_ta, *v_ta, *w_ta, *cv;
   double *u_pre, *v_pre, *w_pre, *t_pre;
   double *ua, *va, *wa, *ut;
-  double dt, re, sc, eta, a1, a2, b1, c0;
+  double dt, re, sc, eta, a1, a2, b1, c0, cd;
 };
 
-__global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, ExplicitArgs x) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  int k = 1 + blockIdx.z;
+// x / (1 + eta cv dt), exactly x where cv is 0
+__device__ inline double pen_div(double x, double eta, double cv, double dt) {
+  return cv == 0.0 ? x : x / (1.0 + eta * cv * dt);
+}
+
+__global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetrics r,
+                                                        ExplicitArgs x) {
+  int bx, by, bz;
+  xcd_block(bx, by, bz);
+  int i = 1 + bx * blockDim.x + threadIdx.x;
+  int j = 1 + by * blockDim.y + threadIdx.y;
+  int k = 1 + bz;
   if (i > g.nx || j > g.ny) return;
   F3<double> U{x.u, g}, V{x.v, g}, W{x.w, g}, T{x.t, g}, RC{x.rc, g}, K{x.thc, g};
   const long long o = g.at(i, j, k);
   const double dt = x.dt, eta = x.eta;
   // u: SET_CONVECT, SET_CONDUCTION, u_l, SET_EXPLICIT, solve_u_temp divide
   {
-    double uv = conv_u(U, V, W, m, i, j, k, dt);
-    double ud = cond_u(U, m, i, j, k, x.c0);
+    double uv = conv_u_r(U, V, W, m, r, i, j, k, dt);
+    double ud = cond_u_r(U, r, i, j, k, x.c0);
     double cv2 = x.cv[g.cv(i, j, k, 2)];
     double ul = eta * cv2 * x.u_ta[o] * dt;
     double ua = x.a1 * uv + x.a2 * x.u_pre[o] + x.b1 * ud + x.b1 * ul;
     x.u_pre[o] = uv;
-    x.ua[o] = (x.u[o] + ua) / (1.0 + eta * cv2 * dt);
+    x.ua[o] = pen_div(x.u[o] + ua, eta, cv2, dt);
   }
   {
-    double vv = conv_v(U, V, W, m, i, j, k, dt);
-    double vd = cond_v(V, m, i, j, k, x.c0);
+    double vv = conv_v_r(U, V, W, m, r, i, j, k, dt);
+    double vd = cond_v_r(V, r, i, j, k, x.c0);
     double cv3 = x.cv[g.cv(i, j, k, 3)];
     double vl = eta * cv3 * x.v_ta[o] * dt;
     double va = x.a1 * vv + x.a2 * x.v_pre[o] + x.b1 * vd + x.b1 * vl;
     x.v_pre[o] = vv;
-    x.va[o] = (x.v[o] + va) / (1.0 + eta * cv3 * dt);
+    x.va[o] = pen_div(x.v[o] + va, eta, cv3, dt);
     // SET_EXPLICIT's reflection v_a(i,ny+1,k) = -v_a(i,ny,k) survives into
     // v_temp (no later routine writes that plane of v)
     if (j == g.ny) x.va[g.at(i, g.ny + 1, k)] = -1.0 * va;
   }
   {
-    double wv = conv_w(U, V, W, m, i, j, k, dt);
-    double wd = cond_w(W, m, i, j, k, x.c0);
+    double wv = conv_w_r(U, V, W, m, r, i, j, k, dt);
+    double wd = cond_w_r(W, r, i, j, k, x.c0);
     double cv4 = x.cv[g.cv(i, j, k, 4)];
     double wl = eta * cv4 * x.w_ta[o] * dt;
     double wa = x.a1 * wv + x.a2 * x.w_pre[o] + x.b1 * wd + x.b1 * wl;
     x.w_pre[o] = wv;
-    x.wa[o] = (x.w[o] + wa) / (1.0 + eta * cv4 * dt);
+    x.wa[o] = pen_div(x.w[o] + wa, eta, cv4, dt);
   }
   // T: convection, diffusion, T_l = 0, combine, history, main-loop divide
   {
-    double vt = conv_scalar(U, V, W, T, RC, m, i, j, k, dt, false);
-    double dtt = diff_scalar(T, K, RC, m, i, j, k, x.re, x.sc, dt);
-    double ut = x.a1 * vt + x.a2 * x.t_pre[o] + x.b1 * dtt + x.b1 * 0.0;
+    double rrc = 1.0 / RC(i, j, k);
+    double vt = conv_scalar_r(U, V, W, T, RC, m, r, i, j, k, dt, rrc);
+    double dtt = diff_scalar_r(T, K, m, r, i, j, k, x.cd, rrc);
+    double ut = x.a1 * vt + x.a2 * x.t_pre[o] + x.b1 * dtt;
     x.t_pre[o] = vt;
-    x.ut[o] = (x.t[o] + ut) / (1.0 + eta * x.cv[g.cv(i, j, k, 1)] * dt);
+    x.ut[o] = pen_div(x.t[o] + ut, eta, x.cv[g.cv(i, j, k, 1)], dt);
   }
 }
 
@@ -156,16 +165,26 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
     x.a1 = 1.5; x.a2 = -0.5; x.b1 = 1.0;
   }
   x.c0 = *dt / *re / 2.0;
-  k_explicit_fused<<<interior_grid(g), interior_block(), 0, s>>>(g, m, x);
+  x.cd = *dt / (2.0 * *re * *sc);
+  RMetrics r;
+  int rrc = make_rmetrics(g, m, r, s);
+  if (rrc) return rrc;
+  {
+    ProfScope ps("explicit_fused", s);
+    k_explicit_fused<<<interior_grid(g), interior_block(), 0, s>>>(g, m, r, x);
+  }
 
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
   double* comps[3] = {dut, dvt, dwt};
   for (int c = 0; c < 3; c++)
-    if (launch_adi_velocity(g, m, comps[c], dcv, c, beta_u, dteta, s))
+    if (launch_adi_velocity(g, m, r, comps[c], dcv, c, beta_u, dteta, s))
       return fail(VK_ERR_BAD_SIZE, "time_step: line length not supported");
-  if (launch_adi_temperature(g, m, twork, dcv, drc, dk, beta_t, dteta, s))
+  if (launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s))
     return fail(VK_ERR_BAD_SIZE, "time_step: line length not supported");
-  k_copy_interior_step<<<interior_grid(g), interior_block(), 0, s>>>(g, twork, dtt);
+  {
+    ProfScope ps("copy_interior", s);
+    k_copy_interior_step<<<interior_grid(g), interior_block(), 0, s>>>(g, twork, dtt);
+  }
   launch_set_bc(g, dut, dvt, dwt, dp, dtt, s);
   MkpOut o{dp, duta, dvta, dwta, de1, de2, de3, ddiv, du, dv, dw};
   int rc = launch_mkp(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s);

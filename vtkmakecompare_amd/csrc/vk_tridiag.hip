@@ -3,6 +3,10 @@
 // the Poisson solve.  Each reference line system is a row functor for the
 // partition solver of vk_partition.h; the coefficient expressions keep the
 // reference's operation order.
+#include <cstring>
+#include <map>
+#include <string>
+
 #include "vk_kernels.h"
 #include "vk_partition.h"
 #include "vk_runtime.h"
@@ -10,109 +14,158 @@
 namespace vk {
 
 // ------------------------------------------------------ y line systems
+//
+// Row coefficients use the reciprocal metric arrays: the reference's
+// -beta/alpha/dy12(j-1)/dy(j) becomes -(beta/alpha) * (1/dy12(j-1) *
+// 1/dy(j)), and the diagonal 1 + beta/alpha*(...) is written 1 - a - c,
+// the same quantity.  beta/alpha is exactly beta where cellvalue is 0.
+// Each system computes a row from its per-point inputs with coef(); the
+// reference row (cellvalue 0, rhoCp = thc = 1) is coef() on those inputs,
+// which is what makes the recorded-multiplier path of vk_partition.h
+// bit-identical to the general one.
+
+__device__ inline double beta_over_alpha(double beta, double dteta, double cv) {
+  return cv == 0.0 ? beta : beta / (dteta * cv + 1.0);
+}
+
+// key of a reference-line table: system kind, size, segmenting, the
+// scalars and the caller's metric array the coefficients come from
+struct RefKey {
+  int kind, n, S, extra;
+  double beta, dteta;
+  const void* metric;
+};
+
+// harmonic face conductance (d1+d2)/(d1/k1+d2/k2) with one division
+__device__ inline double hm(double d1, double k1, double d2, double k2) {
+  return (d1 + d2) * k1 * k2 / (d1 * k2 + d2 * k1);
+}
 
 // implicit_uy (nse_dominik.f:1505-1577) on line (i,k): rows j = 0..ny+1,
 // ghost rows x0 + x1 = 0 and x_ny + x_{ny+1} = 0 (wall reflection); all
 // rows are written back as parallel_uy does (nse_dominik.f:2907-2909).
 struct SysYFace {
   using V = double;
-  static constexpr bool kCyclic = false;
+  static constexpr bool kCyclic = false, kHasRef = true;
   Geom g;
   double* f;
   const double* cv;
-  const double *dy, *dy12;
+  RMetrics r;
   double beta, dteta;
   int ncv;
+  const void* mkey;
   __device__ int n() const { return g.ny + 2; }
   __device__ bool valid(int la, int lb) const { return la < g.nx; }
-  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, double& d) const {
-    int i = la + 1, k = lb + 1;
-    if (r == 0) {
-      a = 0.0; b = 1.0; c = 1.0; d = 0.0;
-    } else if (r == g.ny + 1) {
-      a = 1.0; b = 1.0; c = 0.0; d = 0.0;
+  __device__ void coef(int q, double cvv, double& a, double& b, double& c) const {
+    if (q == 0) {
+      a = 0.0; b = 1.0; c = 1.0;
+    } else if (q == g.ny + 1) {
+      a = 1.0; b = 1.0; c = 0.0;
     } else {
-      int j = r;
-      double alpha = dteta * cv[g.cv(i, j, k, ncv)] + 1.0;
-      a = -beta / alpha / dy12[j - 1] / dy[j];
-      b = 1.0 + beta / alpha * (1.0 / dy12[j] + 1.0 / dy12[j - 1]) / dy[j];
-      c = -beta / alpha / dy12[j] / dy[j];
-      d = f[g.at(i, j, k)];
+      int j = q;
+      double rho = beta_over_alpha(beta, dteta, cvv);
+      a = -(rho * (r.rdy12[j - 1] * r.rdy[j]));
+      c = -(rho * (r.rdy12[j] * r.rdy[j]));
+      b = 1.0 - a - c;
     }
   }
-  __device__ void store(int la, int lb, int r, double x) const {
-    f[g.at(la + 1, r, lb + 1)] = x;
+  __device__ void row(int la, int lb, int q, double& a, double& b, double& c, double& d) const {
+    bool in = q >= 1 && q <= g.ny;
+    coef(q, in ? cv[g.cv(la + 1, q, lb + 1, ncv)] : 0.0, a, b, c);
+    d = in ? f[g.at(la + 1, q, lb + 1)] : 0.0;
+  }
+  __device__ void row_ref(int q, double& a, double& b, double& c) const { coef(q, 0.0, a, b, c); }
+  RefKey ref_key(int nn, int S) const { return RefKey{1, nn, S, ncv, beta, dteta, mkey}; }
+  __device__ void store(int la, int lb, int q, double x) const {
+    f[g.at(la + 1, q, lb + 1)] = x;
   }
 };
 
 // implicit_vy (nse_dominik.f:1583-1648): rows j = 0..ny with x0 = x_ny = 0
 struct SysYV {
   using V = double;
-  static constexpr bool kCyclic = false;
+  static constexpr bool kCyclic = false, kHasRef = true;
   Geom g;
   double* f;
   const double* cv;
-  const double *dy, *dy12;
+  RMetrics r;
   double beta, dteta;
+  const void* mkey;
   __device__ int n() const { return g.ny + 1; }
   __device__ bool valid(int la, int lb) const { return la < g.nx; }
-  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, double& d) const {
-    int i = la + 1, k = lb + 1;
-    if (r == 0 || r == g.ny) {
-      a = 0.0; b = 1.0; c = 0.0; d = 0.0;
+  __device__ void coef(int q, double cvv, double& a, double& b, double& c) const {
+    if (q == 0 || q == g.ny) {
+      a = 0.0; b = 1.0; c = 0.0;
     } else {
-      int j = r;
-      double alpha = dteta * cv[g.cv(i, j, k, 3)] + 1.0;
-      a = -beta / alpha / dy[j] / dy12[j];
-      b = 1.0 + beta / alpha * (1.0 / dy[j + 1] + 1.0 / dy[j]) / dy12[j];
-      c = -beta / alpha / dy[j + 1] / dy12[j];
-      d = f[g.at(i, j, k)];
+      int j = q;
+      double rho = beta_over_alpha(beta, dteta, cvv);
+      a = -(rho * (r.rdy[j] * r.rdy12[j]));
+      c = -(rho * (r.rdy[j + 1] * r.rdy12[j]));
+      b = 1.0 - a - c;
     }
   }
-  __device__ void store(int la, int lb, int r, double x) const {
-    f[g.at(la + 1, r, lb + 1)] = x;
+  __device__ void row(int la, int lb, int q, double& a, double& b, double& c, double& d) const {
+    bool in = q >= 1 && q <= g.ny - 1;
+    coef(q, in ? cv[g.cv(la + 1, q, lb + 1, 3)] : 0.0, a, b, c);
+    d = in ? f[g.at(la + 1, q, lb + 1)] : 0.0;
+  }
+  __device__ void row_ref(int q, double& a, double& b, double& c) const { coef(q, 0.0, a, b, c); }
+  RefKey ref_key(int nn, int S) const { return RefKey{2, nn, S, 3, beta, dteta, mkey}; }
+  __device__ void store(int la, int lb, int q, double x) const {
+    f[g.at(la + 1, q, lb + 1)] = x;
   }
 };
-
-__device__ inline double hm(double d1, double k1, double d2, double k2) {
-  return (d1 + d2) / (d1 / k1 + d2 / k2);
-}
 
 // implicit_Ty (nse_dominik.f:1784-1870) with lly(0) = lly(1) and
 // lly(ny+1) = lly(ny); only rows 1..ny are written back (parallel_Ty).
 struct SysYT {
   using V = double;
-  static constexpr bool kCyclic = false;
+  static constexpr bool kCyclic = false, kHasRef = true;
   Geom g;
   double* f;
   const double *cv, *rc, *thc;
-  const double *dy, *dy12;
+  Metrics m;
+  RMetrics r;
   double beta, dteta;
   __device__ int n() const { return g.ny + 2; }
   __device__ bool valid(int la, int lb) const { return la < g.nx; }
-  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, double& d) const {
-    int i = la + 1, k = lb + 1;
-    if (r == 0) {
-      a = 0.0; b = 1.0; c = 1.0; d = 0.0;
-    } else if (r == g.ny + 1) {
-      a = 1.0; b = 1.0; c = 0.0; d = 0.0;
+  __device__ void coef(int q, double cvv, double l0, double lm, double lp, double rcv, double& a,
+                       double& b, double& c) const {
+    if (q == 0) {
+      a = 0.0; b = 1.0; c = 1.0;
+    } else if (q == g.ny + 1) {
+      a = 1.0; b = 1.0; c = 0.0;
     } else {
-      int j = r;
-      double alpha = dteta * cv[g.cv(i, j, k, 1)] + 1.0;
-      double l0 = thc[g.at(i, j, k)];
-      double lm = (j == 1) ? l0 : thc[g.at(i, j - 1, k)];
-      double lp = (j == g.ny) ? l0 : thc[g.at(i, j + 1, k)];
-      double rr = rc[g.at(i, j, k)];
-      double h1 = hm(dy[j], l0, dy[j - 1], lm);
-      double h2 = hm(dy[j + 1], lp, dy[j], l0);
-      a = -beta / alpha * h1 / dy12[j - 1] / dy[j] / rr;
-      b = 1.0 + beta / alpha * (h2 / dy12[j] / dy[j] + h1 / dy12[j - 1] / dy[j]) / rr;
-      c = -beta / alpha * h2 / dy12[j] / dy[j] / rr;
-      d = f[g.at(i, j, k)];
+      int j = q;
+      double rho = beta_over_alpha(beta, dteta, cvv);
+      double rr = 1.0 / rcv;
+      double h1 = hm(m.dy[j], l0, m.dy[j - 1], lm);
+      double h2 = hm(m.dy[j + 1], lp, m.dy[j], l0);
+      a = -(rho * h1 * (r.rdy12[j - 1] * r.rdy[j]) * rr);
+      c = -(rho * h2 * (r.rdy12[j] * r.rdy[j]) * rr);
+      b = 1.0 - a - c;
     }
   }
-  __device__ void store(int la, int lb, int r, double x) const {
-    if (r >= 1 && r <= g.ny) f[g.at(la + 1, r, lb + 1)] = x;
+  __device__ void row(int la, int lb, int q, double& a, double& b, double& c, double& d) const {
+    if (q == 0 || q == g.ny + 1) {
+      coef(q, 0.0, 1.0, 1.0, 1.0, 1.0, a, b, c);
+      d = 0.0;
+      return;
+    }
+    int i = la + 1, k = lb + 1, j = q;
+    long long o = g.at(i, j, k);
+    double l0 = thc[o];
+    double lm = (j == 1) ? l0 : thc[o - g.sy];
+    double lp = (j == g.ny) ? l0 : thc[o + g.sy];
+    coef(q, cv[g.cv(i, j, k, 1)], l0, lm, lp, rc[o], a, b, c);
+    d = f[o];
+  }
+  __device__ void row_ref(int q, double& a, double& b, double& c) const {
+    coef(q, 0.0, 1.0, 1.0, 1.0, 1.0, a, b, c);
+  }
+  RefKey ref_key(int nn, int S) const { return RefKey{3, nn, S, 1, beta, dteta, m.dy}; }
+  __device__ void store(int la, int lb, int q, double x) const {
+    if (q >= 1 && q <= g.ny) f[g.at(la + 1, q, lb + 1)] = x;
   }
 };
 
@@ -138,23 +191,29 @@ struct LineAxis {
 // x_{n+1} = x1) are written too.
 struct SysPeriodicFace {
   using V = double;
-  static constexpr bool kCyclic = true;
+  static constexpr bool kCyclic = true, kHasRef = true;
   LineAxis ax;
   double* f;
   const double* cv;
-  const double *d, *d12;
+  const double *rd, *rd12;
   double beta, dteta;
   int ncv, nla, nlb;
+  const void* mkey;
   __device__ int n() const { return ax.len(); }
   __device__ bool valid(int la, int lb) const { return la < nla && lb < nlb; }
-  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, double& rhs) const {
+  __device__ void coef(int r, double cvv, double& a, double& b, double& c) const {
     int q = r + 1;
-    double alpha = dteta * cv[ax.cvat(la, lb, q, ncv)] + 1.0;
-    a = -beta / alpha / d[q] / d12[q];
-    b = 1.0 + beta / alpha * (1.0 / d[q + 1] + 1.0 / d[q]) / d12[q];
-    c = -beta / alpha / d[q + 1] / d12[q];
-    rhs = f[ax.at(la, lb, q)];
+    double rho = beta_over_alpha(beta, dteta, cvv);
+    a = -(rho * (rd[q] * rd12[q]));
+    c = -(rho * (rd[q + 1] * rd12[q]));
+    b = 1.0 - a - c;
   }
+  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, double& rhs) const {
+    coef(r, cv[ax.cvat(la, lb, r + 1, ncv)], a, b, c);
+    rhs = f[ax.at(la, lb, r + 1)];
+  }
+  __device__ void row_ref(int r, double& a, double& b, double& c) const { coef(r, 0.0, a, b, c); }
+  RefKey ref_key(int nn, int S) const { return RefKey{4 + ax.dir, nn, S, ncv, beta, dteta, mkey}; }
   __device__ void store(int la, int lb, int r, double x) const {
     int q = r + 1, n = ax.len();
     f[ax.at(la, lb, q)] = x;
@@ -167,29 +226,39 @@ struct SysPeriodicFace {
 // with the conductivity ghosts wrapped periodically; rows 1..n written.
 struct SysPeriodicT {
   using V = double;
-  static constexpr bool kCyclic = true;
+  static constexpr bool kCyclic = true, kHasRef = true;
   LineAxis ax;
   double* f;
   const double *cv, *rc, *thc;
-  const double *d, *d12;
+  const double *d, *rd, *rd12;
   double beta, dteta;
   int nla, nlb;
   __device__ int n() const { return ax.len(); }
   __device__ bool valid(int la, int lb) const { return la < nla && lb < nlb; }
-  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, double& rhs) const {
-    int q = r + 1, n = ax.len();
-    double alpha = dteta * cv[ax.cvat(la, lb, q, 1)] + 1.0;
-    double l0 = thc[ax.at(la, lb, q)];
-    double lm = thc[ax.at(la, lb, q == 1 ? n : q - 1)];
-    double lp = thc[ax.at(la, lb, q == n ? 1 : q + 1)];
-    double rr = rc[ax.at(la, lb, q)];
+  __device__ void coef(int r, double cvv, double l0, double lm, double lp, double rcv, double& a,
+                       double& b, double& c) const {
+    int q = r + 1;
+    double rho = beta_over_alpha(beta, dteta, cvv);
+    double rr = 1.0 / rcv;
     double h1 = hm(d[q], l0, d[q - 1], lm);
     double h2 = hm(d[q + 1], lp, d[q], l0);
-    a = -beta / alpha * h1 / d12[q - 1] / d[q] / rr;
-    b = 1.0 + beta / alpha * (h2 / d12[q] / d[q] + h1 / d12[q - 1] / d[q]) / rr;
-    c = -beta / alpha * h2 / d12[q] / d[q] / rr;
-    rhs = f[ax.at(la, lb, q)];
+    a = -(rho * h1 * (rd12[q - 1] * rd[q]) * rr);
+    c = -(rho * h2 * (rd12[q] * rd[q]) * rr);
+    b = 1.0 - a - c;
   }
+  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, double& rhs) const {
+    int q = r + 1, n = ax.len();
+    long long o = ax.at(la, lb, q);
+    double l0 = thc[o];
+    double lm = thc[ax.at(la, lb, q == 1 ? n : q - 1)];
+    double lp = thc[ax.at(la, lb, q == n ? 1 : q + 1)];
+    coef(r, cv[ax.cvat(la, lb, q, 1)], l0, lm, lp, rc[o], a, b, c);
+    rhs = f[o];
+  }
+  __device__ void row_ref(int r, double& a, double& b, double& c) const {
+    coef(r, 0.0, 1.0, 1.0, 1.0, 1.0, a, b, c);
+  }
+  RefKey ref_key(int nn, int S) const { return RefKey{6 + ax.dir, nn, S, 1, beta, dteta, d}; }
   __device__ void store(int la, int lb, int r, double x) const { f[ax.at(la, lb, r + 1)] = x; }
 };
 
@@ -197,10 +266,11 @@ struct SysPeriodicT {
 
 // SPN + CTRIDIAG (mkp.f:235-258, tridiag.f:43-58) for mode (I,K) =
 // (la, lb), I = 0..nx/2, K = 0..nz-1, rows J = 1..ny with the Neumann
-// fold of the first and last rows.  The zero mode is left to SP0B.
+// fold of the first and last rows.  The zero mode is left to SP0B.  The
+// matrix differs from mode to mode, so there is no reference line.
 struct SysSpectral {
   using V = dcx;
-  static constexpr bool kCyclic = false;
+  static constexpr bool kCyclic = false, kHasRef = false;
   Geom g;
   double* phit;  // interleaved complex
   const double *ac, *kxz;
@@ -223,6 +293,12 @@ struct SysSpectral {
     long long o = 2 * g.spec_at(la, J, lb);
     d = {phit[o], phit[o + 1]};
   }
+  __device__ void row_ref(int, double& a, double& b, double& c) const {
+    a = 0.0;
+    b = 1.0;
+    c = 0.0;
+  }
+  RefKey ref_key(int nn, int S) const { return RefKey{0, nn, S, 0, 0.0, 0.0, nullptr}; }
   __device__ void store(int la, int lb, int r, dcx x) const {
     if (la == 0 && lb == 0) return;
     long long o = 2 * g.spec_at(la, r + 1, lb);
@@ -230,6 +306,20 @@ struct SysSpectral {
     phit[o + 1] = x.y;
   }
 };
+
+// reference-line tables, created on first use of a key
+double* line_table(const void* key, size_t keylen, long total, bool* fresh) {
+  static std::map<std::string, double*> cache;
+  std::string k(static_cast<const char*>(key), keylen);
+  auto it = cache.find(k);
+  *fresh = false;
+  if (it != cache.end()) return it->second;
+  double* p = nullptr;
+  if (hipMalloc(&p, sizeof(double) * (size_t)total) != hipSuccess) return nullptr;
+  cache[k] = p;
+  *fresh = true;
+  return p;
+}
 
 // SP0B (mkp.f:263-290): the zero mode by the forward march with
 // phi(0) = phi(1) = 0; one block loads the mode in parallel, one lane
@@ -274,46 +364,64 @@ __global__ void k_sp0b(Geom g, const double* ac, double* phit) {
 
 int launch_spn(const Geom& g, double* phit, const double* ac, const double* kxz, hipStream_t s) {
   SysSpectral sys{g, phit, ac, kxz};
+  ProfScope ps("line_spectral", s);
   return launch_line_solve<true>(sys, g.ny, g.nx / 2 + 1, g.nz, s);
 }
 
 void launch_sp0b(const Geom& g, double* phit, const double* ac, hipStream_t s) {
+  ProfScope ps("sp0b", s);
   k_sp0b<<<1, 256, 2 * (g.ny + 2) * sizeof(double), s>>>(g, ac, phit);
 }
 
 // solve_u_temp body after the penalisation divide: y, z, x sweeps of one
 // component.  comp 0,1,2 = u, v, w (cellvalue slot comp+2); v's z and x
 // sweeps stop at j = ny-1 (nse_dominik.f:1411,1418).
-int launch_adi_velocity(const Geom& g, const Metrics& m, double* f, const double* cv, int comp,
-                        double beta, double dteta, hipStream_t s) {
+int launch_adi_velocity(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
+                        const double* cv, int comp, double beta, double dteta, hipStream_t s) {
   int rc;
   if (comp == 1) {
-    SysYV sy{g, f, cv, m.dy, m.dy12, beta, dteta};
+    SysYV sy{g, f, cv, r, beta, dteta, m.dy};
+    ProfScope ps("line_y_v", s);
     rc = launch_line_solve<true>(sy, g.ny + 1, g.nx, g.nz, s);
   } else {
-    SysYFace sy{g, f, cv, m.dy, m.dy12, beta, dteta, comp + 2};
+    SysYFace sy{g, f, cv, r, beta, dteta, comp + 2, m.dy};
+    ProfScope ps("line_y_face", s);
     rc = launch_line_solve<true>(sy, g.ny + 2, g.nx, g.nz, s);
   }
   if (rc) return rc;
   int jmax = (comp == 1) ? g.ny - 1 : g.ny;
-  SysPeriodicFace sz{LineAxis{g, 2}, f, cv, m.dz, m.dz12, beta, dteta, comp + 2, g.nx, jmax};
-  rc = launch_line_solve<true>(sz, g.nz, g.nx, jmax, s);
+  SysPeriodicFace sz{LineAxis{g, 2}, f, cv, r.rdz, r.rdz12, beta, dteta, comp + 2, g.nx, jmax,
+                     m.dz};
+  {
+    ProfScope ps("line_z_face", s);
+    rc = launch_line_solve<true>(sz, g.nz, g.nx, jmax, s);
+  }
   if (rc) return rc;
-  SysPeriodicFace sx{LineAxis{g, 0}, f, cv, m.dx, m.dx12, beta, dteta, comp + 2, jmax, g.nz};
+  SysPeriodicFace sx{LineAxis{g, 0}, f, cv, r.rdx, r.rdx12, beta, dteta, comp + 2, jmax, g.nz,
+                     m.dx};
+  ProfScope ps("line_x_face", s);
   return launch_line_solve<false>(sx, g.nx, jmax, g.nz, s);
 }
 
 // solve_t: y, x, z sweeps (nse_dominik.f:1477-1500)
-int launch_adi_temperature(const Geom& g, const Metrics& m, double* f, const double* cv,
-                           const double* rc_, const double* thc, double beta, double dteta,
-                           hipStream_t s) {
-  SysYT sy{g, f, cv, rc_, thc, m.dy, m.dy12, beta, dteta};
-  int rc = launch_line_solve<true>(sy, g.ny + 2, g.nx, g.nz, s);
+int launch_adi_temperature(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
+                           const double* cv, const double* rc_, const double* thc, double beta,
+                           double dteta, hipStream_t s) {
+  SysYT sy{g, f, cv, rc_, thc, m, r, beta, dteta};
+  int rc;
+  {
+    ProfScope ps("line_y_t", s);
+    rc = launch_line_solve<true>(sy, g.ny + 2, g.nx, g.nz, s);
+  }
   if (rc) return rc;
-  SysPeriodicT sx{LineAxis{g, 0}, f, cv, rc_, thc, m.dx, m.dx12, beta, dteta, g.ny, g.nz};
-  rc = launch_line_solve<false>(sx, g.nx, g.ny, g.nz, s);
+  SysPeriodicT sx{LineAxis{g, 0}, f, cv, rc_, thc, m.dx, r.rdx, r.rdx12, beta, dteta, g.ny, g.nz};
+  {
+    ProfScope ps("line_x_t", s);
+    rc = launch_line_solve<false>(sx, g.nx, g.ny, g.nz, s);
+  }
   if (rc) return rc;
-  SysPeriodicT sz{LineAxis{g, 2}, f, cv, rc_, thc, m.dz, m.dz12, beta, dteta, g.nx, g.ny};
+  SysPeriodicT sz{LineAxis{g, 2}, f, cv, rc_, thc, m.dz, r.rdz, r.rdz12, beta, dteta, g.nx, g.ny};
+  ProfScope ps("line_z_t", s);
   return launch_line_solve<true>(sz, g.nz, g.nx, g.ny, s);
 }
 
@@ -347,12 +455,15 @@ int vk_solve_u_temp(double* u_a, double* v_a, double* w_a, const int* nx, const 
   VK_RESOLVE(ddz12, const double, dz12, (g.nz + 2) * sizeof(double));
   Metrics m{ddx, ddy, ddz, ddx12, ddy12, ddz12};
   hipStream_t s = rt().stream;
+  RMetrics r;
+  int rrc = make_rmetrics(g, m, r, s);
+  if (rrc) return rrc;
   double beta = *dt / *re / 2.0, dteta = *dt * *eta;
   double* comps[3] = {dua, dva, dwa};
   const double* olds[3] = {du, dv, dw};
   for (int c = 0; c < 3; c++) {
     launch_pen_divide(g, comps[c], olds[c], dcv, c + 2, *eta, *dt, g.ny, s);
-    if (launch_adi_velocity(g, m, comps[c], dcv, c, beta, dteta, s))
+    if (launch_adi_velocity(g, m, r, comps[c], dcv, c, beta, dteta, s))
       return fail(VK_ERR_BAD_SIZE, "solve_u_temp: line length not supported");
   }
   VK_LAUNCH_CHECK("solve_u_temp");
@@ -379,8 +490,11 @@ int vk_solve_t(double* u_t, const int* nx, const int* ny, const int* nz, const d
   VK_RESOLVE(ddy12, const double, dy12, (g.ny + 2) * sizeof(double));
   VK_RESOLVE(ddz12, const double, dz12, (g.nz + 2) * sizeof(double));
   Metrics m{ddx, ddy, ddz, ddx12, ddy12, ddz12};
+  RMetrics r;
+  int rrc = make_rmetrics(g, m, r, rt().stream);
+  if (rrc) return rrc;
   double beta = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
-  if (launch_adi_temperature(g, m, dut, dcv, drc, dk, beta, dteta, rt().stream))
+  if (launch_adi_temperature(g, m, r, dut, dcv, drc, dk, beta, dteta, rt().stream))
     return fail(VK_ERR_BAD_SIZE, "solve_t: line length not supported");
   VK_LAUNCH_CHECK("solve_t");
   return VK_OK;
