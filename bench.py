@@ -28,6 +28,20 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# device symbol of each timed kernel of the fused step, as rocprofv3 names it
+KERNEL_SYMBOL = {
+    "explicit_fused": "vk::k_explicit_fused",
+    "project": "vk::k_project",
+    "line_y_face": "vk::k_line_solve<32,8,512,true,vk::SysYFace>",
+    "line_y_v": "vk::k_line_solve<32,8,512,true,vk::SysYV>",
+    "line_y_t": "vk::k_line_solve<32,8,512,true,vk::SysYT>",
+    "line_z_t": "vk::k_line_solve<64,8,512,true,vk::SysPeriodicT>",
+    "line_x_t": "vk::k_line_solve<64,8,256,false,vk::SysPeriodicT>",
+    "line_spectral": "vk::k_line_solve<32,8,512,true,vk::SysSpectral>",
+    "rfft_x_div": "vk::k_rfft_x<vk::SampleDiv>",
+    "irfft_x": "vk::k_irfft_x",
+}
+
 
 def kernel_bytes(name: str, g, with_ep2: bool) -> float:
     """Algorithmic HBM bytes of one launch of a kernel of the fused step:
@@ -150,6 +164,13 @@ def main():
     dname, (dms, dcount) = dom
     avg_s = dms / dcount / 1e3
     alg = kernel_bytes(dname, g, with_ep2)
+    # HBM bytes per launch from the committed FETCH_SIZE + WRITE_SIZE summary
+    # of this workload (two rocprofv3 --pmc runs, scaled by a known copy)
+    from vtkmakecompare_amd.profsummary import read_traffic
+    traffic = None
+    if cfg.name == "step384" and not with_ep2 and dname in KERNEL_SYMBOL:
+        traffic = read_traffic(os.path.join(ROOT, "profiles", "step384_traffic.txt"),
+                               KERNEL_SYMBOL[dname])
     achieved = alg / avg_s / 1e9 if alg else 0.0
     kernels = {k: {"total_ms": round(v[0], 3), "launches": v[1],
                    "avg_us": round(1e3 * v[0] / v[1], 2),
@@ -187,7 +208,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "alg_bytes_per_launch": alg, "avg_launch_us": round(avg_s * 1e6, 2),
-                     "traffic": None},
+                     "traffic": traffic},
         "step_roofline": {"alg_bytes_per_step": step_bytes(g, with_ep2),
                           "alg_GBs": round(step_bytes(g, with_ep2) / (ms_step / 1e3) / 1e9, 1),
                           "frac": round(step_bytes(g, with_ep2) / (ms_step / 1e3) / 1e9

@@ -37,6 +37,24 @@ struct Geom {
   __host__ __device__ size_t spec_cells() const {
     return (size_t)(nh() + 1) * (size_t)(ny + 2) * (size_t)(nz + 2);
   }
+  // 32-bit element offsets, for kernels that hold many addresses at once
+  // (an unsigned offset lets a load use a scalar base and one address
+  // register); valid when fits32()
+  __host__ __device__ bool fits32() const {
+    const size_t lim = (size_t)1 << 32;
+    return cells() < lim && 4 * interior() < lim && 2 * spec_cells() < lim;
+  }
+  __host__ __device__ unsigned at32(int i, int j, int k) const {
+    return (unsigned)i + (unsigned)sy * (unsigned)j + (unsigned)sz * (unsigned)k;
+  }
+  __host__ __device__ unsigned cv32(int i, int j, int k, int n) const {
+    return (unsigned)(i - 1) +
+           (unsigned)nx * ((unsigned)(j - 1) + (unsigned)ny * ((unsigned)(k - 1) +
+                                                               (unsigned)nz * (unsigned)(n - 1)));
+  }
+  __host__ __device__ unsigned spec32(int i, int j, int k) const {
+    return (unsigned)i + (unsigned)(nh() + 1) * ((unsigned)j + (unsigned)(ny + 2) * (unsigned)k);
+  }
 };
 
 struct Metrics {

@@ -93,7 +93,8 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
   }
   int rc = launch_p2s_div(g, m, u_temp, v_temp, w_temp, dt, phit, s);
   if (rc) return rc;
-  if (launch_spn(g, phit, ac, kxz, s)) return fail(VK_ERR_BAD_SIZE, "mkp: ny not supported");
+  rc = launch_spn(g, phit, ac, kxz, s);
+  if (rc) return rc;
   launch_sp0b(g, phit, ac, s);
   rc = launch_s2p_inplace(g, phit, phi, s);
   if (rc) return rc;
@@ -124,6 +125,7 @@ int vk_mkp(double* p, const double* u_temp, const double* v_temp, const double* 
   (void)ompnthreads;
   Geom g(*nx, *ny, *nz);
   if (g.nx < 4 || g.ny < 2 || g.nz < 2) return fail(VK_ERR_BAD_SIZE, "mkp: grid too small");
+  if (!g.fits32()) return fail(VK_ERR_BAD_SIZE, "mkp: grid too large");
   size_t fb = g.cells() * sizeof(double);
   VK_RESOLVE(dut, const double, u_temp, fb);
   VK_RESOLVE(dvt, const double, v_temp, fb);

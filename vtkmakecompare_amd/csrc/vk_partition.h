@@ -37,23 +37,40 @@ __device__ inline dcx operator-(dcx a) { return {-a.x, -a.y}; }
 __device__ inline void vzero(double& v) { v = 0.0; }
 __device__ inline void vzero(dcx& v) { v = {0.0, 0.0}; }
 
+// A vector of reduced right-hand sides in LDS: real values at p[i]; complex
+// ones with the imaginary parts one plane (`plane` doubles) further on, so
+// that a lane's access is one 8-byte word.
 template <typename V>
 struct LdsVec;
 template <>
 struct LdsVec<double> {
   double* p;
-  __device__ double get(int q) const { return p[q]; }
-  __device__ void set(int q, double v) const { p[q] = v; }
+  int plane;
+  __device__ double get(int i) const { return p[i]; }
+  __device__ void set(int i, double v) const { p[i] = v; }
 };
 template <>
 struct LdsVec<dcx> {
-  double* p;  // re at [2q], im at [2q+1]
-  __device__ dcx get(int q) const { return {p[2 * q], p[2 * q + 1]}; }
-  __device__ void set(int q, dcx v) const {
-    p[2 * q] = v.x;
-    p[2 * q + 1] = v.y;
+  double* p;
+  int plane;
+  __device__ dcx get(int i) const { return {p[i], p[i + plane]}; }
+  __device__ void set(int i, dcx v) const {
+    p[i] = v.x;
+    p[i + plane] = v.y;
   }
 };
+
+// Where row q of a line's 2S-row reduced system sits in LDS: slot
+// (q & 1) * S + (q >> 1) -- the first rows of all segments, then their last
+// rows -- times `stride` past `base`.  With lines interleaved (stride = the
+// block's line count) the lanes of a wave reach consecutive words for their
+// own rows and for every PCR neighbour, free of bank conflicts.
+template <int S>
+struct LSlot {
+  int base, stride;
+  __device__ int at(int q) const { return base + ((q & 1) * S + (q >> 1)) * stride; }
+};
+
 template <typename V>
 constexpr int vwords() {
   return sizeof(V) / sizeof(double);
@@ -159,33 +176,17 @@ __device__ inline void segment_eliminate(int lo, int m, int seg, double (&a)[MMA
   }
 }
 
-// The same operations on d alone, with the recorded multipliers.
-template <int MMAX, typename V>
-__device__ inline void segment_apply(int lo, int m, int seg, V (&d)[MMAX], const double* tab,
-                                     const TabLayout& L) {
-#pragma unroll
-  for (int r = 0; r < MMAX; r++) {
-    if (r < m) {
-      double f = tab[L.o_f + lo + r];
-      if (r < 2)
-        d[r] = f * d[r];
-      else
-        d[r] = f * (d[r] - tab[L.o_g + lo + r] * d[r - 1]);
-    }
-  }
-#pragma unroll
-  for (int r = MMAX - 3; r >= 1; r--)
-    if (r <= m - 3) d[r] = d[r] - tab[L.o_h + lo + r] * d[r + 1];
-  if (m >= 3) d[0] = tab[L.o_f0 + seg] * (d[0] - tab[L.o_h + lo] * d[1]);
-}
-
-// PCR on the P-row reduced system of one line held in LDS (row q has
-// a[q] x_{q-1} + b[q] x_q + c[q] x_{q+1} = d[q]); this thread owns rows
-// q0 and q0+1 and returns their solutions.  Every thread of the block
-// runs the same number of barriers.
-template <int P, bool REC, typename V>
-__device__ inline void pcr_pair(double* la, double* lb, double* lc, LdsVec<V> ld, int q0,
-                                bool cyclic, V& x0, V& x1, double* tab, const TabLayout& L) {
+// PCR on the P = 2S-row reduced system of one line held in LDS (row q has
+// a[q] x_{q-1} + b[q] x_q + c[q] x_{q+1} = d[q]); the thread of segment `seg`
+// owns rows 2 seg and 2 seg + 1 and returns their solutions.  Every thread
+// of the block runs the same number of barriers.  With REC the multipliers
+// applied to d are recorded in `tab` by natural row index.
+template <int S, bool REC, typename V>
+__device__ inline void pcr_pair(double* la, double* lb, double* lc, LdsVec<V> ld, LSlot<S> sl,
+                                int seg, bool cyclic, V& x0, V& x1, double* tab,
+                                const TabLayout& L) {
+  constexpr int P = 2 * S;
+  const int q0 = 2 * seg;
   const int nsteps_limit = cyclic ? P / 2 : P;
   int step = 0;
   for (int delta = 1; delta < nsteps_limit; delta <<= 1, step++) {
@@ -193,7 +194,7 @@ __device__ inline void pcr_pair(double* la, double* lb, double* lc, LdsVec<V> ld
     V nd[2];
 #pragma unroll
     for (int t = 0; t < 2; t++) {
-      int q = q0 + t;
+      const int q = q0 + t;
       int qm = q - delta, qp = q + delta;
       double am = 0.0, bm = 1.0, cm = 0.0, ap = 0.0, bp = 1.0, cp = 0.0;
       V dm, dp;
@@ -204,39 +205,44 @@ __device__ inline void pcr_pair(double* la, double* lb, double* lc, LdsVec<V> ld
         qp &= (P - 1);
       }
       if (qm >= 0) {
-        am = la[qm]; bm = lb[qm]; cm = lc[qm]; dm = ld.get(qm);
+        const int i = sl.at(qm);
+        am = la[i]; bm = lb[i]; cm = lc[i]; dm = ld.get(i);
       }
       if (qp < P) {
-        ap = la[qp]; bp = lb[qp]; cp = lc[qp]; dp = ld.get(qp);
+        const int i = sl.at(qp);
+        ap = la[i]; bp = lb[i]; cp = lc[i]; dp = ld.get(i);
       }
-      double k1 = la[q] / bm, k2 = lc[q] / bp;
+      const int iq = sl.at(q);
+      double k1 = la[iq] / bm, k2 = lc[iq] / bp;
       if (REC) {
         tab[L.o_k1 + (long)step * P + q] = k1;
         tab[L.o_k2 + (long)step * P + q] = k2;
       }
       na[t] = -(am * k1);
       nc[t] = -(cp * k2);
-      nb[t] = lb[q] - cm * k1 - ap * k2;
-      nd[t] = ld.get(q) - k1 * dm - k2 * dp;
+      nb[t] = lb[iq] - cm * k1 - ap * k2;
+      nd[t] = ld.get(iq) - k1 * dm - k2 * dp;
     }
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < 2; t++) {
-      la[q0 + t] = na[t];
-      lb[q0 + t] = nb[t];
-      lc[q0 + t] = nc[t];
-      ld.set(q0 + t, nd[t]);
+      const int iq = sl.at(q0 + t);
+      la[iq] = na[t];
+      lb[iq] = nb[t];
+      lc[iq] = nc[t];
+      ld.set(iq, nd[t]);
     }
     __syncthreads();
   }
   V xs[2];
 #pragma unroll
   for (int t = 0; t < 2; t++) {
-    int q = q0 + t;
+    const int q = q0 + t;
+    const int iq = sl.at(q);
     if (cyclic) {
-      int qq = (q + P / 2) & (P - 1);
-      double ei = la[q] + lc[q], ej = la[qq] + lc[qq];
-      double bi = lb[q], bj = lb[qq];
+      const int iqq = sl.at((q + P / 2) & (P - 1));
+      double ei = la[iq] + lc[iq], ej = la[iqq] + lc[iqq];
+      double bi = lb[iq], bj = lb[iqq];
       double den = bi * bj - ei * ej;
       double inv = 1.0 / den;
       if (REC) {
@@ -244,28 +250,89 @@ __device__ inline void pcr_pair(double* la, double* lb, double* lc, LdsVec<V> ld
         tab[L.o_fin1 + q] = bj;
         tab[L.o_fin2 + q] = ei;
       }
-      xs[t] = inv * (bj * ld.get(q) - ei * ld.get(qq));
+      xs[t] = inv * (bj * ld.get(iq) - ei * ld.get(iqq));
     } else {
-      double inv = 1.0 / lb[q];
+      double inv = 1.0 / lb[iq];
       if (REC) tab[L.o_fin0 + q] = inv;
-      xs[t] = inv * ld.get(q);
+      xs[t] = inv * ld.get(iq);
     }
   }
   x0 = xs[0];
   x1 = xs[1];
 }
 
-// PCR on d alone with the recorded multipliers
-template <int P, typename V>
-__device__ inline void pcr_apply(LdsVec<V> ld, int q0, bool cyclic, V& x0, V& x1,
-                                 const double* tab, const TabLayout& L) {
+// The reference line's table as staged in LDS by the fast kernel.  The
+// per-row parts (f, g, h, afin, cfin) are laid out by (row within segment,
+// segment), the per-reduced-row parts (k1, k2 per PCR step, fin0..2) by
+// the reduced-row slot (q & 1) * S + (q >> 1), so the lanes of a wave read
+// consecutive or equal words.
+template <int S, int MMAX>
+struct LdsTab {
+  static constexpr int kRow = MMAX * S;
+  static constexpr int P = 2 * S;
+  int steps, f, g, h, afin, cfin, f0, k1, k2, fin0, fin1, fin2, total;
+  __host__ __device__ explicit LdsTab(int nsteps) : steps(nsteps) {
+    f = 0;
+    g = f + kRow;
+    h = g + kRow;
+    afin = h + kRow;
+    cfin = afin + kRow;
+    f0 = cfin + kRow;
+    k1 = f0 + S;
+    k2 = k1 + steps * P;
+    fin0 = k2 + steps * P;
+    fin1 = fin0 + P;
+    fin2 = fin1 + P;
+    total = fin2 + P;
+  }
+  // global table index of LDS slot u (n rows per line); slots of rows past
+  // a short segment map to the segment's first row
+  __device__ long src(int u, int n, const TabLayout& L) const {
+    if (u < f0) {
+      const int part = u / kRow, rem = u % kRow;
+      const int r = rem / S, seg = rem % S;
+      const int lo = (seg * n) / S, m = ((seg + 1) * n) / S - lo;
+      const long base = part == 0 ? L.o_f : part == 1 ? L.o_g : part == 2 ? L.o_h
+                        : part == 3 ? L.o_afin : L.o_cfin;
+      return base + lo + (r < m ? r : 0);
+    }
+    if (u < k1) return L.o_f0 + (u - f0);
+    int v, base;
+    if (u < k2) {
+      v = u - k1;
+      base = (int)L.o_k1;
+    } else if (u < fin0) {
+      v = u - k2;
+      base = (int)L.o_k2;
+    } else if (u < fin1) {
+      v = u - fin0;
+      base = (int)L.o_fin0;
+    } else if (u < fin2) {
+      v = u - fin1;
+      base = (int)L.o_fin1;
+    } else {
+      v = u - fin2;
+      base = (int)L.o_fin2;
+    }
+    const int step = v / P, w = v % P;
+    const int t = w / S, seg = w % S;
+    return (long)base + (long)step * P + 2 * seg + t;
+  }
+};
+
+// PCR on d alone with the recorded multipliers of the LDS table
+template <int S, int MMAX, typename V>
+__device__ inline void pcr_apply(LdsVec<V> ld, LSlot<S> sl, int seg, bool cyclic, V& x0, V& x1,
+                                 const double* st, const LdsTab<S, MMAX>& T) {
+  constexpr int P = 2 * S;
+  const int q0 = 2 * seg;
   const int nsteps_limit = cyclic ? P / 2 : P;
   int step = 0;
   for (int delta = 1; delta < nsteps_limit; delta <<= 1, step++) {
     V nd[2];
 #pragma unroll
     for (int t = 0; t < 2; t++) {
-      int q = q0 + t;
+      const int q = q0 + t;
       int qm = q - delta, qp = q + delta;
       V dm, dp;
       vzero(dm);
@@ -274,112 +341,125 @@ __device__ inline void pcr_apply(LdsVec<V> ld, int q0, bool cyclic, V& x0, V& x1
         qm &= (P - 1);
         qp &= (P - 1);
       }
-      if (qm >= 0) dm = ld.get(qm);
-      if (qp < P) dp = ld.get(qp);
-      double k1 = tab[L.o_k1 + (long)step * P + q], k2 = tab[L.o_k2 + (long)step * P + q];
-      nd[t] = ld.get(q) - k1 * dm - k2 * dp;
+      if (qm >= 0) dm = ld.get(sl.at(qm));
+      if (qp < P) dp = ld.get(sl.at(qp));
+      const int ts = step * P + t * S + seg;
+      const double k1 = st[T.k1 + ts], k2 = st[T.k2 + ts];
+      nd[t] = ld.get(sl.at(q)) - k1 * dm - k2 * dp;
     }
     __syncthreads();
 #pragma unroll
-    for (int t = 0; t < 2; t++) ld.set(q0 + t, nd[t]);
+    for (int t = 0; t < 2; t++) ld.set(sl.at(q0 + t), nd[t]);
     __syncthreads();
   }
   V xs[2];
 #pragma unroll
   for (int t = 0; t < 2; t++) {
-    int q = q0 + t;
+    const int q = q0 + t;
+    const int ts = t * S + seg;
     if (cyclic) {
-      int qq = (q + P / 2) & (P - 1);
-      xs[t] = tab[L.o_fin0 + q] * (tab[L.o_fin1 + q] * ld.get(q) - tab[L.o_fin2 + q] * ld.get(qq));
+      xs[t] = st[T.fin0 + ts] * (st[T.fin1 + ts] * ld.get(sl.at(q)) -
+                                 st[T.fin2 + ts] * ld.get(sl.at((q + P / 2) & (P - 1))));
     } else {
-      xs[t] = tab[L.o_fin0 + q] * ld.get(q);
+      xs[t] = st[T.fin0 + ts] * ld.get(sl.at(q));
     }
   }
   x0 = xs[0];
   x1 = xs[1];
 }
 
-// Solve a batch of lines.  Sys provides
-//   using V;  static constexpr bool kCyclic, kHasRef;
-//   int n() const;                         rows per line
-//   bool valid(int la, int lb) const;      line exists
-//   void row(la, lb, r, a, b, c, d) const; row r of line (la, lb)
-//   void row_ref(r, a, b, c) const;        row r of the reference line
-//   void store(la, lb, r, V x) const;      solution of row r
-// The line index is (la, lb): la = blockIdx.x * LPB + (line within block),
-// lb = blockIdx.y.  When `tab` holds the reference line's multipliers and
-// every row of every line of the block has the reference coefficients
-// (bit for bit), the block applies the recorded multipliers instead of
-// eliminating; otherwise it runs the general elimination.
-template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
-__global__ __launch_bounds__(NT) void k_line_solve(Sys sys, const double* tab, TabLayout L) {
-  using V = typename Sys::V;
-  constexpr int LPB = NT / S;
-  constexpr int P = 2 * S;
-  __shared__ double s_a[LPB * P], s_b[LPB * P], s_c[LPB * P];
-  __shared__ double s_d[LPB * P * vwords<V>()];
-  const int tid = threadIdx.x;
-  const int line = LANE_LINES ? (tid % LPB) : (tid / S);
-  const int seg = LANE_LINES ? (tid / LPB) : (tid % S);
-  const int la = blockIdx.x * LPB + line, lb = blockIdx.y;
-  const bool ok = sys.valid(la, lb);
-  const int n = sys.n();
-  const int lo = (seg * n) / S;
-  const int m = ((seg + 1) * n) / S - lo;
+// Blocks the fast kernel leaves to the general one: `count` entries of
+// `list` (linear block numbers), and `done`, which the general kernel's
+// blocks count up so that the last one can clear both counters for the
+// next sweep on the stream.
+struct LineQueue {
+  unsigned* count;
+  unsigned* done;
+  unsigned* list;
+};
 
-  double a[MMAX], b[MMAX], c[MMAX];
+// A batch of lines.  Sys provides
+//   using V, Pt;  static constexpr bool kCyclic, kHasRef, kNb;
+//   int n() const;                              rows per line
+//   bool valid(int la, int lb) const;           line exists
+//   V rhs(la, lb, r) const;                     right-hand side of row r
+//   Pt pt(la, lb, r) const;                     the per-row inputs of row r's
+//                                               coefficients (r = -1 and n
+//                                               too when kNb: the system's
+//                                               edge rule applies)
+//   bool pt_ref(const Pt&) const;               inputs equal the reference
+//                                               line's (cellvalue 0, ...)
+//   void coef(la, lb, r, p, pm, pp, a, b, c);   row r from its inputs and, when
+//                                               kNb, its neighbours'
+//   void coef_ref(r, a, b, c) const;            row r of the reference line
+//   void store(la, lb, r, V x) const;           solution of row r
+// Block (bx, by) holds lines la = bx * LPB + (line within block), lb = by.
+// Every load of a thread is issued before any value is examined (a thread
+// past the last line reads the block's first line, and rows past a short
+// segment repeat its last row).
+template <int S, int NT, bool LANE_LINES>
+struct LineMap {
+  static constexpr int LPB = NT / S;
+  static constexpr int P = 2 * S;
+  int line, seg;
+  __device__ LineMap(int tid)
+      : line(LANE_LINES ? (tid % LPB) : (tid / S)), seg(LANE_LINES ? (tid / LPB) : (tid % S)) {}
+};
+
+// XCD-aware linear block number: the dispatcher deals workgroups to the
+// eight XCDs round-robin, so consecutive blocks (which share the 128-byte
+// lines of a row when a block covers half of one) are renumbered to run
+// on the same XCD at about the same time.
+__device__ inline unsigned xcd_linear(unsigned p, unsigned nb) {
+  const unsigned X = 8;
+  const unsigned per = nb / X, rem = nb % X;
+  const unsigned x = p % X, r = p / X;
+  return x * per + (x < rem ? x : rem) + r;
+}
+
+// General path for one block: elimination in registers, PCR in LDS.
+template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
+__device__ inline void line_block_general(const Sys& sys, int la0, int lvalid, int by,
+                                          double* s_a, double* s_b, double* s_c, double* s_d) {
+  using V = typename Sys::V;
+  using Pt = typename Sys::Pt;
+  using M = LineMap<S, NT, LANE_LINES>;
+  constexpr int P = M::P;
+  // the thread's segment is made opaque here so that the row coefficients'
+  // metric terms, the same in every block, are not hoisted out of the
+  // caller's block loop and kept in registers across it
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const M mp(tid);
+  const int la = la0 + mp.line, lb = by;
+  const bool ok = sys.valid(la, lb);
+  const int lc = ok ? la : lvalid;
+  const int n = sys.n();
+  const int lo = (mp.seg * n) / S;
+  const int m = ((mp.seg + 1) * n) / S - lo;
+
+  // rows' right-hand sides and coefficients; the coefficient inputs roll
+  // through a three-row window (previous, current, next row)
   V d[MMAX];
-  bool clean = tab != nullptr;
+  double a[MMAX], b[MMAX], c[MMAX];
+  Pt prev = Sys::kNb ? sys.pt(lc, lb, lo - 1) : Pt();
+  Pt cur = sys.pt(lc, lb, lo);
 #pragma unroll
   for (int r = 0; r < MMAX; r++) {
-    a[r] = 0.0;
-    b[r] = 1.0;
-    c[r] = 0.0;
-    vzero(d[r]);
-    if (r < m && ok) {
-      sys.row(la, lb, lo + r, a[r], b[r], c[r], d[r]);
-      if (tab)
-        clean = clean && a[r] == tab[L.o_refa + lo + r] && b[r] == tab[L.o_refb + lo + r] &&
-                c[r] == tab[L.o_refc + lo + r];
+    const int q = lo + (r < m ? r : m - 1);
+    d[r] = sys.rhs(lc, lb, q);
+    Pt next = Pt();
+    if (Sys::kNb) next = sys.pt(lc, lb, r < m - 1 ? q + 1 : lo + m);
+    sys.coef(lc, lb, q, cur, prev, next, a[r], b[r], c[r]);
+    if (r + 1 < MMAX) {
+      prev = cur;
+      cur = Sys::kNb ? next : sys.pt(lc, lb, lo + (r + 1 < m ? r + 1 : m - 1));
     }
   }
-  LdsVec<V> ld{s_d + line * P * vwords<V>()};
-  const int q0 = 2 * seg;
-  V xf, xl;
-  const bool fast = __syncthreads_and(clean ? 1 : 0) != 0;
-  if (fast) {
-    segment_apply<MMAX, V>(lo, m, seg, d, tab, L);
-    V dlast;
-    vzero(dlast);
-#pragma unroll
-    for (int r = 0; r < MMAX; r++)
-      if (r == m - 1) dlast = d[r];
-    ld.set(q0, d[0]);
-    ld.set(q0 + 1, dlast);
-    __syncthreads();
-    pcr_apply<P, V>(ld, q0, Sys::kCyclic, xf, xl, tab, L);
-    if (ok) {
-#pragma unroll
-      for (int r = 0; r < MMAX; r++) {
-        if (r < m) {
-          V x;
-          if (r == 0)
-            x = xf;
-          else if (r == m - 1)
-            x = xl;
-          else
-            x = d[r] - tab[L.o_afin + lo + r] * xf - tab[L.o_cfin + lo + r] * xl;
-          sys.store(la, lb, lo + r, x);
-        }
-      }
-    }
-    return;
-  }
-
-  segment_eliminate<MMAX, false, V>(lo, m, seg, a, b, c, d, nullptr, L);
-  double* la_ = s_a + line * P;
-  double* lb_ = s_b + line * P;
-  double* lc_ = s_c + line * P;
+  TabLayout none{};
+  segment_eliminate<MMAX, false, V>(lo, m, mp.seg, a, b, c, d, nullptr, none);
+  const LSlot<S> sl = LANE_LINES ? LSlot<S>{mp.line, M::LPB} : LSlot<S>{mp.line * P, 1};
+  LdsVec<V> ld{s_d, M::LPB * P};
   double alast = 0.0, clast = 0.0;
   V dlast;
   vzero(dlast);
@@ -390,16 +470,18 @@ __global__ __launch_bounds__(NT) void k_line_solve(Sys sys, const double* tab, T
       clast = c[r];
       dlast = d[r];
     }
-  la_[q0] = a[0];
-  lb_[q0] = 1.0;
-  lc_[q0] = c[0];
-  ld.set(q0, d[0]);
-  la_[q0 + 1] = alast;
-  lb_[q0 + 1] = 1.0;
-  lc_[q0 + 1] = clast;
-  ld.set(q0 + 1, dlast);
+  const int i0 = sl.at(2 * mp.seg), i1 = sl.at(2 * mp.seg + 1);
+  s_a[i0] = a[0];
+  s_b[i0] = 1.0;
+  s_c[i0] = c[0];
+  ld.set(i0, d[0]);
+  s_a[i1] = alast;
+  s_b[i1] = 1.0;
+  s_c[i1] = clast;
+  ld.set(i1, dlast);
   __syncthreads();
-  pcr_pair<P, false, V>(la_, lb_, lc_, ld, q0, Sys::kCyclic, xf, xl, nullptr, L);
+  V xf, xl;
+  pcr_pair<S, false, V>(s_a, s_b, s_c, ld, sl, mp.seg, Sys::kCyclic, xf, xl, nullptr, none);
   if (ok) {
 #pragma unroll
     for (int r = 0; r < MMAX; r++) {
@@ -417,10 +499,153 @@ __global__ __launch_bounds__(NT) void k_line_solve(Sys sys, const double* tab, T
   }
 }
 
-// Factor the reference line (one block, one thread per segment) and record
-// its coefficients and multipliers into `tab`.
+// The general solve over items of `item_lines` lines (a multiple of the
+// block's LPB): item (bx, by) holds lines bx * item_lines ... of lb = by,
+// nbx x nby items in all.  Every item when `queued` is false, else the items
+// the fast kernel queued; a block solves an item in item_lines / LPB
+// passes.  The grid strides over the work; with a queue, the last block to
+// finish clears the queue's counters.
+template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
+__global__ __launch_bounds__(NT) void k_line_general(Sys sys, LineQueue Q, bool queued, int nbx,
+                                                     int nby, int item_lines) {
+  using V = typename Sys::V;
+  constexpr int LPB = NT / S;
+  constexpr int P = 2 * S;
+  __shared__ double s_a[LPB * P], s_b[LPB * P], s_c[LPB * P];
+  __shared__ double s_d[LPB * P * vwords<V>()];
+  const unsigned items = queued ? __atomic_load_n(Q.count, __ATOMIC_RELAXED)
+                                : (unsigned)nbx * (unsigned)nby;
+  const unsigned subs = (unsigned)(item_lines / LPB);
+  const unsigned total = items * subs;
+  for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
+    const unsigned it = t / subs, sub = t % subs;
+    const unsigned blk = queued ? Q.list[it] : xcd_linear(it, items);
+    const int first = (int)(blk % (unsigned)nbx) * item_lines;
+    line_block_general<S, MMAX, NT, LANE_LINES, Sys>(sys, first + (int)sub * LPB, first,
+                                                     (int)(blk / (unsigned)nbx), s_a, s_b, s_c,
+                                                     s_d);
+    __syncthreads();
+  }
+  if (queued && threadIdx.x == 0) {
+    __threadfence();
+    const unsigned prev = atomicAdd(Q.done, 1u);
+    if (prev == gridDim.x - 1) {
+      atomicExch(Q.count, 0u);
+      atomicExch(Q.done, 0u);
+    }
+  }
+}
+
+// The recorded-multiplier path.  The reference line's table (validated for
+// this call by k_line_factor) is staged in LDS from offset o_f on, together
+// with the block's data loads.  A block whose rows all have the reference
+// inputs applies the recorded multipliers: its rows then have the
+// reference coefficients bit for bit, so it performs the same operations on
+// d as the general path.  Any other block queues itself for
+// k_line_general and writes nothing.
+template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
+__global__ __launch_bounds__(NT) void k_line_fast(Sys sys, const double* __restrict__ tab,
+                                                  TabLayout L, LineQueue Q, int nbx) {
+  using V = typename Sys::V;
+  using M = LineMap<S, NT, LANE_LINES>;
+  constexpr int P = M::P;
+  extern __shared__ double smem[];
+  const LdsTab<S, MMAX> T(L.steps);
+  double* st = smem;
+  double* sd = smem + T.total;
+
+  const unsigned nb = gridDim.x;
+  const unsigned blk = xcd_linear(blockIdx.x, nb);
+  const int bx = (int)(blk % (unsigned)nbx), by = (int)(blk / (unsigned)nbx);
+  const M mp(threadIdx.x);
+  const int seg = mp.seg;
+  const int la = bx * M::LPB + mp.line, lb = by;
+  const bool ok = sys.valid(la, lb);
+  const int lc = ok ? la : bx * M::LPB;
+  const int n = sys.n();
+  const int lo = (seg * n) / S;
+  const int m = ((seg + 1) * n) / S - lo;
+
+  V d[MMAX];
+  int clean = 1;
+#pragma unroll
+  for (int r = 0; r < MMAX; r++) {
+    const int q = lo + (r < m ? r : m - 1);
+    d[r] = sys.rhs(lc, lb, q);
+    clean &= int(sys.pt_ref(sys.pt(lc, lb, q)));
+  }
+  if (Sys::kNb) clean &= int(sys.pt_ref(sys.pt(lc, lb, lo - 1))) &
+                         int(sys.pt_ref(sys.pt(lc, lb, lo + m)));
+  // stage the table: KT loads in flight per thread before the first wait
+  constexpr int KT = 8;
+  for (int u0 = 0; u0 < T.total; u0 += KT * NT) {
+    double v[KT];
+#pragma unroll
+    for (int k = 0; k < KT; k++) {
+      const int u = u0 + k * NT + (int)threadIdx.x;
+      v[k] = tab[T.src(u < T.total ? u : T.total - 1, n, L)];
+    }
+#pragma unroll
+    for (int k = 0; k < KT; k++) {
+      const int u = u0 + k * NT + (int)threadIdx.x;
+      if (u < T.total) st[u] = v[k];
+    }
+  }
+  if (!__syncthreads_and(clean)) {
+    if (threadIdx.x == 0) Q.list[atomicAdd(Q.count, 1u)] = blk;
+    return;
+  }
+  // the segment elimination's operations on d, with the recorded multipliers
+#pragma unroll
+  for (int r = 0; r < MMAX; r++) {
+    if (r < m) {
+      const double f = st[T.f + r * S + seg];
+      if (r < 2)
+        d[r] = f * d[r];
+      else
+        d[r] = f * (d[r] - st[T.g + r * S + seg] * d[r - 1]);
+    }
+  }
+#pragma unroll
+  for (int r = MMAX - 3; r >= 1; r--)
+    if (r <= m - 3) d[r] = d[r] - st[T.h + r * S + seg] * d[r + 1];
+  if (m >= 3) d[0] = st[T.f0 + seg] * (d[0] - st[T.h + seg] * d[1]);
+  V dlast;
+  vzero(dlast);
+#pragma unroll
+  for (int r = 0; r < MMAX; r++)
+    if (r == m - 1) dlast = d[r];
+  const LSlot<S> sl = LANE_LINES ? LSlot<S>{mp.line, M::LPB} : LSlot<S>{mp.line * P, 1};
+  LdsVec<V> ld{sd, M::LPB * P};
+  ld.set(sl.at(2 * seg), d[0]);
+  ld.set(sl.at(2 * seg + 1), dlast);
+  __syncthreads();
+  V xf, xl;
+  pcr_apply<S, MMAX, V>(ld, sl, seg, Sys::kCyclic, xf, xl, st, T);
+  if (ok) {
+#pragma unroll
+    for (int r = 0; r < MMAX; r++) {
+      if (r < m) {
+        V x;
+        if (r == 0)
+          x = xf;
+        else if (r == m - 1)
+          x = xl;
+        else
+          x = d[r] - st[T.afin + r * S + seg] * xf - st[T.cfin + r * S + seg] * xl;
+        sys.store(la, lb, lo + r, x);
+      }
+    }
+  }
+}
+
+// Validates the reference line's table for this call and refactors it when
+// needed (one block, one thread per segment): the table is kept when its
+// recorded coefficients equal the reference line's coefficients now and
+// `fresh` is false; otherwise the line is factored again and every
+// multiplier applied to the right-hand side is recorded.
 template <int S, int MMAX, typename Sys>
-__global__ __launch_bounds__(S) void k_line_factor(Sys sys, double* tab, TabLayout L) {
+__global__ __launch_bounds__(S) void k_line_factor(Sys sys, double* tab, TabLayout L, int fresh) {
   constexpr int P = 2 * S;
   __shared__ double s_a[P], s_b[P], s_c[P], s_d[P];
   const int seg = threadIdx.x;
@@ -428,19 +653,25 @@ __global__ __launch_bounds__(S) void k_line_factor(Sys sys, double* tab, TabLayo
   const int lo = (seg * n) / S;
   const int m = ((seg + 1) * n) / S - lo;
   double a[MMAX], b[MMAX], c[MMAX], d[MMAX];
+  int same = fresh ? 0 : 1;
+  // rows past a short segment repeat its last row, so that every load is
+  // unconditional and all are in flight together
 #pragma unroll
   for (int r = 0; r < MMAX; r++) {
-    a[r] = 0.0;
-    b[r] = 1.0;
-    c[r] = 0.0;
+    const int q = lo + (r < m ? r : m - 1);
+    sys.coef_ref(q, a[r], b[r], c[r]);
     d[r] = 0.0;
+    same &= int(a[r] == tab[L.o_refa + q]) & int(b[r] == tab[L.o_refb + q]) &
+            int(c[r] == tab[L.o_refc + q]);
+  }
+  if (__syncthreads_and(same)) return;
+#pragma unroll
+  for (int r = 0; r < MMAX; r++)
     if (r < m) {
-      sys.row_ref(lo + r, a[r], b[r], c[r]);
       tab[L.o_refa + lo + r] = a[r];
       tab[L.o_refb + lo + r] = b[r];
       tab[L.o_refc + lo + r] = c[r];
     }
-  }
   segment_eliminate<MMAX, true, double>(lo, m, seg, a, b, c, d, tab, L);
   double alast = 0.0, clast = 0.0;
 #pragma unroll
@@ -449,67 +680,107 @@ __global__ __launch_bounds__(S) void k_line_factor(Sys sys, double* tab, TabLayo
       alast = a[r];
       clast = c[r];
     }
-  const int q0 = 2 * seg;
-  s_a[q0] = a[0];
-  s_b[q0] = 1.0;
-  s_c[q0] = c[0];
-  s_d[q0] = 0.0;
-  s_a[q0 + 1] = alast;
-  s_b[q0 + 1] = 1.0;
-  s_c[q0 + 1] = clast;
-  s_d[q0 + 1] = 0.0;
+  const LSlot<S> sl{0, 1};
+  const int i0 = sl.at(2 * seg), i1 = sl.at(2 * seg + 1);
+  s_a[i0] = a[0];
+  s_b[i0] = 1.0;
+  s_c[i0] = c[0];
+  s_d[i0] = 0.0;
+  s_a[i1] = alast;
+  s_b[i1] = 1.0;
+  s_c[i1] = clast;
+  s_d[i1] = 0.0;
   __syncthreads();
   double x0, x1;
-  pcr_pair<P, true, double>(s_a, s_b, s_c, LdsVec<double>{s_d}, q0, Sys::kCyclic, x0, x1, tab,
-                            L);
+  pcr_pair<S, true, double>(s_a, s_b, s_c, LdsVec<double>{s_d, P}, sl, seg, Sys::kCyclic, x0, x1,
+                            tab, L);
 }
+
+// The queue shared by the sweeps on the library stream, sized for at
+// least `nblocks` entries.
+LineQueue line_queue(unsigned nblocks, hipStream_t s);
 
 // Host cache of reference-line tables, keyed by the system's description.
 // A stale table is harmless: blocks whose rows do not match its recorded
 // coefficients take the general path.
 double* line_table(const void* key, size_t keylen, long total, bool* fresh);
 
-// Launch helper: the segment count S is the smallest power of two (at
-// most 64) that leaves at most 8 rows per segment (16 when 64 segments
-// are not enough), so a thread keeps its rows in few registers and the
-// reduced system has 2S rows, log2(2S) PCR steps.  x lines (segments
-// across lanes) use 256-thread blocks; lines that are adjacent in memory
-// (LANE_LINES) use at least 16 lines per block, so each row load of a
-// wave covers whole 128-byte lines.
+// One sweep: the general kernel uses SG segments of at most MG rows; the
+// fast kernel SF segments of at most MF rows (fewer, longer segments: its
+// rows need one register each, and fewer segments mean a smaller staged
+// table and fewer PCR steps).  Both use NT-thread blocks; an item of the
+// fast kernel (LPB_F lines) is LPB_F / LPB_G blocks of the general one.
+template <bool LANE_LINES, int SG, int MG, int SF, int MF, int NT, typename Sys>
+inline int line_sweep(const Sys& sys, int n, int nla, int nlb, hipStream_t s) {
+  constexpr int LPB_G = NT / SG, LPB_F = NT / SF;
+  static_assert(LPB_F % LPB_G == 0, "a fast item is whole general blocks");
+  double* tab = nullptr;
+  bool fresh = false;
+  TabLayout L = tab_layout(n, SF, Sys::kCyclic);
+  const size_t lds = sizeof(double) * ((size_t)LdsTab<SF, MF>(L.steps).total +
+                                       (size_t)LPB_F * 2 * SF * vwords<typename Sys::V>());
+  if (Sys::kHasRef && lds <= 65536 && (n + SF - 1) / SF <= MF && n >= 2 * SF) {
+    auto key = sys.ref_key(n, SF);
+    tab = line_table(&key, sizeof(key), L.total, &fresh);  // null: general path only
+  }
+  if (tab) {
+    const int nbx = (nla + LPB_F - 1) / LPB_F;
+    const unsigned nb = (unsigned)nbx * (unsigned)nlb;
+    LineQueue Q = line_queue(nb, s);
+    if (!Q.list) return -2;
+    k_line_factor<SF, MF, Sys><<<1, SF, 0, s>>>(sys, tab, L, fresh ? 1 : 0);
+    k_line_fast<SF, MF, NT, LANE_LINES, Sys><<<nb, NT, lds, s>>>(sys, tab, L, Q, nbx);
+    const unsigned ngen = nb < 1024u ? nb : 1024u;
+    k_line_general<SG, MG, NT, LANE_LINES, Sys><<<ngen, NT, 0, s>>>(sys, Q, true, nbx, nlb,
+                                                                   LPB_F);
+  } else {
+    const int nbx = (nla + LPB_G - 1) / LPB_G;
+    LineQueue Q{nullptr, nullptr, nullptr};
+    k_line_general<SG, MG, NT, LANE_LINES, Sys><<<(unsigned)nbx * (unsigned)nlb, NT, 0, s>>>(
+        sys, Q, false, nbx, nlb, LPB_G);
+  }
+  return 0;
+}
+
+// Launch helper: the general kernel's segment count SG is the smallest
+// power of two (at most 64) that leaves at most 8 rows per segment (16 when
+// 64 segments are not enough), so its reduced system has 2 SG rows; the
+// fast kernel takes half as many segments, of at most 12 or 16 rows.  A
+// sweep with a reference table is three launches: validate the table, the
+// fast kernel over every item, the general kernel over the queued ones.
 template <bool LANE_LINES, typename Sys>
 inline int launch_line_solve(const Sys& sys, int n, int nla, int nlb, hipStream_t s) {
   int S = 1;
   while (S < 64 && (n + S - 1) / S > 8) S <<= 1;
-  int mmax = (n + S - 1) / S;
+  const int mmax = (n + S - 1) / S;
   if (n < 2 || mmax > 16 || n < 2 * S) return -1;
-  TabLayout L = tab_layout(n, S, Sys::kCyclic);
-  double* tab = nullptr;
-  bool fresh = false;
-  if (Sys::kHasRef) {
-    auto key = sys.ref_key(n, S);
-    tab = line_table(&key, sizeof(key), L.total, &fresh);  // null: general path only
+  const bool f12 = S > 1 && (n + S / 2 - 1) / (S / 2) <= 12;
+#define VK_SW(SG, MG, SF, MF) \
+  return line_sweep<LANE_LINES, SG, MG, SF, MF, 512>(sys, n, nla, nlb, s)
+  if (mmax > 8) VK_SW(64, 16, 64, 16);
+  if (!LANE_LINES) {
+    // lines contiguous in memory: a thread's rows are consecutive words, and
+    // short segments keep the lanes' loads close together
+    switch (S) {
+      case 64: return line_sweep<LANE_LINES, 64, 8, 64, 8, 256>(sys, n, nla, nlb, s);
+      case 32: return line_sweep<LANE_LINES, 32, 8, 32, 8, 256>(sys, n, nla, nlb, s);
+      case 16: return line_sweep<LANE_LINES, 16, 8, 16, 8, 256>(sys, n, nla, nlb, s);
+      case 8: return line_sweep<LANE_LINES, 8, 8, 8, 8, 256>(sys, n, nla, nlb, s);
+      case 4: return line_sweep<LANE_LINES, 4, 8, 4, 8, 256>(sys, n, nla, nlb, s);
+      case 2: return line_sweep<LANE_LINES, 2, 8, 2, 8, 256>(sys, n, nla, nlb, s);
+      default: return line_sweep<LANE_LINES, 1, 8, 1, 8, 256>(sys, n, nla, nlb, s);
+    }
   }
-#define VK_LS(SS, MM)                                                              \
-  do {                                                                             \
-    constexpr int NT_ = LANE_LINES ? (16 * SS > 512 ? 512 : (16 * SS < 256 ? 256 : 16 * SS)) \
-                                   : 256;                                          \
-    constexpr int LPB_ = NT_ / SS;                                                 \
-    if (tab && fresh) k_line_factor<SS, MM, Sys><<<1, SS, 0, s>>>(sys, tab, L);    \
-    dim3 grid((nla + LPB_ - 1) / LPB_, nlb);                                       \
-    k_line_solve<SS, MM, NT_, LANE_LINES, Sys><<<grid, NT_, 0, s>>>(sys, tab, L);  \
-    return 0;                                                                      \
-  } while (0)
-  if (mmax > 8) VK_LS(64, 16);
   switch (S) {
-    case 64: VK_LS(64, 8);
-    case 32: VK_LS(32, 8);
-    case 16: VK_LS(16, 8);
-    case 8: VK_LS(8, 8);
-    case 4: VK_LS(4, 8);
-    case 2: VK_LS(2, 8);
-    default: VK_LS(1, 8);
+    case 64: if (f12) VK_SW(64, 8, 32, 12); VK_SW(64, 8, 32, 16);
+    case 32: if (f12) VK_SW(32, 8, 16, 12); VK_SW(32, 8, 16, 16);
+    case 16: if (f12) VK_SW(16, 8, 8, 12); VK_SW(16, 8, 8, 16);
+    case 8: if (f12) VK_SW(8, 8, 4, 12); VK_SW(8, 8, 4, 16);
+    case 4: if (f12) VK_SW(4, 8, 2, 12); VK_SW(4, 8, 2, 16);
+    case 2: if (f12) VK_SW(2, 8, 1, 12); VK_SW(2, 8, 1, 16);
+    default: VK_SW(1, 8, 1, 8);
   }
-#undef VK_LS
+#undef VK_SW
 }
 
 }  // namespace vk

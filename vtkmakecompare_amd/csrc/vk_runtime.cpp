@@ -84,8 +84,9 @@ void* resolve(const void* p, size_t bytes, const char* what) {
   return nullptr;
 }
 
-void* workspace(WorkSlot slot, size_t bytes) {
+void* workspace(WorkSlot slot, size_t bytes, bool* grown) {
   Work& w = g_work[slot];
+  if (grown) *grown = false;
   if (w.bytes >= bytes && w.ptr) return w.ptr;
   if (w.ptr) {
     (void)hipStreamSynchronize(rt().stream);
@@ -99,6 +100,7 @@ void* workspace(WorkSlot slot, size_t bytes) {
     return nullptr;
   }
   w.bytes = bytes;
+  if (grown) *grown = true;
   return w.ptr;
 }
 

@@ -38,9 +38,11 @@ enum WorkSlot {
   kWorkGrad,
   kWorkMisc,
   kWorkRecip,
+  kWorkQueue,
   kWorkSlots
 };
-void* workspace(WorkSlot slot, size_t bytes);
+// `grown` (optional) reports whether this call allocated the slot anew.
+void* workspace(WorkSlot slot, size_t bytes, bool* grown = nullptr);
 
 // Per-kernel timing with HIP events on the library stream, active between
 // vk_profile_begin and vk_profile_end (a no-op otherwise).  A ProfScope

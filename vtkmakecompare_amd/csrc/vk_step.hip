@@ -113,6 +113,7 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
   VK_REQUIRE_INIT();
   Geom g(*nx, *ny, *nz);
   if (g.nx < 4 || g.ny < 4 || g.nz < 4) return fail(VK_ERR_BAD_SIZE, "time_step: n >= 4");
+  if (!g.fits32()) return fail(VK_ERR_BAD_SIZE, "time_step: grid too large");
   const size_t fb = g.cells() * sizeof(double);
   VK_RESOLVE(du, double, u, fb);
   VK_RESOLVE(dv, double, v, fb);
@@ -176,11 +177,14 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
 
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
   double* comps[3] = {dut, dvt, dwt};
-  for (int c = 0; c < 3; c++)
-    if (launch_adi_velocity(g, m, r, comps[c], dcv, c, beta_u, dteta, s))
-      return fail(VK_ERR_BAD_SIZE, "time_step: line length not supported");
-  if (launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s))
-    return fail(VK_ERR_BAD_SIZE, "time_step: line length not supported");
+  for (int c = 0; c < 3; c++) {
+    int lrc = launch_adi_velocity(g, m, r, comps[c], dcv, c, beta_u, dteta, s);
+    if (lrc) return lrc;
+  }
+  {
+    int lrc = launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s);
+    if (lrc) return lrc;
+  }
   {
     ProfScope ps("copy_interior", s);
     k_copy_interior_step<<<interior_grid(g), interior_block(), 0, s>>>(g, twork, dtt);

@@ -41,12 +41,35 @@ __device__ inline double hm(double d1, double k1, double d2, double k2) {
   return (d1 + d2) * k1 * k2 / (d1 * k2 + d2 * k1);
 }
 
+__device__ inline int clampi(int q, int lo, int hi) { return q < lo ? lo : (q > hi ? hi : q); }
+
+// `v` where `keep`, else `other`, by bit masks: a select between a loaded
+// value and a constant is turned into a branch around the load (which then
+// waits for it before the join), so a thread's row loads would be issued
+// one at a time
+__device__ inline double keep_or(double v, bool keep, double other) {
+  const unsigned long long m = 0ull - (unsigned long long)keep;
+  return __longlong_as_double((__double_as_longlong(v) & (long long)m) |
+                              (__double_as_longlong(other) & (long long)~m));
+}
+
+// per-row coefficient inputs: cellvalue alone, or cellvalue with the
+// conductivity and heat capacity of the temperature systems
+struct PtCv {
+  double cv;
+};
+struct PtT {
+  double cv, th, rc;
+};
+struct PtNone {};
+
 // implicit_uy (nse_dominik.f:1505-1577) on line (i,k): rows j = 0..ny+1,
 // ghost rows x0 + x1 = 0 and x_ny + x_{ny+1} = 0 (wall reflection); all
 // rows are written back as parallel_uy does (nse_dominik.f:2907-2909).
 struct SysYFace {
   using V = double;
-  static constexpr bool kCyclic = false, kHasRef = true;
+  using Pt = PtCv;
+  static constexpr bool kCyclic = false, kHasRef = true, kNb = false;
   Geom g;
   double* f;
   const double* cv;
@@ -56,35 +79,45 @@ struct SysYFace {
   const void* mkey;
   __device__ int n() const { return g.ny + 2; }
   __device__ bool valid(int la, int lb) const { return la < g.nx; }
+  // ghost rows 0 and ny+1 are x0 + x1 = 0 and x_ny + x_{ny+1} = 0; the
+  // interior expression is evaluated for every row (at a clamped j) so that
+  // no metric load sits behind a branch
   __device__ void coef(int q, double cvv, double& a, double& b, double& c) const {
-    if (q == 0) {
-      a = 0.0; b = 1.0; c = 1.0;
-    } else if (q == g.ny + 1) {
-      a = 1.0; b = 1.0; c = 0.0;
-    } else {
-      int j = q;
-      double rho = beta_over_alpha(beta, dteta, cvv);
-      a = -(rho * (r.rdy12[j - 1] * r.rdy[j]));
-      c = -(rho * (r.rdy12[j] * r.rdy[j]));
-      b = 1.0 - a - c;
-    }
+    const int j = clampi(q, 1, g.ny);
+    const double rho = beta_over_alpha(beta, dteta, cvv);
+    const double ai = -(rho * (r.rdy12[j - 1] * r.rdy[j]));
+    const double ci = -(rho * (r.rdy12[j] * r.rdy[j]));
+    const double bi = 1.0 - ai - ci;
+    const bool in = q >= 1 && q <= g.ny;
+    a = keep_or(ai, in, q == 0 ? 0.0 : 1.0);
+    b = keep_or(bi, in, 1.0);
+    c = keep_or(ci, in, q == 0 ? 1.0 : 0.0);
   }
-  __device__ void row(int la, int lb, int q, double& a, double& b, double& c, double& d) const {
-    bool in = q >= 1 && q <= g.ny;
-    coef(q, in ? cv[g.cv(la + 1, q, lb + 1, ncv)] : 0.0, a, b, c);
-    d = in ? f[g.at(la + 1, q, lb + 1)] : 0.0;
+  __device__ double rhs(int la, int lb, int q) const {
+    double v = f[g.at32(la + 1, clampi(q, 1, g.ny), lb + 1)];
+    return keep_or(v, q >= 1 && q <= g.ny, 0.0);
   }
-  __device__ void row_ref(int q, double& a, double& b, double& c) const { coef(q, 0.0, a, b, c); }
+  __device__ Pt pt(int la, int lb, int q) const {
+    double v = cv[g.cv32(la + 1, clampi(q, 1, g.ny), lb + 1, ncv)];
+    return Pt{keep_or(v, q >= 1 && q <= g.ny, 0.0)};
+  }
+  __device__ bool pt_ref(const Pt& p) const { return p.cv == 0.0; }
+  __device__ void coef(int, int, int q, const Pt& p, const Pt&, const Pt&, double& a, double& b,
+                       double& c) const {
+    coef(q, p.cv, a, b, c);
+  }
+  __device__ void coef_ref(int q, double& a, double& b, double& c) const { coef(q, 0.0, a, b, c); }
   RefKey ref_key(int nn, int S) const { return RefKey{1, nn, S, ncv, beta, dteta, mkey}; }
   __device__ void store(int la, int lb, int q, double x) const {
-    f[g.at(la + 1, q, lb + 1)] = x;
+    f[g.at32(la + 1, q, lb + 1)] = x;
   }
 };
 
 // implicit_vy (nse_dominik.f:1583-1648): rows j = 0..ny with x0 = x_ny = 0
 struct SysYV {
   using V = double;
-  static constexpr bool kCyclic = false, kHasRef = true;
+  using Pt = PtCv;
+  static constexpr bool kCyclic = false, kHasRef = true, kNb = false;
   Geom g;
   double* f;
   const double* cv;
@@ -94,33 +127,44 @@ struct SysYV {
   __device__ int n() const { return g.ny + 1; }
   __device__ bool valid(int la, int lb) const { return la < g.nx; }
   __device__ void coef(int q, double cvv, double& a, double& b, double& c) const {
-    if (q == 0 || q == g.ny) {
-      a = 0.0; b = 1.0; c = 0.0;
-    } else {
-      int j = q;
-      double rho = beta_over_alpha(beta, dteta, cvv);
-      a = -(rho * (r.rdy[j] * r.rdy12[j]));
-      c = -(rho * (r.rdy[j + 1] * r.rdy12[j]));
-      b = 1.0 - a - c;
-    }
+    const int j = clampi(q, 1, g.ny - 1);
+    const double rho = beta_over_alpha(beta, dteta, cvv);
+    const double ai = -(rho * (r.rdy[j] * r.rdy12[j]));
+    const double ci = -(rho * (r.rdy[j + 1] * r.rdy12[j]));
+    const double bi = 1.0 - ai - ci;
+    const bool in = q >= 1 && q <= g.ny - 1;
+    a = keep_or(ai, in, 0.0);
+    b = keep_or(bi, in, 1.0);
+    c = keep_or(ci, in, 0.0);
   }
-  __device__ void row(int la, int lb, int q, double& a, double& b, double& c, double& d) const {
-    bool in = q >= 1 && q <= g.ny - 1;
-    coef(q, in ? cv[g.cv(la + 1, q, lb + 1, 3)] : 0.0, a, b, c);
-    d = in ? f[g.at(la + 1, q, lb + 1)] : 0.0;
+  __device__ double rhs(int la, int lb, int q) const {
+    double v = f[g.at32(la + 1, clampi(q, 1, g.ny - 1), lb + 1)];
+    return keep_or(v, q >= 1 && q <= g.ny - 1, 0.0);
   }
-  __device__ void row_ref(int q, double& a, double& b, double& c) const { coef(q, 0.0, a, b, c); }
+  __device__ Pt pt(int la, int lb, int q) const {
+    double v = cv[g.cv32(la + 1, clampi(q, 1, g.ny - 1), lb + 1, 3)];
+    return Pt{keep_or(v, q >= 1 && q <= g.ny - 1, 0.0)};
+  }
+  __device__ bool pt_ref(const Pt& p) const { return p.cv == 0.0; }
+  __device__ void coef(int, int, int q, const Pt& p, const Pt&, const Pt&, double& a, double& b,
+                       double& c) const {
+    coef(q, p.cv, a, b, c);
+  }
+  __device__ void coef_ref(int q, double& a, double& b, double& c) const { coef(q, 0.0, a, b, c); }
   RefKey ref_key(int nn, int S) const { return RefKey{2, nn, S, 3, beta, dteta, mkey}; }
   __device__ void store(int la, int lb, int q, double x) const {
-    f[g.at(la + 1, q, lb + 1)] = x;
+    f[g.at32(la + 1, q, lb + 1)] = x;
   }
 };
 
 // implicit_Ty (nse_dominik.f:1784-1870) with lly(0) = lly(1) and
 // lly(ny+1) = lly(ny); only rows 1..ny are written back (parallel_Ty).
+// The inputs of ghost rows 0 and ny+1 carry the wall cell's conductivity,
+// which is the neighbour value rows 1 and ny use.
 struct SysYT {
   using V = double;
-  static constexpr bool kCyclic = false, kHasRef = true;
+  using Pt = PtT;
+  static constexpr bool kCyclic = false, kHasRef = true, kNb = true;
   Geom g;
   double* f;
   const double *cv, *rc, *thc;
@@ -131,41 +175,43 @@ struct SysYT {
   __device__ bool valid(int la, int lb) const { return la < g.nx; }
   __device__ void coef(int q, double cvv, double l0, double lm, double lp, double rcv, double& a,
                        double& b, double& c) const {
-    if (q == 0) {
-      a = 0.0; b = 1.0; c = 1.0;
-    } else if (q == g.ny + 1) {
-      a = 1.0; b = 1.0; c = 0.0;
-    } else {
-      int j = q;
-      double rho = beta_over_alpha(beta, dteta, cvv);
-      double rr = 1.0 / rcv;
-      double h1 = hm(m.dy[j], l0, m.dy[j - 1], lm);
-      double h2 = hm(m.dy[j + 1], lp, m.dy[j], l0);
-      a = -(rho * h1 * (r.rdy12[j - 1] * r.rdy[j]) * rr);
-      c = -(rho * h2 * (r.rdy12[j] * r.rdy[j]) * rr);
-      b = 1.0 - a - c;
-    }
+    const int j = clampi(q, 1, g.ny);
+    const double rho = beta_over_alpha(beta, dteta, cvv);
+    const double rr = 1.0 / rcv;
+    const double h1 = hm(m.dy[j], l0, m.dy[j - 1], lm);
+    const double h2 = hm(m.dy[j + 1], lp, m.dy[j], l0);
+    const double ai = -(rho * h1 * (r.rdy12[j - 1] * r.rdy[j]) * rr);
+    const double ci = -(rho * h2 * (r.rdy12[j] * r.rdy[j]) * rr);
+    const double bi = 1.0 - ai - ci;
+    const bool in = q >= 1 && q <= g.ny;
+    a = keep_or(ai, in, q == 0 ? 0.0 : 1.0);
+    b = keep_or(bi, in, 1.0);
+    c = keep_or(ci, in, q == 0 ? 1.0 : 0.0);
   }
-  __device__ void row(int la, int lb, int q, double& a, double& b, double& c, double& d) const {
-    if (q == 0 || q == g.ny + 1) {
-      coef(q, 0.0, 1.0, 1.0, 1.0, 1.0, a, b, c);
-      d = 0.0;
-      return;
-    }
-    int i = la + 1, k = lb + 1, j = q;
-    long long o = g.at(i, j, k);
-    double l0 = thc[o];
-    double lm = (j == 1) ? l0 : thc[o - g.sy];
-    double lp = (j == g.ny) ? l0 : thc[o + g.sy];
-    coef(q, cv[g.cv(i, j, k, 1)], l0, lm, lp, rc[o], a, b, c);
-    d = f[o];
+  __device__ double rhs(int la, int lb, int q) const {
+    double v = f[g.at32(la + 1, clampi(q, 1, g.ny), lb + 1)];
+    return keep_or(v, q >= 1 && q <= g.ny, 0.0);
   }
-  __device__ void row_ref(int q, double& a, double& b, double& c) const {
+  __device__ Pt pt(int la, int lb, int q) const {
+    const int j = clampi(q, 1, g.ny), i = la + 1, k = lb + 1;
+    const unsigned o = g.at32(i, j, k);
+    const bool in = q >= 1 && q <= g.ny;
+    double c = cv[g.cv32(i, j, k, 1)], t = thc[o], h = rc[o];
+    return Pt{keep_or(c, in, 0.0), t, keep_or(h, in, 1.0)};
+  }
+  __device__ bool pt_ref(const Pt& p) const {
+    return (p.cv == 0.0) & (p.th == 1.0) & (p.rc == 1.0);
+  }
+  __device__ void coef(int, int, int q, const Pt& p, const Pt& pm, const Pt& pp, double& a,
+                       double& b, double& c) const {
+    coef(q, p.cv, p.th, pm.th, pp.th, p.rc, a, b, c);
+  }
+  __device__ void coef_ref(int q, double& a, double& b, double& c) const {
     coef(q, 0.0, 1.0, 1.0, 1.0, 1.0, a, b, c);
   }
   RefKey ref_key(int nn, int S) const { return RefKey{3, nn, S, 1, beta, dteta, m.dy}; }
   __device__ void store(int la, int lb, int q, double x) const {
-    if (q >= 1 && q <= g.ny) f[g.at(la + 1, q, lb + 1)] = x;
+    if (q >= 1 && q <= g.ny) f[g.at32(la + 1, q, lb + 1)] = x;
   }
 };
 
@@ -177,11 +223,11 @@ struct SysYT {
 struct LineAxis {
   Geom g;
   int dir;
-  __device__ long long at(int la, int lb, int q) const {
-    return dir == 2 ? g.at(la + 1, lb + 1, q) : g.at(q, la + 1, lb + 1);
+  __device__ unsigned at(int la, int lb, int q) const {
+    return dir == 2 ? g.at32(la + 1, lb + 1, q) : g.at32(q, la + 1, lb + 1);
   }
-  __device__ long long cvat(int la, int lb, int q, int n) const {
-    return dir == 2 ? g.cv(la + 1, lb + 1, q, n) : g.cv(q, la + 1, lb + 1, n);
+  __device__ unsigned cvat(int la, int lb, int q, int n) const {
+    return dir == 2 ? g.cv32(la + 1, lb + 1, q, n) : g.cv32(q, la + 1, lb + 1, n);
   }
   __device__ int len() const { return dir == 2 ? g.nz : g.nx; }
 };
@@ -191,7 +237,8 @@ struct LineAxis {
 // x_{n+1} = x1) are written too.
 struct SysPeriodicFace {
   using V = double;
-  static constexpr bool kCyclic = true, kHasRef = true;
+  using Pt = PtCv;
+  static constexpr bool kCyclic = true, kHasRef = true, kNb = false;
   LineAxis ax;
   double* f;
   const double* cv;
@@ -208,11 +255,14 @@ struct SysPeriodicFace {
     c = -(rho * (rd[q + 1] * rd12[q]));
     b = 1.0 - a - c;
   }
-  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, double& rhs) const {
-    coef(r, cv[ax.cvat(la, lb, r + 1, ncv)], a, b, c);
-    rhs = f[ax.at(la, lb, r + 1)];
+  __device__ double rhs(int la, int lb, int r) const { return f[ax.at(la, lb, r + 1)]; }
+  __device__ Pt pt(int la, int lb, int r) const { return Pt{cv[ax.cvat(la, lb, r + 1, ncv)]}; }
+  __device__ bool pt_ref(const Pt& p) const { return p.cv == 0.0; }
+  __device__ void coef(int, int, int r, const Pt& p, const Pt&, const Pt&, double& a, double& b,
+                       double& c) const {
+    coef(r, p.cv, a, b, c);
   }
-  __device__ void row_ref(int r, double& a, double& b, double& c) const { coef(r, 0.0, a, b, c); }
+  __device__ void coef_ref(int r, double& a, double& b, double& c) const { coef(r, 0.0, a, b, c); }
   RefKey ref_key(int nn, int S) const { return RefKey{4 + ax.dir, nn, S, ncv, beta, dteta, mkey}; }
   __device__ void store(int la, int lb, int r, double x) const {
     int q = r + 1, n = ax.len();
@@ -226,7 +276,8 @@ struct SysPeriodicFace {
 // with the conductivity ghosts wrapped periodically; rows 1..n written.
 struct SysPeriodicT {
   using V = double;
-  static constexpr bool kCyclic = true, kHasRef = true;
+  using Pt = PtT;
+  static constexpr bool kCyclic = true, kHasRef = true, kNb = true;
   LineAxis ax;
   double* f;
   const double *cv, *rc, *thc;
@@ -246,16 +297,22 @@ struct SysPeriodicT {
     c = -(rho * h2 * (rd12[q] * rd[q]) * rr);
     b = 1.0 - a - c;
   }
-  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, double& rhs) const {
-    int q = r + 1, n = ax.len();
-    long long o = ax.at(la, lb, q);
-    double l0 = thc[o];
-    double lm = thc[ax.at(la, lb, q == 1 ? n : q - 1)];
-    double lp = thc[ax.at(la, lb, q == n ? 1 : q + 1)];
-    coef(r, cv[ax.cvat(la, lb, q, 1)], l0, lm, lp, rc[o], a, b, c);
-    rhs = f[o];
+  __device__ double rhs(int la, int lb, int r) const { return f[ax.at(la, lb, r + 1)]; }
+  // r = -1 and r = n wrap to the last and first rows
+  __device__ Pt pt(int la, int lb, int r) const {
+    const int n = ax.len();
+    const int q = r < 0 ? n : (r >= n ? 1 : r + 1);
+    const unsigned o = ax.at(la, lb, q);
+    return Pt{cv[ax.cvat(la, lb, q, 1)], thc[o], rc[o]};
   }
-  __device__ void row_ref(int r, double& a, double& b, double& c) const {
+  __device__ bool pt_ref(const Pt& p) const {
+    return (p.cv == 0.0) & (p.th == 1.0) & (p.rc == 1.0);
+  }
+  __device__ void coef(int, int, int r, const Pt& p, const Pt& pm, const Pt& pp, double& a,
+                       double& b, double& c) const {
+    coef(r, p.cv, p.th, pm.th, pp.th, p.rc, a, b, c);
+  }
+  __device__ void coef_ref(int r, double& a, double& b, double& c) const {
     coef(r, 0.0, 1.0, 1.0, 1.0, 1.0, a, b, c);
   }
   RefKey ref_key(int nn, int S) const { return RefKey{6 + ax.dir, nn, S, 1, beta, dteta, d}; }
@@ -270,30 +327,34 @@ struct SysPeriodicT {
 // matrix differs from mode to mode, so there is no reference line.
 struct SysSpectral {
   using V = dcx;
-  static constexpr bool kCyclic = false, kHasRef = false;
+  using Pt = PtNone;
+  static constexpr bool kCyclic = false, kHasRef = false, kNb = false;
   Geom g;
   double* phit;  // interleaved complex
   const double *ac, *kxz;
   __device__ int n() const { return g.ny; }
   __device__ bool valid(int la, int lb) const { return la <= g.nx / 2 && lb < g.nz; }
-  __device__ void row(int la, int lb, int r, double& a, double& b, double& c, dcx& d) const {
-    int J = r + 1, ny = g.ny;
-    double a1 = ac[J - 1], a2 = ac[J - 1 + ny] - kxz[la + (g.nx / 2 + 1) * lb], a3 = ac[J - 1 + 2 * ny];
-    if (J == 1) {
-      a2 = a2 + a1;
-      a1 = 0.0;
-    }
-    if (J == ny) {
-      a2 = a2 + a3;
-      a3 = 0.0;
-    }
-    a = a1;
-    b = a2;
-    c = a3;
-    long long o = 2 * g.spec_at(la, J, lb);
-    d = {phit[o], phit[o + 1]};
+  __device__ dcx rhs(int la, int lb, int r) const {
+    unsigned o = 2 * g.spec32(la, r + 1, lb);
+    return {phit[o], phit[o + 1]};
   }
-  __device__ void row_ref(int, double& a, double& b, double& c) const {
+  __device__ Pt pt(int, int, int) const { return Pt{}; }
+  __device__ bool pt_ref(const Pt&) const { return false; }
+  __device__ void coef(int la, int lb, int r, const Pt&, const Pt&, const Pt&, double& a,
+                       double& b, double& c) const {
+    const int J = r + 1, ny = g.ny;
+    const double a1 = ac[J - 1], a3 = ac[J - 1 + 2 * ny];
+    double a2 = ac[J - 1 + ny] - kxz[la + (g.nx / 2 + 1) * lb];
+    // Neumann fold of the first and last rows, without a branch around the
+    // loads
+    const double f1 = a2 + a1, f3 = a2 + a3;
+    a2 = keep_or(a2, J != 1, f1);
+    a2 = keep_or(a2, J != ny, J == 1 ? f1 + a3 : f3);
+    a = keep_or(a1, J != 1, 0.0);
+    b = a2;
+    c = keep_or(a3, J != ny, 0.0);
+  }
+  __device__ void coef_ref(int, double& a, double& b, double& c) const {
     a = 0.0;
     b = 1.0;
     c = 0.0;
@@ -301,11 +362,23 @@ struct SysSpectral {
   RefKey ref_key(int nn, int S) const { return RefKey{0, nn, S, 0, 0.0, 0.0, nullptr}; }
   __device__ void store(int la, int lb, int r, dcx x) const {
     if (la == 0 && lb == 0) return;
-    long long o = 2 * g.spec_at(la, r + 1, lb);
+    unsigned o = 2 * g.spec32(la, r + 1, lb);
     phit[o] = x.x;
     phit[o + 1] = x.y;
   }
 };
+
+// the fast kernel's queue: two counters, each on its own 128-byte line,
+// then the list; cleared when allocated, afterwards by the general kernel
+LineQueue line_queue(unsigned nblocks, hipStream_t s) {
+  bool grown = false;
+  unsigned* p = static_cast<unsigned*>(
+      workspace(kWorkQueue, sizeof(unsigned) * (64 + (size_t)nblocks), &grown));
+  if (!p) return LineQueue{nullptr, nullptr, nullptr};
+  if (grown && hipMemsetAsync(p, 0, 64 * sizeof(unsigned), s) != hipSuccess)
+    return LineQueue{nullptr, nullptr, nullptr};
+  return LineQueue{p, p + 32, p + 64};
+}
 
 // reference-line tables, created on first use of a key
 double* line_table(const void* key, size_t keylen, long total, bool* fresh) {
@@ -362,10 +435,18 @@ __global__ void k_sp0b(Geom g, const double* ac, double* phit) {
   }
 }
 
+// launch_line_solve's result as a status: -1 is a line length the solver
+// does not take, -2 a failed allocation (whose message is already set)
+static int line_status(int rc, const char* what) {
+  if (rc == 0) return VK_OK;
+  if (rc == -2) return VK_ERR_HIP;
+  return fail(VK_ERR_BAD_SIZE, std::string(what) + ": line length not supported");
+}
+
 int launch_spn(const Geom& g, double* phit, const double* ac, const double* kxz, hipStream_t s) {
   SysSpectral sys{g, phit, ac, kxz};
   ProfScope ps("line_spectral", s);
-  return launch_line_solve<true>(sys, g.ny, g.nx / 2 + 1, g.nz, s);
+  return line_status(launch_line_solve<true>(sys, g.ny, g.nx / 2 + 1, g.nz, s), "spn");
 }
 
 void launch_sp0b(const Geom& g, double* phit, const double* ac, hipStream_t s) {
@@ -382,11 +463,11 @@ int launch_adi_velocity(const Geom& g, const Metrics& m, const RMetrics& r, doub
   if (comp == 1) {
     SysYV sy{g, f, cv, r, beta, dteta, m.dy};
     ProfScope ps("line_y_v", s);
-    rc = launch_line_solve<true>(sy, g.ny + 1, g.nx, g.nz, s);
+    rc = line_status(launch_line_solve<true>(sy, g.ny + 1, g.nx, g.nz, s), "y sweep");
   } else {
     SysYFace sy{g, f, cv, r, beta, dteta, comp + 2, m.dy};
     ProfScope ps("line_y_face", s);
-    rc = launch_line_solve<true>(sy, g.ny + 2, g.nx, g.nz, s);
+    rc = line_status(launch_line_solve<true>(sy, g.ny + 2, g.nx, g.nz, s), "y sweep");
   }
   if (rc) return rc;
   int jmax = (comp == 1) ? g.ny - 1 : g.ny;
@@ -394,13 +475,13 @@ int launch_adi_velocity(const Geom& g, const Metrics& m, const RMetrics& r, doub
                      m.dz};
   {
     ProfScope ps("line_z_face", s);
-    rc = launch_line_solve<true>(sz, g.nz, g.nx, jmax, s);
+    rc = line_status(launch_line_solve<true>(sz, g.nz, g.nx, jmax, s), "z sweep");
   }
   if (rc) return rc;
   SysPeriodicFace sx{LineAxis{g, 0}, f, cv, r.rdx, r.rdx12, beta, dteta, comp + 2, jmax, g.nz,
                      m.dx};
   ProfScope ps("line_x_face", s);
-  return launch_line_solve<false>(sx, g.nx, jmax, g.nz, s);
+  return line_status(launch_line_solve<false>(sx, g.nx, jmax, g.nz, s), "x sweep");
 }
 
 // solve_t: y, x, z sweeps (nse_dominik.f:1477-1500)
@@ -411,18 +492,18 @@ int launch_adi_temperature(const Geom& g, const Metrics& m, const RMetrics& r, d
   int rc;
   {
     ProfScope ps("line_y_t", s);
-    rc = launch_line_solve<true>(sy, g.ny + 2, g.nx, g.nz, s);
+    rc = line_status(launch_line_solve<true>(sy, g.ny + 2, g.nx, g.nz, s), "T y sweep");
   }
   if (rc) return rc;
   SysPeriodicT sx{LineAxis{g, 0}, f, cv, rc_, thc, m.dx, r.rdx, r.rdx12, beta, dteta, g.ny, g.nz};
   {
     ProfScope ps("line_x_t", s);
-    rc = launch_line_solve<false>(sx, g.nx, g.ny, g.nz, s);
+    rc = line_status(launch_line_solve<false>(sx, g.nx, g.ny, g.nz, s), "T x sweep");
   }
   if (rc) return rc;
   SysPeriodicT sz{LineAxis{g, 2}, f, cv, rc_, thc, m.dz, r.rdz, r.rdz12, beta, dteta, g.nx, g.ny};
   ProfScope ps("line_z_t", s);
-  return launch_line_solve<true>(sz, g.nz, g.nx, g.ny, s);
+  return line_status(launch_line_solve<true>(sz, g.nz, g.nx, g.ny, s), "T z sweep");
 }
 
 }  // namespace vk
@@ -439,6 +520,7 @@ int vk_solve_u_temp(double* u_a, double* v_a, double* w_a, const int* nx, const 
   VK_REQUIRE_INIT();
   Geom g(*nx, *ny, *nz);
   if (g.nx < 4 || g.ny < 4 || g.nz < 4) return fail(VK_ERR_BAD_SIZE, "solve_u_temp: n >= 4");
+  if (!g.fits32()) return fail(VK_ERR_BAD_SIZE, "solve_u_temp: grid too large");
   size_t fb = g.cells() * sizeof(double);
   VK_RESOLVE(dua, double, u_a, fb);
   VK_RESOLVE(dva, double, v_a, fb);
@@ -463,8 +545,8 @@ int vk_solve_u_temp(double* u_a, double* v_a, double* w_a, const int* nx, const 
   const double* olds[3] = {du, dv, dw};
   for (int c = 0; c < 3; c++) {
     launch_pen_divide(g, comps[c], olds[c], dcv, c + 2, *eta, *dt, g.ny, s);
-    if (launch_adi_velocity(g, m, r, comps[c], dcv, c, beta, dteta, s))
-      return fail(VK_ERR_BAD_SIZE, "solve_u_temp: line length not supported");
+    int rc = launch_adi_velocity(g, m, r, comps[c], dcv, c, beta, dteta, s);
+    if (rc) return rc;
   }
   VK_LAUNCH_CHECK("solve_u_temp");
   return VK_OK;
@@ -478,6 +560,7 @@ int vk_solve_t(double* u_t, const int* nx, const int* ny, const int* nz, const d
   VK_REQUIRE_INIT();
   Geom g(*nx, *ny, *nz);
   if (g.nx < 4 || g.ny < 4 || g.nz < 4) return fail(VK_ERR_BAD_SIZE, "solve_t: n >= 4");
+  if (!g.fits32()) return fail(VK_ERR_BAD_SIZE, "solve_t: grid too large");
   size_t fb = g.cells() * sizeof(double);
   VK_RESOLVE(dut, double, u_t, fb);
   VK_RESOLVE(drc, const double, rhocp, fb);
@@ -494,8 +577,8 @@ int vk_solve_t(double* u_t, const int* nx, const int* ny, const int* nz, const d
   int rrc = make_rmetrics(g, m, r, rt().stream);
   if (rrc) return rrc;
   double beta = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
-  if (launch_adi_temperature(g, m, r, dut, dcv, drc, dk, beta, dteta, rt().stream))
-    return fail(VK_ERR_BAD_SIZE, "solve_t: line length not supported");
+  int lrc = launch_adi_temperature(g, m, r, dut, dcv, drc, dk, beta, dteta, rt().stream);
+  if (lrc) return lrc;
   VK_LAUNCH_CHECK("solve_t");
   return VK_OK;
 }
@@ -505,11 +588,12 @@ int vk_spn(const int* nx, const int* ny, const int* nz, const double* ac, const 
   VK_REQUIRE_INIT();
   Geom g(*nx, *ny, *nz);
   if (g.nx % 2 || g.ny < 2) return fail(VK_ERR_BAD_SIZE, "spn: nx even, ny >= 2");
+  if (!g.fits32()) return fail(VK_ERR_BAD_SIZE, "spn: grid too large");
   VK_RESOLVE(dac, const double, ac, 3 * g.ny * sizeof(double));
   VK_RESOLVE(dk, const double, kxz, (g.nx / 2 + 1) * (g.nz + 1) * sizeof(double));
   VK_RESOLVE(dp, double, phit, 2 * g.spec_cells() * sizeof(double));
-  if (launch_spn(g, dp, dac, dk, rt().stream))
-    return fail(VK_ERR_BAD_SIZE, "spn: ny not supported");
+  int lrc = launch_spn(g, dp, dac, dk, rt().stream);
+  if (lrc) return lrc;
   VK_LAUNCH_CHECK("spn");
   return VK_OK;
 }

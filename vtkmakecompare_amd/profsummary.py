@@ -2,12 +2,13 @@
 
     python -m vtkmakecompare_amd.profsummary trace  <results.db> [--steps K]
     python -m vtkmakecompare_amd.profsummary traffic <fetch.db> <write.db> \
-        --points N [--ref KERNEL --ref-bytes B]
+        --points N [--calib-fetch F.db --calib-write W.db --calib-bytes B]
 
 `trace` lists per-kernel call counts and average/total durations of a
 --kernel-trace run; `traffic` lists FETCH_SIZE and WRITE_SIZE per dispatch
-(KiB as reported, and in units of one interior double field of N points),
-optionally scaled by what a reference kernel of known traffic reads.
+scaled to bytes by what the same counters read for a copy of known size
+(vtkmakecompare_amd.calibrate), and in units of one interior double field
+of N points.
 """
 from __future__ import annotations
 
@@ -43,25 +44,46 @@ def _per_kernel(db: str, counter: str):
     return {k: (sum(v) / len(v), len(v)) for k, v in d.items()}
 
 
-def traffic(fetch_db: str, write_db: str, points: int, ref: str | None = None,
-            ref_bytes: float | None = None) -> str:
+def calib_scale(fetch_db: str, write_db: str, kernel: str, nbytes: float):
+    """Factors that turn FETCH_SIZE and WRITE_SIZE (KiB per dispatch) into
+    bytes, from a copy kernel that reads and writes `nbytes` per launch."""
+    f = _per_kernel(fetch_db, "FETCH_SIZE")
+    w = _per_kernel(write_db, "WRITE_SIZE")
+    if kernel not in f or kernel not in w:
+        raise SystemExit(f"calibration kernel {kernel} not found")
+    return nbytes / (f[kernel][0] * 1024.0), nbytes / (w[kernel][0] * 1024.0)
+
+
+def traffic(fetch_db: str, write_db: str, points: int, fetch_scale: float = 1.0,
+            write_scale: float = 1.0) -> str:
+    """Per-kernel memory traffic per dispatch; `fields` is the traffic in
+    units of one interior double field of `points` points."""
     f = _per_kernel(fetch_db, "FETCH_SIZE")
     w = _per_kernel(write_db, "WRITE_SIZE")
     field = points * 8.0
-    scale_f = scale_w = 1.0
-    out = []
-    if ref and ref in f and ref_bytes:
-        scale_f = ref_bytes / (f[ref][0] * 1024.0)
-        scale_w = ref_bytes / (w[ref][0] * 1024.0) if ref in w else 1.0
-        out.append(f"scaled by reference kernel {ref}: fetch x{scale_f:.3f}, write x{scale_w:.3f}")
-    out.append(f"{'kernel':70s} {'fetch_MB':>10s} {'fields':>7s} {'write_MB':>10s} {'fields':>7s} "
-               f"{'n':>5s}")
+    out = [f"counter scale: fetch x{fetch_scale:.4f}, write x{write_scale:.4f}",
+           f"{'kernel':64s} {'fetch_MB':>10s} {'fields':>7s} {'write_MB':>10s} {'fields':>7s} "
+           f"{'total_B':>14s} {'n':>5s}"]
     for k in sorted(f, key=lambda k: -f[k][0]):
-        fb = f[k][0] * 1024.0 * scale_f
-        wb = w.get(k, (0.0, 0))[0] * 1024.0 * scale_w
-        out.append(f"{k[:70]:70s} {fb / 1e6:10.1f} {fb / field:7.2f} {wb / 1e6:10.1f} "
-                   f"{wb / field:7.2f} {f[k][1]:5d}")
+        fb = f[k][0] * 1024.0 * fetch_scale
+        wb = w.get(k, (0.0, 0))[0] * 1024.0 * write_scale
+        out.append(f"{k.replace(' ', '')[:64]:64s} {fb / 1e6:10.1f} {fb / field:7.2f} "
+                   f"{wb / 1e6:10.1f} {wb / field:7.2f} {fb + wb:14.0f} {f[k][1]:5d}")
     return "\n".join(out)
+
+
+def read_traffic(path: str, kernel: str):
+    """HBM bytes per dispatch (fetch + write) of `kernel` from a summary
+    written by `traffic`, or None."""
+    try:
+        with open(path) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 7 and parts[0] == kernel:
+                    return float(parts[5])
+    except OSError:
+        return None
+    return None
 
 
 def main():
@@ -74,13 +96,19 @@ def main():
     m.add_argument("fetch_db")
     m.add_argument("write_db")
     m.add_argument("--points", type=int, required=True)
-    m.add_argument("--ref")
-    m.add_argument("--ref-bytes", type=float)
+    m.add_argument("--calib-fetch", help="FETCH_SIZE results of the calibration copy")
+    m.add_argument("--calib-write", help="WRITE_SIZE results of the calibration copy")
+    m.add_argument("--calib-kernel", default="vk::k_copy_interior")
+    m.add_argument("--calib-bytes", type=float,
+                   help="bytes the calibration copy reads (and writes) per launch")
     a = ap.parse_args()
     if a.cmd == "trace":
         print(trace(a.db, a.steps))
     else:
-        print(traffic(a.fetch_db, a.write_db, a.points, a.ref, a.ref_bytes))
+        fs = ws = 1.0
+        if a.calib_fetch and a.calib_write and a.calib_bytes:
+            fs, ws = calib_scale(a.calib_fetch, a.calib_write, a.calib_kernel, a.calib_bytes)
+        print(traffic(a.fetch_db, a.write_db, a.points, fs, ws))
 
 
 if __name__ == "__main__":
