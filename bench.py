@@ -1,4 +1,5 @@
-"""Benchmark: the fused time step of the channel solver on one MI355X.
+"""Benchmark: the fused time step of the channel solver on one MI355X, in
+its FP64 form and its mixed-precision form.
 
 Workload: step384 -- 384 x 193 x 384 channel (Lx = 2 pi, Lz = pi, walls at
 y = +-1 with tanh stretching beta = 1.8), Re = 5600, Pr = 0.71, 8 seeded
@@ -7,12 +8,16 @@ noise; synthetic inputs, every field resident in HBM before timing.
 One step is one pass of the reference's time-loop body
 (nse_dominik.f:290-448): halos, explicit terms with penalisation and AB2,
 the nine ADI line sweeps plus three for temperature, the spectral
-pressure projection.
+pressure projection (with one FP64 refinement pass in the mixed form).
 
-Prints one JSON line with the throughput in M grid-point-steps/s, the
-roofline of the dominant kernel (algorithmic bytes per launch over its
-average duration, timed with HIP events on the library stream during the
-timed steps) and the oracle's CPU time on a bounded sample.
+Both forms start from the same state and run W + K steps; the line
+reports the mixed form's throughput as `value` (M grid-point-steps/s),
+the FP64 form's beside it, the ratio, the mixed form's normwise error
+against the FP64 form per field after the same steps, the post-projection
+divergence of each, the roofline of each form's dominant kernel
+(algorithmic bytes per launch over its average duration, timed with HIP
+events on the library stream during the timed steps) and the oracle's CPU
+time on a bounded sample.
 """
 from __future__ import annotations
 
@@ -28,49 +33,77 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# device symbol of each timed kernel of the fused step, as rocprofv3 names it
+# device symbol of each timed kernel, as rocprofv3 names it (the line
+# sweeps are two kernels each -- the recorded-multiplier kernel and the
+# general kernel for blocks next to the solid -- and are reported by their
+# sweep name only)
 KERNEL_SYMBOL = {
     "explicit_fused": "vk::k_explicit_fused",
+    "explicit_mixed": "vk::k_explicit_mixed",
     "project": "vk::k_project",
-    "line_y_face": "vk::k_line_solve<32,8,512,true,vk::SysYFace>",
-    "line_y_v": "vk::k_line_solve<32,8,512,true,vk::SysYV>",
-    "line_y_t": "vk::k_line_solve<32,8,512,true,vk::SysYT>",
-    "line_z_t": "vk::k_line_solve<64,8,512,true,vk::SysPeriodicT>",
-    "line_x_t": "vk::k_line_solve<64,8,256,false,vk::SysPeriodicT>",
-    "line_spectral": "vk::k_line_solve<32,8,512,true,vk::SysSpectral>",
-    "rfft_x_div": "vk::k_rfft_x<vk::SampleDiv>",
-    "irfft_x": "vk::k_irfft_x",
+    "project_mixed": "vk::k_project_mixed",
+    "rfft_x_div": "vk::k_rfft_x<vk::SampleDiv,double>",
+    "irfft_x": "vk::k_irfft_x<double>",
 }
 
 
-def kernel_bytes(name: str, g, with_ep2: bool) -> float:
+def kernel_bytes(name: str, g, with_ep2: bool = False, mixed: bool = False) -> float:
     """Algorithmic HBM bytes of one launch of a kernel of the fused step:
     each array the kernel must read or write counted once over the
-    interior (halos ignored), 8 bytes per double, 16 per complex."""
+    interior (halos ignored).  FP64 form: 8 bytes per double, 16 per
+    complex.  Mixed form: the velocity, u_temp, p and divu in double, the
+    temperature, history, u_ta, cellvalue, rhoCp, thc, the spectra and
+    the Poisson solutions in float."""
     N = g.nx * g.ny * g.nz
-    S = (g.nx // 2 + 1) * g.ny * g.nz * 16.0
     f = 8.0 * N
+    if not mixed:
+        S = (g.nx // 2 + 1) * g.ny * g.nz * 16.0
+        table = {
+            # u,v,w,T,rhoCp,thc, 4 history, 3 u_ta, 4 cellvalue in; 4 rhs, 4 history out
+            "explicit_fused": 25 * f,
+            "line_y_face": 3 * f, "line_y_v": 3 * f, "line_z_face": 3 * f, "line_x_face": 3 * f,
+            # T lines also read rhoCp and thc
+            "line_y_t": 5 * f, "line_x_t": 5 * f, "line_z_t": 5 * f,
+            "rfft_x_div": 3 * f + S, "rfft_x": f + S,
+            "cfft_z_fwd": 2 * S, "cfft_z_inv": 2 * S, "line_spectral": 2 * S,
+            "irfft_x": S + f,
+            "project": (18 if with_ep2 else 12) * f,
+            "copy_interior": 2 * f,
+        }
+        return table.get(name, 0.0)
+    h = 4.0 * N  # one float field
+    S = (g.nx // 2 + 1) * g.ny * g.nz * 8.0
     table = {
-        # u,v,w,T,rhoCp,thc, 4 history, 3 u_ta, 4 cellvalue in; 4 rhs, 4 history out
-        "explicit_fused": 25 * f,
-        "line_y_face": 3 * f, "line_y_v": 3 * f, "line_z_face": 3 * f, "line_x_face": 3 * f,
-        # T lines also read rhoCp and thc
-        "line_y_t": 5 * f, "line_x_t": 5 * f, "line_z_t": 5 * f,
-        "rfft_x_div": 3 * f + S, "rfft_x": f + S,
-        "cfft_z_fwd": 2 * S, "cfft_z_inv": 2 * S, "line_spectral": 2 * S,
-        "irfft_x": S + f,
-        "project": (18 if with_ep2 else 12) * f,
-        "copy_interior": 2 * f,
+        # in: u,v,w (double), T, rhoCp, thc, 4 history, 3 u_ta, 4 cellvalue (float);
+        # out: u_temp, v_temp, w_temp (double), T work, 4 history (float)
+        "explicit_mixed": 3 * f + 14 * h + 3 * f + 5 * h,
+        # velocity lines: double rhs and solution, float cellvalue
+        "line_y_face": 2 * f + h, "line_y_v": 2 * f + h, "line_z_face": 2 * f + h,
+        "line_x_face": 2 * f + h,
+        # temperature lines: all float (rhs, cellvalue, rhoCp, thc, solution)
+        "line_y_t": 5 * h, "line_x_t": 5 * h, "line_z_t": 5 * h,
+        "rfft_x_div": 3 * f + S, "rfft_x_resid": 3 * f + h + S,
+        "cfft_z_fwd": 2 * S, "cfft_z_fwd2": 2 * S, "cfft_z_inv": 2 * S, "cfft_z_inv2": 2 * S,
+        "line_spectral": 2 * S,
+        "irfft_x": S + h, "irfft_x2": S + h,
+        # phi1, delta, u_temp x3 in; u, v, w, p, divu (double), 3 u_ta (float) out
+        "project_mixed": 2 * h + 3 * f + 5 * f + 3 * h,
+        "copy_interior": 2 * h,
     }
     return table.get(name, 0.0)
 
 
-def step_bytes(g, with_ep2: bool) -> float:
-    per_step = {"explicit_fused": 1, "line_y_face": 2, "line_y_v": 1, "line_z_face": 3,
-                "line_x_face": 3, "line_y_t": 1, "line_x_t": 1, "line_z_t": 1,
-                "rfft_x_div": 1, "cfft_z_fwd": 1, "cfft_z_inv": 1, "line_spectral": 1,
-                "irfft_x": 1, "project": 1, "copy_interior": 1}
-    return sum(c * kernel_bytes(k, g, with_ep2) for k, c in per_step.items())
+STEP_LAUNCHES = {"explicit_fused": 1, "explicit_mixed": 1, "line_y_face": 2, "line_y_v": 1,
+                 "line_z_face": 3, "line_x_face": 3, "line_y_t": 1, "line_x_t": 1,
+                 "line_z_t": 1, "rfft_x_div": 1, "rfft_x_resid": 1, "cfft_z_fwd": 1,
+                 "cfft_z_fwd2": 1, "cfft_z_inv": 1, "cfft_z_inv2": 1, "irfft_x": 1,
+                 "irfft_x2": 1, "project": 1, "project_mixed": 1, "copy_interior": 1}
+
+
+def step_bytes(g, with_ep2: bool = False, mixed: bool = False) -> float:
+    per_step = dict(STEP_LAUNCHES)
+    per_step["line_spectral"] = 2 if mixed else 1
+    return sum(c * kernel_bytes(k, g, with_ep2, mixed) for k, c in per_step.items())
 
 
 def profile_entries(lib):
@@ -108,16 +141,57 @@ def cpu_baseline(cfg, g, st, cellvalue, steps: int):
                       f"({g.nx}x{g.ny}x{g.nz}), oracle C/OpenMP, {dt:.1f} s"}
 
 
+def timed_run(lib, run, steps: int, warmup: int):
+    """W untimed steps, then K timed steps with per-kernel HIP-event timing."""
+    from vtkmakecompare_amd import _lib, api
+    run.step(warmup)
+    api.synchronize()
+    _lib.check(lib.vk_profile_begin(), "profile")
+    t0 = time.perf_counter()
+    run.step(steps)
+    api.synchronize()
+    t1 = time.perf_counter()
+    _lib.check(lib.vk_profile_end(), "profile")
+    return t1 - t0, profile_entries(lib)
+
+
+def kernel_table(prof, g, steps, with_ep2, mixed):
+    return {k: {"total_ms": round(v[0], 3), "launches": v[1],
+                "avg_us": round(1e3 * v[0] / v[1], 2),
+                "alg_GBs": round(kernel_bytes(k, g, with_ep2, mixed) / (v[0] / v[1] / 1e3) / 1e9, 1)
+                if kernel_bytes(k, g, with_ep2, mixed) else None}
+            for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+
+
+def roofline(prof, g, with_ep2, mixed, traffic_file):
+    """The dominant kernel (by total time): algorithmic bytes per launch
+    over its average launch duration, against the 8 TB/s HBM peak."""
+    from vtkmakecompare_amd.profsummary import read_traffic
+    dname, (dms, dcount) = max(prof.items(), key=lambda kv: kv[1][0])
+    avg_s = dms / dcount / 1e3
+    alg = kernel_bytes(dname, g, with_ep2, mixed)
+    traffic = None
+    if dname in KERNEL_SYMBOL and not with_ep2:
+        traffic = read_traffic(traffic_file, KERNEL_SYMBOL[dname])
+    achieved = alg / avg_s / 1e9 if alg else 0.0
+    return {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "alg_bytes_per_launch": alg, "avg_launch_us": round(avg_s * 1e6, 2),
+            "traffic": traffic}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="step384")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=("both", "fp64", "mixed"), default="both",
+                    help="run one form only (for profiling)")
     ap.add_argument("--diagnostics", action="store_true",
-                    help="also compute MKP's u_ep2 splitting-error fields")
+                    help="also compute MKP's u_ep2 splitting-error fields (FP64 form)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -134,90 +208,93 @@ def main():
     cfg = G.make_config(args.config)
     g = cfg.grid()
     st = G.initial_state(cfg, g)
+    npts = g.npoints
     print(f"[bench] {cfg.name} {g.nx}x{g.ny}x{g.nz}: level set on device", file=sys.stderr,
           flush=True)
-    run = api.ChannelRun(cfg, g, st, diagnostics=args.diagnostics)
+    r64 = api.ChannelRun(cfg, g, st, diagnostics=args.diagnostics)
     api.synchronize()
-    run.step(args.warmup)
-    api.synchronize()
-    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+    cv = r64.get("cellvalue")
+    res = {}
+    if args.only in ("both", "fp64"):
+        el, prof = timed_run(lib, r64, args.steps, args.warmup)
+        res["fp64"] = (el, prof, r64)
+        print(f"[bench] fp64: {1e3 * el / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+    if args.only in ("both", "mixed"):
+        if args.only == "mixed":
+            r64 = None
+            res.pop("fp64", None)
+        rmx = api.ChannelRun(cfg, g, st, cellvalue=cv, precision="mixed")
+        el, prof = timed_run(lib, rmx, args.steps, args.warmup)
+        res["mixed"] = (el, prof, rmx)
+        print(f"[bench] mixed: {1e3 * el / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
 
-    _lib.check(lib.vk_profile_begin(), "profile")
-    t0 = time.perf_counter()
-    run.step(args.steps)
-    api.synchronize()
-    t1 = time.perf_counter()
-    _lib.check(lib.vk_profile_end(), "profile")
-    prof = profile_entries(lib)
-    elapsed = t1 - t0
-    npts = g.npoints
-    value = npts * args.steps / elapsed / 1e6
-    ms_step = 1e3 * elapsed / args.steps
+    forms = {}
+    for name, (el, prof, run) in res.items():
+        mixed = name == "mixed"
+        with_ep2 = args.diagnostics and not mixed
+        ms = 1e3 * el / args.steps
+        sb = step_bytes(g, with_ep2, mixed)
+        tfile = os.path.join(ROOT, "profiles",
+                             f"{cfg.name}_{'mixed_' if mixed else ''}traffic.txt")
+        forms[name] = {
+            "value": round(npts * args.steps / el / 1e6, 2), "ms_per_step": round(ms, 4),
+            "roofline": roofline(prof, g, with_ep2, mixed, tfile),
+            "step_roofline": {"alg_bytes_per_step": sb,
+                              "alg_GBs": round(sb / (ms / 1e3) / 1e9, 1),
+                              "frac": round(sb / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                              "kernel_time_ms_per_step":
+                                  round(sum(v[0] for v in prof.values()) / args.steps, 4)},
+            "kernels": kernel_table(prof, g, args.steps, with_ep2, mixed),
+            "max_abs_divu": api.field_absmax(run.d["divu"], g.nx, g.ny, g.nz),
+        }
 
-    # divergence check of the final state (post-projection, absolute)
-    div = api.field_absmax(run.d["divu"], g.nx, g.ny, g.nz)
-    umax = api.field_absmax(run.d["u"], g.nx, g.ny, g.nz)
-
-    # dominant kernel by total time
-    with_ep2 = args.diagnostics
-    dom = max(prof.items(), key=lambda kv: kv[1][0])
-    dname, (dms, dcount) = dom
-    avg_s = dms / dcount / 1e3
-    alg = kernel_bytes(dname, g, with_ep2)
-    # HBM bytes per launch from the committed FETCH_SIZE + WRITE_SIZE summary
-    # of this workload (two rocprofv3 --pmc runs, scaled by a known copy)
-    from vtkmakecompare_amd.profsummary import read_traffic
-    traffic = None
-    if cfg.name == "step384" and not with_ep2 and dname in KERNEL_SYMBOL:
-        traffic = read_traffic(os.path.join(ROOT, "profiles", "step384_traffic.txt"),
-                               KERNEL_SYMBOL[dname])
-    achieved = alg / avg_s / 1e9 if alg else 0.0
-    kernels = {k: {"total_ms": round(v[0], 3), "launches": v[1],
-                   "avg_us": round(1e3 * v[0] / v[1], 2),
-                   "alg_GBs": round(kernel_bytes(k, g, with_ep2) / (v[0] / v[1] / 1e3) / 1e9, 1)
-                   if kernel_bytes(k, g, with_ep2) else None}
-               for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
-    kernel_sum_ms = sum(v[0] for v in prof.values()) / args.steps
+    # error of the mixed form against the FP64 form after the same steps
+    errors = None
+    if "fp64" in res and "mixed" in res:
+        a, b = res["mixed"][2], res["fp64"][2]
+        errors = {}
+        for k in ("u", "v", "w", "p", "t"):
+            x = a.get(k)[1:-1, 1:-1, 1:-1]
+            y = b.get(k)[1:-1, 1:-1, 1:-1]
+            errors[k] = float(np.max(np.abs(x - y)) / max(float(np.max(np.abs(y))), 1e-300))
+            del x, y
+        errors["velocity"] = max(errors["u"], errors["v"], errors["w"])
+        errors["steps_compared"] = args.warmup + args.steps
 
     cpu = None
     if not args.no_cpu_baseline:
-        print(f"[bench] GPU done ({ms_step:.3f} ms/step); CPU baseline "
-              f"{args.cpu_steps} steps", file=sys.stderr, flush=True)
-        cv = run.get("cellvalue")
+        print(f"[bench] CPU baseline {args.cpu_steps} steps", file=sys.stderr, flush=True)
         st2 = G.initial_state(cfg, g)
         cpu = cpu_baseline(cfg, g, st2, cv, args.cpu_steps)
 
+    head = forms.get("mixed") or forms["fp64"]
+    is_mixed = "mixed" in forms
     out = {
         "metric": "M grid-pt-steps/s @384x193x384: mixed vs FP64 at <=1e-6 rel err; HBM GB/s",
-        "value": round(value, 2),
+        "value": head["value"],
         "unit": "M grid-pt-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64 (mixed)" if is_mixed else "f64",
         "data": "synthetic (seeded counter-based noise, 8 random spheres)",
-        "config": {"workload": f"{cfg.name}: full FP64 time step, {g.nx}x{g.ny}x{g.nz} "
-                               f"channel, Re={cfg.re:g}, 8 spheres, tanh-stretched y",
-                   "precision": "fp64", "grid": [g.nx, g.ny, g.nz],
-                   "mkp_diagnostics": with_ep2},
-        "roofline": {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "alg_bytes_per_launch": alg, "avg_launch_us": round(avg_s * 1e6, 2),
-                     "traffic": traffic},
-        "step_roofline": {"alg_bytes_per_step": step_bytes(g, with_ep2),
-                          "alg_GBs": round(step_bytes(g, with_ep2) / (ms_step / 1e3) / 1e9, 1),
-                          "frac": round(step_bytes(g, with_ep2) / (ms_step / 1e3) / 1e9
-                                        / HBM_PEAK_GBS, 4),
-                          "kernel_time_ms_per_step": round(kernel_sum_ms, 4)},
-        "kernels": kernels,
-        "check": {"max_abs_divu": div, "max_abs_u": umax},
+        "config": {"workload": f"{cfg.name}: full time step, {g.nx}x{g.ny}x{g.nz} channel, "
+                               f"Re={cfg.re:g}, 8 spheres, tanh-stretched y; value is the "
+                               f"{'mixed-precision' if is_mixed else 'FP64'} step",
+                   "precision": "mixed" if is_mixed else "fp64", "grid": [g.nx, g.ny, g.nz],
+                   "mkp_diagnostics": args.diagnostics},
+        "roofline": head["roofline"],
+        "mixed_vs_fp64": None,
+        "errors_mixed_vs_fp64": errors,
+        "forms": forms,
         "cpu_baseline": cpu,
     }
+    if "fp64" in forms and "mixed" in forms:
+        out["mixed_vs_fp64"] = round(forms["mixed"]["value"] / forms["fp64"]["value"], 4)
     print(json.dumps(out))
 
 

@@ -232,6 +232,26 @@ int vk_time_step(double *u, double *v, double *w, double *p, double *t,
                  const double *eta, const double *xl, const double *zl,
                  const int *l);
 
+/* The mixed-precision form of vk_time_step (no reference counterpart:
+   the reference is FP64 throughout).  Same arrays and layouts, with these
+   held in float: t, u_pre, v_pre, w_pre, t_pre, u_ta, v_ta, w_ta,
+   cellvalue, rhocp, thc.  u, v, w, p, u_temp, v_temp, w_temp and divu stay
+   double.  The explicit phase and the FFTs compute in FP32, the line
+   solves in FP64 on the mixed storage; the projection adds one FP64
+   iterative-refinement pass, so the post-projection divergence is at
+   FP64 round-off.  MKP's u_ep2 diagnostic is not produced. */
+int vk_time_step_mixed(double *u, double *v, double *w, double *p, float *t,
+                       float *u_pre, float *v_pre, float *w_pre, float *t_pre,
+                       float *u_ta, float *v_ta, float *w_ta, double *u_temp,
+                       double *v_temp, double *w_temp, double *divu,
+                       float *cellvalue, const float *rhocp, const float *thc,
+                       const double *dx, const double *dy, const double *dz,
+                       const double *dx12, const double *dy12,
+                       const double *dz12, const int *nx, const int *ny,
+                       const int *nz, const double *dt, const double *re,
+                       const double *sc, const double *eta, const double *xl,
+                       const double *zl, const int *l);
+
 /* dst := src over a whole halo field, or over its interior when
    *interior_only is non-zero: the inline array copies of program main
    (u_pre := u at nse_dominik.f:246-257, u_temp := u_a at :407-417,

@@ -155,20 +155,37 @@ STEP_FIELDS = ("u", "v", "w", "p", "t", "u_pre", "v_pre", "w_pre", "t_pre",
                "rhocp", "thc", "u_ep2", "v_ep2", "w_ep2")
 
 
+# fields the mixed-precision step keeps in float (with cellvalue)
+MIXED_F32 = ("t", "u_pre", "v_pre", "w_pre", "t_pre", "u_ta", "v_ta", "w_ta", "rhocp", "thc")
+
+
 class ChannelRun:
-    """A configuration resident in HBM, advanced by vk_time_step.
+    """A configuration resident in HBM, advanced by vk_time_step
+    (``precision="fp64"``) or vk_time_step_mixed (``precision="mixed"``).
 
     ``state`` is the dict of initial_state(); the level set is either
-    given (``cellvalue``) or computed on the device by set_levelset.
+    given (``cellvalue``) or computed on the device by set_levelset.  In
+    the mixed form the fields of MIXED_F32 and cellvalue are held in
+    float, the rest in double; the MKP splitting-error diagnostic is
+    not produced.
     """
 
     def __init__(self, cfg: G.Config, g: G.Grid, state: dict, cellvalue=None,
-                 diagnostics=True):
+                 diagnostics=True, precision: str = "fp64"):
         init()
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
+        self.precision = precision
+        mixed = precision == "mixed"
         self.cfg, self.g = cfg, g
         self.d = {}
         for n in STEP_FIELDS:
-            self.d[n] = DeviceArray.from_host(state[n])
+            if mixed and n in ("u_ep2", "v_ep2", "w_ep2"):
+                continue
+            a = state[n]
+            if mixed and n in MIXED_F32:
+                a = np.asarray(a, dtype=np.float32)
+            self.d[n] = DeviceArray.from_host(a)
         for n in ("dx", "dy", "dz", "dx12", "dy12", "dz12", "x", "y", "z"):
             self.d[n] = DeviceArray.from_host(getattr(g, n))
         if cellvalue is None:
@@ -178,13 +195,29 @@ class ChannelRun:
             d = self.d
             set_levelset(g.nx, g.ny, g.nz, d["dx"], d["dx12"], d["dy12"], d["dz12"], d["phi0"],
                          d["phi"], d["x"], d["y"], d["z"], d["cellvalue"])
+            if mixed:
+                cv = self.d["cellvalue"].download()
+                self.d["cellvalue"].free()
+                self.d["cellvalue"] = DeviceArray.from_host(cv.astype(np.float32))
         else:
-            self.d["cellvalue"] = DeviceArray.from_host(cellvalue)
-        self.diagnostics = diagnostics
+            cv = np.asarray(cellvalue, dtype=np.float32 if mixed else np.float64)
+            self.d["cellvalue"] = DeviceArray.from_host(cv)
+        self.diagnostics = diagnostics and not mixed
         self.l = 0
 
     def step(self, n: int = 1) -> None:
         d, g, c = self.d, self.g, self.cfg
+        if self.precision == "mixed":
+            for _ in range(n):
+                self.l += 1
+                check(lib().vk_time_step_mixed(
+                    d["u"], d["v"], d["w"], d["p"], d["t"], d["u_pre"], d["v_pre"],
+                    d["w_pre"], d["t_pre"], d["u_ta"], d["v_ta"], d["w_ta"], d["u_temp"],
+                    d["v_temp"], d["w_temp"], d["divu"], d["cellvalue"], d["rhocp"],
+                    d["thc"], d["dx"], d["dy"], d["dz"], d["dx12"], d["dy12"], d["dz12"],
+                    I(g.nx), I(g.ny), I(g.nz), D(c.dt), D(c.re), D(c.sc), D(c.eta), D(g.xl),
+                    D(g.zl), I(self.l)), f"mixed time step {self.l}")
+            return
         ep = (d["u_ep2"], d["v_ep2"], d["w_ep2"]) if self.diagnostics else (None, None, None)
         for _ in range(n):
             self.l += 1
@@ -197,4 +230,5 @@ class ChannelRun:
                 I(self.l)), f"time step {self.l}")
 
     def get(self, name: str) -> np.ndarray:
-        return self.d[name].download()
+        """A field downloaded to the host, as float64 whatever its storage."""
+        return self.d[name].download().astype(np.float64, copy=False)

@@ -18,50 +18,70 @@
 
 namespace vk {
 
+// complex type and table of real type R
+template <typename R>
+struct Cx;
+template <>
+struct Cx<double> {
+  using T = double2;
+};
+template <>
+struct Cx<float> {
+  using T = float2;
+};
+
+template <typename R>
 struct FftPlan {
+  using C = typename Cx<R>::T;
   int n = 0;
   int nst = 0;
   int r[20] = {0};
-  const double2* tw = nullptr;  // e^{-2 pi i m / n}, m = 0..n-1
+  const C* tw = nullptr;  // e^{-2 pi i m / n}, m = 0..n-1
 };
 
 namespace {
-std::map<int, FftPlan>& plans() {
-  static std::map<int, FftPlan> p;
+template <typename R>
+std::map<int, FftPlan<R>>& plans() {
+  static std::map<int, FftPlan<R>> p;
   return p;
 }
-std::map<int, const double2*>& tables() {
-  static std::map<int, const double2*> t;
+template <typename R>
+std::map<int, const typename Cx<R>::T*>& tables() {
+  static std::map<int, const typename Cx<R>::T*> t;
   return t;
 }
 }  // namespace
 
-// e^{-2 pi i m / n} for m = 0..n-1, built once per n.
-const double2* twiddle_table(int n) {
-  auto it = tables().find(n);
-  if (it != tables().end()) return it->second;
-  std::vector<double2> h(n);
+// e^{-2 pi i m / n} for m = 0..n-1 in R, rounded once from extended
+// precision, built once per (n, R).
+template <typename R>
+const typename Cx<R>::T* twiddle_table(int n) {
+  using C = typename Cx<R>::T;
+  auto it = tables<R>().find(n);
+  if (it != tables<R>().end()) return it->second;
+  std::vector<C> h(n);
   const long double pi = 3.141592653589793238462643383279502884L;
   for (int m = 0; m < n; m++) {
     long double ang = -2.0L * pi * (long double)m / (long double)n;
-    h[m] = make_double2((double)cosl(ang), (double)sinl(ang));
+    h[m].x = (R)cosl(ang);
+    h[m].y = (R)sinl(ang);
   }
-  double2* d = nullptr;
-  if (hipMalloc(&d, sizeof(double2) * n) != hipSuccess) return nullptr;
-  if (hipMemcpy(d, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice) != hipSuccess)
-    return nullptr;
-  tables()[n] = d;
+  C* d = nullptr;
+  if (hipMalloc(&d, sizeof(C) * n) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), sizeof(C) * n, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+  tables<R>()[n] = d;
   return d;
 }
 
 // Factor n into radix 4, 3, 2 passes; false if another prime divides n.
-static bool fft_plan(int n, FftPlan& p) {
-  auto it = plans().find(n);
-  if (it != plans().end()) {
+template <typename R>
+static bool fft_plan(int n, FftPlan<R>& p) {
+  auto it = plans<R>().find(n);
+  if (it != plans<R>().end()) {
     p = it->second;
     return true;
   }
-  FftPlan q;
+  FftPlan<R> q;
   q.n = n;
   int m = n;
   while (m % 4 == 0 && m > 1) {
@@ -77,87 +97,114 @@ static bool fft_plan(int n, FftPlan& p) {
     m /= 2;
   }
   if (m != 1) return false;
-  q.tw = twiddle_table(n);
+  q.tw = twiddle_table<R>(n);
   if (!q.tw) return false;
-  plans()[n] = q;
+  plans<R>()[n] = q;
   p = q;
   return true;
 }
 
-__device__ inline double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+template <typename C>
+__device__ inline C mkc(decltype(C().x) x, decltype(C().x) y) {
+  C c;
+  c.x = x;
+  c.y = y;
+  return c;
 }
-__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ inline double2 conjd(double2 a) { return make_double2(a.x, -a.y); }
+template <typename C>
+__device__ inline C cmul(C a, C b) {
+  return mkc<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename C>
+__device__ inline C cadd(C a, C b) {
+  return mkc<C>(a.x + b.x, a.y + b.y);
+}
+template <typename C>
+__device__ inline C csub(C a, C b) {
+  return mkc<C>(a.x - b.x, a.y - b.y);
+}
+template <typename C>
+__device__ inline C conjc(C a) {
+  return mkc<C>(a.x, -a.y);
+}
+template <typename C>
+__device__ inline C cscale(decltype(C().x) s, C a) {
+  return mkc<C>(s * a.x, s * a.y);
+}
 // multiply by sign * i
-__device__ inline double2 mul_si(double2 a, int sign) {
-  return sign < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+template <typename C>
+__device__ inline C mul_si(C a, int sign) {
+  return sign < 0 ? mkc<C>(a.y, -a.x) : mkc<C>(-a.y, a.x);
 }
 
 // Stockham autosort FFT of length p.n in LDS, threads t = 0..T-1 of one
 // group.  sign = -1 forward, +1 backward, unnormalised.  Returns the
 // buffer holding the result (a or b).  All threads of the block must call
 // it together (it contains block barriers).
-__device__ double2* fft_lds(double2* a, double2* b, int t, int T, const FftPlan& p, int sign) {
+template <typename R>
+__device__ typename Cx<R>::T* fft_lds(typename Cx<R>::T* a, typename Cx<R>::T* b, int t, int T,
+                                      const FftPlan<R>& p, int sign) {
+  using C = typename Cx<R>::T;
   const int n = p.n;
   int ns = 1;
-  double2* src = a;
-  double2* dst = b;
+  C* src = a;
+  C* dst = b;
   for (int s = 0; s < p.nst; s++) {
-    const int R = p.r[s];
-    const int nb = n / R;
-    const int twstep = n / (ns * R);
+    const int Rd = p.r[s];
+    const int nb = n / Rd;
+    const int twstep = n / (ns * Rd);
     for (int j = t; j < nb; j += T) {
       const int kk = j % ns;
-      double2 v[4];
+      C v[4];
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        if (r < R) {
+        if (r < Rd) {
           v[r] = src[j + r * nb];
           if (r > 0) {
-            double2 w = p.tw[(r * kk * twstep) % n];
-            if (sign > 0) w = conjd(w);
+            C w = p.tw[(r * kk * twstep) % n];
+            if (sign > 0) w = conjc(w);
             v[r] = cmul(v[r], w);
           }
         }
-      double2 o[4];
-      if (R == 4) {
-        double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-        double2 s13 = cadd(v[1], v[3]), d13 = mul_si(csub(v[1], v[3]), sign);
+      C o[4];
+      if (Rd == 4) {
+        C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        C s13 = cadd(v[1], v[3]), d13 = mul_si(csub(v[1], v[3]), sign);
         o[0] = cadd(s02, s13);
         o[2] = csub(s02, s13);
         o[1] = cadd(d02, d13);
         o[3] = csub(d02, d13);
-      } else if (R == 3) {
-        const double h = 0.86602540378443864676;  // sqrt(3)/2
-        double2 sm = cadd(v[1], v[2]), df = csub(v[1], v[2]);
+      } else if (Rd == 3) {
+        const R h = (R)0.86602540378443864676;  // sqrt(3)/2
+        C sm = cadd(v[1], v[2]), df = csub(v[1], v[2]);
         o[0] = cadd(v[0], sm);
-        double2 tt = make_double2(v[0].x - 0.5 * sm.x, v[0].y - 0.5 * sm.y);
-        double2 rot = mul_si(make_double2(h * df.x, h * df.y), sign);
+        C tt = mkc<C>(v[0].x - (R)0.5 * sm.x, v[0].y - (R)0.5 * sm.y);
+        C rot = mul_si(cscale(h, df), sign);
         o[1] = cadd(tt, rot);
         o[2] = csub(tt, rot);
       } else {
         o[0] = cadd(v[0], v[1]);
         o[1] = csub(v[0], v[1]);
       }
-      const int base = (j / ns) * ns * R + kk;
+      const int base = (j / ns) * ns * Rd + kk;
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        if (r < R) dst[base + r * ns] = o[r];
+        if (r < Rd) dst[base + r * ns] = o[r];
     }
     __syncthreads();
-    double2* tmp = src;
+    C* tmp = src;
     src = dst;
     dst = tmp;
-    ns *= R;
+    ns *= Rd;
   }
   return src;
 }
 
-// Which sample feeds the forward x pass: a stored field, or the Poisson
+// Which sample feeds the forward x pass: a stored field, the Poisson
 // right-hand side div(u_temp)/dt computed on the fly (mkp.f:88-121, where
-// index 0 of x and z is the periodic image of nx and nz).
+// index 0 of x and z is the periodic image of nx and nz), or the residual
+// of a Poisson solution against that right-hand side.  Samples are
+// computed in FP64 whatever the transform's precision.
 struct SampleField {
   const double* fp;
   __device__ double operator()(const Geom& g, int i, int j, int k) const {
@@ -175,34 +222,56 @@ struct SampleDiv {
            dt;
   }
 };
+// div(u_temp)/dt - L phi for a float solution phi with filled halos, L
+// the discrete divergence of the discrete gradient that the projection
+// applies (mkp.f:141-151, 209-217): the divergence the projected velocity
+// would have, over dt.  All FP64.
+struct SampleResid {
+  SampleDiv div;
+  const float* phi;
+  const double *rdx12, *rdy12, *rdz12, *rdx, *rdy, *rdz;
+  __device__ double operator()(const Geom& g, int i, int j, int k) const {
+    int ii = (i == 0) ? g.nx : i, kk = (k == 0) ? g.nz : k;
+    long long o = g.at(ii, j, kk);
+    const double c = phi[o];
+    const double lx = (((double)phi[o + 1] - c) * rdx12[ii] -
+                       (c - (double)phi[o - 1]) * rdx12[ii - 1]) * rdx[ii];
+    const double ly = (((double)phi[o + g.sy] - c) * rdy12[j] -
+                       (c - (double)phi[o - g.sy]) * rdy12[j - 1]) * rdy[j];
+    const double lz = (((double)phi[o + g.sz] - c) * rdz12[kk] -
+                       (c - (double)phi[o - g.sz]) * rdz12[kk - 1]) * rdz[kk];
+    return div(g, i, j, k) - (lx + ly + lz);
+  }
+};
 
 constexpr int kRowsPerBlock = 4;
 constexpr int kRowThreads = 64;
 
 // forward x pass: rows (j, k) = (1 + blockIdx.x*4 + row, blockIdx.y)
-template <typename Sample>
-__global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_rfft_x(Geom g, Sample smp,
-                                                                        double* phit, FftPlan p,
-                                                                        const double2* twl) {
-  extern __shared__ double2 sh[];
+template <typename Sample, typename R>
+__global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_rfft_x(
+    Geom g, Sample smp, R* phit, FftPlan<R> p, const typename Cx<R>::T* twl) {
+  using C = typename Cx<R>::T;
+  extern __shared__ unsigned char shraw[];
+  C* sh = reinterpret_cast<C*>(shraw);
   const int M = p.n;  // nx/2
   const int row = threadIdx.x / kRowThreads, t = threadIdx.x % kRowThreads;
   const int j = 1 + blockIdx.x * kRowsPerBlock + row, k = blockIdx.y;
-  double2* a = sh + row * 2 * M;
-  double2* b = a + M;
+  C* a = sh + row * 2 * M;
+  C* b = a + M;
   const bool ok = j <= g.ny;
-  double* ar = reinterpret_cast<double*>(a);
-  for (int i = t; i < 2 * M; i += kRowThreads) ar[i] = ok ? smp(g, i, j, k) : 0.0;
+  R* ar = reinterpret_cast<R*>(a);
+  for (int i = t; i < 2 * M; i += kRowThreads) ar[i] = ok ? (R)smp(g, i, j, k) : (R)0;
   __syncthreads();
-  double2* z = fft_lds(a, b, t, kRowThreads, p, -1);
+  C* z = fft_lds<R>(a, b, t, kRowThreads, p, -1);
   if (!ok) return;
   // X_k = E_k + e^{-2 pi i k/nx} O_k with E, O from Z_k and conj(Z_{M-k})
   for (int q = t; q <= M; q += kRowThreads) {
-    double2 zk = z[q % M], zc = conjd(z[(M - q) % M]);
-    double2 e = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
-    double2 dd = csub(zk, zc);
-    double2 o = make_double2(0.5 * dd.y, -0.5 * dd.x);  // -i/2 (zk - zc)
-    double2 x = cadd(e, cmul(twl[q], o));
+    C zk = z[q % M], zc = conjc(z[(M - q) % M]);
+    C e = mkc<C>((R)0.5 * (zk.x + zc.x), (R)0.5 * (zk.y + zc.y));
+    C dd = csub(zk, zc);
+    C o = mkc<C>((R)0.5 * dd.y, (R)-0.5 * dd.x);  // -i/2 (zk - zc)
+    C x = cadd(e, cmul(twl[q], o));
     long long off = 2 * g.spec_at(q, j, k);
     phit[off] = x.x;
     phit[off + 1] = x.y;
@@ -212,9 +281,12 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_rfft_x(Geom g, S
 // z pass in place over PHIT columns (I, j) for I = blockIdx.x*C + c,
 // j = 1 + blockIdx.y.  Forward divides by nx*nz (p2s_fftw2d.f:101).
 constexpr int kCols = 4;
-__global__ __launch_bounds__(256) void k_cfft_z(Geom g, double* phit, FftPlan p, int sign,
+template <typename R>
+__global__ __launch_bounds__(256) void k_cfft_z(Geom g, R* phit, FftPlan<R> p, int sign,
                                                 int scale) {
-  extern __shared__ double2 sh[];
+  using C = typename Cx<R>::T;
+  extern __shared__ unsigned char shraw[];
+  C* sh = reinterpret_cast<C*>(shraw);
   const int N = p.n;  // nz
   const int nh1 = g.nx / 2 + 1;
   const int I0 = blockIdx.x * kCols, j = 1 + blockIdx.y;
@@ -222,26 +294,26 @@ __global__ __launch_bounds__(256) void k_cfft_z(Geom g, double* phit, FftPlan p,
   for (int e = threadIdx.x; e < kCols * N; e += blockDim.x) {
     int c = e % kCols, k = e / kCols;
     int I = I0 + c;
-    double2 v = make_double2(0.0, 0.0);
+    C v = mkc<C>((R)0, (R)0);
     if (I < nh1) {
       long long off = 2 * g.spec_at(I, j, k);
-      v = make_double2(phit[off], phit[off + 1]);
+      v = mkc<C>(phit[off], phit[off + 1]);
     }
     sh[c * 2 * N + k] = v;
   }
   __syncthreads();
   const int c = threadIdx.x / 64, t = threadIdx.x % 64;
-  double2* res = fft_lds(sh + c * 2 * N, sh + c * 2 * N + N, t, 64, p, sign);
+  C* res = fft_lds<R>(sh + c * 2 * N, sh + c * 2 * N + N, t, 64, p, sign);
   // each column's result buffer is the same half for every column
   const int half = (res == sh + c * 2 * N) ? 0 : N;
   __syncthreads();
-  const double sc = (double)(g.nx * g.nz);
+  const R sc = (R)(g.nx * g.nz);
   for (int e = threadIdx.x; e < kCols * N; e += blockDim.x) {
     int cc = e % kCols, k = e / kCols;
     int I = I0 + cc;
     if (I >= nh1) continue;
-    double2 v = sh[cc * 2 * N + half + k];
-    if (scale) v = make_double2(v.x / sc, v.y / sc);
+    C v = sh[cc * 2 * N + half + k];
+    if (scale) v = mkc<C>(v.x / sc, v.y / sc);
     long long off = 2 * g.spec_at(I, j, k);
     phit[off] = v.x;
     phit[off + 1] = v.y;
@@ -251,36 +323,37 @@ __global__ __launch_bounds__(256) void k_cfft_z(Geom g, double* phit, FftPlan p,
 // inverse x pass (c2r) of row (j, k), then FP(0..nx-1, j, k), the wrap
 // FP(nx, j, k) = FP(0, j, k) and, from row k = 0, the plane k = nz
 // (p2s_fftw2d.f:205-216).
-__global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_irfft_x(Geom g,
-                                                                         const double* phit,
-                                                                         double* fp, FftPlan p,
-                                                                         const double2* twl) {
-  extern __shared__ double2 sh[];
+template <typename R>
+__global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_irfft_x(
+    Geom g, const R* phit, R* fp, FftPlan<R> p, const typename Cx<R>::T* twl) {
+  using C = typename Cx<R>::T;
+  extern __shared__ unsigned char shraw[];
+  C* sh = reinterpret_cast<C*>(shraw);
   const int M = p.n;
   const int row = threadIdx.x / kRowThreads, t = threadIdx.x % kRowThreads;
   const int j = 1 + blockIdx.x * kRowsPerBlock + row, k = blockIdx.y;
-  double2* a = sh + row * 2 * M;
-  double2* b = a + M;
+  C* a = sh + row * 2 * M;
+  C* b = a + M;
   const bool ok = j <= g.ny;
   // Z_k = (X_k + conj X_{M-k}) + i e^{2 pi i k/nx} (X_k - conj X_{M-k}),
   // with the imaginary parts of X_0 and X_M dropped as a c2r transform does
   for (int q = t; q < M; q += kRowThreads) {
-    double2 xk = make_double2(0.0, 0.0), xc = make_double2(0.0, 0.0);
+    C xk = mkc<C>((R)0, (R)0), xc = mkc<C>((R)0, (R)0);
     if (ok) {
       long long o1 = 2 * g.spec_at(q, j, k), o2 = 2 * g.spec_at(M - q, j, k);
-      xk = make_double2(phit[o1], q == 0 ? 0.0 : phit[o1 + 1]);
-      xc = make_double2(phit[o2], (q == 0) ? 0.0 : -phit[o2 + 1]);
+      xk = mkc<C>(phit[o1], q == 0 ? (R)0 : phit[o1 + 1]);
+      xc = mkc<C>(phit[o2], (q == 0) ? (R)0 : -phit[o2 + 1]);
     }
-    double2 sum = cadd(xk, xc), dif = csub(xk, xc);
-    double2 r = cmul(conjd(twl[q]), dif);
-    a[q] = make_double2(sum.x - r.y, sum.y + r.x);
+    C sum = cadd(xk, xc), dif = csub(xk, xc);
+    C r = cmul(conjc(twl[q]), dif);
+    a[q] = mkc<C>(sum.x - r.y, sum.y + r.x);
   }
   __syncthreads();
-  double2* z = fft_lds(a, b, t, kRowThreads, p, +1);
+  C* z = fft_lds<R>(a, b, t, kRowThreads, p, +1);
   if (!ok) return;
-  const double* zr = reinterpret_cast<const double*>(z);
+  const R* zr = reinterpret_cast<const R*>(z);
   for (int i = t; i < 2 * M; i += kRowThreads) {
-    double x = zr[i];
+    R x = zr[i];
     fp[g.at(i, j, k)] = x;
     if (i == 0) fp[g.at(g.nx, j, k)] = x;
     if (k == 0) {
@@ -290,61 +363,87 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_irfft_x(Geom g,
   }
 }
 
-static int plans_for(const Geom& g, FftPlan& px, FftPlan& pz, const double2*& twl) {
+template <typename R>
+static int plans_for(const Geom& g, FftPlan<R>& px, FftPlan<R>& pz,
+                     const typename Cx<R>::T*& twl) {
   if (g.nx % 2 != 0) return fail(VK_ERR_BAD_SIZE, "FFT: nx must be even");
-  if (!fft_plan(g.nx / 2, px) || !fft_plan(g.nz, pz))
+  if (!fft_plan<R>(g.nx / 2, px) || !fft_plan<R>(g.nz, pz))
     return fail(VK_ERR_BAD_SIZE, "FFT: nx/2 and nz must be products of 2 and 3");
-  twl = twiddle_table(g.nx);
+  twl = twiddle_table<R>(g.nx);
   if (!twl) return fail(VK_ERR_HIP, "FFT: twiddle table");
   if (g.nz > 2048 || g.nx > 4096) return fail(VK_ERR_BAD_SIZE, "FFT: size above the LDS plan");
   return VK_OK;
 }
 
-template <typename Sample>
-static int launch_forward(const Geom& g, Sample smp, double* phit, const char* xname,
-                          hipStream_t s) {
-  FftPlan px, pz;
-  const double2* twl;
-  int rc = plans_for(g, px, pz, twl);
+template <typename R, typename Sample>
+static int launch_forward(const Geom& g, Sample smp, R* phit, const char* xname,
+                          const char* zname, hipStream_t s) {
+  using C = typename Cx<R>::T;
+  FftPlan<R> px, pz;
+  const C* twl;
+  int rc = plans_for<R>(g, px, pz, twl);
   if (rc) return rc;
-  size_t shx = sizeof(double2) * 2 * (g.nx / 2) * kRowsPerBlock;
+  size_t shx = sizeof(C) * 2 * (g.nx / 2) * kRowsPerBlock;
   {
     ProfScope ps(xname, s);
-    k_rfft_x<Sample><<<dim3((g.ny + kRowsPerBlock - 1) / kRowsPerBlock, g.nz),
-                       kRowsPerBlock * kRowThreads, shx, s>>>(g, smp, phit, px, twl);
+    k_rfft_x<Sample, R><<<dim3((g.ny + kRowsPerBlock - 1) / kRowsPerBlock, g.nz),
+                          kRowsPerBlock * kRowThreads, shx, s>>>(g, smp, phit, px, twl);
   }
-  size_t shz = sizeof(double2) * 2 * g.nz * kCols;
-  ProfScope ps("cfft_z_fwd", s);
-  k_cfft_z<<<dim3((g.nx / 2 + 1 + kCols - 1) / kCols, g.ny), 256, shz, s>>>(g, phit, pz, -1, 1);
+  size_t shz = sizeof(C) * 2 * g.nz * kCols;
+  ProfScope ps(zname, s);
+  k_cfft_z<R><<<dim3((g.nx / 2 + 1 + kCols - 1) / kCols, g.ny), 256, shz, s>>>(g, phit, pz, -1,
+                                                                              1);
   return VK_OK;
 }
 
 int launch_p2s(const Geom& g, const double* fp, double* phit, hipStream_t s) {
-  return launch_forward(g, SampleField{fp}, phit, "rfft_x", s);
+  return launch_forward<double>(g, SampleField{fp}, phit, "rfft_x", "cfft_z_fwd", s);
 }
 
+template <typename R>
 int launch_p2s_div(const Geom& g, const Metrics& m, const double* u, const double* v,
-                   const double* w, double dt, double* phit, hipStream_t s) {
-  return launch_forward(g, SampleDiv{u, v, w, m.dx, m.dy, m.dz, dt}, phit, "rfft_x_div", s);
+                   const double* w, double dt, R* phit, hipStream_t s) {
+  return launch_forward<R>(g, SampleDiv{u, v, w, m.dx, m.dy, m.dz, dt}, phit, "rfft_x_div",
+                           "cfft_z_fwd", s);
+}
+template int launch_p2s_div<double>(const Geom&, const Metrics&, const double*, const double*,
+                                    const double*, double, double*, hipStream_t);
+template int launch_p2s_div<float>(const Geom&, const Metrics&, const double*, const double*,
+                                   const double*, double, float*, hipStream_t);
+
+int launch_p2s_resid(const Geom& g, const Metrics& m, const RMetrics& r, const double* u,
+                     const double* v, const double* w, double dt, const float* phi,
+                     float* phit, hipStream_t s) {
+  SampleResid smp{SampleDiv{u, v, w, m.dx, m.dy, m.dz, dt}, phi, r.rdx12, r.rdy12, r.rdz12,
+                  r.rdx, r.rdy, r.rdz};
+  return launch_forward<float>(g, smp, phit, "rfft_x_resid", "cfft_z_fwd2", s);
 }
 
 // S2P from a spectrum the call may overwrite (the z pass runs in place)
-int launch_s2p_inplace(const Geom& g, double* phit, double* fp, hipStream_t s) {
-  FftPlan px, pz;
-  const double2* twl;
-  int rc = plans_for(g, px, pz, twl);
+template <typename R>
+int launch_s2p_inplace(const Geom& g, R* phit, R* fp, hipStream_t s, const char* zname,
+                       const char* xname) {
+  using C = typename Cx<R>::T;
+  FftPlan<R> px, pz;
+  const C* twl;
+  int rc = plans_for<R>(g, px, pz, twl);
   if (rc) return rc;
-  size_t shz = sizeof(double2) * 2 * g.nz * kCols;
+  size_t shz = sizeof(C) * 2 * g.nz * kCols;
   {
-    ProfScope ps("cfft_z_inv", s);
-    k_cfft_z<<<dim3((g.nx / 2 + 1 + kCols - 1) / kCols, g.ny), 256, shz, s>>>(g, phit, pz, +1, 0);
+    ProfScope ps(zname, s);
+    k_cfft_z<R><<<dim3((g.nx / 2 + 1 + kCols - 1) / kCols, g.ny), 256, shz, s>>>(g, phit, pz, +1,
+                                                                                0);
   }
-  size_t shx = sizeof(double2) * 2 * (g.nx / 2) * kRowsPerBlock;
-  ProfScope ps("irfft_x", s);
-  k_irfft_x<<<dim3((g.ny + kRowsPerBlock - 1) / kRowsPerBlock, g.nz),
-              kRowsPerBlock * kRowThreads, shx, s>>>(g, phit, fp, px, twl);
+  size_t shx = sizeof(C) * 2 * (g.nx / 2) * kRowsPerBlock;
+  ProfScope ps(xname, s);
+  k_irfft_x<R><<<dim3((g.ny + kRowsPerBlock - 1) / kRowsPerBlock, g.nz),
+                 kRowsPerBlock * kRowThreads, shx, s>>>(g, phit, fp, px, twl);
   return VK_OK;
 }
+template int launch_s2p_inplace<double>(const Geom&, double*, double*, hipStream_t, const char*,
+                                        const char*);
+template int launch_s2p_inplace<float>(const Geom&, float*, float*, hipStream_t, const char*,
+                                       const char*);
 
 }  // namespace vk
 
@@ -374,7 +473,7 @@ int vk_s2p(const double* fpt, double* fp, const int* nx, const int* ny, const in
   int rc = hip_check(hipMemcpyAsync(work, dft, sb, hipMemcpyDeviceToDevice, rt().stream),
                      "s2p copy");
   if (rc) return rc;
-  rc = launch_s2p_inplace(g, work, dfp, rt().stream);
+  rc = launch_s2p_inplace<double>(g, work, dfp, rt().stream, "cfft_z_inv", "irfft_x");
   if (rc) return rc;
   VK_LAUNCH_CHECK("s2p");
   return VK_OK;

@@ -10,33 +10,53 @@
 
 namespace vk {
 
-// halos (vk_stencil.hip)
-void launch_set_bc(const Geom& g, double* u, double* v, double* w, double* p, double* t,
+// halos (vk_stencil.hip); TT / T = double or float
+template <typename TT>
+void launch_set_bc(const Geom& g, double* u, double* v, double* w, double* p, TT* t,
                    hipStream_t s);
-void launch_halo_neumann(const Geom& g, double* f, hipStream_t s);
+template <typename T>
+void launch_halo_neumann(const Geom& g, T* f, hipStream_t s);
 void launch_convect(const Geom& g, const Metrics& m, const double* u, const double* v,
                     const double* w, double* uv, double* vv, double* wv, double dt,
                     hipStream_t s);
 void launch_pen_divide(const Geom& g, double* x, const double* old, const double* cv, int n,
                        double eta, double dt, int jmax, hipStream_t s);
-void launch_cv_faces(const Geom& g, double* cv, hipStream_t s);
+template <typename T>
+void launch_cv_faces(const Geom& g, T* cv, hipStream_t s);
 
 // line solves (vk_tridiag.hip); return non-zero for unsupported lengths
-int launch_spn(const Geom& g, double* phit, const double* ac, const double* kxz, hipStream_t s);
-void launch_sp0b(const Geom& g, double* phit, const double* ac, hipStream_t s);
+// The spectrum may be stored in double or float (TS); the velocity
+// systems take cellvalue in double or float (TC) with the field in double;
+// the temperature systems store field, cellvalue, rhoCp and thc in TT.
+// The elimination itself is FP64 arithmetic in every case.
+template <typename TS>
+int launch_spn(const Geom& g, TS* phit, const double* ac, const double* kxz, hipStream_t s);
+template <typename TS>
+void launch_sp0b(const Geom& g, TS* phit, const double* ac, hipStream_t s);
+template <typename TC>
 int launch_adi_velocity(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
-                        const double* cv, int comp, double beta, double dteta, hipStream_t s);
-int launch_adi_temperature(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
-                           const double* cv, const double* rc, const double* thc, double beta,
-                           double dteta, hipStream_t s);
+                        const TC* cv, int comp, double beta, double dteta, hipStream_t s);
+template <typename TT>
+int launch_adi_temperature(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
+                           const TT* cv, const TT* rc, const TT* thc, double beta, double dteta,
+                           hipStream_t s);
 // reciprocal metric arrays in workspace (recomputed each call, 6(n+2) doubles)
 int make_rmetrics(const Geom& g, const Metrics& m, RMetrics& r, hipStream_t s);
 
 // FFT (vk_fft.hip); return a VK_ status
 int launch_p2s(const Geom& g, const double* fp, double* phit, hipStream_t s);
+// R = double or float: the transform's arithmetic and the spectrum's and
+// output's storage; the divergence and residual samples are FP64.
+template <typename R>
 int launch_p2s_div(const Geom& g, const Metrics& m, const double* u, const double* v,
-                   const double* w, double dt, double* phit, hipStream_t s);
-int launch_s2p_inplace(const Geom& g, double* phit, double* fp, hipStream_t s);
+                   const double* w, double dt, R* phit, hipStream_t s);
+// spectrum of div(u_temp)/dt - L phi for a float phi with filled halos
+int launch_p2s_resid(const Geom& g, const Metrics& m, const RMetrics& r, const double* u,
+                     const double* v, const double* w, double dt, const float* phi,
+                     float* phit, hipStream_t s);
+template <typename R>
+int launch_s2p_inplace(const Geom& g, R* phit, R* fp, hipStream_t s, const char* zname,
+                       const char* xname);
 
 // projection (vk_mkp.hip)
 struct MkpOut {
@@ -45,6 +65,16 @@ struct MkpOut {
 int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const double* v_temp,
                const double* w_temp, double dt, double xl, double zl, const MkpOut& o,
                hipStream_t s);
+
+// the mixed-precision projection: FP64 velocity, p and divu, float u_ta
+struct MkpOutMixed {
+  double* p;
+  float *u_ta, *v_ta, *w_ta;
+  double *divu, *u, *v, *w;
+};
+int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
+                     const double* v_temp, const double* w_temp, double dt, double xl, double zl,
+                     const MkpOutMixed& o, hipStream_t s);
 
 // explicit phase of the fused step (vk_step.hip)
 }  // namespace vk

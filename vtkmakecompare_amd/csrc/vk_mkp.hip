@@ -96,7 +96,7 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
   rc = launch_spn(g, phit, ac, kxz, s);
   if (rc) return rc;
   launch_sp0b(g, phit, ac, s);
-  rc = launch_s2p_inplace(g, phit, phi, s);
+  rc = launch_s2p_inplace<double>(g, phit, phi, s, "cfft_z_inv", "irfft_x");
   if (rc) return rc;
   launch_halo_neumann(g, phi, s);
   RMetrics r;
@@ -105,6 +105,93 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
   dim3 blk(64, 4, 1), grd((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
   ProfScope ps("project", s);
   k_project<<<grd, blk, 0, s>>>(g, r, phi, u_temp, v_temp, w_temp, dt, o);
+  return VK_OK;
+}
+
+// Mixed-precision projection: phi = phi1 + delta from two float Poisson
+// solutions, summed in FP64 at every point read; the corrected velocity,
+// p and divu are FP64, u_ta is stored in float.  Same index ranges and
+// formulas as k_project (mkp.f:141-217); the u_ep2 diagnostic is not
+// produced.
+__global__ void k_project_mixed(Geom g, RMetrics r, const float* phi1, const float* dphi,
+                                const double* ut, const double* vt, const double* wt, double dt,
+                                MkpOutMixed o) {
+  int bx, by, bz;
+  xcd_block(bx, by, bz);
+  int i = bx * blockDim.x + threadIdx.x;
+  int j = by * blockDim.y + threadIdx.y;
+  int k = bz;
+  if (i > g.nx || j > g.ny) return;
+  const long long c = g.at(i, j, k);
+  const long long sy = g.sy, sz = g.sz;
+  auto phi = [&](long long q) { return (double)phi1[q] + (double)dphi[q]; };
+  const double ph = phi(c);
+  const double gx = (phi(c + 1) - ph) * r.rdx12[i];
+  const double gy = (phi(c + sy) - ph) * r.rdy12[j];
+  const double gz = (phi(c + sz) - ph) * r.rdz12[k];
+  const double un = ut[c] - dt * gx, vn = vt[c] - dt * gy, wn = wt[c] - dt * gz;
+  o.u[c] = un;
+  o.v[c] = vn;
+  o.w[c] = wn;
+  if (i >= 1 && j >= 1 && k >= 1) {
+    o.p[c] = ph;
+    const double um = ut[c - 1] - dt * ((ph - phi(c - 1)) * r.rdx12[i - 1]);
+    const double vm = vt[c - sy] - dt * ((ph - phi(c - sy)) * r.rdy12[j - 1]);
+    const double wm = wt[c - sz] - dt * ((ph - phi(c - sz)) * r.rdz12[k - 1]);
+    o.divu[c] = (un - um) * r.rdx[i] + (vn - vm) * r.rdy[j] + (wn - wm) * r.rdz[k];
+  }
+  o.u_ta[c] = (float)(dt * gx);
+  o.v_ta[c] = (float)(dt * gy);
+  o.w_ta[c] = (float)(dt * gz);
+}
+
+// The projection of the mixed-precision step.  Pass 1 solves
+// L phi1 = div(u_temp)/dt with float transforms and spectra (the right-hand
+// side is formed in FP64, every tridiagonal elimination, SP0B's march of
+// the zero mode included, runs in FP64 arithmetic).  Pass 2 is one
+// iterative-refinement step: the residual div(u_temp)/dt - L phi1 is
+// formed in FP64 from the FP64 velocity and the stored phi1, and solved
+// the same way for delta.  The projection uses phi1 + delta in FP64.
+int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
+                     const double* v_temp, const double* w_temp, double dt, double xl, double zl,
+                     const MkpOutMixed& o, hipStream_t s) {
+  const int nh = g.nx / 2;
+  size_t nk = (size_t)(nh + 1) * (g.nz + 1) + 3 * (size_t)g.ny;
+  double* coef = static_cast<double*>(workspace(kWorkMisc, (nk + 8) * sizeof(double)));
+  float* phit = static_cast<float*>(workspace(kWorkSpec, 2 * g.spec_cells() * sizeof(float)));
+  float* phi1 = static_cast<float*>(workspace(kWorkPhi, g.cells() * sizeof(float)));
+  float* dphi = static_cast<float*>(workspace(kWorkPhi2, g.cells() * sizeof(float)));
+  if (!coef || !phit || !phi1 || !dphi) return VK_ERR_HIP;
+  double* kxz = coef;
+  double* ac = coef + (size_t)(nh + 1) * (g.nz + 1);
+  {
+    ProfScope ps("mkp_coef", s);
+    k_mkp_coefficients<<<((int)nk + 255) / 256, 256, 0, s>>>(g, m, xl, zl, kxz, ac);
+  }
+  RMetrics r;
+  int rc = make_rmetrics(g, m, r, s);
+  if (rc) return rc;
+  // pass 1
+  rc = launch_p2s_div<float>(g, m, u_temp, v_temp, w_temp, dt, phit, s);
+  if (rc) return rc;
+  rc = launch_spn<float>(g, phit, ac, kxz, s);
+  if (rc) return rc;
+  launch_sp0b<float>(g, phit, ac, s);
+  rc = launch_s2p_inplace<float>(g, phit, phi1, s, "cfft_z_inv", "irfft_x");
+  if (rc) return rc;
+  launch_halo_neumann<float>(g, phi1, s);
+  // pass 2: refinement on the FP64 residual
+  rc = launch_p2s_resid(g, m, r, u_temp, v_temp, w_temp, dt, phi1, phit, s);
+  if (rc) return rc;
+  rc = launch_spn<float>(g, phit, ac, kxz, s);
+  if (rc) return rc;
+  launch_sp0b<float>(g, phit, ac, s);
+  rc = launch_s2p_inplace<float>(g, phit, dphi, s, "cfft_z_inv2", "irfft_x2");
+  if (rc) return rc;
+  launch_halo_neumann<float>(g, dphi, s);
+  dim3 blk(64, 4, 1), grd((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
+  ProfScope ps("project_mixed", s);
+  k_project_mixed<<<grd, blk, 0, s>>>(g, r, phi1, dphi, u_temp, v_temp, w_temp, dt, o);
   return VK_OK;
 }
 

@@ -39,6 +39,8 @@ enum WorkSlot {
   kWorkMisc,
   kWorkRecip,
   kWorkQueue,
+  kWorkPhi2,
+  kWorkRecipF,
   kWorkSlots
 };
 // `grown` (optional) reports whether this call allocated the slot anew.

@@ -32,16 +32,18 @@ __device__ inline bool map_y(int j, int ny, int mode, int& jm, double& s) {
   return true;
 }
 
-__device__ inline void halo_cell(double* f, const Geom& g, int i, int j, int k, int mode) {
+template <typename T>
+__device__ inline void halo_cell(T* f, const Geom& g, int i, int j, int k, int mode) {
   int im = (i == 0) ? g.nx : (i == g.nx + 1) ? 1 : i;
   int km = (k == 0) ? g.nz : (k == g.nz + 1) ? 1 : k;
   int jm;
   double s;
   bool nz = map_y(j, g.ny, mode, jm, s);
-  f[g.at(i, j, k)] = nz ? s * f[g.at(im, jm, km)] : 0.0;
+  f[g.at(i, j, k)] = nz ? static_cast<T>(s * f[g.at(im, jm, km)]) : static_cast<T>(0);
 }
 
-__global__ void k_set_bc(Geom g, double* u, double* v, double* w, double* p, double* t) {
+template <typename TT>
+__global__ void k_set_bc(Geom g, double* u, double* v, double* w, double* p, TT* t) {
   long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   if (!halo_decode(g, q, true, i, j, k)) return;
@@ -56,25 +58,34 @@ __global__ void k_set_bc(Geom g, double* u, double* v, double* w, double* p, dou
 
 // full halo of one field with a given wall rule (PHIBC uses Neumann and
 // agrees with mkp.f:293-320 on every cell PHIBC defines)
-__global__ void k_halo_one(Geom g, double* f, int mode) {
+template <typename T>
+__global__ void k_halo_one(Geom g, T* f, int mode) {
   long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   if (!halo_decode(g, q, false, i, j, k)) return;
   halo_cell(f, g, i, j, k, mode);
 }
 
-void launch_set_bc(const Geom& g, double* u, double* v, double* w, double* p, double* t,
+template <typename TT>
+void launch_set_bc(const Geom& g, double* u, double* v, double* w, double* p, TT* t,
                    hipStream_t s) {
   long long n = halo_count(g, true);
   ProfScope ps("set_bc", s);
-  k_set_bc<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, u, v, w, p, t);
+  k_set_bc<TT><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, u, v, w, p, t);
 }
+template void launch_set_bc<double>(const Geom&, double*, double*, double*, double*, double*,
+                                    hipStream_t);
+template void launch_set_bc<float>(const Geom&, double*, double*, double*, double*, float*,
+                                   hipStream_t);
 
-void launch_halo_neumann(const Geom& g, double* f, hipStream_t s) {
+template <typename T>
+void launch_halo_neumann(const Geom& g, T* f, hipStream_t s) {
   long long n = halo_count(g, false);
   ProfScope ps("halo_neumann", s);
-  k_halo_one<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, f, kNeumann);
+  k_halo_one<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, f, kNeumann);
 }
+template void launch_halo_neumann<double>(const Geom&, double*, hipStream_t);
+template void launch_halo_neumann<float>(const Geom&, float*, hipStream_t);
 
 // ----------------------------------------------------- explicit terms
 
@@ -123,12 +134,14 @@ __global__ void k_diffusion(Geom g, Metrics m, const double* t, const double* th
 // SET_IMMERSED_BOUNDARY l = 1 branch (nse_dominik.f:1037-1080): face
 // values cv2..4 as averages of neighbouring cv1, copying cv1 at the last
 // face (no periodic wrap there).
-__global__ void k_cv_faces(Geom g, double* cv) {
+template <typename T>
+__global__ void k_cv_faces(Geom g, T* cv) {
   INTERIOR_INDEX;
-  double c = cv[g.cv(i, j, k, 1)];
-  cv[g.cv(i, j, k, 2)] = (i < g.nx) ? (c + cv[g.cv(i + 1, j, k, 1)]) * 0.50 : c;
-  cv[g.cv(i, j, k, 3)] = (j < g.ny) ? (c + cv[g.cv(i, j + 1, k, 1)]) * 0.50 : c;
-  cv[g.cv(i, j, k, 4)] = (k < g.nz) ? (c + cv[g.cv(i, j, k + 1, 1)]) * 0.50 : c;
+  T c = cv[g.cv(i, j, k, 1)];
+  const T h = static_cast<T>(0.50);
+  cv[g.cv(i, j, k, 2)] = (i < g.nx) ? (c + cv[g.cv(i + 1, j, k, 1)]) * h : c;
+  cv[g.cv(i, j, k, 3)] = (j < g.ny) ? (c + cv[g.cv(i, j + 1, k, 1)]) * h : c;
+  cv[g.cv(i, j, k, 4)] = (k < g.nz) ? (c + cv[g.cv(i, j, k + 1, 1)]) * h : c;
 }
 
 // SET_IMMERSED_BOUNDARY every step (nse_dominik.f:1086-1126)
@@ -387,10 +400,13 @@ int make_rmetrics(const Geom& g, const Metrics& m, RMetrics& r, hipStream_t s) {
   return VK_OK;
 }
 
-void launch_cv_faces(const Geom& g, double* cv, hipStream_t s) {
+template <typename T>
+void launch_cv_faces(const Geom& g, T* cv, hipStream_t s) {
   ProfScope ps("cv_faces", s);
-  k_cv_faces<<<interior_grid(g), interior_block(), 0, s>>>(g, cv);
+  k_cv_faces<T><<<interior_grid(g), interior_block(), 0, s>>>(g, cv);
 }
+template void launch_cv_faces<double>(const Geom&, double*, hipStream_t);
+template void launch_cv_faces<float>(const Geom&, float*, hipStream_t);
 
 }  // namespace vk
 

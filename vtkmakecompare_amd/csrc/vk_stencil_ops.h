@@ -160,152 +160,149 @@ __device__ inline double diff_scalar(const F3<double>& T, const F3<double>& K,
 }
 
 // ---------------------------------------------------------------------
-// Reciprocal forms used by the fused time step.  Same expressions with
+// Reciprocal forms used by the fused time steps.  Same expressions with
 // every division by a metric replaced by a multiplication with its
 // precomputed reciprocal, and the harmonic conductance evaluated with one
 // division; results agree with the forms above to rounding (about 1e-16
-// relative per term).
+// relative per term in FP64).  Templates over the arithmetic type R, the
+// field views (F3 or F3as) and the metric sets: R = double with Metrics /
+// RMetrics for the FP64 step, R = float with float metric copies for the
+// mixed-precision step.
+
+// a halo field of T read as R
+template <typename T, typename R>
+struct F3as {
+  const T* __restrict__ p;
+  Geom g;
+  __device__ R operator()(int i, int j, int k) const { return static_cast<R>(p[g.at(i, j, k)]); }
+};
 
 // SET_CONVECT u (nse_dominik.f:774-785)
-__device__ inline double conv_u_r(const F3<double>& U, const F3<double>& V, const F3<double>& W,
-                                  const Metrics& m, const RMetrics& r, int i, int j, int k,
-                                  double dt) {
-  double c = U(i, j, k);
-  double up = U(i + 1, j, k), um = U(i - 1, j, k);
-  double u1 = ((c + up) * (c + up) - (c + um) * (c + um)) * r.rdx12[i];
-  double u2 = ((V(i + 1, j, k) * m.dx[i + 1] + V(i, j, k) * m.dx[i]) * (U(i, j + 1, k) + c) -
-               (V(i + 1, j - 1, k) * m.dx[i + 1] + V(i, j - 1, k) * m.dx[i]) *
-                   (c + U(i, j - 1, k))) *
-              (r.rdx12[i] * r.rdy[j]);
-  double u3 = ((W(i + 1, j, k) * m.dx[i + 1] + W(i, j, k) * m.dx[i]) * (U(i, j, k + 1) + c) -
-               (W(i + 1, j, k - 1) * m.dx[i + 1] + W(i, j, k - 1) * m.dx[i]) *
-                   (c + U(i, j, k - 1))) *
-              (r.rdx12[i] * r.rdz[k]);
-  return 0.25 * (-u1 - u2 - u3) * dt + dt;
+template <typename R, typename VU, typename VV, typename VW, typename MT, typename RT>
+__device__ inline R conv_u_r(const VU& U, const VV& V, const VW& W, const MT& m, const RT& r,
+                             int i, int j, int k, R dt) {
+  R c = U(i, j, k);
+  R up = U(i + 1, j, k), um = U(i - 1, j, k);
+  R u1 = ((c + up) * (c + up) - (c + um) * (c + um)) * r.rdx12[i];
+  R u2 = ((V(i + 1, j, k) * m.dx[i + 1] + V(i, j, k) * m.dx[i]) * (U(i, j + 1, k) + c) -
+          (V(i + 1, j - 1, k) * m.dx[i + 1] + V(i, j - 1, k) * m.dx[i]) * (c + U(i, j - 1, k))) *
+         (r.rdx12[i] * r.rdy[j]);
+  R u3 = ((W(i + 1, j, k) * m.dx[i + 1] + W(i, j, k) * m.dx[i]) * (U(i, j, k + 1) + c) -
+          (W(i + 1, j, k - 1) * m.dx[i + 1] + W(i, j, k - 1) * m.dx[i]) * (c + U(i, j, k - 1))) *
+         (r.rdx12[i] * r.rdz[k]);
+  return R(0.25) * (-u1 - u2 - u3) * dt + dt;
 }
 
 // SET_CONVECT v (nse_dominik.f:794-805)
-__device__ inline double conv_v_r(const F3<double>& U, const F3<double>& V, const F3<double>& W,
-                                  const Metrics& m, const RMetrics& r, int i, int j, int k,
-                                  double dt) {
-  double c = V(i, j, k);
-  double v1 = ((U(i, j + 1, k) * m.dy[j + 1] + U(i, j, k) * m.dy[j]) * (V(i + 1, j, k) + c) -
-               (U(i - 1, j + 1, k) * m.dy[j + 1] + U(i - 1, j, k) * m.dy[j]) *
-                   (c + V(i - 1, j, k))) *
-              (r.rdy12[j] * r.rdx[i]);
-  double vp = V(i, j + 1, k), vm = V(i, j - 1, k);
-  double v2 = ((c + vp) * (c + vp) - (c + vm) * (c + vm)) * r.rdy12[j];
-  double v3 = ((W(i, j + 1, k) * m.dy[j + 1] + W(i, j, k) * m.dy[j]) * (V(i, j, k + 1) + c) -
-               (W(i, j + 1, k - 1) * m.dy[j + 1] + W(i, j, k - 1) * m.dy[j]) *
-                   (c + V(i, j, k - 1))) *
-              (r.rdy12[j] * r.rdz[k]);
-  return 0.25 * (-v1 - v2 - v3) * dt;
+template <typename R, typename VU, typename VV, typename VW, typename MT, typename RT>
+__device__ inline R conv_v_r(const VU& U, const VV& V, const VW& W, const MT& m, const RT& r,
+                             int i, int j, int k, R dt) {
+  R c = V(i, j, k);
+  R v1 = ((U(i, j + 1, k) * m.dy[j + 1] + U(i, j, k) * m.dy[j]) * (V(i + 1, j, k) + c) -
+          (U(i - 1, j + 1, k) * m.dy[j + 1] + U(i - 1, j, k) * m.dy[j]) * (c + V(i - 1, j, k))) *
+         (r.rdy12[j] * r.rdx[i]);
+  R vp = V(i, j + 1, k), vm = V(i, j - 1, k);
+  R v2 = ((c + vp) * (c + vp) - (c + vm) * (c + vm)) * r.rdy12[j];
+  R v3 = ((W(i, j + 1, k) * m.dy[j + 1] + W(i, j, k) * m.dy[j]) * (V(i, j, k + 1) + c) -
+          (W(i, j + 1, k - 1) * m.dy[j + 1] + W(i, j, k - 1) * m.dy[j]) * (c + V(i, j, k - 1))) *
+         (r.rdy12[j] * r.rdz[k]);
+  return R(0.25) * (-v1 - v2 - v3) * dt;
 }
 
 // SET_CONVECT w (nse_dominik.f:814-825)
-__device__ inline double conv_w_r(const F3<double>& U, const F3<double>& V, const F3<double>& W,
-                                  const Metrics& m, const RMetrics& r, int i, int j, int k,
-                                  double dt) {
-  double c = W(i, j, k);
-  double w1 = ((U(i, j, k + 1) * m.dz[k + 1] + U(i, j, k) * m.dz[k]) * (W(i + 1, j, k) + c) -
-               (U(i - 1, j, k + 1) * m.dz[k + 1] + U(i - 1, j, k) * m.dz[k]) *
-                   (c + W(i - 1, j, k))) *
-              (r.rdz12[k] * r.rdx[i]);
-  double w2 = ((V(i, j, k + 1) * m.dz[k + 1] + V(i, j, k) * m.dz[k]) * (W(i, j + 1, k) + c) -
-               (V(i, j - 1, k + 1) * m.dz[k + 1] + V(i, j - 1, k) * m.dz[k]) *
-                   (c + W(i, j - 1, k))) *
-              (r.rdz12[k] * r.rdy[j]);
-  double wp = W(i, j, k + 1), wm = W(i, j, k - 1);
-  double w3 = ((c + wp) * (c + wp) - (c + wm) * (c + wm)) * r.rdz12[k];
-  return 0.25 * (-w1 - w2 - w3) * dt;
+template <typename R, typename VU, typename VV, typename VW, typename MT, typename RT>
+__device__ inline R conv_w_r(const VU& U, const VV& V, const VW& W, const MT& m, const RT& r,
+                             int i, int j, int k, R dt) {
+  R c = W(i, j, k);
+  R w1 = ((U(i, j, k + 1) * m.dz[k + 1] + U(i, j, k) * m.dz[k]) * (W(i + 1, j, k) + c) -
+          (U(i - 1, j, k + 1) * m.dz[k + 1] + U(i - 1, j, k) * m.dz[k]) * (c + W(i - 1, j, k))) *
+         (r.rdz12[k] * r.rdx[i]);
+  R w2 = ((V(i, j, k + 1) * m.dz[k + 1] + V(i, j, k) * m.dz[k]) * (W(i, j + 1, k) + c) -
+          (V(i, j - 1, k + 1) * m.dz[k + 1] + V(i, j - 1, k) * m.dz[k]) * (c + W(i, j - 1, k))) *
+         (r.rdz12[k] * r.rdy[j]);
+  R wp = W(i, j, k + 1), wm = W(i, j, k - 1);
+  R w3 = ((c + wp) * (c + wp) - (c + wm) * (c + wm)) * r.rdz12[k];
+  return R(0.25) * (-w1 - w2 - w3) * dt;
 }
 
 // SET_CONVECTION_FOR_SCALAR (nse_dominik.f:852-879), rrc = 1/rhoCp(i,j,k)
-__device__ inline double conv_scalar_r(const F3<double>& U, const F3<double>& V,
-                                       const F3<double>& W, const F3<double>& T,
-                                       const F3<double>& RC, const Metrics& m,
-                                       const RMetrics& r, int i, int j, int k, double dt,
-                                       double rrc) {
-  double rc = RC(i, j, k), tc = T(i, j, k);
-  double rt = rc * tc;
-  double ut = (-((RC(i - 1, j, k) * T(i - 1, j, k) * m.dx[i] + rt * m.dx[i - 1]) *
-                 U(i - 1, j, k)) *
-                   (0.5 * r.rdx12[i - 1]) +
-               ((rt * m.dx[i + 1] + RC(i + 1, j, k) * T(i + 1, j, k) * m.dx[i]) * U(i, j, k)) *
-                   (0.5 * r.rdx12[i])) *
-              r.rdx[i];
-  double vt = (-((RC(i, j - 1, k) * T(i, j - 1, k) * m.dy[j] + rt * m.dy[j - 1]) *
-                 V(i, j - 1, k)) *
-                   (0.5 * r.rdy12[j - 1]) +
-               ((rt * m.dy[j + 1] + RC(i, j + 1, k) * T(i, j + 1, k) * m.dy[j]) * V(i, j, k)) *
-                   (0.5 * r.rdy12[j])) *
-              r.rdy[j];
-  double wt = (-((RC(i, j, k - 1) * T(i, j, k - 1) * m.dz[k] + rt * m.dz[k - 1]) *
-                 W(i, j, k - 1)) *
-                   (0.5 * r.rdz12[k - 1]) +
-               ((rt * m.dz[k + 1] + RC(i, j, k + 1) * T(i, j, k + 1) * m.dz[k]) * W(i, j, k)) *
-                   (0.5 * r.rdz12[k])) *
-              r.rdz[k];
+template <typename R, typename VU, typename VV, typename VW, typename VT, typename VR,
+          typename MT, typename RT>
+__device__ inline R conv_scalar_r(const VU& U, const VV& V, const VW& W, const VT& T,
+                                  const VR& RC, const MT& m, const RT& r, int i, int j, int k,
+                                  R dt, R rrc) {
+  R rc = RC(i, j, k), tc = T(i, j, k);
+  R rt = rc * tc;
+  R ut = (-((RC(i - 1, j, k) * T(i - 1, j, k) * m.dx[i] + rt * m.dx[i - 1]) * U(i - 1, j, k)) *
+              (R(0.5) * r.rdx12[i - 1]) +
+          ((rt * m.dx[i + 1] + RC(i + 1, j, k) * T(i + 1, j, k) * m.dx[i]) * U(i, j, k)) *
+              (R(0.5) * r.rdx12[i])) *
+         r.rdx[i];
+  R vt = (-((RC(i, j - 1, k) * T(i, j - 1, k) * m.dy[j] + rt * m.dy[j - 1]) * V(i, j - 1, k)) *
+              (R(0.5) * r.rdy12[j - 1]) +
+          ((rt * m.dy[j + 1] + RC(i, j + 1, k) * T(i, j + 1, k) * m.dy[j]) * V(i, j, k)) *
+              (R(0.5) * r.rdy12[j])) *
+         r.rdy[j];
+  R wt = (-((RC(i, j, k - 1) * T(i, j, k - 1) * m.dz[k] + rt * m.dz[k - 1]) * W(i, j, k - 1)) *
+              (R(0.5) * r.rdz12[k - 1]) +
+          ((rt * m.dz[k + 1] + RC(i, j, k + 1) * T(i, j, k + 1) * m.dz[k]) * W(i, j, k)) *
+              (R(0.5) * r.rdz12[k])) *
+         r.rdz[k];
   return (-ut - vt - wt) * dt * rrc + dt;
 }
 
 // SET_CONDUCTION (nse_dominik.f:903-939); c0 = DT/RE/2
-__device__ inline double cond_u_r(const F3<double>& U, const RMetrics& r, int i, int j, int k,
-                                  double c0) {
-  double c = U(i, j, k);
-  double u1 = ((U(i + 1, j, k) - c) * r.rdx[i + 1] - (c - U(i - 1, j, k)) * r.rdx[i]) * r.rdx12[i];
-  double u2 =
-      ((U(i, j + 1, k) - c) * r.rdy12[j] - (c - U(i, j - 1, k)) * r.rdy12[j - 1]) * r.rdy[j];
-  double u3 =
-      ((U(i, j, k + 1) - c) * r.rdz12[k] - (c - U(i, j, k - 1)) * r.rdz12[k - 1]) * r.rdz[k];
+template <typename R, typename VU, typename RT>
+__device__ inline R cond_u_r(const VU& U, const RT& r, int i, int j, int k, R c0) {
+  R c = U(i, j, k);
+  R u1 = ((U(i + 1, j, k) - c) * r.rdx[i + 1] - (c - U(i - 1, j, k)) * r.rdx[i]) * r.rdx12[i];
+  R u2 = ((U(i, j + 1, k) - c) * r.rdy12[j] - (c - U(i, j - 1, k)) * r.rdy12[j - 1]) * r.rdy[j];
+  R u3 = ((U(i, j, k + 1) - c) * r.rdz12[k] - (c - U(i, j, k - 1)) * r.rdz12[k - 1]) * r.rdz[k];
   return c0 * (u1 + u2 + u3);
 }
-__device__ inline double cond_v_r(const F3<double>& V, const RMetrics& r, int i, int j, int k,
-                                  double c0) {
-  double c = V(i, j, k);
-  double v1 =
-      ((V(i + 1, j, k) - c) * r.rdx12[i] - (c - V(i - 1, j, k)) * r.rdx12[i - 1]) * r.rdx[i];
-  double v2 = ((V(i, j + 1, k) - c) * r.rdy[j + 1] - (c - V(i, j - 1, k)) * r.rdy[j]) * r.rdy12[j];
-  double v3 =
-      ((V(i, j, k + 1) - c) * r.rdz12[k] - (c - V(i, j, k - 1)) * r.rdz12[k - 1]) * r.rdz[k];
+template <typename R, typename VV, typename RT>
+__device__ inline R cond_v_r(const VV& V, const RT& r, int i, int j, int k, R c0) {
+  R c = V(i, j, k);
+  R v1 = ((V(i + 1, j, k) - c) * r.rdx12[i] - (c - V(i - 1, j, k)) * r.rdx12[i - 1]) * r.rdx[i];
+  R v2 = ((V(i, j + 1, k) - c) * r.rdy[j + 1] - (c - V(i, j - 1, k)) * r.rdy[j]) * r.rdy12[j];
+  R v3 = ((V(i, j, k + 1) - c) * r.rdz12[k] - (c - V(i, j, k - 1)) * r.rdz12[k - 1]) * r.rdz[k];
   return c0 * (v1 + v2 + v3);
 }
-__device__ inline double cond_w_r(const F3<double>& W, const RMetrics& r, int i, int j, int k,
-                                  double c0) {
-  double c = W(i, j, k);
-  double w1 =
-      ((W(i + 1, j, k) - c) * r.rdx12[i] - (c - W(i - 1, j, k)) * r.rdx12[i - 1]) * r.rdx[i];
-  double w2 =
-      ((W(i, j + 1, k) - c) * r.rdy12[j] - (c - W(i, j - 1, k)) * r.rdy12[j - 1]) * r.rdy[j];
-  double w3 = ((W(i, j, k + 1) - c) * r.rdz[k + 1] - (c - W(i, j, k - 1)) * r.rdz[k]) * r.rdz12[k];
+template <typename R, typename VW, typename RT>
+__device__ inline R cond_w_r(const VW& W, const RT& r, int i, int j, int k, R c0) {
+  R c = W(i, j, k);
+  R w1 = ((W(i + 1, j, k) - c) * r.rdx12[i] - (c - W(i - 1, j, k)) * r.rdx12[i - 1]) * r.rdx[i];
+  R w2 = ((W(i, j + 1, k) - c) * r.rdy12[j] - (c - W(i, j - 1, k)) * r.rdy12[j - 1]) * r.rdy[j];
+  R w3 = ((W(i, j, k + 1) - c) * r.rdz[k + 1] - (c - W(i, j, k - 1)) * r.rdz[k]) * r.rdz12[k];
   return c0 * (w1 + w2 + w3);
 }
 
 // (d1+d2)/(d1/k1+d2/k2) with one division
-__device__ inline double hmean_r(double d1, double k1, double d2, double k2) {
+template <typename R>
+__device__ inline R hmean_r(R d1, R k1, R d2, R k2) {
   return (d1 + d2) * k1 * k2 / (d1 * k2 + d2 * k1);
 }
 
 // SET_DIFFUSION_FOR_SCALAR (nse_dominik.f:969-999); cd = DT/(2 RE SC)
-__device__ inline double diff_scalar_r(const F3<double>& T, const F3<double>& K,
-                                       const Metrics& m, const RMetrics& r, int i, int j, int k,
-                                       double cd, double rrc) {
-  double tc = T(i, j, k), kc = K(i, j, k);
-  double ut1 = (hmean_r(m.dx[i + 1], K(i + 1, j, k), m.dx[i], kc) * (T(i + 1, j, k) - tc) *
-                    r.rdx12[i] -
-                hmean_r(m.dx[i - 1], K(i - 1, j, k), m.dx[i], kc) * (tc - T(i - 1, j, k)) *
-                    r.rdx12[i - 1]) *
-               r.rdx[i];
-  double ut2 = (hmean_r(m.dy[j + 1], K(i, j + 1, k), m.dy[j], kc) * (T(i, j + 1, k) - tc) *
-                    r.rdy12[j] -
-                hmean_r(m.dy[j - 1], K(i, j - 1, k), m.dy[j], kc) * (tc - T(i, j - 1, k)) *
-                    r.rdy12[j - 1]) *
-               r.rdy[j];
-  double ut3 = (hmean_r(m.dz[k + 1], K(i, j, k + 1), m.dz[k], kc) * (T(i, j, k + 1) - tc) *
-                    r.rdz12[k] -
-                hmean_r(m.dz[k - 1], K(i, j, k - 1), m.dz[k], kc) * (tc - T(i, j, k - 1)) *
-                    r.rdz12[k - 1]) *
-               r.rdz[k];
+template <typename R, typename VT, typename VK, typename MT, typename RT>
+__device__ inline R diff_scalar_r(const VT& T, const VK& K, const MT& m, const RT& r, int i,
+                                  int j, int k, R cd, R rrc) {
+  R tc = T(i, j, k), kc = K(i, j, k);
+  R ut1 = (hmean_r<R>(m.dx[i + 1], K(i + 1, j, k), m.dx[i], kc) * (T(i + 1, j, k) - tc) *
+               r.rdx12[i] -
+           hmean_r<R>(m.dx[i - 1], K(i - 1, j, k), m.dx[i], kc) * (tc - T(i - 1, j, k)) *
+               r.rdx12[i - 1]) *
+          r.rdx[i];
+  R ut2 = (hmean_r<R>(m.dy[j + 1], K(i, j + 1, k), m.dy[j], kc) * (T(i, j + 1, k) - tc) *
+               r.rdy12[j] -
+           hmean_r<R>(m.dy[j - 1], K(i, j - 1, k), m.dy[j], kc) * (tc - T(i, j - 1, k)) *
+               r.rdy12[j - 1]) *
+          r.rdy[j];
+  R ut3 = (hmean_r<R>(m.dz[k + 1], K(i, j, k + 1), m.dz[k], kc) * (T(i, j, k + 1) - tc) *
+               r.rdz12[k] -
+           hmean_r<R>(m.dz[k - 1], K(i, j, k - 1), m.dz[k], kc) * (tc - T(i, j, k - 1)) *
+               r.rdz12[k - 1]) *
+          r.rdz[k];
   return (ut1 + ut2 + ut3) * cd * rrc;
 }
 

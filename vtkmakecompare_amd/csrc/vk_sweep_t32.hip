@@ -1,0 +1,10 @@
+// vk_sweep_t32.hip -- instantiation of launch_adi_temperature<float> (vk_linesys.h).
+#include "vk_linesys.h"
+
+namespace vk {
+
+template int launch_adi_temperature<float>(const Geom&, const Metrics&, const RMetrics&, float*,
+                                           const float*, const float*, const float*, double,
+                                           double, hipStream_t);
+
+}  // namespace vk

@@ -1,9 +1,8 @@
-// vk_tridiag.hip -- the implicit line solves: solve_u_temp and solve_t
-// (ADI sweeps y -> z -> x for u, v, w; y -> x -> z for T), SPN and SP0B of
-// the Poisson solve.  Each reference line system is a row functor for the
-// partition solver of vk_partition.h; the coefficient expressions keep the
-// reference's operation order.
-#include <cstring>
+// vk_tridiag.hip -- entry points of the implicit line solves: solve_u_temp
+// and solve_t (ADI sweeps y -> z -> x for u, v, w; y -> x -> z for T), SPN
+// and SP0B of the Poisson solve; the line solver's shared queue and
+// reference-line tables; SP0B's march.  The line systems and the sweep
+// launchers are in vk_linesys.h, instantiated in vk_sweep_*.hip.
 #include <map>
 #include <string>
 
@@ -12,361 +11,6 @@
 #include "vk_runtime.h"
 
 namespace vk {
-
-// ------------------------------------------------------ y line systems
-//
-// Row coefficients use the reciprocal metric arrays: the reference's
-// -beta/alpha/dy12(j-1)/dy(j) becomes -(beta/alpha) * (1/dy12(j-1) *
-// 1/dy(j)), and the diagonal 1 + beta/alpha*(...) is written 1 - a - c,
-// the same quantity.  beta/alpha is exactly beta where cellvalue is 0.
-// Each system computes a row from its per-point inputs with coef(); the
-// reference row (cellvalue 0, rhoCp = thc = 1) is coef() on those inputs,
-// which is what makes the recorded-multiplier path of vk_partition.h
-// bit-identical to the general one.
-
-__device__ inline double beta_over_alpha(double beta, double dteta, double cv) {
-  return cv == 0.0 ? beta : beta / (dteta * cv + 1.0);
-}
-
-// key of a reference-line table: system kind, size, segmenting, the
-// scalars and the caller's metric array the coefficients come from
-struct RefKey {
-  int kind, n, S, extra;
-  double beta, dteta;
-  const void* metric;
-};
-
-// harmonic face conductance (d1+d2)/(d1/k1+d2/k2) with one division
-__device__ inline double hm(double d1, double k1, double d2, double k2) {
-  return (d1 + d2) * k1 * k2 / (d1 * k2 + d2 * k1);
-}
-
-__device__ inline int clampi(int q, int lo, int hi) { return q < lo ? lo : (q > hi ? hi : q); }
-
-// `v` where `keep`, else `other`, by bit masks: a select between a loaded
-// value and a constant is turned into a branch around the load (which then
-// waits for it before the join), so a thread's row loads would be issued
-// one at a time
-__device__ inline double keep_or(double v, bool keep, double other) {
-  const unsigned long long m = 0ull - (unsigned long long)keep;
-  return __longlong_as_double((__double_as_longlong(v) & (long long)m) |
-                              (__double_as_longlong(other) & (long long)~m));
-}
-
-// per-row coefficient inputs: cellvalue alone, or cellvalue with the
-// conductivity and heat capacity of the temperature systems
-struct PtCv {
-  double cv;
-};
-struct PtT {
-  double cv, th, rc;
-};
-struct PtNone {};
-
-// implicit_uy (nse_dominik.f:1505-1577) on line (i,k): rows j = 0..ny+1,
-// ghost rows x0 + x1 = 0 and x_ny + x_{ny+1} = 0 (wall reflection); all
-// rows are written back as parallel_uy does (nse_dominik.f:2907-2909).
-struct SysYFace {
-  using V = double;
-  using Pt = PtCv;
-  static constexpr bool kCyclic = false, kHasRef = true, kNb = false;
-  Geom g;
-  double* f;
-  const double* cv;
-  RMetrics r;
-  double beta, dteta;
-  int ncv;
-  const void* mkey;
-  __device__ int n() const { return g.ny + 2; }
-  __device__ bool valid(int la, int lb) const { return la < g.nx; }
-  // ghost rows 0 and ny+1 are x0 + x1 = 0 and x_ny + x_{ny+1} = 0; the
-  // interior expression is evaluated for every row (at a clamped j) so that
-  // no metric load sits behind a branch
-  __device__ void coef(int q, double cvv, double& a, double& b, double& c) const {
-    const int j = clampi(q, 1, g.ny);
-    const double rho = beta_over_alpha(beta, dteta, cvv);
-    const double ai = -(rho * (r.rdy12[j - 1] * r.rdy[j]));
-    const double ci = -(rho * (r.rdy12[j] * r.rdy[j]));
-    const double bi = 1.0 - ai - ci;
-    const bool in = q >= 1 && q <= g.ny;
-    a = keep_or(ai, in, q == 0 ? 0.0 : 1.0);
-    b = keep_or(bi, in, 1.0);
-    c = keep_or(ci, in, q == 0 ? 1.0 : 0.0);
-  }
-  __device__ double rhs(int la, int lb, int q) const {
-    double v = f[g.at32(la + 1, clampi(q, 1, g.ny), lb + 1)];
-    return keep_or(v, q >= 1 && q <= g.ny, 0.0);
-  }
-  __device__ Pt pt(int la, int lb, int q) const {
-    double v = cv[g.cv32(la + 1, clampi(q, 1, g.ny), lb + 1, ncv)];
-    return Pt{keep_or(v, q >= 1 && q <= g.ny, 0.0)};
-  }
-  __device__ bool pt_ref(const Pt& p) const { return p.cv == 0.0; }
-  __device__ void coef(int, int, int q, const Pt& p, const Pt&, const Pt&, double& a, double& b,
-                       double& c) const {
-    coef(q, p.cv, a, b, c);
-  }
-  __device__ void coef_ref(int q, double& a, double& b, double& c) const { coef(q, 0.0, a, b, c); }
-  RefKey ref_key(int nn, int S) const { return RefKey{1, nn, S, ncv, beta, dteta, mkey}; }
-  __device__ void store(int la, int lb, int q, double x) const {
-    f[g.at32(la + 1, q, lb + 1)] = x;
-  }
-};
-
-// implicit_vy (nse_dominik.f:1583-1648): rows j = 0..ny with x0 = x_ny = 0
-struct SysYV {
-  using V = double;
-  using Pt = PtCv;
-  static constexpr bool kCyclic = false, kHasRef = true, kNb = false;
-  Geom g;
-  double* f;
-  const double* cv;
-  RMetrics r;
-  double beta, dteta;
-  const void* mkey;
-  __device__ int n() const { return g.ny + 1; }
-  __device__ bool valid(int la, int lb) const { return la < g.nx; }
-  __device__ void coef(int q, double cvv, double& a, double& b, double& c) const {
-    const int j = clampi(q, 1, g.ny - 1);
-    const double rho = beta_over_alpha(beta, dteta, cvv);
-    const double ai = -(rho * (r.rdy[j] * r.rdy12[j]));
-    const double ci = -(rho * (r.rdy[j + 1] * r.rdy12[j]));
-    const double bi = 1.0 - ai - ci;
-    const bool in = q >= 1 && q <= g.ny - 1;
-    a = keep_or(ai, in, 0.0);
-    b = keep_or(bi, in, 1.0);
-    c = keep_or(ci, in, 0.0);
-  }
-  __device__ double rhs(int la, int lb, int q) const {
-    double v = f[g.at32(la + 1, clampi(q, 1, g.ny - 1), lb + 1)];
-    return keep_or(v, q >= 1 && q <= g.ny - 1, 0.0);
-  }
-  __device__ Pt pt(int la, int lb, int q) const {
-    double v = cv[g.cv32(la + 1, clampi(q, 1, g.ny - 1), lb + 1, 3)];
-    return Pt{keep_or(v, q >= 1 && q <= g.ny - 1, 0.0)};
-  }
-  __device__ bool pt_ref(const Pt& p) const { return p.cv == 0.0; }
-  __device__ void coef(int, int, int q, const Pt& p, const Pt&, const Pt&, double& a, double& b,
-                       double& c) const {
-    coef(q, p.cv, a, b, c);
-  }
-  __device__ void coef_ref(int q, double& a, double& b, double& c) const { coef(q, 0.0, a, b, c); }
-  RefKey ref_key(int nn, int S) const { return RefKey{2, nn, S, 3, beta, dteta, mkey}; }
-  __device__ void store(int la, int lb, int q, double x) const {
-    f[g.at32(la + 1, q, lb + 1)] = x;
-  }
-};
-
-// implicit_Ty (nse_dominik.f:1784-1870) with lly(0) = lly(1) and
-// lly(ny+1) = lly(ny); only rows 1..ny are written back (parallel_Ty).
-// The inputs of ghost rows 0 and ny+1 carry the wall cell's conductivity,
-// which is the neighbour value rows 1 and ny use.
-struct SysYT {
-  using V = double;
-  using Pt = PtT;
-  static constexpr bool kCyclic = false, kHasRef = true, kNb = true;
-  Geom g;
-  double* f;
-  const double *cv, *rc, *thc;
-  Metrics m;
-  RMetrics r;
-  double beta, dteta;
-  __device__ int n() const { return g.ny + 2; }
-  __device__ bool valid(int la, int lb) const { return la < g.nx; }
-  __device__ void coef(int q, double cvv, double l0, double lm, double lp, double rcv, double& a,
-                       double& b, double& c) const {
-    const int j = clampi(q, 1, g.ny);
-    const double rho = beta_over_alpha(beta, dteta, cvv);
-    const double rr = 1.0 / rcv;
-    const double h1 = hm(m.dy[j], l0, m.dy[j - 1], lm);
-    const double h2 = hm(m.dy[j + 1], lp, m.dy[j], l0);
-    const double ai = -(rho * h1 * (r.rdy12[j - 1] * r.rdy[j]) * rr);
-    const double ci = -(rho * h2 * (r.rdy12[j] * r.rdy[j]) * rr);
-    const double bi = 1.0 - ai - ci;
-    const bool in = q >= 1 && q <= g.ny;
-    a = keep_or(ai, in, q == 0 ? 0.0 : 1.0);
-    b = keep_or(bi, in, 1.0);
-    c = keep_or(ci, in, q == 0 ? 1.0 : 0.0);
-  }
-  __device__ double rhs(int la, int lb, int q) const {
-    double v = f[g.at32(la + 1, clampi(q, 1, g.ny), lb + 1)];
-    return keep_or(v, q >= 1 && q <= g.ny, 0.0);
-  }
-  __device__ Pt pt(int la, int lb, int q) const {
-    const int j = clampi(q, 1, g.ny), i = la + 1, k = lb + 1;
-    const unsigned o = g.at32(i, j, k);
-    const bool in = q >= 1 && q <= g.ny;
-    double c = cv[g.cv32(i, j, k, 1)], t = thc[o], h = rc[o];
-    return Pt{keep_or(c, in, 0.0), t, keep_or(h, in, 1.0)};
-  }
-  __device__ bool pt_ref(const Pt& p) const {
-    return (p.cv == 0.0) & (p.th == 1.0) & (p.rc == 1.0);
-  }
-  __device__ void coef(int, int, int q, const Pt& p, const Pt& pm, const Pt& pp, double& a,
-                       double& b, double& c) const {
-    coef(q, p.cv, p.th, pm.th, pp.th, p.rc, a, b, c);
-  }
-  __device__ void coef_ref(int q, double& a, double& b, double& c) const {
-    coef(q, 0.0, 1.0, 1.0, 1.0, 1.0, a, b, c);
-  }
-  RefKey ref_key(int nn, int S) const { return RefKey{3, nn, S, 1, beta, dteta, m.dy}; }
-  __device__ void store(int la, int lb, int q, double x) const {
-    if (q >= 1 && q <= g.ny) f[g.at32(la + 1, q, lb + 1)] = x;
-  }
-};
-
-// --------------------------------------------------- periodic systems
-
-// A periodic line along axis `dir` (0 = x, 2 = z).  Lines are indexed
-// (la, lb): for z lines (i, j) = (la+1, lb+1); for x lines (j, k) =
-// (la+1, lb+1).  Row r is grid index q = r + 1.
-struct LineAxis {
-  Geom g;
-  int dir;
-  __device__ unsigned at(int la, int lb, int q) const {
-    return dir == 2 ? g.at32(la + 1, lb + 1, q) : g.at32(q, la + 1, lb + 1);
-  }
-  __device__ unsigned cvat(int la, int lb, int q, int n) const {
-    return dir == 2 ? g.cv32(la + 1, lb + 1, q, n) : g.cv32(q, la + 1, lb + 1, n);
-  }
-  __device__ int len() const { return dir == 2 ? g.nz : g.nx; }
-};
-
-// implicit_solver + cyclic (periodic_implicit.f:1-93) on rows q = 1..n;
-// the ghost values the reference writes at q = 0 and n+1 (x0 = xn,
-// x_{n+1} = x1) are written too.
-struct SysPeriodicFace {
-  using V = double;
-  using Pt = PtCv;
-  static constexpr bool kCyclic = true, kHasRef = true, kNb = false;
-  LineAxis ax;
-  double* f;
-  const double* cv;
-  const double *rd, *rd12;
-  double beta, dteta;
-  int ncv, nla, nlb;
-  const void* mkey;
-  __device__ int n() const { return ax.len(); }
-  __device__ bool valid(int la, int lb) const { return la < nla && lb < nlb; }
-  __device__ void coef(int r, double cvv, double& a, double& b, double& c) const {
-    int q = r + 1;
-    double rho = beta_over_alpha(beta, dteta, cvv);
-    a = -(rho * (rd[q] * rd12[q]));
-    c = -(rho * (rd[q + 1] * rd12[q]));
-    b = 1.0 - a - c;
-  }
-  __device__ double rhs(int la, int lb, int r) const { return f[ax.at(la, lb, r + 1)]; }
-  __device__ Pt pt(int la, int lb, int r) const { return Pt{cv[ax.cvat(la, lb, r + 1, ncv)]}; }
-  __device__ bool pt_ref(const Pt& p) const { return p.cv == 0.0; }
-  __device__ void coef(int, int, int r, const Pt& p, const Pt&, const Pt&, double& a, double& b,
-                       double& c) const {
-    coef(r, p.cv, a, b, c);
-  }
-  __device__ void coef_ref(int r, double& a, double& b, double& c) const { coef(r, 0.0, a, b, c); }
-  RefKey ref_key(int nn, int S) const { return RefKey{4 + ax.dir, nn, S, ncv, beta, dteta, mkey}; }
-  __device__ void store(int la, int lb, int r, double x) const {
-    int q = r + 1, n = ax.len();
-    f[ax.at(la, lb, q)] = x;
-    if (q == n) f[ax.at(la, lb, 0)] = x;
-    if (q == 1) f[ax.at(la, lb, n + 1)] = x;
-  }
-};
-
-// implicit_Tx (nse_dominik.f:1725-1781), used along x and z by solve_t,
-// with the conductivity ghosts wrapped periodically; rows 1..n written.
-struct SysPeriodicT {
-  using V = double;
-  using Pt = PtT;
-  static constexpr bool kCyclic = true, kHasRef = true, kNb = true;
-  LineAxis ax;
-  double* f;
-  const double *cv, *rc, *thc;
-  const double *d, *rd, *rd12;
-  double beta, dteta;
-  int nla, nlb;
-  __device__ int n() const { return ax.len(); }
-  __device__ bool valid(int la, int lb) const { return la < nla && lb < nlb; }
-  __device__ void coef(int r, double cvv, double l0, double lm, double lp, double rcv, double& a,
-                       double& b, double& c) const {
-    int q = r + 1;
-    double rho = beta_over_alpha(beta, dteta, cvv);
-    double rr = 1.0 / rcv;
-    double h1 = hm(d[q], l0, d[q - 1], lm);
-    double h2 = hm(d[q + 1], lp, d[q], l0);
-    a = -(rho * h1 * (rd12[q - 1] * rd[q]) * rr);
-    c = -(rho * h2 * (rd12[q] * rd[q]) * rr);
-    b = 1.0 - a - c;
-  }
-  __device__ double rhs(int la, int lb, int r) const { return f[ax.at(la, lb, r + 1)]; }
-  // r = -1 and r = n wrap to the last and first rows
-  __device__ Pt pt(int la, int lb, int r) const {
-    const int n = ax.len();
-    const int q = r < 0 ? n : (r >= n ? 1 : r + 1);
-    const unsigned o = ax.at(la, lb, q);
-    return Pt{cv[ax.cvat(la, lb, q, 1)], thc[o], rc[o]};
-  }
-  __device__ bool pt_ref(const Pt& p) const {
-    return (p.cv == 0.0) & (p.th == 1.0) & (p.rc == 1.0);
-  }
-  __device__ void coef(int, int, int r, const Pt& p, const Pt& pm, const Pt& pp, double& a,
-                       double& b, double& c) const {
-    coef(r, p.cv, p.th, pm.th, pp.th, p.rc, a, b, c);
-  }
-  __device__ void coef_ref(int r, double& a, double& b, double& c) const {
-    coef(r, 0.0, 1.0, 1.0, 1.0, 1.0, a, b, c);
-  }
-  RefKey ref_key(int nn, int S) const { return RefKey{6 + ax.dir, nn, S, 1, beta, dteta, d}; }
-  __device__ void store(int la, int lb, int r, double x) const { f[ax.at(la, lb, r + 1)] = x; }
-};
-
-// -------------------------------------------------- spectral y solves
-
-// SPN + CTRIDIAG (mkp.f:235-258, tridiag.f:43-58) for mode (I,K) =
-// (la, lb), I = 0..nx/2, K = 0..nz-1, rows J = 1..ny with the Neumann
-// fold of the first and last rows.  The zero mode is left to SP0B.  The
-// matrix differs from mode to mode, so there is no reference line.
-struct SysSpectral {
-  using V = dcx;
-  using Pt = PtNone;
-  static constexpr bool kCyclic = false, kHasRef = false, kNb = false;
-  Geom g;
-  double* phit;  // interleaved complex
-  const double *ac, *kxz;
-  __device__ int n() const { return g.ny; }
-  __device__ bool valid(int la, int lb) const { return la <= g.nx / 2 && lb < g.nz; }
-  __device__ dcx rhs(int la, int lb, int r) const {
-    unsigned o = 2 * g.spec32(la, r + 1, lb);
-    return {phit[o], phit[o + 1]};
-  }
-  __device__ Pt pt(int, int, int) const { return Pt{}; }
-  __device__ bool pt_ref(const Pt&) const { return false; }
-  __device__ void coef(int la, int lb, int r, const Pt&, const Pt&, const Pt&, double& a,
-                       double& b, double& c) const {
-    const int J = r + 1, ny = g.ny;
-    const double a1 = ac[J - 1], a3 = ac[J - 1 + 2 * ny];
-    double a2 = ac[J - 1 + ny] - kxz[la + (g.nx / 2 + 1) * lb];
-    // Neumann fold of the first and last rows, without a branch around the
-    // loads
-    const double f1 = a2 + a1, f3 = a2 + a3;
-    a2 = keep_or(a2, J != 1, f1);
-    a2 = keep_or(a2, J != ny, J == 1 ? f1 + a3 : f3);
-    a = keep_or(a1, J != 1, 0.0);
-    b = a2;
-    c = keep_or(a3, J != ny, 0.0);
-  }
-  __device__ void coef_ref(int, double& a, double& b, double& c) const {
-    a = 0.0;
-    b = 1.0;
-    c = 0.0;
-  }
-  RefKey ref_key(int nn, int S) const { return RefKey{0, nn, S, 0, 0.0, 0.0, nullptr}; }
-  __device__ void store(int la, int lb, int r, dcx x) const {
-    if (la == 0 && lb == 0) return;
-    unsigned o = 2 * g.spec32(la, r + 1, lb);
-    phit[o] = x.x;
-    phit[o + 1] = x.y;
-  }
-};
 
 // the fast kernel's queue: two counters, each on its own 128-byte line,
 // then the list; cleared when allocated, afterwards by the general kernel
@@ -397,7 +41,8 @@ double* line_table(const void* key, size_t keylen, long total, bool* fresh) {
 // SP0B (mkp.f:263-290): the zero mode by the forward march with
 // phi(0) = phi(1) = 0; one block loads the mode in parallel, one lane
 // marches, the block stores.
-__global__ void k_sp0b(Geom g, const double* ac, double* phit) {
+template <typename TS>
+__global__ void k_sp0b(Geom g, const double* ac, TS* phit) {
   extern __shared__ double sh[];  // 2*(ny+2) doubles
   int ny = g.ny;
   for (int J = threadIdx.x; J <= ny + 1; J += blockDim.x) {
@@ -430,81 +75,18 @@ __global__ void k_sp0b(Geom g, const double* ac, double* phit) {
   __syncthreads();
   for (int J = threadIdx.x; J <= ny + 1; J += blockDim.x) {
     long long o = 2 * g.spec_at(0, J, 0);
-    phit[o] = sh[2 * J];
-    phit[o + 1] = sh[2 * J + 1];
+    phit[o] = static_cast<TS>(sh[2 * J]);
+    phit[o + 1] = static_cast<TS>(sh[2 * J + 1]);
   }
 }
 
-// launch_line_solve's result as a status: -1 is a line length the solver
-// does not take, -2 a failed allocation (whose message is already set)
-static int line_status(int rc, const char* what) {
-  if (rc == 0) return VK_OK;
-  if (rc == -2) return VK_ERR_HIP;
-  return fail(VK_ERR_BAD_SIZE, std::string(what) + ": line length not supported");
-}
-
-int launch_spn(const Geom& g, double* phit, const double* ac, const double* kxz, hipStream_t s) {
-  SysSpectral sys{g, phit, ac, kxz};
-  ProfScope ps("line_spectral", s);
-  return line_status(launch_line_solve<true>(sys, g.ny, g.nx / 2 + 1, g.nz, s), "spn");
-}
-
-void launch_sp0b(const Geom& g, double* phit, const double* ac, hipStream_t s) {
+template <typename TS>
+void launch_sp0b(const Geom& g, TS* phit, const double* ac, hipStream_t s) {
   ProfScope ps("sp0b", s);
-  k_sp0b<<<1, 256, 2 * (g.ny + 2) * sizeof(double), s>>>(g, ac, phit);
+  k_sp0b<TS><<<1, 256, 2 * (g.ny + 2) * sizeof(double), s>>>(g, ac, phit);
 }
-
-// solve_u_temp body after the penalisation divide: y, z, x sweeps of one
-// component.  comp 0,1,2 = u, v, w (cellvalue slot comp+2); v's z and x
-// sweeps stop at j = ny-1 (nse_dominik.f:1411,1418).
-int launch_adi_velocity(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
-                        const double* cv, int comp, double beta, double dteta, hipStream_t s) {
-  int rc;
-  if (comp == 1) {
-    SysYV sy{g, f, cv, r, beta, dteta, m.dy};
-    ProfScope ps("line_y_v", s);
-    rc = line_status(launch_line_solve<true>(sy, g.ny + 1, g.nx, g.nz, s), "y sweep");
-  } else {
-    SysYFace sy{g, f, cv, r, beta, dteta, comp + 2, m.dy};
-    ProfScope ps("line_y_face", s);
-    rc = line_status(launch_line_solve<true>(sy, g.ny + 2, g.nx, g.nz, s), "y sweep");
-  }
-  if (rc) return rc;
-  int jmax = (comp == 1) ? g.ny - 1 : g.ny;
-  SysPeriodicFace sz{LineAxis{g, 2}, f, cv, r.rdz, r.rdz12, beta, dteta, comp + 2, g.nx, jmax,
-                     m.dz};
-  {
-    ProfScope ps("line_z_face", s);
-    rc = line_status(launch_line_solve<true>(sz, g.nz, g.nx, jmax, s), "z sweep");
-  }
-  if (rc) return rc;
-  SysPeriodicFace sx{LineAxis{g, 0}, f, cv, r.rdx, r.rdx12, beta, dteta, comp + 2, jmax, g.nz,
-                     m.dx};
-  ProfScope ps("line_x_face", s);
-  return line_status(launch_line_solve<false>(sx, g.nx, jmax, g.nz, s), "x sweep");
-}
-
-// solve_t: y, x, z sweeps (nse_dominik.f:1477-1500)
-int launch_adi_temperature(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
-                           const double* cv, const double* rc_, const double* thc, double beta,
-                           double dteta, hipStream_t s) {
-  SysYT sy{g, f, cv, rc_, thc, m, r, beta, dteta};
-  int rc;
-  {
-    ProfScope ps("line_y_t", s);
-    rc = line_status(launch_line_solve<true>(sy, g.ny + 2, g.nx, g.nz, s), "T y sweep");
-  }
-  if (rc) return rc;
-  SysPeriodicT sx{LineAxis{g, 0}, f, cv, rc_, thc, m.dx, r.rdx, r.rdx12, beta, dteta, g.ny, g.nz};
-  {
-    ProfScope ps("line_x_t", s);
-    rc = line_status(launch_line_solve<false>(sx, g.nx, g.ny, g.nz, s), "T x sweep");
-  }
-  if (rc) return rc;
-  SysPeriodicT sz{LineAxis{g, 2}, f, cv, rc_, thc, m.dz, r.rdz, r.rdz12, beta, dteta, g.nx, g.ny};
-  ProfScope ps("line_z_t", s);
-  return line_status(launch_line_solve<true>(sz, g.nz, g.nx, g.ny, s), "T z sweep");
-}
+template void launch_sp0b<double>(const Geom&, double*, const double*, hipStream_t);
+template void launch_sp0b<float>(const Geom&, float*, const double*, hipStream_t);
 
 }  // namespace vk
 
