@@ -252,6 +252,50 @@ int vk_time_step_mixed(double *u, double *v, double *w, double *p, float *t,
                        const double *sc, const double *eta, const double *xl,
                        const double *zl, const int *l);
 
+/* ------------------------------------------------- adjoint gradient */
+/* No reference counterpart: adjoint.f holds only operator kernels (above),
+   with no driver, objective or gradient.
+
+   The objective is the wall heat flux through both walls, integrated over
+   the walls and over the run:
+     J = sum_{n=1..nsteps} dt sum_{i,k} dx(i) dz(k)
+           [2 T^n(i,1,k)/dy12(0) + 2 T^n(i,ny,k)/dy12(ny)],
+   T^n the temperature after step n of the FP64 fused step (vk_time_step)
+   from the given initial state (arrays as for vk_time_step; none is
+   modified).  cellvalue slot 1 is the control: slots 2-4 are derived from
+   it at step 1 as SET_IMMERSED_BOUNDARY does.
+
+   vk_objective returns J.  vk_adjoint_gradient returns J and grad =
+   dJ/dcellvalue(:,:,:,1) (nx*ny*nz doubles, x fastest): the exact
+   discrete adjoint of the forward step, run backward over the trajectory
+   with a checkpoint every `checkpoint_every` steps held in device memory
+   and each segment recomputed from its checkpoint. */
+int vk_objective(const double *u, const double *v, const double *w,
+                 const double *p, const double *t, const double *u_pre,
+                 const double *v_pre, const double *w_pre, const double *t_pre,
+                 const double *u_ta, const double *v_ta, const double *w_ta,
+                 const double *cellvalue, const double *rhocp,
+                 const double *thc, const double *dx, const double *dy,
+                 const double *dz, const double *dx12, const double *dy12,
+                 const double *dz12, const int *nx, const int *ny,
+                 const int *nz, const double *dt, const double *re,
+                 const double *sc, const double *eta, const double *xl,
+                 const double *zl, const int *nsteps, double *objective);
+int vk_adjoint_gradient(const double *u, const double *v, const double *w,
+                        const double *p, const double *t, const double *u_pre,
+                        const double *v_pre, const double *w_pre,
+                        const double *t_pre, const double *u_ta,
+                        const double *v_ta, const double *w_ta,
+                        const double *cellvalue, const double *rhocp,
+                        const double *thc, const double *dx, const double *dy,
+                        const double *dz, const double *dx12,
+                        const double *dy12, const double *dz12, const int *nx,
+                        const int *ny, const int *nz, const double *dt,
+                        const double *re, const double *sc, const double *eta,
+                        const double *xl, const double *zl, const int *nsteps,
+                        const int *checkpoint_every, double *objective,
+                        double *grad);
+
 /* dst := src over a whole halo field, or over its interior when
    *interior_only is non-zero: the inline array copies of program main
    (u_pre := u at nse_dominik.f:246-257, u_temp := u_a at :407-417,

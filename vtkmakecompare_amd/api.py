@@ -148,6 +148,55 @@ def field_absmax(f, nx, ny, nz) -> float:
     return r.value
 
 
+# ------------------------------------------------------------- adjoint
+
+ADJ_FIELDS = ("u", "v", "w", "p", "t", "u_pre", "v_pre", "w_pre", "t_pre", "u_ta", "v_ta",
+              "w_ta")
+
+
+class AdjointProblem:
+    """An initial state, a level set and a run length for the wall-heat-flux
+    objective and its adjoint gradient with respect to cellvalue(...,1)
+    (vk_objective, vk_adjoint_gradient); every array resident in HBM."""
+
+    def __init__(self, cfg: G.Config, g: G.Grid, state: dict, cellvalue: np.ndarray,
+                 nsteps: int):
+        init()
+        self.cfg, self.g, self.nsteps = cfg, g, nsteps
+        self.d = {n: DeviceArray.from_host(state[n]) for n in ADJ_FIELDS + ("rhocp", "thc")}
+        for n in ("dx", "dy", "dz", "dx12", "dy12", "dz12"):
+            self.d[n] = DeviceArray.from_host(getattr(g, n))
+        self.cv = DeviceArray.from_host(np.asarray(cellvalue, dtype=np.float64))
+
+    def _args(self, cellvalue):
+        d, g, c = self.d, self.g, self.cfg
+        return ([d[n] for n in ADJ_FIELDS] + [cellvalue, d["rhocp"], d["thc"]] +
+                [d[n] for n in ("dx", "dy", "dz", "dx12", "dy12", "dz12")] +
+                [I(g.nx), I(g.ny), I(g.nz), D(c.dt), D(c.re), D(c.sc), D(c.eta), D(g.xl),
+                 D(g.zl), I(self.nsteps)])
+
+    def objective(self, cellvalue=None) -> float:
+        """J for the stored level set, or for `cellvalue` (host array, all
+        four slots; slots 2-4 are recomputed at step 1)."""
+        import ctypes
+        cv = self.cv
+        if cellvalue is not None:
+            cv = DeviceArray.from_host(np.asarray(cellvalue, dtype=np.float64))
+        out = ctypes.c_double(0.0)
+        check(lib().vk_objective(*self._args(cv), ctypes.byref(out)), "objective")
+        return out.value
+
+    def gradient(self, checkpoint_every: int = 20):
+        """(J, dJ/dcellvalue(...,1)) with the gradient as an (nz, ny, nx) array."""
+        import ctypes
+        g = self.g
+        grad = DeviceArray((g.nz, g.ny, g.nx))
+        out = ctypes.c_double(0.0)
+        check(lib().vk_adjoint_gradient(*self._args(self.cv), I(checkpoint_every),
+                                        ctypes.byref(out), grad), "adjoint_gradient")
+        return out.value, grad.download()
+
+
 # ------------------------------------------------------------------ runs
 
 STEP_FIELDS = ("u", "v", "w", "p", "t", "u_pre", "v_pre", "w_pre", "t_pre",

@@ -244,6 +244,15 @@ struct SampleResid {
   }
 };
 
+// f(i,j,k) / dy(j) with x and z index 0 read as the periodic image n
+struct SampleWScaled {
+  const double *f, *rdy;
+  __device__ double operator()(const Geom& g, int i, int j, int k) const {
+    int ii = (i == 0) ? g.nx : i, kk = (k == 0) ? g.nz : k;
+    return f[g.at(ii, j, kk)] * rdy[j];
+  }
+};
+
 constexpr int kRowsPerBlock = 4;
 constexpr int kRowThreads = 64;
 
@@ -417,6 +426,11 @@ int launch_p2s_resid(const Geom& g, const Metrics& m, const RMetrics& r, const d
   SampleResid smp{SampleDiv{u, v, w, m.dx, m.dy, m.dz, dt}, phi, r.rdx12, r.rdy12, r.rdz12,
                   r.rdx, r.rdy, r.rdz};
   return launch_forward<float>(g, smp, phit, "rfft_x_resid", "cfft_z_fwd2", s);
+}
+
+int launch_p2s_wscaled(const Geom& g, const double* f, const double* rdy, double* phit,
+                       hipStream_t s) {
+  return launch_forward<double>(g, SampleWScaled{f, rdy}, phit, "rfft_x_adj", "cfft_z_adj", s);
 }
 
 // S2P from a spectrum the call may overwrite (the z pass runs in place)

@@ -40,6 +40,23 @@ template <typename TT>
 int launch_adi_temperature(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
                            const TT* cv, const TT* rc, const TT* thc, double beta, double dteta,
                            hipStream_t s);
+// one sweep: velocity dir 0, 1, 2 = y, z, x; temperature dir 0, 1, 2 = y, x, z
+template <typename TC>
+int launch_velocity_sweep(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
+                          const TC* cv, int comp, int dir, double beta, double dteta,
+                          hipStream_t s);
+template <typename TT>
+int launch_temperature_sweep(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
+                             const TT* cv, const TT* rc, const TT* thc, int dir, double beta,
+                             double dteta, hipStream_t s);
+// the transposed sweeps (vk_sweep_adj.hip), in place on the adjoint field
+int launch_velocity_sweep_adj(const Geom& g, const Metrics& m, const RMetrics& r, double* lam,
+                              const double* cv, int comp, int dir, double beta, double dteta,
+                              hipStream_t s);
+int launch_temperature_sweep_adj(const Geom& g, const Metrics& m, const RMetrics& r,
+                                 double* lam, const double* cv, const double* rc,
+                                 const double* thc, int dir, double beta, double dteta,
+                                 hipStream_t s);
 // reciprocal metric arrays in workspace (recomputed each call, 6(n+2) doubles)
 int make_rmetrics(const Geom& g, const Metrics& m, RMetrics& r, hipStream_t s);
 
@@ -76,5 +93,21 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
                      const double* v_temp, const double* w_temp, double dt, double xl, double zl,
                      const MkpOutMixed& o, hipStream_t s);
 
-// explicit phase of the fused step (vk_step.hip)
+// explicit phase of the FP64 fused step (vk_step.hip): all explicit terms,
+// penalisation, AB2 and both divides; writes u_temp.. (ua, va, wa), the
+// temperature work field (ut) and the history arrays in place
+struct ExplicitArgs {
+  const double *u, *v, *w, *t, *rc, *thc, *u_ta, *v_ta, *w_ta, *cv;
+  double *u_pre, *v_pre, *w_pre, *t_pre;
+  double *ua, *va, *wa, *ut;
+  double dt, re, sc, eta, a1, a2, b1, c0, cd;
+};
+// the scalar part of ExplicitArgs for step l (AB2 weights, CN factors)
+ExplicitArgs explicit_args(double dt, double re, double sc, double eta, int l);
+void launch_explicit_fused(const Geom& g, const Metrics& m, const RMetrics& r,
+                           const ExplicitArgs& x, hipStream_t s);
+// the spectrum (double) of f(i,j,k) / dy(j), index 0 of x and z read as
+// the periodic image n (for the transposed Poisson solve)
+int launch_p2s_wscaled(const Geom& g, const double* f, const double* rdy, double* phit,
+                       hipStream_t s);
 }  // namespace vk

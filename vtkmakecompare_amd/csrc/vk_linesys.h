@@ -6,6 +6,7 @@
 #pragma once
 
 #include <string>
+#include <type_traits>
 
 #include "vk_kernels.h"
 #include "vk_partition.h"
@@ -109,6 +110,9 @@ struct SysYFace {
   }
   __device__ void coef_ref(int q, double& a, double& b, double& c) const { coef(q, 0.0, a, b, c); }
   RefKey ref_key(int nn, int S) const { return RefKey{1, nn, S, ncv, beta, dteta, mkey}; }
+  // element of row q, and whether the solve reads row q of its input
+  __device__ unsigned addr(int la, int lb, int q) const { return g.at32(la + 1, q, lb + 1); }
+  __device__ bool in_row(int q) const { return q >= 1 && q <= g.ny; }
   __device__ void store(int la, int lb, int q, double x) const {
     f[g.at32(la + 1, q, lb + 1)] = static_cast<TF>(x);
   }
@@ -154,6 +158,8 @@ struct SysYV {
   }
   __device__ void coef_ref(int q, double& a, double& b, double& c) const { coef(q, 0.0, a, b, c); }
   RefKey ref_key(int nn, int S) const { return RefKey{2, nn, S, 3, beta, dteta, mkey}; }
+  __device__ unsigned addr(int la, int lb, int q) const { return g.at32(la + 1, q, lb + 1); }
+  __device__ bool in_row(int q) const { return q >= 1 && q <= g.ny - 1; }
   __device__ void store(int la, int lb, int q, double x) const {
     f[g.at32(la + 1, q, lb + 1)] = static_cast<TF>(x);
   }
@@ -213,6 +219,8 @@ struct SysYT {
     coef(q, 0.0, 1.0, 1.0, 1.0, 1.0, a, b, c);
   }
   RefKey ref_key(int nn, int S) const { return RefKey{3, nn, S, 1, beta, dteta, m.dy}; }
+  __device__ unsigned addr(int la, int lb, int q) const { return g.at32(la + 1, q, lb + 1); }
+  __device__ bool in_row(int q) const { return q >= 1 && q <= g.ny; }
   __device__ void store(int la, int lb, int q, double x) const {
     if (q >= 1 && q <= g.ny) f[g.at32(la + 1, q, lb + 1)] = static_cast<TF>(x);
   }
@@ -268,6 +276,8 @@ struct SysPeriodicFace {
   }
   __device__ void coef_ref(int r, double& a, double& b, double& c) const { coef(r, 0.0, a, b, c); }
   RefKey ref_key(int nn, int S) const { return RefKey{4 + ax.dir, nn, S, ncv, beta, dteta, mkey}; }
+  __device__ unsigned addr(int la, int lb, int r) const { return ax.at(la, lb, r + 1); }
+  __device__ bool in_row(int) const { return true; }
   __device__ void store(int la, int lb, int r, double x) const {
     int q = r + 1, n = ax.len();
     const TF xs = static_cast<TF>(x);
@@ -322,6 +332,8 @@ struct SysPeriodicT {
     coef(r, 0.0, 1.0, 1.0, 1.0, 1.0, a, b, c);
   }
   RefKey ref_key(int nn, int S) const { return RefKey{6 + ax.dir, nn, S, 1, beta, dteta, d}; }
+  __device__ unsigned addr(int la, int lb, int r) const { return ax.at(la, lb, r + 1); }
+  __device__ bool in_row(int) const { return true; }
   __device__ void store(int la, int lb, int r, double x) const {
     f[ax.at(la, lb, r + 1)] = static_cast<TF>(x);
   }
@@ -392,60 +404,172 @@ int launch_spn(const Geom& g, TS* phit, const double* ac, const double* kxz, hip
   return line_status(launch_line_solve<true>(sys, g.ny, g.nx / 2 + 1, g.nz, s), "spn");
 }
 
+// The transpose of a line system: row q of A^T holds (c_{q-1}, b_q,
+// a_{q+1}) of A (indices mod n when cyclic, zero past the ends
+// otherwise).  The right-hand side is the adjoint of the solve's output,
+// read from `lam` on the rows the forward solve reads from its input (the
+// others are constant rows of the forward system, zero here); the solution
+// overwrites `lam` on those rows.  General path only (no reference-line
+// table): the adjoint sweeps run once per step of the backward pass.
+template <typename S>
+struct SysT {
+  using V = double;
+  using Pt = PtNone;
+  static constexpr bool kCyclic = S::kCyclic, kHasRef = false, kNb = false;
+  S s;
+  double* lam;
+  __device__ int n() const { return s.n(); }
+  __device__ bool valid(int la, int lb) const { return s.valid(la, lb); }
+  __device__ void row(int la, int lb, int q, double& a, double& b, double& c) const {
+    typename S::Pt p = s.pt(la, lb, q);
+    typename S::Pt pm = S::kNb ? s.pt(la, lb, q - 1) : typename S::Pt();
+    typename S::Pt pp = S::kNb ? s.pt(la, lb, q + 1) : typename S::Pt();
+    s.coef(la, lb, q, p, pm, pp, a, b, c);
+  }
+  __device__ double rhs(int la, int lb, int q) const {
+    const double v = lam[s.addr(la, lb, q)];
+    return s.in_row(q) ? v : 0.0;
+  }
+  __device__ Pt pt(int, int, int) const { return Pt{}; }
+  __device__ bool pt_ref(const Pt&) const { return false; }
+  __device__ void coef(int la, int lb, int q, const Pt&, const Pt&, const Pt&, double& a,
+                       double& b, double& c) const {
+    const int nn = s.n();
+    double a0, b0, c0, am, bm, cm, ap, bp, cp;
+    row(la, lb, q, a0, b0, c0);
+    if (kCyclic) {
+      row(la, lb, q == 0 ? nn - 1 : q - 1, am, bm, cm);
+      row(la, lb, q == nn - 1 ? 0 : q + 1, ap, bp, cp);
+      a = cm;
+      c = ap;
+    } else {
+      a = 0.0;
+      c = 0.0;
+      if (q >= 1) {
+        row(la, lb, q - 1, am, bm, cm);
+        a = cm;
+      }
+      if (q <= nn - 2) {
+        row(la, lb, q + 1, ap, bp, cp);
+        c = ap;
+      }
+    }
+    b = b0;
+  }
+  __device__ void coef_ref(int, double& a, double& b, double& c) const {
+    a = 0.0;
+    b = 1.0;
+    c = 0.0;
+  }
+  RefKey ref_key(int nn, int S_) const { return RefKey{-1, nn, S_, 0, 0.0, 0.0, nullptr}; }
+  __device__ void store(int la, int lb, int q, double x) const {
+    if (s.in_row(q)) lam[s.addr(la, lb, q)] = x;
+  }
+};
+
+// Runs fn(sys, n, nla, nlb, lane_lines, name) on the system of one sweep
+// of solve_u_temp: comp 0,1,2 = u, v, w (cellvalue slot comp+2), dir 0, 1,
+// 2 = the y, z, x sweep; v's z and x sweeps stop at j = ny-1
+// (nse_dominik.f:1411,1418).
+template <typename TC, typename Fn>
+int with_velocity_system(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
+                         const TC* cv, int comp, int dir, double beta, double dteta, Fn&& fn) {
+  const int jmax = (comp == 1) ? g.ny - 1 : g.ny;
+  if (dir == 0) {
+    if (comp == 1) {
+      SysYV<double, TC> sy{g, f, cv, r, beta, dteta, m.dy};
+      return fn(sy, g.ny + 1, g.nx, g.nz, std::true_type(), "line_y_v");
+    }
+    SysYFace<double, TC> sy{g, f, cv, r, beta, dteta, comp + 2, m.dy};
+    return fn(sy, g.ny + 2, g.nx, g.nz, std::true_type(), "line_y_face");
+  }
+  if (dir == 1) {
+    SysPeriodicFace<double, TC> sz{LineAxis{g, 2}, f,    cv,   r.rdz, r.rdz12, beta,
+                                   dteta,          comp + 2, g.nx, jmax,  m.dz};
+    return fn(sz, g.nz, g.nx, jmax, std::true_type(), "line_z_face");
+  }
+  SysPeriodicFace<double, TC> sx{LineAxis{g, 0}, f,    cv,   r.rdx, r.rdx12, beta,
+                                 dteta,          comp + 2, jmax, g.nz,  m.dx};
+  return fn(sx, g.nx, jmax, g.nz, std::false_type(), "line_x_face");
+}
+
+// the same for solve_t: dir 0, 1, 2 = the y, x, z sweep (nse_dominik.f:1477-1500)
+template <typename TT, typename Fn>
+int with_temperature_system(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
+                            const TT* cv, const TT* rc_, const TT* thc, int dir, double beta,
+                            double dteta, Fn&& fn) {
+  if (dir == 0) {
+    SysYT<TT, TT> sy{g, f, cv, rc_, thc, m, r, beta, dteta};
+    return fn(sy, g.ny + 2, g.nx, g.nz, std::true_type(), "line_y_t");
+  }
+  if (dir == 1) {
+    SysPeriodicT<TT, TT> sx{LineAxis{g, 0}, f, cv, rc_, thc, m.dx, r.rdx, r.rdx12, beta, dteta,
+                            g.ny, g.nz};
+    return fn(sx, g.nx, g.ny, g.nz, std::false_type(), "line_x_t");
+  }
+  SysPeriodicT<TT, TT> sz{LineAxis{g, 2}, f, cv, rc_, thc, m.dz, r.rdz, r.rdz12, beta, dteta,
+                          g.nx, g.ny};
+  return fn(sz, g.nz, g.nx, g.ny, std::true_type(), "line_z_t");
+}
+
+// a forward solve of the system handed over by with_*_system
+struct SolveFwd {
+  hipStream_t s;
+  template <typename Sys, typename Lane>
+  int operator()(const Sys& sys, int n, int nla, int nlb, Lane, const char* name) const {
+    ProfScope ps(name, s);
+    return line_status(launch_line_solve<Lane::value>(sys, n, nla, nlb, s), name);
+  }
+};
+
+// the transposed solve on the adjoint field `lam`
+struct SolveAdj {
+  hipStream_t s;
+  double* lam;
+  template <typename Sys, typename Lane>
+  int operator()(const Sys& sys, int n, int nla, int nlb, Lane, const char* name) const {
+    SysT<Sys> st{sys, lam};
+    ProfScope ps("line_adjoint", s);
+    return line_status(launch_line_solve<Lane::value>(st, n, nla, nlb, s), name);
+  }
+};
+
+template <typename TC>
+int launch_velocity_sweep(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
+                          const TC* cv, int comp, int dir, double beta, double dteta,
+                          hipStream_t s) {
+  return with_velocity_system(g, m, r, f, cv, comp, dir, beta, dteta, SolveFwd{s});
+}
+
+template <typename TT>
+int launch_temperature_sweep(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
+                             const TT* cv, const TT* rc_, const TT* thc, int dir, double beta,
+                             double dteta, hipStream_t s) {
+  return with_temperature_system(g, m, r, f, cv, rc_, thc, dir, beta, dteta, SolveFwd{s});
+}
+
 // solve_u_temp body after the penalisation divide: y, z, x sweeps of one
-// component.  comp 0,1,2 = u, v, w (cellvalue slot comp+2); v's z and x
-// sweeps stop at j = ny-1 (nse_dominik.f:1411,1418).
+// component
 template <typename TC>
 int launch_adi_velocity(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
                         const TC* cv, int comp, double beta, double dteta, hipStream_t s) {
-  int rc;
-  if (comp == 1) {
-    SysYV<double, TC> sy{g, f, cv, r, beta, dteta, m.dy};
-    ProfScope ps("line_y_v", s);
-    rc = line_status(launch_line_solve<true>(sy, g.ny + 1, g.nx, g.nz, s), "y sweep");
-  } else {
-    SysYFace<double, TC> sy{g, f, cv, r, beta, dteta, comp + 2, m.dy};
-    ProfScope ps("line_y_face", s);
-    rc = line_status(launch_line_solve<true>(sy, g.ny + 2, g.nx, g.nz, s), "y sweep");
+  for (int dir = 0; dir < 3; dir++) {
+    int rc = launch_velocity_sweep<TC>(g, m, r, f, cv, comp, dir, beta, dteta, s);
+    if (rc) return rc;
   }
-  if (rc) return rc;
-  int jmax = (comp == 1) ? g.ny - 1 : g.ny;
-  SysPeriodicFace<double, TC> sz{LineAxis{g, 2}, f,    cv,   r.rdz, r.rdz12, beta,
-                                 dteta,          comp + 2, g.nx, jmax,  m.dz};
-  {
-    ProfScope ps("line_z_face", s);
-    rc = line_status(launch_line_solve<true>(sz, g.nz, g.nx, jmax, s), "z sweep");
-  }
-  if (rc) return rc;
-  SysPeriodicFace<double, TC> sx{LineAxis{g, 0}, f,    cv,   r.rdx, r.rdx12, beta,
-                                 dteta,          comp + 2, jmax, g.nz,  m.dx};
-  ProfScope ps("line_x_face", s);
-  return line_status(launch_line_solve<false>(sx, g.nx, jmax, g.nz, s), "x sweep");
+  return VK_OK;
 }
 
-// solve_t: y, x, z sweeps (nse_dominik.f:1477-1500)
+// solve_t: y, x, z sweeps
 template <typename TT>
 int launch_adi_temperature(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
                            const TT* cv, const TT* rc_, const TT* thc, double beta, double dteta,
                            hipStream_t s) {
-  SysYT<TT, TT> sy{g, f, cv, rc_, thc, m, r, beta, dteta};
-  int rc;
-  {
-    ProfScope ps("line_y_t", s);
-    rc = line_status(launch_line_solve<true>(sy, g.ny + 2, g.nx, g.nz, s), "T y sweep");
+  for (int dir = 0; dir < 3; dir++) {
+    int rc = launch_temperature_sweep<TT>(g, m, r, f, cv, rc_, thc, dir, beta, dteta, s);
+    if (rc) return rc;
   }
-  if (rc) return rc;
-  SysPeriodicT<TT, TT> sx{LineAxis{g, 0}, f, cv, rc_, thc, m.dx, r.rdx, r.rdx12, beta, dteta,
-                          g.ny, g.nz};
-  {
-    ProfScope ps("line_x_t", s);
-    rc = line_status(launch_line_solve<false>(sx, g.nx, g.ny, g.nz, s), "T x sweep");
-  }
-  if (rc) return rc;
-  SysPeriodicT<TT, TT> sz{LineAxis{g, 2}, f, cv, rc_, thc, m.dz, r.rdz, r.rdz12, beta, dteta,
-                          g.nx, g.ny};
-  ProfScope ps("line_z_t", s);
-  return line_status(launch_line_solve<true>(sz, g.nz, g.nx, g.ny, s), "T z sweep");
+  return VK_OK;
 }
 
 }  // namespace vk

@@ -21,13 +21,6 @@
 
 namespace vk {
 
-struct ExplicitArgs {
-  const double *u, *v, *w, *t, *rc, *thc, *u_ta, *v_ta, *w_ta, *cv;
-  double *u_pre, *v_pre, *w_pre, *t_pre;
-  double *ua, *va, *wa, *ut;
-  double dt, re, sc, eta, a1, a2, b1, c0, cd;
-};
-
 // x / (1 + eta cv dt), exactly x where cv is 0
 __device__ inline double pen_div(double x, double eta, double cv, double dt) {
   return cv == 0.0 ? x : x / (1.0 + eta * cv * dt);
@@ -95,6 +88,25 @@ __global__ void k_copy_interior_step(Geom g, const double* src, double* dst) {
   dst[o] = src[o];
 }
 
+void launch_explicit_fused(const Geom& g, const Metrics& m, const RMetrics& r,
+                           const ExplicitArgs& x, hipStream_t s) {
+  ProfScope ps("explicit_fused", s);
+  k_explicit_fused<<<interior_grid(g), interior_block(), 0, s>>>(g, m, r, x);
+}
+
+ExplicitArgs explicit_args(double dt, double re, double sc, double eta, int l) {
+  ExplicitArgs x{};
+  x.dt = dt; x.re = re; x.sc = sc; x.eta = eta;
+  if (l == 1) {
+    x.a1 = 1.0; x.a2 = 0.0; x.b1 = 1.0;
+  } else {
+    x.a1 = 1.5; x.a2 = -0.5; x.b1 = 1.0;
+  }
+  x.c0 = dt / re / 2.0;
+  x.cd = dt / (2.0 * re * sc);
+  return x;
+}
+
 }  // namespace vk
 
 using namespace vk;
@@ -154,26 +166,15 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
 
   launch_set_bc(g, du, dv, dw, dp, dtt, s);
   if (*l == 1) launch_cv_faces(g, dcv, s);
-  ExplicitArgs x;
+  ExplicitArgs x = explicit_args(*dt, *re, *sc, *eta, *l);
   x.u = du; x.v = dv; x.w = dw; x.t = dtt; x.rc = drc; x.thc = dk;
   x.u_ta = duta; x.v_ta = dvta; x.w_ta = dwta; x.cv = dcv;
   x.u_pre = dup; x.v_pre = dvp; x.w_pre = dwp; x.t_pre = dtp;
   x.ua = dut; x.va = dvt; x.wa = dwt; x.ut = twork;
-  x.dt = *dt; x.re = *re; x.sc = *sc; x.eta = *eta;
-  if (*l == 1) {
-    x.a1 = 1.0; x.a2 = 0.0; x.b1 = 1.0;
-  } else {
-    x.a1 = 1.5; x.a2 = -0.5; x.b1 = 1.0;
-  }
-  x.c0 = *dt / *re / 2.0;
-  x.cd = *dt / (2.0 * *re * *sc);
   RMetrics r;
   int rrc = make_rmetrics(g, m, r, s);
   if (rrc) return rrc;
-  {
-    ProfScope ps("explicit_fused", s);
-    k_explicit_fused<<<interior_grid(g), interior_block(), 0, s>>>(g, m, r, x);
-  }
+  launch_explicit_fused(g, m, r, x, s);
 
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
   double* comps[3] = {dut, dvt, dwt};

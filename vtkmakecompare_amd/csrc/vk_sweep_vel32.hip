@@ -3,6 +3,8 @@
 
 namespace vk {
 
+template int launch_velocity_sweep<float>(const Geom&, const Metrics&, const RMetrics&, double*,
+                                          const float*, int, int, double, double, hipStream_t);
 template int launch_adi_velocity<float>(const Geom&, const Metrics&, const RMetrics&, double*,
                                         const float*, int, double, double, hipStream_t);
 

@@ -3,6 +3,8 @@
 
 namespace vk {
 
+template int launch_velocity_sweep<double>(const Geom&, const Metrics&, const RMetrics&, double*,
+                                           const double*, int, int, double, double, hipStream_t);
 template int launch_adi_velocity<double>(const Geom&, const Metrics&, const RMetrics&, double*,
                                          const double*, int, double, double, hipStream_t);
 
