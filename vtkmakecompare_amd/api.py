@@ -148,6 +148,20 @@ def field_absmax(f, nx, ny, nz) -> float:
     return r.value
 
 
+def levelset_cellvalue(g: G.Grid, phi0: np.ndarray) -> np.ndarray:
+    """cellvalue (4, nz, ny, nx) from set_levelset on the device (slot 1
+    filled, slots 2-4 zero until a time step derives them)."""
+    d = {n: DeviceArray.from_host(getattr(g, n)) for n in ("dx", "dx12", "dy12", "dz12",
+                                                           "x", "y", "z")}
+    p0 = DeviceArray.from_host(phi0)
+    phi = DeviceArray(g.shape)
+    cv = DeviceArray((4, g.nz, g.ny, g.nx))
+    cv.fill_zero()
+    set_levelset(g.nx, g.ny, g.nz, d["dx"], d["dx12"], d["dy12"], d["dz12"], p0, phi, d["x"],
+                 d["y"], d["z"], cv)
+    return cv.download()
+
+
 # ------------------------------------------------------------- adjoint
 
 ADJ_FIELDS = ("u", "v", "w", "p", "t", "u_pre", "v_pre", "w_pre", "t_pre", "u_ta", "v_ta",
