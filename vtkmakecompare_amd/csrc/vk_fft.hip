@@ -265,7 +265,9 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_rfft_x(
   C* sh = reinterpret_cast<C*>(shraw);
   const int M = p.n;  // nx/2
   const int row = threadIdx.x / kRowThreads, t = threadIdx.x % kRowThreads;
-  const int j = 1 + blockIdx.x * kRowsPerBlock + row, k = blockIdx.y;
+  int bx, by, bz;
+  xcd_block(bx, by, bz);
+  const int j = 1 + bx * kRowsPerBlock + row, k = by;
   C* a = sh + row * 2 * M;
   C* b = a + M;
   const bool ok = j <= g.ny;
@@ -287,21 +289,25 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_rfft_x(
   }
 }
 
-// z pass in place over PHIT columns (I, j) for I = blockIdx.x*C + c,
-// j = 1 + blockIdx.y.  Forward divides by nx*nz (p2s_fftw2d.f:101).
-constexpr int kCols = 4;
-template <typename R>
+// z pass in place over PHIT columns (I, j) for I = bx*CC + c, j = 1 + by.
+// Forward divides by nx*nz (p2s_fftw2d.f:101).  CC columns per block, so
+// that a block's loads cover whole 128-byte lines where the LDS allows
+// (8 float or 4 double columns at nz = 384); 256 / CC threads per column.
+template <typename R, int CC>
 __global__ __launch_bounds__(256) void k_cfft_z(Geom g, R* phit, FftPlan<R> p, int sign,
                                                 int scale) {
   using C = typename Cx<R>::T;
+  constexpr int TPC = 256 / CC;
   extern __shared__ unsigned char shraw[];
   C* sh = reinterpret_cast<C*>(shraw);
   const int N = p.n;  // nz
   const int nh1 = g.nx / 2 + 1;
-  const int I0 = blockIdx.x * kCols, j = 1 + blockIdx.y;
-  // load: element e = c + C*k
-  for (int e = threadIdx.x; e < kCols * N; e += blockDim.x) {
-    int c = e % kCols, k = e / kCols;
+  int bx, by, bz;
+  xcd_block(bx, by, bz);
+  const int I0 = bx * CC, j = 1 + by;
+  // load: element e = c + CC*k
+  for (int e = threadIdx.x; e < CC * N; e += blockDim.x) {
+    int c = e % CC, k = e / CC;
     int I = I0 + c;
     C v = mkc<C>((R)0, (R)0);
     if (I < nh1) {
@@ -311,14 +317,14 @@ __global__ __launch_bounds__(256) void k_cfft_z(Geom g, R* phit, FftPlan<R> p, i
     sh[c * 2 * N + k] = v;
   }
   __syncthreads();
-  const int c = threadIdx.x / 64, t = threadIdx.x % 64;
-  C* res = fft_lds<R>(sh + c * 2 * N, sh + c * 2 * N + N, t, 64, p, sign);
+  const int c = threadIdx.x / TPC, t = threadIdx.x % TPC;
+  C* res = fft_lds<R>(sh + c * 2 * N, sh + c * 2 * N + N, t, TPC, p, sign);
   // each column's result buffer is the same half for every column
   const int half = (res == sh + c * 2 * N) ? 0 : N;
   __syncthreads();
   const R sc = (R)(g.nx * g.nz);
-  for (int e = threadIdx.x; e < kCols * N; e += blockDim.x) {
-    int cc = e % kCols, k = e / kCols;
+  for (int e = threadIdx.x; e < CC * N; e += blockDim.x) {
+    int cc = e % CC, k = e / CC;
     int I = I0 + cc;
     if (I >= nh1) continue;
     C v = sh[cc * 2 * N + half + k];
@@ -326,6 +332,24 @@ __global__ __launch_bounds__(256) void k_cfft_z(Geom g, R* phit, FftPlan<R> p, i
     long long off = 2 * g.spec_at(I, j, k);
     phit[off] = v.x;
     phit[off + 1] = v.y;
+  }
+}
+
+// the z pass with the widest column group whose LDS fits in 64 KB
+template <typename R>
+static void launch_cfft_z(const Geom& g, R* phit, const FftPlan<R>& pz, int sign, int scale,
+                          hipStream_t s) {
+  using C = typename Cx<R>::T;
+  const size_t per_col = sizeof(C) * 2 * (size_t)g.nz;
+  const int nh1 = g.nx / 2 + 1;
+  if (per_col * 8 <= 65536) {
+    k_cfft_z<R, 8><<<dim3((nh1 + 7) / 8, g.ny), 256, per_col * 8, s>>>(g, phit, pz, sign, scale);
+  } else if (per_col * 4 <= 65536) {
+    k_cfft_z<R, 4><<<dim3((nh1 + 3) / 4, g.ny), 256, per_col * 4, s>>>(g, phit, pz, sign, scale);
+  } else if (per_col * 2 <= 65536) {
+    k_cfft_z<R, 2><<<dim3((nh1 + 1) / 2, g.ny), 256, per_col * 2, s>>>(g, phit, pz, sign, scale);
+  } else {
+    k_cfft_z<R, 1><<<dim3(nh1, g.ny), 256, per_col, s>>>(g, phit, pz, sign, scale);
   }
 }
 
@@ -340,7 +364,9 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_irfft_x(
   C* sh = reinterpret_cast<C*>(shraw);
   const int M = p.n;
   const int row = threadIdx.x / kRowThreads, t = threadIdx.x % kRowThreads;
-  const int j = 1 + blockIdx.x * kRowsPerBlock + row, k = blockIdx.y;
+  int bx, by, bz;
+  xcd_block(bx, by, bz);
+  const int j = 1 + bx * kRowsPerBlock + row, k = by;
   C* a = sh + row * 2 * M;
   C* b = a + M;
   const bool ok = j <= g.ny;
@@ -380,7 +406,10 @@ static int plans_for(const Geom& g, FftPlan<R>& px, FftPlan<R>& pz,
     return fail(VK_ERR_BAD_SIZE, "FFT: nx/2 and nz must be products of 2 and 3");
   twl = twiddle_table<R>(g.nx);
   if (!twl) return fail(VK_ERR_HIP, "FFT: twiddle table");
-  if (g.nz > 2048 || g.nx > 4096) return fail(VK_ERR_BAD_SIZE, "FFT: size above the LDS plan");
+  // one column (2 nz complex) and four x rows (2 nx complex each) in 64 KB
+  if (2 * (size_t)g.nz * sizeof(typename Cx<R>::T) > 65536 ||
+      4 * 2 * (size_t)(g.nx / 2) * sizeof(typename Cx<R>::T) > 65536)
+    return fail(VK_ERR_BAD_SIZE, "FFT: size above the LDS plan");
   return VK_OK;
 }
 
@@ -398,10 +427,8 @@ static int launch_forward(const Geom& g, Sample smp, R* phit, const char* xname,
     k_rfft_x<Sample, R><<<dim3((g.ny + kRowsPerBlock - 1) / kRowsPerBlock, g.nz),
                           kRowsPerBlock * kRowThreads, shx, s>>>(g, smp, phit, px, twl);
   }
-  size_t shz = sizeof(C) * 2 * g.nz * kCols;
   ProfScope ps(zname, s);
-  k_cfft_z<R><<<dim3((g.nx / 2 + 1 + kCols - 1) / kCols, g.ny), 256, shz, s>>>(g, phit, pz, -1,
-                                                                              1);
+  launch_cfft_z<R>(g, phit, pz, -1, 1, s);
   return VK_OK;
 }
 
@@ -442,11 +469,9 @@ int launch_s2p_inplace(const Geom& g, R* phit, R* fp, hipStream_t s, const char*
   const C* twl;
   int rc = plans_for<R>(g, px, pz, twl);
   if (rc) return rc;
-  size_t shz = sizeof(C) * 2 * g.nz * kCols;
   {
     ProfScope ps(zname, s);
-    k_cfft_z<R><<<dim3((g.nx / 2 + 1 + kCols - 1) / kCols, g.ny), 256, shz, s>>>(g, phit, pz, +1,
-                                                                                0);
+    launch_cfft_z<R>(g, phit, pz, +1, 0, s);
   }
   size_t shx = sizeof(C) * 2 * (g.nx / 2) * kRowsPerBlock;
   ProfScope ps(xname, s);

@@ -19,18 +19,19 @@ module vk_capi
     type(c_ptr) function vk_last_error() bind(C, name="vk_last_error")
       import :: c_ptr
     end function
+    ! assumed-type: double or float host arrays
     integer(c_int) function vk_register(host, bytes) bind(C, name="vk_register")
-      import :: c_int, c_double, c_size_t
-      real(c_double) :: host(*)
+      import :: c_int, c_size_t
+      type(*), dimension(*) :: host
       integer(c_size_t), intent(in) :: bytes
     end function
     integer(c_int) function vk_upload(host) bind(C, name="vk_upload")
-      import :: c_int, c_double
-      real(c_double) :: host(*)
+      import :: c_int
+      type(*), dimension(*) :: host
     end function
     integer(c_int) function vk_download(host) bind(C, name="vk_download")
-      import :: c_int, c_double
-      real(c_double) :: host(*)
+      import :: c_int
+      type(*), dimension(*) :: host
     end function
     integer(c_int) function vk_synchronize() bind(C, name="vk_synchronize")
       import :: c_int
@@ -145,6 +146,43 @@ module vk_capi
       real(c_double) :: dt, re, sc, eta, xl, zl
       integer(c_int) :: nx, ny, nz, l
     end function
+    ! the mixed-precision step: t, the history, u_ta, cellvalue, rhoCp and
+    ! thc in real(c_float), the rest in real(c_double)
+    integer(c_int) function vk_time_step_mixed(u, v, w, p, t, u_pre, v_pre, w_pre, t_pre, &
+        u_ta, v_ta, w_ta, u_temp, v_temp, w_temp, divu, cellvalue, rhocp, thc, &
+        dx, dy, dz, dx12, dy12, dz12, nx, ny, nz, dt, re, sc, eta, xl, zl, l) &
+        bind(C, name="vk_time_step_mixed")
+      import :: c_int, c_double, c_float
+      real(c_double) :: u(*), v(*), w(*), p(*), u_temp(*), v_temp(*), w_temp(*), divu(*)
+      real(c_float) :: t(*), u_pre(*), v_pre(*), w_pre(*), t_pre(*), u_ta(*), v_ta(*), w_ta(*)
+      real(c_float) :: cellvalue(*), rhocp(*), thc(*)
+      real(c_double) :: dx(*), dy(*), dz(*), dx12(*), dy12(*), dz12(*)
+      real(c_double) :: dt, re, sc, eta, xl, zl
+      integer(c_int) :: nx, ny, nz, l
+    end function
+    ! wall-heat-flux objective of an nsteps run, and its adjoint gradient
+    ! with respect to cellvalue(:,:,:,1) (checkpoint every `every` steps)
+    integer(c_int) function vk_objective(u, v, w, p, t, u_pre, v_pre, w_pre, t_pre, &
+        u_ta, v_ta, w_ta, cellvalue, rhocp, thc, dx, dy, dz, dx12, dy12, dz12, &
+        nx, ny, nz, dt, re, sc, eta, xl, zl, nsteps, objective) bind(C, name="vk_objective")
+      import :: c_int, c_double
+      real(c_double) :: u(*), v(*), w(*), p(*), t(*), u_pre(*), v_pre(*), w_pre(*), t_pre(*)
+      real(c_double) :: u_ta(*), v_ta(*), w_ta(*), cellvalue(*), rhocp(*), thc(*)
+      real(c_double) :: dx(*), dy(*), dz(*), dx12(*), dy12(*), dz12(*)
+      real(c_double) :: dt, re, sc, eta, xl, zl, objective
+      integer(c_int) :: nx, ny, nz, nsteps
+    end function
+    integer(c_int) function vk_adjoint_gradient(u, v, w, p, t, u_pre, v_pre, w_pre, t_pre, &
+        u_ta, v_ta, w_ta, cellvalue, rhocp, thc, dx, dy, dz, dx12, dy12, dz12, &
+        nx, ny, nz, dt, re, sc, eta, xl, zl, nsteps, every, objective, grad) &
+        bind(C, name="vk_adjoint_gradient")
+      import :: c_int, c_double
+      real(c_double) :: u(*), v(*), w(*), p(*), t(*), u_pre(*), v_pre(*), w_pre(*), t_pre(*)
+      real(c_double) :: u_ta(*), v_ta(*), w_ta(*), cellvalue(*), rhocp(*), thc(*)
+      real(c_double) :: dx(*), dy(*), dz(*), dx12(*), dy12(*), dz12(*)
+      real(c_double) :: dt, re, sc, eta, xl, zl, objective, grad(*)
+      integer(c_int) :: nx, ny, nz, nsteps, every
+    end function
   end interface
 
 contains
@@ -168,11 +206,19 @@ contains
     error stop 2
   end subroutine
 
-  ! Registers a host array and copies it to its device mirror.
+  ! Registers a host array of n doubles and copies it to its device mirror.
   subroutine vk_resident(a, n)
     real(c_double) :: a(*)
     integer(c_size_t), intent(in) :: n
     call vk_check(vk_register(a, n * 8_c_size_t), 'vk_register')
+    call vk_check(vk_upload(a), 'vk_upload')
+  end subroutine
+
+  ! The same for an array of n floats.
+  subroutine vk_resident_f32(a, n)
+    real(c_float) :: a(*)
+    integer(c_size_t), intent(in) :: n
+    call vk_check(vk_register(a, n * 4_c_size_t), 'vk_register')
     call vk_check(vk_upload(a), 'vk_upload')
   end subroutine
 end module vk_capi

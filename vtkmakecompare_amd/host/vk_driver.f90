@@ -3,21 +3,27 @@
 !
 ! The loop body calls the reference routines by their own names and
 ! argument lists (vk_reference.f90), in the reference's order
-! (nse_dominik.f:290-448); mode 1 calls the fused step instead.  Arrays are
+! (nse_dominik.f:290-448); mode 1 calls the fused step instead, mode 2 the
+! mixed-precision step (float copies of T, the history, u_ta, cellvalue,
+! rhoCp and thc), mode 3 runs nsteps for the wall-heat-flux objective and
+! its adjoint gradient with respect to cellvalue(:,:,:,1).  Arrays are
 ! allocated on the host, made device-resident once, and copied back only
 ! for the output file.
 !
 ! usage: vk_driver input.bin output.bin
 !
 ! input (stream, native endian):
-!   int32   nx, ny, nz, nsteps, mode (0 = routine calls, 1 = fused step),
-!           levelset (1 = set_levelset from phi0, 0 = cellvalue given), 0, 0
+!   int32   nx, ny, nz, nsteps, mode (0 = routine calls, 1 = fused step,
+!           2 = mixed-precision step, 3 = objective and adjoint gradient),
+!           levelset (1 = set_levelset from phi0, 0 = cellvalue given),
+!           checkpoint interval (mode 3), 0
 !   float64 dt, Re, Sc, eta, xl, zl, 0, 0
 !   float64 dx, dy, dz, dx12, dy12, dz12, x, y, z   (n+2 each)
 !   float64 u, v, w, T, phi0                          (halo fields)
 !   float64 cellvalue                                 (nx*ny*nz*4)
 ! output: u, v, w, p, T, u_ta, v_ta, w_ta, divu, u_temp, v_temp, w_temp
-!         (halo fields) and cellvalue
+!         (halo fields) and cellvalue; mode 3: the objective, then the
+!         gradient (nx*ny*nz)
 program vk_driver
   use vk_capi
   implicit none
@@ -28,7 +34,12 @@ program vk_driver
       w_d, u_a, v_a, w_a, u_temp, v_temp, w_temp, u_pre, v_pre, w_pre, u_ta, v_ta, w_ta, &
       u_l, v_l, w_l, u_vt, u_dt, u_t, t_pre, t_l, rhocp, thc, divu, u_ep2, v_ep2, w_ep2, &
       phi, phi0
-  double precision, allocatable :: cellvalue(:,:,:,:)
+  double precision, allocatable :: cellvalue(:,:,:,:), grad(:,:,:)
+  real(c_float), allocatable, dimension(:,:,:) :: tf, u_pref, v_pref, w_pref, t_pref, u_taf, &
+      v_taf, w_taf, rhocpf, thcf
+  real(c_float), allocatable :: cellvaluef(:,:,:,:)
+  double precision :: objective
+  integer :: every
   character(len=512) :: fin, fout
   integer(c_size_t) :: nf, nc
   integer :: ios
@@ -95,7 +106,59 @@ program vk_driver
   if (lsflag == 1) call set_levelset(nx, ny, nz, dx, dx12, dy12, dz12, phi0, phi, x, y, z, &
                                      cellvalue)
 
+  if (mode == 3) then
+    every = max(1, hdr(7))
+    allocate (grad(1:nx,1:ny,1:nz))
+    grad = 0.0d0
+    call vk_resident(grad, int(nx, c_size_t) * int(ny, c_size_t) * int(nz, c_size_t))
+    call vk_check(vk_adjoint_gradient(u, v, w, p, t, u_pre, v_pre, w_pre, t_pre, u_ta, v_ta, &
+                                      w_ta, cellvalue, rhocp, thc, dx, dy, dz, dx12, dy12, &
+                                      dz12, nx, ny, nz, dt, re, sc, eta, xl, zl, nsteps, every, &
+                                      objective, grad), 'adjoint gradient')
+    call vk_check(vk_download(grad), 'download')
+    open (11, file=trim(fout), access='stream', form='unformatted', status='replace')
+    write (11) objective, grad
+    close (11)
+    call vk_check(vk_finalize(), 'vk_finalize')
+    write (6, '(a,i0,a,es23.15)') 'vk_driver: adjoint over ', nsteps, ' steps, J = ', objective
+    stop
+  end if
+
+  if (mode == 2) then
+    ! float copies of the fields the mixed step keeps in float
+    call vk_check(vk_download(cellvalue), 'download')
+    allocate (tf(0:nx+1,0:ny+1,0:nz+1))
+    allocate (u_pref, v_pref, w_pref, t_pref, u_taf, v_taf, w_taf, rhocpf, thcf, mold=tf)
+    allocate (cellvaluef(1:nx,1:ny,1:nz,1:4))
+    tf = real(t, c_float); u_pref = real(u_pre, c_float); v_pref = real(v_pre, c_float)
+    w_pref = real(w_pre, c_float); t_pref = real(t_pre, c_float)
+    u_taf = real(u_ta, c_float); v_taf = real(v_ta, c_float); w_taf = real(w_ta, c_float)
+    rhocpf = real(rhocp, c_float); thcf = real(thc, c_float)
+    cellvaluef = real(cellvalue, c_float)
+    call vk_resident_f32(tf, nf); call vk_resident_f32(u_pref, nf)
+    call vk_resident_f32(v_pref, nf); call vk_resident_f32(w_pref, nf)
+    call vk_resident_f32(t_pref, nf); call vk_resident_f32(u_taf, nf)
+    call vk_resident_f32(v_taf, nf); call vk_resident_f32(w_taf, nf)
+    call vk_resident_f32(rhocpf, nf); call vk_resident_f32(thcf, nf)
+    call vk_resident_f32(cellvaluef, nc)
+    do l = 1, nsteps
+      call vk_check(vk_time_step_mixed(u, v, w, p, tf, u_pref, v_pref, w_pref, t_pref, u_taf, &
+                                       v_taf, w_taf, u_temp, v_temp, w_temp, divu, cellvaluef, &
+                                       rhocpf, thcf, dx, dy, dz, dx12, dy12, dz12, nx, ny, nz, &
+                                       dt, re, sc, eta, xl, zl, l), 'mixed time step')
+    end do
+    call vk_check(vk_download(tf), 'download'); call vk_check(vk_download(u_taf), 'download')
+    call vk_check(vk_download(v_taf), 'download'); call vk_check(vk_download(w_taf), 'download')
+    call vk_check(vk_download(cellvaluef), 'download')
+    t = real(tf, c_double); u_ta = real(u_taf, c_double); v_ta = real(v_taf, c_double)
+    w_ta = real(w_taf, c_double); cellvalue = real(cellvaluef, c_double)
+    call vk_check(vk_upload(t), 'upload'); call vk_check(vk_upload(u_ta), 'upload')
+    call vk_check(vk_upload(v_ta), 'upload'); call vk_check(vk_upload(w_ta), 'upload')
+    call vk_check(vk_upload(cellvalue), 'upload')
+  end if
+
   do l = 1, nsteps
+    if (mode == 2) exit
     if (mode == 1) then
       call vk_check(vk_time_step(u, v, w, p, t, u_pre, v_pre, w_pre, t_pre, u_ta, v_ta, w_ta, &
                                  u_temp, v_temp, w_temp, divu, cellvalue, rhocp, thc, u_ep2, &
