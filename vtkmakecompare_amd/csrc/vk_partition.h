@@ -368,10 +368,10 @@ __device__ inline void pcr_apply(LdsVec<V> ld, LSlot<S> sl, int seg, bool cyclic
   x1 = xs[1];
 }
 
-// Blocks the fast kernel leaves to the general one: `count` entries of
-// `list` (linear block numbers), and `done`, which the general kernel's
-// blocks count up so that the last one can clear both counters for the
-// next sweep on the stream.
+// Lines the fast kernel leaves to the general one: `count` entries of
+// `list`, each a line number la + nla_pad * lb, and `done`, which the
+// general kernel's blocks count up so that the last one can clear both
+// counters for the next sweep on the stream.
 struct LineQueue {
   unsigned* count;
   unsigned* done;
@@ -417,10 +417,13 @@ __device__ inline unsigned xcd_linear(unsigned p, unsigned nb) {
   return x * per + (x < rem ? x : rem) + r;
 }
 
-// General path for one block: elimination in registers, PCR in LDS.
+// General path for one block: elimination in registers, PCR in LDS.  Each
+// thread's line is (la, lb), a valid line; `ok` says whether its solution
+// is stored (threads past the last line load a valid line and store
+// nothing).
 template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
-__device__ inline void line_block_general(const Sys& sys, int la0, int lvalid, int by,
-                                          double* s_a, double* s_b, double* s_c, double* s_d) {
+__device__ inline void line_block_general(const Sys& sys, int la, int lb, bool ok, double* s_a,
+                                          double* s_b, double* s_c, double* s_d) {
   using V = typename Sys::V;
   using Pt = typename Sys::Pt;
   using M = LineMap<S, NT, LANE_LINES>;
@@ -431,9 +434,6 @@ __device__ inline void line_block_general(const Sys& sys, int la0, int lvalid, i
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
   const M mp(tid);
-  const int la = la0 + mp.line, lb = by;
-  const bool ok = sys.valid(la, lb);
-  const int lc = ok ? la : lvalid;
   const int n = sys.n();
   const int lo = (mp.seg * n) / S;
   const int m = ((mp.seg + 1) * n) / S - lo;
@@ -442,18 +442,18 @@ __device__ inline void line_block_general(const Sys& sys, int la0, int lvalid, i
   // through a three-row window (previous, current, next row)
   V d[MMAX];
   double a[MMAX], b[MMAX], c[MMAX];
-  Pt prev = Sys::kNb ? sys.pt(lc, lb, lo - 1) : Pt();
-  Pt cur = sys.pt(lc, lb, lo);
+  Pt prev = Sys::kNb ? sys.pt(la, lb, lo - 1) : Pt();
+  Pt cur = sys.pt(la, lb, lo);
 #pragma unroll
   for (int r = 0; r < MMAX; r++) {
     const int q = lo + (r < m ? r : m - 1);
-    d[r] = sys.rhs(lc, lb, q);
+    d[r] = sys.rhs(la, lb, q);
     Pt next = Pt();
-    if (Sys::kNb) next = sys.pt(lc, lb, r < m - 1 ? q + 1 : lo + m);
-    sys.coef(lc, lb, q, cur, prev, next, a[r], b[r], c[r]);
+    if (Sys::kNb) next = sys.pt(la, lb, r < m - 1 ? q + 1 : lo + m);
+    sys.coef(la, lb, q, cur, prev, next, a[r], b[r], c[r]);
     if (r + 1 < MMAX) {
       prev = cur;
-      cur = Sys::kNb ? next : sys.pt(lc, lb, lo + (r + 1 < m ? r + 1 : m - 1));
+      cur = Sys::kNb ? next : sys.pt(la, lb, lo + (r + 1 < m ? r + 1 : m - 1));
     }
   }
   TabLayout none{};
@@ -499,34 +499,50 @@ __device__ inline void line_block_general(const Sys& sys, int la0, int lvalid, i
   }
 }
 
-// The general solve over items of `item_lines` lines (a multiple of the
-// block's LPB): item (bx, by) holds lines bx * item_lines ... of lb = by,
-// nbx x nby items in all.  Every item when `queued` is false, else the items
-// the fast kernel queued; a block solves an item in item_lines / LPB
-// passes.  The grid strides over the work; with a queue, the last block to
-// finish clears the queue's counters.
+// The general solve of every line, block (bx, by) holding lines
+// la = bx * LPB + (line within block), lb = by of an nbx x nby grid of
+// blocks, in XCD-aware order.
 template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
-__global__ __launch_bounds__(NT) void k_line_general(Sys sys, LineQueue Q, bool queued, int nbx,
-                                                     int nby, int item_lines) {
+__global__ __launch_bounds__(NT) void k_line_general(Sys sys, int nbx, int nby) {
   using V = typename Sys::V;
+  using M = LineMap<S, NT, LANE_LINES>;
   constexpr int LPB = NT / S;
   constexpr int P = 2 * S;
   __shared__ double s_a[LPB * P], s_b[LPB * P], s_c[LPB * P];
   __shared__ double s_d[LPB * P * vwords<V>()];
-  const unsigned items = queued ? __atomic_load_n(Q.count, __ATOMIC_RELAXED)
-                                : (unsigned)nbx * (unsigned)nby;
-  const unsigned subs = (unsigned)(item_lines / LPB);
-  const unsigned total = items * subs;
-  for (unsigned t = blockIdx.x; t < total; t += gridDim.x) {
-    const unsigned it = t / subs, sub = t % subs;
-    const unsigned blk = queued ? Q.list[it] : xcd_linear(it, items);
-    const int first = (int)(blk % (unsigned)nbx) * item_lines;
-    line_block_general<S, MMAX, NT, LANE_LINES, Sys>(sys, first + (int)sub * LPB, first,
-                                                     (int)(blk / (unsigned)nbx), s_a, s_b, s_c,
+  const unsigned blk = xcd_linear(blockIdx.x, gridDim.x);
+  const int bx = (int)(blk % (unsigned)nbx), by = (int)(blk / (unsigned)nbx);
+  const M mp(threadIdx.x);
+  const int la = bx * LPB + mp.line;
+  const bool ok = sys.valid(la, by);
+  line_block_general<S, MMAX, NT, LANE_LINES, Sys>(sys, ok ? la : bx * LPB, by, ok, s_a, s_b,
+                                                   s_c, s_d);
+}
+
+// The general solve of the lines the fast kernel queued, LPB at a time; the
+// grid strides over the queue and the last block to finish clears the
+// queue's counters.
+template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
+__global__ __launch_bounds__(NT) void k_line_general_q(Sys sys, LineQueue Q, unsigned nla_pad) {
+  using V = typename Sys::V;
+  using M = LineMap<S, NT, LANE_LINES>;
+  constexpr int LPB = NT / S;
+  constexpr int P = 2 * S;
+  __shared__ double s_a[LPB * P], s_b[LPB * P], s_c[LPB * P];
+  __shared__ double s_d[LPB * P * vwords<V>()];
+  const unsigned count = __atomic_load_n(Q.count, __ATOMIC_RELAXED);
+  const unsigned groups = (count + LPB - 1) / LPB;
+  const M mp(threadIdx.x);
+  for (unsigned gi = blockIdx.x; gi < groups; gi += gridDim.x) {
+    const unsigned e = gi * LPB + (unsigned)mp.line;
+    const bool ok = e < count;
+    const unsigned id = Q.list[ok ? e : gi * LPB];
+    line_block_general<S, MMAX, NT, LANE_LINES, Sys>(sys, (int)(id % nla_pad),
+                                                     (int)(id / nla_pad), ok, s_a, s_b, s_c,
                                                      s_d);
     __syncthreads();
   }
-  if (queued && threadIdx.x == 0) {
+  if (threadIdx.x == 0) {
     __threadfence();
     const unsigned prev = atomicAdd(Q.done, 1u);
     if (prev == gridDim.x - 1) {
@@ -538,11 +554,11 @@ __global__ __launch_bounds__(NT) void k_line_general(Sys sys, LineQueue Q, bool 
 
 // The recorded-multiplier path.  The reference line's table (validated for
 // this call by k_line_factor) is staged in LDS from offset o_f on, together
-// with the block's data loads.  A block whose rows all have the reference
+// with the block's data loads.  A line whose rows all have the reference
 // inputs applies the recorded multipliers: its rows then have the
 // reference coefficients bit for bit, so it performs the same operations on
-// d as the general path.  Any other block queues itself for
-// k_line_general and writes nothing.
+// d as the general path.  Any other line is queued for k_line_general_q and
+// not written here.
 template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
 __global__ __launch_bounds__(NT) void k_line_fast(Sys sys, const double* __restrict__ tab,
                                                   TabLayout L, LineQueue Q, int nbx) {
@@ -550,6 +566,7 @@ __global__ __launch_bounds__(NT) void k_line_fast(Sys sys, const double* __restr
   using M = LineMap<S, NT, LANE_LINES>;
   constexpr int P = M::P;
   extern __shared__ double smem[];
+  __shared__ int lclean[M::LPB];
   const LdsTab<S, MMAX> T(L.steps);
   double* st = smem;
   double* sd = smem + T.total;
@@ -565,6 +582,7 @@ __global__ __launch_bounds__(NT) void k_line_fast(Sys sys, const double* __restr
   const int n = sys.n();
   const int lo = (seg * n) / S;
   const int m = ((seg + 1) * n) / S - lo;
+  if (seg == 0) lclean[mp.line] = 1;
 
   V d[MMAX];
   int clean = 1;
@@ -591,10 +609,8 @@ __global__ __launch_bounds__(NT) void k_line_fast(Sys sys, const double* __restr
       if (u < T.total) st[u] = v[k];
     }
   }
-  if (!__syncthreads_and(clean)) {
-    if (threadIdx.x == 0) Q.list[atomicAdd(Q.count, 1u)] = blk;
-    return;
-  }
+  __syncthreads();
+  if (!clean) lclean[mp.line] = 0;
   // the segment elimination's operations on d, with the recorded multipliers
 #pragma unroll
   for (int r = 0; r < MMAX; r++) {
@@ -620,9 +636,12 @@ __global__ __launch_bounds__(NT) void k_line_fast(Sys sys, const double* __restr
   ld.set(sl.at(2 * seg), d[0]);
   ld.set(sl.at(2 * seg + 1), dlast);
   __syncthreads();
+  const bool line_clean = lclean[mp.line] != 0;
+  if (!line_clean && seg == 0 && ok)
+    Q.list[atomicAdd(Q.count, 1u)] = (unsigned)la + (unsigned)(nbx * M::LPB) * (unsigned)lb;
   V xf, xl;
   pcr_apply<S, MMAX, V>(ld, sl, seg, Sys::kCyclic, xf, xl, st, T);
-  if (ok) {
+  if (ok && line_clean) {
 #pragma unroll
     for (int r = 0; r < MMAX; r++) {
       if (r < m) {
@@ -697,8 +716,8 @@ __global__ __launch_bounds__(S) void k_line_factor(Sys sys, double* tab, TabLayo
 }
 
 // The queue shared by the sweeps on the library stream, sized for at
-// least `nblocks` entries.
-LineQueue line_queue(unsigned nblocks, hipStream_t s);
+// least `nlines` entries.
+LineQueue line_queue(unsigned nlines, hipStream_t s);
 
 // Host cache of reference-line tables, keyed by the system's description.
 // A stale table is harmless: blocks whose rows do not match its recorded
@@ -708,12 +727,10 @@ double* line_table(const void* key, size_t keylen, long total, bool* fresh);
 // One sweep: the general kernel uses SG segments of at most MG rows; the
 // fast kernel SF segments of at most MF rows (fewer, longer segments: its
 // rows need one register each, and fewer segments mean a smaller staged
-// table and fewer PCR steps).  Both use NT-thread blocks; an item of the
-// fast kernel (LPB_F lines) is LPB_F / LPB_G blocks of the general one.
+// table and fewer PCR steps).  Both use NT-thread blocks.
 template <bool LANE_LINES, int SG, int MG, int SF, int MF, int NT, typename Sys>
 inline int line_sweep(const Sys& sys, int n, int nla, int nlb, hipStream_t s) {
   constexpr int LPB_G = NT / SG, LPB_F = NT / SF;
-  static_assert(LPB_F % LPB_G == 0, "a fast item is whole general blocks");
   double* tab = nullptr;
   bool fresh = false;
   TabLayout L = tab_layout(n, SF, Sys::kCyclic);
@@ -726,26 +743,25 @@ inline int line_sweep(const Sys& sys, int n, int nla, int nlb, hipStream_t s) {
   if (tab) {
     const int nbx = (nla + LPB_F - 1) / LPB_F;
     const unsigned nb = (unsigned)nbx * (unsigned)nlb;
-    LineQueue Q = line_queue(nb, s);
+    const unsigned nla_pad = (unsigned)nbx * (unsigned)LPB_F;
+    LineQueue Q = line_queue(nla_pad * (unsigned)nlb, s);
     if (!Q.list) return -2;
     k_line_factor<SF, MF, Sys><<<1, SF, 0, s>>>(sys, tab, L, fresh ? 1 : 0);
     k_line_fast<SF, MF, NT, LANE_LINES, Sys><<<nb, NT, lds, s>>>(sys, tab, L, Q, nbx);
     const unsigned ngen = nb < 1024u ? nb : 1024u;
-    k_line_general<SG, MG, NT, LANE_LINES, Sys><<<ngen, NT, 0, s>>>(sys, Q, true, nbx, nlb,
-                                                                   LPB_F);
+    k_line_general_q<SG, MG, NT, LANE_LINES, Sys><<<ngen, NT, 0, s>>>(sys, Q, nla_pad);
   } else {
     const int nbx = (nla + LPB_G - 1) / LPB_G;
-    LineQueue Q{nullptr, nullptr, nullptr};
     k_line_general<SG, MG, NT, LANE_LINES, Sys><<<(unsigned)nbx * (unsigned)nlb, NT, 0, s>>>(
-        sys, Q, false, nbx, nlb, LPB_G);
+        sys, nbx, nlb);
   }
   return 0;
 }
 
 // Launch helper: the general kernel's segment count SG is the smallest
-// power of two (at most 64) that leaves at most 8 rows per segment (16 when
-// 64 segments are not enough), so its reduced system has 2 SG rows; the
-// fast kernel takes half as many segments, of at most 12 or 16 rows.  A
+// power of two (at most 64) that leaves at most 8 rows per segment (12 or
+// 16 when 64 segments are not enough), so its reduced system has 2 SG rows;
+// the fast kernel takes half as many segments, of at most 12 or 16 rows.  A
 // sweep with a reference table is three launches: validate the table, the
 // fast kernel over every item, the general kernel over the queued ones.
 template <bool LANE_LINES, typename Sys>
@@ -757,7 +773,12 @@ inline int launch_line_solve(const Sys& sys, int n, int nla, int nlb, hipStream_
   const bool f12 = S > 1 && (n + S / 2 - 1) / (S / 2) <= 12;
 #define VK_SW(SG, MG, SF, MF) \
   return line_sweep<LANE_LINES, SG, MG, SF, MF, 512>(sys, n, nla, nlb, s)
-  if (mmax > 8) VK_SW(64, 16, 64, 16);
+  if (mmax > 8) {
+    // long lines: 64 segments of up to 12 rows keep the fast kernel's
+    // staged table within 64 KB of LDS (768-row lines), 16 rows beyond
+    if ((n + 63) / 64 <= 12) VK_SW(64, 12, 64, 12);
+    VK_SW(64, 16, 64, 16);
+  }
   if (!LANE_LINES) {
     // lines contiguous in memory: a thread's rows are consecutive words, and
     // short segments keep the lanes' loads close together

@@ -14,10 +14,10 @@ namespace vk {
 
 // the fast kernel's queue: two counters, each on its own 128-byte line,
 // then the list; cleared when allocated, afterwards by the general kernel
-LineQueue line_queue(unsigned nblocks, hipStream_t s) {
+LineQueue line_queue(unsigned nlines, hipStream_t s) {
   bool grown = false;
   unsigned* p = static_cast<unsigned*>(
-      workspace(kWorkQueue, sizeof(unsigned) * (64 + (size_t)nblocks), &grown));
+      workspace(kWorkQueue, sizeof(unsigned) * (64 + (size_t)nlines), &grown));
   if (!p) return LineQueue{nullptr, nullptr, nullptr};
   if (grown && hipMemsetAsync(p, 0, 64 * sizeof(unsigned), s) != hipSuccess)
     return LineQueue{nullptr, nullptr, nullptr};
