@@ -31,19 +31,14 @@ __global__ void k_mkp_coefficients(Geom g, Metrics m, double xl, double zl, doub
   }
 }
 
-// mkp.f:141-217 over i,j,k = 0..n: gradient of PHI at the faces, the
+// mkp.f:141-217 at one (i,j,k) of 0..n: gradient of PHI at the faces, the
 // corrected velocity, and on the interior p = PHI, the diagnostic
 // u_ep2 = -dt grad(p) + u_ta (read before u_ta is replaced; p's halo at
 // nx+1, ny+1, nz+1 is the caller's, as in the reference), u_ta = dt grad,
 // and divu of the corrected velocity.
-__global__ void k_project(Geom g, RMetrics r, const double* phi, const double* ut,
-                          const double* vt, const double* wt, double dt, MkpOut o) {
-  int bx, by, bz;
-  xcd_block(bx, by, bz);
-  int i = bx * blockDim.x + threadIdx.x;
-  int j = by * blockDim.y + threadIdx.y;
-  int k = bz;
-  if (i > g.nx || j > g.ny) return;
+__device__ inline void project_point(const Geom& g, const RMetrics& r, const double* phi,
+                                     const double* ut, const double* vt, const double* wt,
+                                     double dt, const MkpOut& o, int i, int j, int k) {
   long long c = g.at(i, j, k);
   const long long sy = g.sy, sz = g.sz;
   double ph = phi[c];
@@ -75,6 +70,44 @@ __global__ void k_project(Geom g, RMetrics r, const double* phi, const double* u
   o.w_ta[c] = dt * gz;
 }
 
+// the interior points, in full 64 x 4 tiles along (i, j)
+__global__ void k_project(Geom g, RMetrics r, const double* phi, const double* ut,
+                          const double* vt, const double* wt, double dt, MkpOut o) {
+  int bx, by, bz;
+  xcd_block(bx, by, bz);
+  const int i = 1 + bx * blockDim.x + threadIdx.x;
+  const int j = 1 + by * blockDim.y + threadIdx.y;
+  if (i > g.nx || j > g.ny) return;
+  project_point(g, r, phi, ut, vt, wt, dt, o, i, j, 1 + bz);
+}
+
+// the points of 0..n with an index 0: the faces i = 0, j = 0, k = 0
+template <typename Fn>
+__device__ inline void zero_faces(const Geom& g, long long q, Fn&& fn) {
+  const long long a = (long long)(g.ny + 1) * (g.nz + 1);         // i = 0
+  const long long b = (long long)g.nx * (g.nz + 1);               // j = 0, i >= 1
+  const long long c = (long long)g.nx * g.ny;                     // k = 0, i, j >= 1
+  if (q < a) {
+    fn(0, (int)(q % (g.ny + 1)), (int)(q / (g.ny + 1)));
+  } else if (q < a + b) {
+    q -= a;
+    fn(1 + (int)(q % g.nx), 0, (int)(q / g.nx));
+  } else if (q < a + b + c) {
+    q -= a + b;
+    fn(1 + (int)(q % g.nx), 1 + (int)(q / g.nx), 0);
+  }
+}
+inline long long zero_face_count(const Geom& g) {
+  return (long long)(g.ny + 1) * (g.nz + 1) + (long long)g.nx * (g.nz + 1) +
+         (long long)g.nx * g.ny;
+}
+
+__global__ void k_project_faces(Geom g, RMetrics r, const double* phi, const double* ut,
+                                const double* vt, const double* wt, double dt, MkpOut o) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  zero_faces(g, q, [&](int i, int j, int k) { project_point(g, r, phi, ut, vt, wt, dt, o, i, j, k); });
+}
+
 int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const double* v_temp,
                const double* w_temp, double dt, double xl, double zl, const MkpOut& o,
                hipStream_t s) {
@@ -102,26 +135,24 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
   RMetrics r;
   rc = make_rmetrics(g, m, r, s);
   if (rc) return rc;
-  dim3 blk(64, 4, 1), grd((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
   ProfScope ps("project", s);
-  k_project<<<grd, blk, 0, s>>>(g, r, phi, u_temp, v_temp, w_temp, dt, o);
+  k_project<<<interior_grid(g), interior_block(), 0, s>>>(g, r, phi, u_temp, v_temp, w_temp, dt,
+                                                         o);
+  const long long nf = zero_face_count(g);
+  k_project_faces<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(g, r, phi, u_temp, v_temp,
+                                                               w_temp, dt, o);
   return VK_OK;
 }
 
 // Mixed-precision projection: phi = phi1 + delta from two float Poisson
 // solutions, summed in FP64 at every point read; the corrected velocity,
 // p and divu are FP64, u_ta is stored in float.  Same index ranges and
-// formulas as k_project (mkp.f:141-217); the u_ep2 diagnostic is not
+// formulas as project_point (mkp.f:141-217); the u_ep2 diagnostic is not
 // produced.
-__global__ void k_project_mixed(Geom g, RMetrics r, const float* phi1, const float* dphi,
-                                const double* ut, const double* vt, const double* wt, double dt,
-                                MkpOutMixed o) {
-  int bx, by, bz;
-  xcd_block(bx, by, bz);
-  int i = bx * blockDim.x + threadIdx.x;
-  int j = by * blockDim.y + threadIdx.y;
-  int k = bz;
-  if (i > g.nx || j > g.ny) return;
+__device__ inline void project_mixed_point(const Geom& g, const RMetrics& r, const float* phi1,
+                                           const float* dphi, const double* ut,
+                                           const double* vt, const double* wt, double dt,
+                                           const MkpOutMixed& o, int i, int j, int k) {
   const long long c = g.at(i, j, k);
   const long long sy = g.sy, sz = g.sz;
   auto phi = [&](long long q) { return (double)phi1[q] + (double)dphi[q]; };
@@ -143,6 +174,26 @@ __global__ void k_project_mixed(Geom g, RMetrics r, const float* phi1, const flo
   o.u_ta[c] = (float)(dt * gx);
   o.v_ta[c] = (float)(dt * gy);
   o.w_ta[c] = (float)(dt * gz);
+}
+
+__global__ void k_project_mixed(Geom g, RMetrics r, const float* phi1, const float* dphi,
+                                const double* ut, const double* vt, const double* wt, double dt,
+                                MkpOutMixed o) {
+  int bx, by, bz;
+  xcd_block(bx, by, bz);
+  const int i = 1 + bx * blockDim.x + threadIdx.x;
+  const int j = 1 + by * blockDim.y + threadIdx.y;
+  if (i > g.nx || j > g.ny) return;
+  project_mixed_point(g, r, phi1, dphi, ut, vt, wt, dt, o, i, j, 1 + bz);
+}
+
+__global__ void k_project_mixed_faces(Geom g, RMetrics r, const float* phi1, const float* dphi,
+                                      const double* ut, const double* vt, const double* wt,
+                                      double dt, MkpOutMixed o) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  zero_faces(g, q, [&](int i, int j, int k) {
+    project_mixed_point(g, r, phi1, dphi, ut, vt, wt, dt, o, i, j, k);
+  });
 }
 
 // The projection of the mixed-precision step.  Pass 1 solves
@@ -189,9 +240,12 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
   rc = launch_s2p_inplace<float>(g, phit, dphi, s, "cfft_z_inv2", "irfft_x2");
   if (rc) return rc;
   launch_halo_neumann<float>(g, dphi, s);
-  dim3 blk(64, 4, 1), grd((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
   ProfScope ps("project_mixed", s);
-  k_project_mixed<<<grd, blk, 0, s>>>(g, r, phi1, dphi, u_temp, v_temp, w_temp, dt, o);
+  k_project_mixed<<<interior_grid(g), interior_block(), 0, s>>>(g, r, phi1, dphi, u_temp, v_temp,
+                                                               w_temp, dt, o);
+  const long long nf = zero_face_count(g);
+  k_project_mixed_faces<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(g, r, phi1, dphi, u_temp,
+                                                                     v_temp, w_temp, dt, o);
   return VK_OK;
 }
 
