@@ -68,7 +68,6 @@ def kernel_bytes(name: str, g, with_ep2: bool = False, mixed: bool = False) -> f
             "cfft_z_fwd": 2 * S, "cfft_z_inv": 2 * S, "line_spectral": 2 * S,
             "irfft_x": S + f,
             "project": (18 if with_ep2 else 12) * f,
-            "copy_interior": 2 * f,
         }
         return table.get(name, 0.0)
     h = 4.0 * N  # one float field
@@ -82,13 +81,13 @@ def kernel_bytes(name: str, g, with_ep2: bool = False, mixed: bool = False) -> f
         "line_x_face": 2 * f + h,
         # temperature lines: all float (rhs, cellvalue, rhoCp, thc, solution)
         "line_y_t": 5 * h, "line_x_t": 5 * h, "line_z_t": 5 * h,
-        "rfft_x_div": 3 * f + S, "rfft_x_resid": 3 * f + h + S,
+        # pass 1 keeps the FP64 right-hand side for the residual of pass 2
+        "rfft_x_div": 3 * f + f + S, "rfft_x_resid": f + h + S,
         "cfft_z_fwd": 2 * S, "cfft_z_fwd2": 2 * S, "cfft_z_inv": 2 * S, "cfft_z_inv2": 2 * S,
         "line_spectral": 2 * S,
         "irfft_x": S + h, "irfft_x2": S + h,
         # phi1, delta, u_temp x3 in; u, v, w, p, divu (double), 3 u_ta (float) out
         "project_mixed": 2 * h + 3 * f + 5 * f + 3 * h,
-        "copy_interior": 2 * h,
     }
     return table.get(name, 0.0)
 
@@ -97,7 +96,7 @@ STEP_LAUNCHES = {"explicit_fused": 1, "explicit_mixed": 1, "line_y_face": 2, "li
                  "line_z_face": 3, "line_x_face": 3, "line_y_t": 1, "line_x_t": 1,
                  "line_z_t": 1, "rfft_x_div": 1, "rfft_x_resid": 1, "cfft_z_fwd": 1,
                  "cfft_z_fwd2": 1, "cfft_z_inv": 1, "cfft_z_inv2": 1, "irfft_x": 1,
-                 "irfft_x2": 1, "project": 1, "project_mixed": 1, "copy_interior": 1}
+                 "irfft_x2": 1, "project": 1, "project_mixed": 1}
 
 
 def step_bytes(g, with_ep2: bool = False, mixed: bool = False) -> float:

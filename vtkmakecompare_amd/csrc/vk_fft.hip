@@ -222,12 +222,25 @@ struct SampleDiv {
            dt;
   }
 };
-// div(u_temp)/dt - L phi for a float solution phi with filled halos, L
-// the discrete divergence of the discrete gradient that the projection
-// applies (mkp.f:141-151, 209-217): the divergence the projected velocity
-// would have, over dt.  All FP64.
-struct SampleResid {
+// div(u_temp)/dt, also stored in FP64 at its interior point (every
+// interior point is sampled exactly once by the forward x pass)
+struct SampleDivStore {
   SampleDiv div;
+  double* out;
+  __device__ double operator()(const Geom& g, int i, int j, int k) const {
+    const double v = div(g, i, j, k);
+    out[g.at((i == 0) ? g.nx : i, j, (k == 0) ? g.nz : k)] = v;
+    return v;
+  }
+};
+
+// rhs - L phi for a float solution phi with filled halos and the stored
+// FP64 right-hand side rhs = div(u_temp)/dt, L the discrete divergence of
+// the discrete gradient that the projection applies (mkp.f:141-151,
+// 209-217): the divergence the projected velocity would have, over dt.
+// All FP64.
+struct SampleResid {
+  const double* rhs;
   const float* phi;
   const double *rdx12, *rdy12, *rdz12, *rdx, *rdy, *rdz;
   __device__ double operator()(const Geom& g, int i, int j, int k) const {
@@ -240,7 +253,7 @@ struct SampleResid {
                        (c - (double)phi[o - g.sy]) * rdy12[j - 1]) * rdy[j];
     const double lz = (((double)phi[o + g.sz] - c) * rdz12[kk] -
                        (c - (double)phi[o - g.sz]) * rdz12[kk - 1]) * rdz[kk];
-    return div(g, i, j, k) - (lx + ly + lz);
+    return rhs[o] - (lx + ly + lz);
   }
 };
 
@@ -447,11 +460,15 @@ template int launch_p2s_div<double>(const Geom&, const Metrics&, const double*, 
 template int launch_p2s_div<float>(const Geom&, const Metrics&, const double*, const double*,
                                    const double*, double, float*, hipStream_t);
 
-int launch_p2s_resid(const Geom& g, const Metrics& m, const RMetrics& r, const double* u,
-                     const double* v, const double* w, double dt, const float* phi,
+int launch_p2s_div_store(const Geom& g, const Metrics& m, const double* u, const double* v,
+                         const double* w, double dt, double* rhs, float* phit, hipStream_t s) {
+  return launch_forward<float>(g, SampleDivStore{SampleDiv{u, v, w, m.dx, m.dy, m.dz, dt}, rhs},
+                               phit, "rfft_x_div", "cfft_z_fwd", s);
+}
+
+int launch_p2s_resid(const Geom& g, const RMetrics& r, const double* rhs, const float* phi,
                      float* phit, hipStream_t s) {
-  SampleResid smp{SampleDiv{u, v, w, m.dx, m.dy, m.dz, dt}, phi, r.rdx12, r.rdy12, r.rdz12,
-                  r.rdx, r.rdy, r.rdz};
+  SampleResid smp{rhs, phi, r.rdx12, r.rdy12, r.rdz12, r.rdx, r.rdy, r.rdz};
   return launch_forward<float>(g, smp, phit, "rfft_x_resid", "cfft_z_fwd2", s);
 }
 

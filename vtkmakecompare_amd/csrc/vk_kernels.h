@@ -39,7 +39,7 @@ int launch_adi_velocity(const Geom& g, const Metrics& m, const RMetrics& r, doub
 template <typename TT>
 int launch_adi_temperature(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
                            const TT* cv, const TT* rc, const TT* thc, double beta, double dteta,
-                           hipStream_t s);
+                           hipStream_t s, TT* out = nullptr);
 // one sweep: velocity dir 0, 1, 2 = y, z, x; temperature dir 0, 1, 2 = y, x, z
 template <typename TC>
 int launch_velocity_sweep(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
@@ -48,7 +48,7 @@ int launch_velocity_sweep(const Geom& g, const Metrics& m, const RMetrics& r, do
 template <typename TT>
 int launch_temperature_sweep(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
                              const TT* cv, const TT* rc, const TT* thc, int dir, double beta,
-                             double dteta, hipStream_t s);
+                             double dteta, hipStream_t s, TT* out = nullptr);
 // the transposed sweeps (vk_sweep_adj.hip), in place on the adjoint field
 int launch_velocity_sweep_adj(const Geom& g, const Metrics& m, const RMetrics& r, double* lam,
                               const double* cv, int comp, int dir, double beta, double dteta,
@@ -67,9 +67,11 @@ int launch_p2s(const Geom& g, const double* fp, double* phit, hipStream_t s);
 template <typename R>
 int launch_p2s_div(const Geom& g, const Metrics& m, const double* u, const double* v,
                    const double* w, double dt, R* phit, hipStream_t s);
-// spectrum of div(u_temp)/dt - L phi for a float phi with filled halos
-int launch_p2s_resid(const Geom& g, const Metrics& m, const RMetrics& r, const double* u,
-                     const double* v, const double* w, double dt, const float* phi,
+// float spectrum of div(u_temp)/dt, which is also stored in FP64 in `rhs`
+int launch_p2s_div_store(const Geom& g, const Metrics& m, const double* u, const double* v,
+                         const double* w, double dt, double* rhs, float* phit, hipStream_t s);
+// float spectrum of rhs - L phi for a float phi with filled halos
+int launch_p2s_resid(const Geom& g, const RMetrics& r, const double* rhs, const float* phi,
                      float* phit, hipStream_t s);
 template <typename R>
 int launch_s2p_inplace(const Geom& g, R* phit, R* fp, hipStream_t s, const char* zname,

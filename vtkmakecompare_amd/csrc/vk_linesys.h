@@ -25,8 +25,10 @@ namespace vk {
 // which is what makes the recorded-multiplier path of vk_partition.h
 // bit-identical to the general one.
 
+// beta / alpha with alpha = 1 + dt eta cv; exactly beta where cv is 0 (the
+// divisor is exactly 1), so no branch is needed
 __device__ inline double beta_over_alpha(double beta, double dteta, double cv) {
-  return cv == 0.0 ? beta : beta / (dteta * cv + 1.0);
+  return beta / (dteta * cv + 1.0);
 }
 
 // key of a reference-line table: system kind, size, segmenting, the
@@ -180,6 +182,7 @@ struct SysYT {
   Metrics m;
   RMetrics r;
   double beta, dteta;
+  TF* out = nullptr;  // where the solution goes (f itself when null)
   __device__ int n() const { return g.ny + 2; }
   __device__ bool valid(int la, int lb) const { return la < g.nx; }
   __device__ void coef(int q, double cvv, double l0, double lm, double lp, double rcv, double& a,
@@ -222,7 +225,7 @@ struct SysYT {
   __device__ unsigned addr(int la, int lb, int q) const { return g.at32(la + 1, q, lb + 1); }
   __device__ bool in_row(int q) const { return q >= 1 && q <= g.ny; }
   __device__ void store(int la, int lb, int q, double x) const {
-    if (q >= 1 && q <= g.ny) f[g.at32(la + 1, q, lb + 1)] = static_cast<TF>(x);
+    if (q >= 1 && q <= g.ny) (out ? out : f)[g.at32(la + 1, q, lb + 1)] = static_cast<TF>(x);
   }
 };
 
@@ -497,9 +500,9 @@ int with_velocity_system(const Geom& g, const Metrics& m, const RMetrics& r, dou
 template <typename TT, typename Fn>
 int with_temperature_system(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
                             const TT* cv, const TT* rc_, const TT* thc, int dir, double beta,
-                            double dteta, Fn&& fn) {
+                            double dteta, Fn&& fn, TT* out = nullptr) {
   if (dir == 0) {
-    SysYT<TT, TT> sy{g, f, cv, rc_, thc, m, r, beta, dteta};
+    SysYT<TT, TT> sy{g, f, cv, rc_, thc, m, r, beta, dteta, out};
     return fn(sy, g.ny + 2, g.nx, g.nz, std::true_type(), "line_y_t");
   }
   if (dir == 1) {
@@ -544,8 +547,8 @@ int launch_velocity_sweep(const Geom& g, const Metrics& m, const RMetrics& r, do
 template <typename TT>
 int launch_temperature_sweep(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
                              const TT* cv, const TT* rc_, const TT* thc, int dir, double beta,
-                             double dteta, hipStream_t s) {
-  return with_temperature_system(g, m, r, f, cv, rc_, thc, dir, beta, dteta, SolveFwd{s});
+                             double dteta, hipStream_t s, TT* out) {
+  return with_temperature_system(g, m, r, f, cv, rc_, thc, dir, beta, dteta, SolveFwd{s}, out);
 }
 
 // solve_u_temp body after the penalisation divide: y, z, x sweeps of one
@@ -560,13 +563,16 @@ int launch_adi_velocity(const Geom& g, const Metrics& m, const RMetrics& r, doub
   return VK_OK;
 }
 
-// solve_t: y, x, z sweeps
+// solve_t: y, x, z sweeps; with `out` the y sweep writes the interior of
+// `out` and the x and z sweeps work there (f is then only read)
 template <typename TT>
 int launch_adi_temperature(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
                            const TT* cv, const TT* rc_, const TT* thc, double beta, double dteta,
-                           hipStream_t s) {
+                           hipStream_t s, TT* out) {
   for (int dir = 0; dir < 3; dir++) {
-    int rc = launch_temperature_sweep<TT>(g, m, r, f, cv, rc_, thc, dir, beta, dteta, s);
+    TT* src = (dir == 0 || !out) ? f : out;
+    int rc = launch_temperature_sweep<TT>(g, m, r, src, cv, rc_, thc, dir, beta, dteta, s,
+                                          dir == 0 ? out : nullptr);
     if (rc) return rc;
   }
   return VK_OK;

@@ -82,10 +82,10 @@ struct ExplicitMixedArgs {
   double dtd, etad;
 };
 
-// (x) / (1 + eta cv dt), exactly x where cv is 0
+// x / (1 + eta cv dt): exactly x where cv is 0 (the divisor is exactly 1)
 template <typename R>
 __device__ inline R pen_div_m(R x, R eta, R cv, R dt) {
-  return cv == R(0) ? x : x / (R(1) + eta * cv * dt);
+  return x / (R(1) + eta * cv * dt);
 }
 
 // The explicit phase of one step: the same terms as k_explicit_fused
@@ -139,15 +139,6 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
     x.t_pre[o] = vt;
     x.ut[o] = pen_div_m<float>(x.t[o] + ut, eta, x.cv[g.cv(i, j, k, 1)], dt);
   }
-}
-
-__global__ void k_copy_interior_f(Geom g, const float* src, float* dst) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 1 + blockIdx.z;
-  if (i > g.nx || j > g.ny) return;
-  const long long o = g.at(i, j, k);
-  dst[o] = src[o];
 }
 
 }  // namespace vk
@@ -229,12 +220,9 @@ extern "C" int vk_time_step_mixed(double* u, double* v, double* w, double* p, fl
     rc = launch_adi_velocity<float>(g, m, r, comps[c], dcv, c, beta_u, dteta, s);
     if (rc) return rc;
   }
-  rc = launch_adi_temperature<float>(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s);
+  // T = T work after the sweeps: the y sweep writes T's interior
+  rc = launch_adi_temperature<float>(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s, dtt);
   if (rc) return rc;
-  {
-    ProfScope ps("copy_interior", s);
-    k_copy_interior_f<<<interior_grid(g), interior_block(), 0, s>>>(g, twork, dtt);
-  }
   launch_set_bc<float>(g, dut, dvt, dwt, dp, dtt, s);
   MkpOutMixed o{dp, duta, dvta, dwta, ddiv, du, dv, dw};
   rc = launch_mkp_mixed(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s);

@@ -198,11 +198,11 @@ __global__ void k_project_mixed_faces(Geom g, RMetrics r, const float* phi1, con
 
 // The projection of the mixed-precision step.  Pass 1 solves
 // L phi1 = div(u_temp)/dt with float transforms and spectra (the right-hand
-// side is formed in FP64, every tridiagonal elimination, SP0B's march of
-// the zero mode included, runs in FP64 arithmetic).  Pass 2 is one
-// iterative-refinement step: the residual div(u_temp)/dt - L phi1 is
-// formed in FP64 from the FP64 velocity and the stored phi1, and solved
-// the same way for delta.  The projection uses phi1 + delta in FP64.
+// side is formed, and kept, in FP64; every tridiagonal elimination, SP0B's
+// march of the zero mode included, runs in FP64 arithmetic).  Pass 2 is
+// one iterative-refinement step: the residual div(u_temp)/dt - L phi1 is
+// formed in FP64 from the kept right-hand side and the stored phi1, and
+// solved the same way for delta.  The projection uses phi1 + delta in FP64.
 int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
                      const double* v_temp, const double* w_temp, double dt, double xl, double zl,
                      const MkpOutMixed& o, hipStream_t s) {
@@ -212,7 +212,8 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
   float* phit = static_cast<float*>(workspace(kWorkSpec, 2 * g.spec_cells() * sizeof(float)));
   float* phi1 = static_cast<float*>(workspace(kWorkPhi, g.cells() * sizeof(float)));
   float* dphi = static_cast<float*>(workspace(kWorkPhi2, g.cells() * sizeof(float)));
-  if (!coef || !phit || !phi1 || !dphi) return VK_ERR_HIP;
+  double* rhs = static_cast<double*>(workspace(kWorkRhs, g.cells() * sizeof(double)));
+  if (!coef || !phit || !phi1 || !dphi || !rhs) return VK_ERR_HIP;
   double* kxz = coef;
   double* ac = coef + (size_t)(nh + 1) * (g.nz + 1);
   {
@@ -223,7 +224,7 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
   int rc = make_rmetrics(g, m, r, s);
   if (rc) return rc;
   // pass 1
-  rc = launch_p2s_div<float>(g, m, u_temp, v_temp, w_temp, dt, phit, s);
+  rc = launch_p2s_div_store(g, m, u_temp, v_temp, w_temp, dt, rhs, phit, s);
   if (rc) return rc;
   rc = launch_spn<float>(g, phit, ac, kxz, s);
   if (rc) return rc;
@@ -232,7 +233,7 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
   if (rc) return rc;
   launch_halo_neumann<float>(g, phi1, s);
   // pass 2: refinement on the FP64 residual
-  rc = launch_p2s_resid(g, m, r, u_temp, v_temp, w_temp, dt, phi1, phit, s);
+  rc = launch_p2s_resid(g, r, rhs, phi1, phit, s);
   if (rc) return rc;
   rc = launch_spn<float>(g, phit, ac, kxz, s);
   if (rc) return rc;

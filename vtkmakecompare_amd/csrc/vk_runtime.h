@@ -41,6 +41,7 @@ enum WorkSlot {
   kWorkQueue,
   kWorkPhi2,
   kWorkRecipF,
+  kWorkRhs,
   kWorkSlots
 };
 // `grown` (optional) reports whether this call allocated the slot anew.

@@ -21,9 +21,10 @@
 
 namespace vk {
 
-// x / (1 + eta cv dt), exactly x where cv is 0
+// x / (1 + eta cv dt); where cv is 0 the divisor is exactly 1 and the
+// quotient exactly x, so no branch is needed to keep fluid cells exact
 __device__ inline double pen_div(double x, double eta, double cv, double dt) {
-  return cv == 0.0 ? x : x / (1.0 + eta * cv * dt);
+  return x / (1.0 + eta * cv * dt);
 }
 
 __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetrics r,
@@ -77,15 +78,6 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
     x.t_pre[o] = vt;
     x.ut[o] = pen_div(x.t[o] + ut, eta, x.cv[g.cv(i, j, k, 1)], dt);
   }
-}
-
-__global__ void k_copy_interior_step(Geom g, const double* src, double* dst) {
-  int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  int k = 1 + blockIdx.z;
-  if (i > g.nx || j > g.ny) return;
-  long long o = g.at(i, j, k);
-  dst[o] = src[o];
 }
 
 void launch_explicit_fused(const Geom& g, const Metrics& m, const RMetrics& r,
@@ -183,12 +175,10 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
     if (lrc) return lrc;
   }
   {
-    int lrc = launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s);
+    // T = T work after the sweeps (nse_dominik.f:429-437): the y sweep
+    // writes T's interior and the x and z sweeps run there
+    int lrc = launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s, dtt);
     if (lrc) return lrc;
-  }
-  {
-    ProfScope ps("copy_interior", s);
-    k_copy_interior_step<<<interior_grid(g), interior_block(), 0, s>>>(g, twork, dtt);
   }
   launch_set_bc(g, dut, dvt, dwt, dp, dtt, s);
   MkpOut o{dp, duta, dvta, dwta, de1, de2, de3, ddiv, du, dv, dw};

@@ -5,9 +5,9 @@ namespace vk {
 
 template int launch_temperature_sweep<float>(const Geom&, const Metrics&, const RMetrics&,
                                              float*, const float*, const float*, const float*,
-                                             int, double, double, hipStream_t);
+                                             int, double, double, hipStream_t, float*);
 template int launch_adi_temperature<float>(const Geom&, const Metrics&, const RMetrics&, float*,
                                            const float*, const float*, const float*, double,
-                                           double, hipStream_t);
+                                           double, hipStream_t, float*);
 
 }  // namespace vk

@@ -5,9 +5,11 @@ namespace vk {
 
 template int launch_temperature_sweep<double>(const Geom&, const Metrics&, const RMetrics&,
                                               double*, const double*, const double*,
-                                              const double*, int, double, double, hipStream_t);
+                                              const double*, int, double, double, hipStream_t,
+                                              double*);
 template int launch_adi_temperature<double>(const Geom&, const Metrics&, const RMetrics&,
                                             double*, const double*, const double*,
-                                            const double*, double, double, hipStream_t);
+                                            const double*, double, double, hipStream_t,
+                                            double*);
 
 }  // namespace vk
