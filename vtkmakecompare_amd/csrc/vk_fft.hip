@@ -211,15 +211,17 @@ struct SampleField {
     return fp[g.at(i, j, k)];
   }
 };
+// (mkp.f:88-99 divides by the metrics and by dt; multiplying by their
+// reciprocals agrees to rounding)
 struct SampleDiv {
-  const double *u, *v, *w, *dx, *dy, *dz;
-  double dt;
+  const double *u, *v, *w, *rdx, *rdy, *rdz;
+  double rdt;
   __device__ double operator()(const Geom& g, int i, int j, int k) const {
     int ii = (i == 0) ? g.nx : i, kk = (k == 0) ? g.nz : k;
     long long o = g.at(ii, j, kk);
-    return ((u[o] - u[o - 1]) / dx[ii] + (v[o] - v[o - g.sy]) / dy[j] +
-            (w[o] - w[o - g.sz]) / dz[kk]) /
-           dt;
+    return ((u[o] - u[o - 1]) * rdx[ii] + (v[o] - v[o - g.sy]) * rdy[j] +
+            (w[o] - w[o - g.sz]) * rdz[kk]) *
+           rdt;
   }
 };
 // div(u_temp)/dt, also stored in FP64 at its interior point (every
@@ -335,13 +337,13 @@ __global__ __launch_bounds__(256) void k_cfft_z(Geom g, R* phit, FftPlan<R> p, i
   // each column's result buffer is the same half for every column
   const int half = (res == sh + c * 2 * N) ? 0 : N;
   __syncthreads();
-  const R sc = (R)(g.nx * g.nz);
+  const R rsc = (R)(1.0 / ((double)g.nx * (double)g.nz));
   for (int e = threadIdx.x; e < CC * N; e += blockDim.x) {
     int cc = e % CC, k = e / CC;
     int I = I0 + cc;
     if (I >= nh1) continue;
     C v = sh[cc * 2 * N + half + k];
-    if (scale) v = mkc<C>(v.x / sc, v.y / sc);
+    if (scale) v = mkc<C>(v.x * rsc, v.y * rsc);
     long long off = 2 * g.spec_at(I, j, k);
     phit[off] = v.x;
     phit[off + 1] = v.y;
@@ -450,20 +452,20 @@ int launch_p2s(const Geom& g, const double* fp, double* phit, hipStream_t s) {
 }
 
 template <typename R>
-int launch_p2s_div(const Geom& g, const Metrics& m, const double* u, const double* v,
+int launch_p2s_div(const Geom& g, const RMetrics& r, const double* u, const double* v,
                    const double* w, double dt, R* phit, hipStream_t s) {
-  return launch_forward<R>(g, SampleDiv{u, v, w, m.dx, m.dy, m.dz, dt}, phit, "rfft_x_div",
+  return launch_forward<R>(g, SampleDiv{u, v, w, r.rdx, r.rdy, r.rdz, 1.0 / dt}, phit,
+                           "rfft_x_div",
                            "cfft_z_fwd", s);
 }
-template int launch_p2s_div<double>(const Geom&, const Metrics&, const double*, const double*,
+template int launch_p2s_div<double>(const Geom&, const RMetrics&, const double*, const double*,
                                     const double*, double, double*, hipStream_t);
-template int launch_p2s_div<float>(const Geom&, const Metrics&, const double*, const double*,
-                                   const double*, double, float*, hipStream_t);
 
-int launch_p2s_div_store(const Geom& g, const Metrics& m, const double* u, const double* v,
+int launch_p2s_div_store(const Geom& g, const RMetrics& r, const double* u, const double* v,
                          const double* w, double dt, double* rhs, float* phit, hipStream_t s) {
-  return launch_forward<float>(g, SampleDivStore{SampleDiv{u, v, w, m.dx, m.dy, m.dz, dt}, rhs},
-                               phit, "rfft_x_div", "cfft_z_fwd", s);
+  return launch_forward<float>(
+      g, SampleDivStore{SampleDiv{u, v, w, r.rdx, r.rdy, r.rdz, 1.0 / dt}, rhs}, phit,
+      "rfft_x_div", "cfft_z_fwd", s);
 }
 
 int launch_p2s_resid(const Geom& g, const RMetrics& r, const double* rhs, const float* phi,

@@ -65,10 +65,10 @@ int launch_p2s(const Geom& g, const double* fp, double* phit, hipStream_t s);
 // R = double or float: the transform's arithmetic and the spectrum's and
 // output's storage; the divergence and residual samples are FP64.
 template <typename R>
-int launch_p2s_div(const Geom& g, const Metrics& m, const double* u, const double* v,
+int launch_p2s_div(const Geom& g, const RMetrics& r, const double* u, const double* v,
                    const double* w, double dt, R* phit, hipStream_t s);
 // float spectrum of div(u_temp)/dt, which is also stored in FP64 in `rhs`
-int launch_p2s_div_store(const Geom& g, const Metrics& m, const double* u, const double* v,
+int launch_p2s_div_store(const Geom& g, const RMetrics& r, const double* u, const double* v,
                          const double* w, double dt, double* rhs, float* phit, hipStream_t s);
 // float spectrum of rhs - L phi for a float phi with filled halos
 int launch_p2s_resid(const Geom& g, const RMetrics& r, const double* rhs, const float* phi,

@@ -124,7 +124,10 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
     ProfScope ps("mkp_coef", s);
     k_mkp_coefficients<<<(nthreads + 255) / 256, 256, 0, s>>>(g, m, xl, zl, kxz, ac);
   }
-  int rc = launch_p2s_div(g, m, u_temp, v_temp, w_temp, dt, phit, s);
+  RMetrics r;
+  int rc = make_rmetrics(g, m, r, s);
+  if (rc) return rc;
+  rc = launch_p2s_div(g, r, u_temp, v_temp, w_temp, dt, phit, s);
   if (rc) return rc;
   rc = launch_spn(g, phit, ac, kxz, s);
   if (rc) return rc;
@@ -132,9 +135,6 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
   rc = launch_s2p_inplace<double>(g, phit, phi, s, "cfft_z_inv", "irfft_x");
   if (rc) return rc;
   launch_halo_neumann(g, phi, s);
-  RMetrics r;
-  rc = make_rmetrics(g, m, r, s);
-  if (rc) return rc;
   ProfScope ps("project", s);
   k_project<<<interior_grid(g), interior_block(), 0, s>>>(g, r, phi, u_temp, v_temp, w_temp, dt,
                                                          o);
@@ -224,7 +224,7 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
   int rc = make_rmetrics(g, m, r, s);
   if (rc) return rc;
   // pass 1
-  rc = launch_p2s_div_store(g, m, u_temp, v_temp, w_temp, dt, rhs, phit, s);
+  rc = launch_p2s_div_store(g, r, u_temp, v_temp, w_temp, dt, rhs, phit, s);
   if (rc) return rc;
   rc = launch_spn<float>(g, phit, ac, kxz, s);
   if (rc) return rc;
