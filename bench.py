@@ -233,8 +233,7 @@ def main():
         with_ep2 = args.diagnostics and not mixed
         ms = 1e3 * el / args.steps
         sb = step_bytes(g, with_ep2, mixed)
-        tfile = os.path.join(ROOT, "profiles",
-                             f"{cfg.name}_{'mixed_' if mixed else ''}traffic.txt")
+        tfile = os.path.join(ROOT, "profiles", f"{cfg.name}_traffic.txt")
         forms[name] = {
             "value": round(npts * args.steps / el / 1e6, 2), "ms_per_step": round(ms, 4),
             "roofline": roofline(prof, g, with_ep2, mixed, tfile),
