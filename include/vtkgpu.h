@@ -52,6 +52,11 @@ int vk_free(void *dptr);
 int vk_memcpy_htod(void *dst, const void *src, const size_t *bytes);
 int vk_memcpy_dtoh(void *dst, const void *src, const size_t *bytes);
 int vk_memset(void *dst, const int *value, const size_t *bytes);
+/* Bytes from p to the end of the registered host array or device
+   allocation that holds it.  Every entry point makes the same check on
+   its arguments and refuses (VK_ERR_ARGUMENT) an array too small for the
+   grid it is given. */
+int vk_extent(const void *p, size_t *bytes);
 
 /* Kernel timing on the library stream with HIP events, for benchmarks:
    between vk_profile_begin and vk_profile_end every kernel launch is

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libvtkgpu.so")
 EXPORTS = [
     "vk_init", "vk_finalize", "vk_last_error", "vk_register", "vk_unregister",
     "vk_upload", "vk_download", "vk_synchronize", "vk_malloc", "vk_free",
-    "vk_memcpy_htod", "vk_memcpy_dtoh", "vk_memset",
+    "vk_memcpy_htod", "vk_memcpy_dtoh", "vk_memset", "vk_extent",
     "vk_set_bc", "vk_set_convect", "vk_set_convection_for_scalar",
     "vk_set_conduction", "vk_set_diffusion_for_scalar",
     "vk_set_immersed_boundary", "vk_set_explicit",
@@ -31,7 +31,11 @@ EXPORTS = [
 
 
 class VKError(RuntimeError):
-    pass
+    """A non-zero status of the library; ``status`` is the VK_ERR_* code."""
+
+    def __init__(self, msg: str, status: int | None = None):
+        super().__init__(msg)
+        self.status = status
 
 
 _lib = None
@@ -56,7 +60,7 @@ def lib() -> ctypes.CDLL:
 def check(rc: int, what: str = "") -> None:
     if rc != 0:
         msg = lib().vk_last_error().decode(errors="replace")
-        raise VKError(f"{what}: status {rc}: {msg}")
+        raise VKError(f"{what}: status {rc}: {msg}", rc)
 
 
 def I(v: int):

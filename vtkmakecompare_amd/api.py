@@ -141,6 +141,15 @@ def set_adj_ns_scalar(dt, nx, ny, nz, dx12, dy12, dz12, t, u_s, v_s, w_s, u_fina
           "set_adj_NS_scalar")
 
 
+def extent(a) -> int:
+    """Bytes from ``a`` (a DeviceArray, a device pointer or a registered
+    host array) to the end of its allocation."""
+    import ctypes
+    n = ctypes.c_size_t(0)
+    check(lib().vk_extent(a, ctypes.byref(n)), "extent")
+    return n.value
+
+
 def field_absmax(f, nx, ny, nz) -> float:
     import ctypes
     r = ctypes.c_double(0.0)

@@ -657,7 +657,7 @@ int resolve_problem(Problem& P, Inputs& in, const double* u, const double* v, co
   for (int q = 0; q < 15; q++) {
     const size_t bytes = (q == 12) ? g.interior() * 4 * sizeof(double) : fb;
     d[q] = static_cast<const double*>(resolve(a[q], bytes, names[q]));
-    if (!d[q]) return VK_ERR_NOT_RESIDENT;
+    if (!d[q]) return last_status();
   }
   in = Inputs{d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11],
               d[12], d[13], d[14]};
@@ -666,7 +666,7 @@ int resolve_problem(Problem& P, Inputs& in, const double* u, const double* v, co
   const double* md[6];
   for (int q = 0; q < 6; q++) {
     md[q] = static_cast<const double*>(resolve(mm[q], len[q] * sizeof(double), "metric"));
-    if (!md[q]) return VK_ERR_NOT_RESIDENT;
+    if (!md[q]) return last_status();
   }
   P.m = Metrics{md[0], md[1], md[2], md[3], md[4], md[5]};
   P.rc = in.rc;
@@ -717,7 +717,9 @@ int vk_objective(const double* u, const double* v, const double* w, const double
   if (*nsteps < 1) return fail(VK_ERR_ARGUMENT, "objective: nsteps >= 1");
   Problem P{Geom(*nx, *ny, *nz), Metrics{}, *dt, *re, *sc, *eta, *xl, *zl, nullptr, nullptr,
             nullptr};
+  if (P.g.nx < 4 || P.g.ny < 4 || P.g.nz < 4) return fail(VK_ERR_BAD_SIZE, "objective: n >= 4");
   if (!P.g.fits32()) return fail(VK_ERR_BAD_SIZE, "objective: grid too large");
+  if (int frc = fft_sizes_ok(P.g)) return frc;
   Inputs in;
   int rc = resolve_problem(P, in, u, v, w, p, t, u_pre, v_pre, w_pre, t_pre, u_ta, v_ta, w_ta,
                            cellvalue, rhocp, thc, dx, dy, dz, dx12, dy12, dz12);
@@ -759,12 +761,13 @@ int vk_adjoint_gradient(const double* u, const double* v, const double* w, const
   const Geom& g = P.g;
   if (g.nx < 4 || g.ny < 4 || g.nz < 4) return fail(VK_ERR_BAD_SIZE, "adjoint: n >= 4");
   if (!g.fits32()) return fail(VK_ERR_BAD_SIZE, "adjoint: grid too large");
+  if (int frc = fft_sizes_ok(g)) return frc;
   Inputs in;
   int rc = resolve_problem(P, in, u, v, w, p, t, u_pre, v_pre, w_pre, t_pre, u_ta, v_ta, w_ta,
                            cellvalue, rhocp, thc, dx, dy, dz, dx12, dy12, dz12);
   if (rc) return rc;
   double* dgrad = static_cast<double*>(resolve(grad, g.interior() * sizeof(double), "grad"));
-  if (!dgrad) return VK_ERR_NOT_RESIDENT;
+  if (!dgrad) return last_status();
   Dev dev;
   const size_t nc = g.cells();
   P.cv = dev.alloc(g.interior() * 4);

@@ -413,6 +413,25 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_irfft_x(
   }
 }
 
+// The sizes the spectral solve takes: nx even, nx/2 and nz products of 2
+// and 3, one FP64 column (2 nz complex) and four FP64 x rows (2 nx complex
+// each) within 64 KB of LDS.  Checked before a step launches anything, so
+// a refused size leaves every field as it was.
+int fft_sizes_ok(const Geom& g) {
+  auto smooth = [](int n) {
+    if (n < 1) return false;
+    while (n % 2 == 0) n /= 2;
+    while (n % 3 == 0) n /= 3;
+    return n == 1;
+  };
+  if (g.nx % 2 != 0) return fail(VK_ERR_BAD_SIZE, "FFT: nx must be even");
+  if (!smooth(g.nx / 2) || !smooth(g.nz))
+    return fail(VK_ERR_BAD_SIZE, "FFT: nx/2 and nz must be products of 2 and 3");
+  if (2 * (size_t)g.nz * 16 > 65536 || 4 * 2 * (size_t)(g.nx / 2) * 16 > 65536)
+    return fail(VK_ERR_BAD_SIZE, "FFT: size above the LDS plan");
+  return VK_OK;
+}
+
 template <typename R>
 static int plans_for(const Geom& g, FftPlan<R>& px, FftPlan<R>& pz,
                      const typename Cx<R>::T*& twl) {

@@ -77,6 +77,9 @@ template <typename R>
 int launch_s2p_inplace(const Geom& g, R* phit, R* fp, hipStream_t s, const char* zname,
                        const char* xname);
 
+// VK_OK when the spectral Poisson solve supports the grid (vk_fft.hip)
+int fft_sizes_ok(const Geom& g);
+
 // projection (vk_mkp.hip)
 struct MkpOut {
   double *p, *u_ta, *v_ta, *w_ta, *u_ep2, *v_ep2, *w_ep2, *divu, *u, *v, *w;

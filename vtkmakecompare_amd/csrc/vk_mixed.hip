@@ -159,6 +159,7 @@ extern "C" int vk_time_step_mixed(double* u, double* v, double* w, double* p, fl
   Geom g(*nx, *ny, *nz);
   if (g.nx < 4 || g.ny < 4 || g.nz < 4) return fail(VK_ERR_BAD_SIZE, "time_step_mixed: n >= 4");
   if (!g.fits32()) return fail(VK_ERR_BAD_SIZE, "time_step_mixed: grid too large");
+  if (int frc = fft_sizes_ok(g)) return frc;
   const size_t fd = g.cells() * sizeof(double), ff = g.cells() * sizeof(float);
   VK_RESOLVE(du, double, u, fd);
   VK_RESOLVE(dv, double, v, fd);

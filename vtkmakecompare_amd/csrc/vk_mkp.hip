@@ -111,6 +111,7 @@ __global__ void k_project_faces(Geom g, RMetrics r, const double* phi, const dou
 int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const double* v_temp,
                const double* w_temp, double dt, double xl, double zl, const MkpOut& o,
                hipStream_t s) {
+  if (int frc = fft_sizes_ok(g)) return frc;
   const int nh = g.nx / 2;
   size_t nk = (size_t)(nh + 1) * (g.nz + 1) + 3 * (size_t)g.ny;
   double* coef = static_cast<double*>(workspace(kWorkMisc, (nk + 8) * sizeof(double)));
@@ -206,6 +207,7 @@ __global__ void k_project_mixed_faces(Geom g, RMetrics r, const float* phi1, con
 int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
                      const double* v_temp, const double* w_temp, double dt, double xl, double zl,
                      const MkpOutMixed& o, hipStream_t s) {
+  if (int frc = fft_sizes_ok(g)) return frc;
   const int nh = g.nx / 2;
   size_t nk = (size_t)(nh + 1) * (g.nz + 1) + 3 * (size_t)g.ny;
   double* coef = static_cast<double*>(workspace(kWorkMisc, (nk + 8) * sizeof(double)));

@@ -29,6 +29,8 @@ std::string& last_error() {
   return s;
 }
 
+int g_last_status = VK_OK;
+
 struct Work {
   void* ptr = nullptr;
   size_t bytes = 0;
@@ -44,8 +46,11 @@ Runtime& rt() {
 
 int fail(int code, const std::string& msg) {
   last_error() = msg;
+  g_last_status = code;
   return code;
 }
+
+int last_status() { return g_last_status; }
 
 int hip_check(hipError_t e, const char* what) {
   if (e == hipSuccess) return VK_OK;
@@ -66,7 +71,7 @@ void* resolve(const void* p, size_t bytes, const char* what) {
     if (a >= base && a < base + it->second.bytes) {
       size_t off = a - base;
       if (off + bytes > it->second.bytes) {
-        fail(VK_ERR_NOT_RESIDENT, std::string(what) + ": registered array is " +
+        fail(VK_ERR_ARGUMENT, std::string(what) + ": registered array is " +
                                       std::to_string(it->second.bytes) + " bytes, need " +
                                       std::to_string(off + bytes));
         return nullptr;
@@ -77,11 +82,37 @@ void* resolve(const void* p, size_t bytes, const char* what) {
   hipPointerAttribute_t attr;
   std::memset(&attr, 0, sizeof(attr));
   hipError_t e = hipPointerGetAttributes(&attr, p);
-  if (e == hipSuccess && attr.type == hipMemoryTypeDevice) return const_cast<void*>(p);
+  if (e == hipSuccess && attr.type == hipMemoryTypeDevice) {
+    // a device pointer must leave room for the whole field in its
+    // allocation; where the runtime cannot say (memory mapped by the
+    // virtual-memory API) the caller's size is taken on trust
+    size_t ext = 0;
+    if (device_extent(p, &ext) && ext < bytes) {
+      fail(VK_ERR_ARGUMENT, std::string(what) + ": device allocation holds " +
+                                std::to_string(ext) + " bytes from this address, need " +
+                                std::to_string(bytes));
+      return nullptr;
+    }
+    return const_cast<void*>(p);
+  }
   (void)hipGetLastError();  // clear the sticky error of the query
   fail(VK_ERR_NOT_RESIDENT,
        std::string(what) + ": neither a registered host array nor device memory");
   return nullptr;
+}
+
+bool device_extent(const void* p, size_t* bytes) {
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  hipError_t e = hipMemGetAddressRange(&base, &size, const_cast<void*>(p));
+  if (e != hipSuccess || base == nullptr) {
+    (void)hipGetLastError();
+    return false;
+  }
+  uintptr_t a = reinterpret_cast<uintptr_t>(p), b = reinterpret_cast<uintptr_t>(base);
+  if (a < b || a - b > size) return false;
+  *bytes = size - (a - b);
+  return true;
 }
 
 void* workspace(WorkSlot slot, size_t bytes, bool* grown) {
@@ -316,6 +347,31 @@ int vk_profile_entry(const int* index, char* name, const int* name_len, double* 
   std::snprintf(name, (size_t)*name_len, "%s", ps.name.c_str());
   *total_ms = tot;
   *launches = (int)ps.used;
+  return VK_OK;
+}
+
+int vk_extent(const void* p, size_t* bytes) {
+  VK_REQUIRE_INIT();
+  if (!p || !bytes) return fail(VK_ERR_ARGUMENT, "vk_extent: null argument");
+  auto& reg = registry();
+  uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  auto it = reg.upper_bound(a);
+  if (it != reg.begin()) {
+    --it;
+    if (a >= it->first && a < it->first + it->second.bytes) {
+      *bytes = it->second.bytes - (a - it->first);
+      return VK_OK;
+    }
+  }
+  hipPointerAttribute_t attr;
+  std::memset(&attr, 0, sizeof(attr));
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess || attr.type != hipMemoryTypeDevice) {
+    (void)hipGetLastError();
+    return fail(VK_ERR_NOT_RESIDENT, "vk_extent: neither a registered host array nor device memory");
+  }
+  if (!device_extent(p, bytes))
+    return fail(VK_ERR_ARGUMENT, "vk_extent: the runtime reports no allocation range here");
   return VK_OK;
 }
 

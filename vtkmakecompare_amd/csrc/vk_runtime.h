@@ -21,6 +21,8 @@ Runtime& rt();
 
 // Records an error message and returns the code.
 int fail(int code, const std::string& msg);
+// the code of the last fail(): what resolve() refused an array with
+int last_status();
 // Checks a HIP status; records and returns VK_ERR_HIP on failure.
 int hip_check(hipError_t e, const char* what);
 
@@ -29,6 +31,8 @@ int hip_check(hipError_t e, const char* what);
 // device memory is used as is.  Returns nullptr and records the error
 // otherwise.
 void* resolve(const void* p, size_t bytes, const char* what);
+// bytes from p to the end of its device allocation, when the runtime knows
+bool device_extent(const void* p, size_t* bytes);
 
 // Grow-only device workspace, one per slot.
 enum WorkSlot {
@@ -68,7 +72,7 @@ struct ProfScope {
 
 #define VK_RESOLVE(var, type, ptr, bytes)                                   \
   type* var = static_cast<type*>(vk::resolve((const void*)(ptr), (bytes), #ptr)); \
-  if (!var) return VK_ERR_NOT_RESIDENT
+  if (!var) return vk::last_status()
 
 #define VK_LAUNCH_CHECK(what)                                               \
   do {                                                                      \

@@ -587,7 +587,7 @@ int vk_set_explicit(const double* u_v, double* u_pre, const double* u_d, double*
   double* d[15];
   for (int q = 0; q < 15; q++) {
     d[q] = static_cast<double*>(resolve(in[q], g.cells() * sizeof(double), "set_explicit"));
-    if (!d[q]) return VK_ERR_NOT_RESIDENT;
+    if (!d[q]) return last_status();
   }
   double a1, a2, b1;
   ab2(*l, a1, a2, b1);

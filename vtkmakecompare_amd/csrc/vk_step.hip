@@ -118,6 +118,7 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
   Geom g(*nx, *ny, *nz);
   if (g.nx < 4 || g.ny < 4 || g.nz < 4) return fail(VK_ERR_BAD_SIZE, "time_step: n >= 4");
   if (!g.fits32()) return fail(VK_ERR_BAD_SIZE, "time_step: grid too large");
+  if (int frc = fft_sizes_ok(g)) return frc;
   const size_t fb = g.cells() * sizeof(double);
   VK_RESOLVE(du, double, u, fb);
   VK_RESOLVE(dv, double, v, fb);
@@ -143,7 +144,7 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
     de1 = static_cast<double*>(resolve(u_ep2, fb, "u_ep2"));
     de2 = static_cast<double*>(resolve(v_ep2, fb, "v_ep2"));
     de3 = static_cast<double*>(resolve(w_ep2, fb, "w_ep2"));
-    if (!de1 || !de2 || !de3) return VK_ERR_NOT_RESIDENT;
+    if (!de1 || !de2 || !de3) return last_status();
   }
   VK_RESOLVE(ddx, const double, dx, (g.nx + 2) * sizeof(double));
   VK_RESOLVE(ddy, const double, dy, (g.ny + 2) * sizeof(double));

@@ -36,6 +36,11 @@ module vk_capi
     integer(c_int) function vk_synchronize() bind(C, name="vk_synchronize")
       import :: c_int
     end function
+    integer(c_int) function vk_extent(host, bytes) bind(C, name="vk_extent")
+      import :: c_int, c_size_t
+      type(*), dimension(*) :: host
+      integer(c_size_t), intent(out) :: bytes
+    end function
 
     integer(c_int) function vk_set_bc(u, v, w, p, nx, ny, nz, t) bind(C, name="vk_set_bc")
       import :: c_int, c_double
