@@ -553,15 +553,17 @@ __global__ __launch_bounds__(NT) void k_line_general_q(Sys sys, LineQueue Q, uns
 }
 
 // The recorded-multiplier path.  The reference line's table (validated for
-// this call by k_line_factor) is staged in LDS from offset o_f on, together
-// with the block's data loads.  A line whose rows all have the reference
+// this call by k_line_factor) is staged in LDS from offset o_f on, once per
+// block and together with its first data loads; the block then takes GPB
+// groups of LPB lines in turn.  A line whose rows all have the reference
 // inputs applies the recorded multipliers: its rows then have the
 // reference coefficients bit for bit, so it performs the same operations on
 // d as the general path.  Any other line is queued for k_line_general_q and
 // not written here.
 template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
 __global__ __launch_bounds__(NT) void k_line_fast(Sys sys, const double* __restrict__ tab,
-                                                  TabLayout L, LineQueue Q, int nbx) {
+                                                  TabLayout L, LineQueue Q, int nbx,
+                                                  unsigned ngrp) {
   using V = typename Sys::V;
   using M = LineMap<S, NT, LANE_LINES>;
   constexpr int P = M::P;
@@ -571,88 +573,101 @@ __global__ __launch_bounds__(NT) void k_line_fast(Sys sys, const double* __restr
   double* st = smem;
   double* sd = smem + T.total;
 
-  const unsigned nb = gridDim.x;
-  const unsigned blk = xcd_linear(blockIdx.x, nb);
-  const int bx = (int)(blk % (unsigned)nbx), by = (int)(blk / (unsigned)nbx);
+  // a block takes GPB consecutive groups of lines and stages the table once
+  // (x lines: four lines per block would otherwise stage a table larger
+  // than their data; y and z blocks hold 16-32 lines and take one group)
+  constexpr int GPB = LANE_LINES ? 1 : 4;
+  const unsigned first = xcd_linear(blockIdx.x, gridDim.x) * (unsigned)GPB;
   const M mp(threadIdx.x);
   const int seg = mp.seg;
-  const int la = bx * M::LPB + mp.line, lb = by;
-  const bool ok = sys.valid(la, lb);
-  const int lc = ok ? la : bx * M::LPB;
   const int n = sys.n();
   const int lo = (seg * n) / S;
   const int m = ((seg + 1) * n) / S - lo;
-  if (seg == 0) lclean[mp.line] = 1;
+#pragma unroll 1
+  for (int gi = 0; gi < GPB; gi++) {
+    const unsigned blk = first + (unsigned)gi;
+    if (blk >= ngrp) break;  // the same for every thread of the block
+    // the previous group's reads of lclean and of the reduced system are done
+    if (gi > 0) __syncthreads();
+    const int bx = (int)(blk % (unsigned)nbx), by = (int)(blk / (unsigned)nbx);
+    const int la = bx * M::LPB + mp.line, lb = by;
+    const bool ok = sys.valid(la, lb);
+    const int lc = ok ? la : bx * M::LPB;
+    if (seg == 0) lclean[mp.line] = 1;
 
-  V d[MMAX];
-  int clean = 1;
+    V d[MMAX];
+    int clean = 1;
 #pragma unroll
-  for (int r = 0; r < MMAX; r++) {
-    const int q = lo + (r < m ? r : m - 1);
-    d[r] = sys.rhs(lc, lb, q);
-    clean &= int(sys.pt_ref(sys.pt(lc, lb, q)));
-  }
-  if (Sys::kNb) clean &= int(sys.pt_ref(sys.pt(lc, lb, lo - 1))) &
-                         int(sys.pt_ref(sys.pt(lc, lb, lo + m)));
-  // stage the table: KT loads in flight per thread before the first wait
-  constexpr int KT = 8;
-  for (int u0 = 0; u0 < T.total; u0 += KT * NT) {
-    double v[KT];
-#pragma unroll
-    for (int k = 0; k < KT; k++) {
-      const int u = u0 + k * NT + (int)threadIdx.x;
-      v[k] = tab[T.src(u < T.total ? u : T.total - 1, n, L)];
+    for (int r = 0; r < MMAX; r++) {
+      const int q = lo + (r < m ? r : m - 1);
+      d[r] = sys.rhs(lc, lb, q);
+      clean &= int(sys.pt_ref(sys.pt(lc, lb, q)));
     }
+    if (Sys::kNb) clean &= int(sys.pt_ref(sys.pt(lc, lb, lo - 1))) &
+                           int(sys.pt_ref(sys.pt(lc, lb, lo + m)));
+    if (gi == 0) {
+      // stage the table while the first group's rows are in flight: KT
+      // loads in flight per thread before the first wait
+      constexpr int KT = 8;
+      for (int u0 = 0; u0 < T.total; u0 += KT * NT) {
+        double v[KT];
 #pragma unroll
-    for (int k = 0; k < KT; k++) {
-      const int u = u0 + k * NT + (int)threadIdx.x;
-      if (u < T.total) st[u] = v[k];
+        for (int k = 0; k < KT; k++) {
+          const int u = u0 + k * NT + (int)threadIdx.x;
+          v[k] = tab[T.src(u < T.total ? u : T.total - 1, n, L)];
+        }
+#pragma unroll
+        for (int k = 0; k < KT; k++) {
+          const int u = u0 + k * NT + (int)threadIdx.x;
+          if (u < T.total) st[u] = v[k];
+        }
+      }
     }
-  }
-  __syncthreads();
-  if (!clean) lclean[mp.line] = 0;
-  // the segment elimination's operations on d, with the recorded multipliers
-#pragma unroll
-  for (int r = 0; r < MMAX; r++) {
-    if (r < m) {
-      const double f = st[T.f + r * S + seg];
-      if (r < 2)
-        d[r] = f * d[r];
-      else
-        d[r] = f * (d[r] - st[T.g + r * S + seg] * d[r - 1]);
-    }
-  }
-#pragma unroll
-  for (int r = MMAX - 3; r >= 1; r--)
-    if (r <= m - 3) d[r] = d[r] - st[T.h + r * S + seg] * d[r + 1];
-  if (m >= 3) d[0] = st[T.f0 + seg] * (d[0] - st[T.h + seg] * d[1]);
-  V dlast;
-  vzero(dlast);
-#pragma unroll
-  for (int r = 0; r < MMAX; r++)
-    if (r == m - 1) dlast = d[r];
-  const LSlot<S> sl = LANE_LINES ? LSlot<S>{mp.line, M::LPB} : LSlot<S>{mp.line * P, 1};
-  LdsVec<V> ld{sd, M::LPB * P};
-  ld.set(sl.at(2 * seg), d[0]);
-  ld.set(sl.at(2 * seg + 1), dlast);
-  __syncthreads();
-  const bool line_clean = lclean[mp.line] != 0;
-  if (!line_clean && seg == 0 && ok)
-    Q.list[atomicAdd(Q.count, 1u)] = (unsigned)la + (unsigned)(nbx * M::LPB) * (unsigned)lb;
-  V xf, xl;
-  pcr_apply<S, MMAX, V>(ld, sl, seg, Sys::kCyclic, xf, xl, st, T);
-  if (ok && line_clean) {
+    __syncthreads();
+    if (!clean) lclean[mp.line] = 0;
+    // the segment elimination's operations on d, with the recorded multipliers
 #pragma unroll
     for (int r = 0; r < MMAX; r++) {
       if (r < m) {
-        V x;
-        if (r == 0)
-          x = xf;
-        else if (r == m - 1)
-          x = xl;
+        const double f = st[T.f + r * S + seg];
+        if (r < 2)
+          d[r] = f * d[r];
         else
-          x = d[r] - st[T.afin + r * S + seg] * xf - st[T.cfin + r * S + seg] * xl;
-        sys.store(la, lb, lo + r, x);
+          d[r] = f * (d[r] - st[T.g + r * S + seg] * d[r - 1]);
+      }
+    }
+#pragma unroll
+    for (int r = MMAX - 3; r >= 1; r--)
+      if (r <= m - 3) d[r] = d[r] - st[T.h + r * S + seg] * d[r + 1];
+    if (m >= 3) d[0] = st[T.f0 + seg] * (d[0] - st[T.h + seg] * d[1]);
+    V dlast;
+    vzero(dlast);
+#pragma unroll
+    for (int r = 0; r < MMAX; r++)
+      if (r == m - 1) dlast = d[r];
+    const LSlot<S> sl = LANE_LINES ? LSlot<S>{mp.line, M::LPB} : LSlot<S>{mp.line * P, 1};
+    LdsVec<V> ld{sd, M::LPB * P};
+    ld.set(sl.at(2 * seg), d[0]);
+    ld.set(sl.at(2 * seg + 1), dlast);
+    __syncthreads();
+    const bool line_clean = lclean[mp.line] != 0;
+    if (!line_clean && seg == 0 && ok)
+      Q.list[atomicAdd(Q.count, 1u)] = (unsigned)la + (unsigned)(nbx * M::LPB) * (unsigned)lb;
+    V xf, xl;
+    pcr_apply<S, MMAX, V>(ld, sl, seg, Sys::kCyclic, xf, xl, st, T);
+    if (ok && line_clean) {
+#pragma unroll
+      for (int r = 0; r < MMAX; r++) {
+        if (r < m) {
+          V x;
+          if (r == 0)
+            x = xf;
+          else if (r == m - 1)
+            x = xl;
+          else
+            x = d[r] - st[T.afin + r * S + seg] * xf - st[T.cfin + r * S + seg] * xl;
+          sys.store(la, lb, lo + r, x);
+        }
       }
     }
   }
@@ -746,8 +761,10 @@ inline int line_sweep(const Sys& sys, int n, int nla, int nlb, hipStream_t s) {
     const unsigned nla_pad = (unsigned)nbx * (unsigned)LPB_F;
     LineQueue Q = line_queue(nla_pad * (unsigned)nlb, s);
     if (!Q.list) return -2;
+    constexpr unsigned GPB = LANE_LINES ? 1u : 4u;  // groups per block, as in k_line_fast
+    const unsigned ngb = (nb + GPB - 1) / GPB;
     k_line_factor<SF, MF, Sys><<<1, SF, 0, s>>>(sys, tab, L, fresh ? 1 : 0);
-    k_line_fast<SF, MF, NT, LANE_LINES, Sys><<<nb, NT, lds, s>>>(sys, tab, L, Q, nbx);
+    k_line_fast<SF, MF, NT, LANE_LINES, Sys><<<ngb, NT, lds, s>>>(sys, tab, L, Q, nbx, nb);
     const unsigned ngen = nb < 1024u ? nb : 1024u;
     k_line_general_q<SG, MG, NT, LANE_LINES, Sys><<<ngen, NT, 0, s>>>(sys, Q, nla_pad);
   } else {
