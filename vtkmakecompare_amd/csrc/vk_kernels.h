@@ -24,6 +24,46 @@ void launch_pen_divide(const Geom& g, double* x, const double* old, const double
 template <typename T>
 void launch_cv_faces(const Geom& g, T* cv, hipStream_t s);
 
+// Dirty-line masks of one step, for the line sweeps' recorded-multiplier
+// path: bit 0 (temperature: cellvalue slot 1 non-zero, rhoCp or thc not 1)
+// and bits 1-3 (u, v, w: cellvalue slot 2-4 non-zero) of the word of every
+// line holding such a point.  fx[(j-1) + ny (k-1)] are the x lines,
+// fy[(i-1) + nx (k-1)] the y lines, fz[(i-1) + nx (j-1)] the z lines.  A
+// clear bit lets a sweep skip the per-point test of its line's inputs; a
+// set bit sends the line to the general path, so masks that mark too much
+// only cost time.  Null pointers: each sweep tests its points itself.
+struct LineFlags {
+  unsigned *fx, *fy, *fz;
+};
+// the masks in workspace, cleared on the stream (vk_step.hip)
+int line_flags_clear(const Geom& g, LineFlags& lf, hipStream_t s);
+
+// Sets bits mk of interior point (i, j, k) in the masks of its three
+// lines.  Called by every active lane of an interior block (64 x 4): the
+// lanes of a wave then hold consecutive i of one (j, k), and the x line's
+// word takes a single atomic per wave.
+static_assert(kBX == 64, "a wave of an interior block must cover one x row");
+__device__ inline void mark_dirty_lines(const LineFlags& lf, const Geom& g, int i, int j, int k,
+                                        unsigned mk) {
+  unsigned mx = 0;
+#pragma unroll
+  for (int b = 0; b < 4; b++)
+    if (__ballot((mk >> b) & 1u)) mx |= 1u << b;
+  if (mk) {
+    atomicOr(&lf.fy[(i - 1) + g.nx * (k - 1)], mk);
+    atomicOr(&lf.fz[(i - 1) + g.nx * (j - 1)], mk);
+  }
+  const unsigned long long act = __ballot(1);
+  if (mx && (int)__lane_id() == __ffsll(act) - 1) atomicOr(&lf.fx[(j - 1) + g.ny * (k - 1)], mx);
+}
+
+// the mask bits of a point from its cellvalue slots 1-4, rhoCp and thc
+template <typename T>
+__device__ inline unsigned dirty_bits(T cv1, T cv2, T cv3, T cv4, T rc, T thc) {
+  return ((cv1 != T(0)) | (rc != T(1)) | (thc != T(1)) ? 1u : 0u) | (cv2 != T(0) ? 2u : 0u) |
+         (cv3 != T(0) ? 4u : 0u) | (cv4 != T(0) ? 8u : 0u);
+}
+
 // line solves (vk_tridiag.hip); return non-zero for unsupported lengths
 // The spectrum may be stored in double or float (TS); the velocity
 // systems take cellvalue in double or float (TC) with the field in double;
@@ -35,20 +75,22 @@ template <typename TS>
 void launch_sp0b(const Geom& g, TS* phit, const double* ac, hipStream_t s);
 template <typename TC>
 int launch_adi_velocity(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
-                        const TC* cv, int comp, double beta, double dteta, hipStream_t s);
+                        const TC* cv, int comp, double beta, double dteta, hipStream_t s,
+                        const LineFlags* fl = nullptr);
 template <typename TT>
 int launch_adi_temperature(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
                            const TT* cv, const TT* rc, const TT* thc, double beta, double dteta,
-                           hipStream_t s, TT* out = nullptr);
+                           hipStream_t s, TT* out = nullptr, const LineFlags* fl = nullptr);
 // one sweep: velocity dir 0, 1, 2 = y, z, x; temperature dir 0, 1, 2 = y, x, z
 template <typename TC>
 int launch_velocity_sweep(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
                           const TC* cv, int comp, int dir, double beta, double dteta,
-                          hipStream_t s);
+                          hipStream_t s, const LineFlags* fl = nullptr);
 template <typename TT>
 int launch_temperature_sweep(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
                              const TT* cv, const TT* rc, const TT* thc, int dir, double beta,
-                             double dteta, hipStream_t s, TT* out = nullptr);
+                             double dteta, hipStream_t s, TT* out = nullptr,
+                             const LineFlags* fl = nullptr);
 // the transposed sweeps (vk_sweep_adj.hip), in place on the adjoint field
 int launch_velocity_sweep_adj(const Geom& g, const Metrics& m, const RMetrics& r, double* lam,
                               const double* cv, int comp, int dir, double beta, double dteta,
@@ -106,6 +148,7 @@ struct ExplicitArgs {
   double *u_pre, *v_pre, *w_pre, *t_pre;
   double *ua, *va, *wa, *ut;
   double dt, re, sc, eta, a1, a2, b1, c0, cd;
+  LineFlags lf;  // filled when non-null
 };
 // the scalar part of ExplicitArgs for step l (AB2 weights, CN factors)
 ExplicitArgs explicit_args(double dt, double re, double sc, double eta, int l);

@@ -118,6 +118,11 @@ struct SysYFace {
   __device__ void store(int la, int lb, int q, double x) const {
     f[g.at32(la + 1, q, lb + 1)] = static_cast<TF>(x);
   }
+  // dirty-line mask of this sweep (LineFlags): word la + lstride lb, bit
+  // lbit; null when the sweep tests its points itself
+  const unsigned* lflag = nullptr;
+  unsigned lbit = 0;
+  int lstride = 0;
 };
 
 // implicit_vy (nse_dominik.f:1583-1648): rows j = 0..ny with x0 = x_ny = 0
@@ -165,6 +170,11 @@ struct SysYV {
   __device__ void store(int la, int lb, int q, double x) const {
     f[g.at32(la + 1, q, lb + 1)] = static_cast<TF>(x);
   }
+  // dirty-line mask of this sweep (LineFlags): word la + lstride lb, bit
+  // lbit; null when the sweep tests its points itself
+  const unsigned* lflag = nullptr;
+  unsigned lbit = 0;
+  int lstride = 0;
 };
 
 // implicit_Ty (nse_dominik.f:1784-1870) with lly(0) = lly(1) and
@@ -227,6 +237,11 @@ struct SysYT {
   __device__ void store(int la, int lb, int q, double x) const {
     if (q >= 1 && q <= g.ny) (out ? out : f)[g.at32(la + 1, q, lb + 1)] = static_cast<TF>(x);
   }
+  // dirty-line mask of this sweep (LineFlags): word la + lstride lb, bit
+  // lbit; null when the sweep tests its points itself
+  const unsigned* lflag = nullptr;
+  unsigned lbit = 0;
+  int lstride = 0;
 };
 
 // --------------------------------------------------- periodic systems
@@ -288,6 +303,11 @@ struct SysPeriodicFace {
     if (q == n) f[ax.at(la, lb, 0)] = xs;
     if (q == 1) f[ax.at(la, lb, n + 1)] = xs;
   }
+  // dirty-line mask of this sweep (LineFlags): word la + lstride lb, bit
+  // lbit; null when the sweep tests its points itself
+  const unsigned* lflag = nullptr;
+  unsigned lbit = 0;
+  int lstride = 0;
 };
 
 // implicit_Tx (nse_dominik.f:1725-1781), used along x and z by solve_t,
@@ -340,6 +360,11 @@ struct SysPeriodicT {
   __device__ void store(int la, int lb, int r, double x) const {
     f[ax.at(la, lb, r + 1)] = static_cast<TF>(x);
   }
+  // dirty-line mask of this sweep (LineFlags): word la + lstride lb, bit
+  // lbit; null when the sweep tests its points itself
+  const unsigned* lflag = nullptr;
+  unsigned lbit = 0;
+  int lstride = 0;
 };
 
 // -------------------------------------------------- spectral y solves
@@ -390,6 +415,11 @@ struct SysSpectral {
     phit[o] = static_cast<TS>(x.x);
     phit[o + 1] = static_cast<TS>(x.y);
   }
+  // dirty-line mask of this sweep (LineFlags): word la + lstride lb, bit
+  // lbit; null when the sweep tests its points itself
+  const unsigned* lflag = nullptr;
+  unsigned lbit = 0;
+  int lstride = 0;
 };
 
 // launch_line_solve's result as a status: -1 is a line length the solver
@@ -468,31 +498,49 @@ struct SysT {
   __device__ void store(int la, int lb, int q, double x) const {
     if (s.in_row(q)) lam[s.addr(la, lb, q)] = x;
   }
+  // dirty-line mask of this sweep (LineFlags): word la + lstride lb, bit
+  // lbit; null when the sweep tests its points itself
+  const unsigned* lflag = nullptr;
+  unsigned lbit = 0;
+  int lstride = 0;
 };
 
 // Runs fn(sys, n, nla, nlb, lane_lines, name) on the system of one sweep
 // of solve_u_temp: comp 0,1,2 = u, v, w (cellvalue slot comp+2), dir 0, 1,
 // 2 = the y, z, x sweep; v's z and x sweeps stop at j = ny-1
 // (nse_dominik.f:1411,1418).
+template <typename Sys>
+inline void use_flags(Sys& sys, const unsigned* words, unsigned bit, int stride) {
+  sys.lflag = words;
+  sys.lbit = bit;
+  sys.lstride = stride;
+}
+
 template <typename TC, typename Fn>
 int with_velocity_system(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
-                         const TC* cv, int comp, int dir, double beta, double dteta, Fn&& fn) {
+                         const TC* cv, int comp, int dir, double beta, double dteta, Fn&& fn,
+                         const LineFlags* fl = nullptr) {
   const int jmax = (comp == 1) ? g.ny - 1 : g.ny;
+  const unsigned bit = 1u << (comp + 1);
   if (dir == 0) {
     if (comp == 1) {
       SysYV<double, TC> sy{g, f, cv, r, beta, dteta, m.dy};
+      if (fl) use_flags(sy, fl->fy, bit, g.nx);
       return fn(sy, g.ny + 1, g.nx, g.nz, std::true_type(), "line_y_v");
     }
     SysYFace<double, TC> sy{g, f, cv, r, beta, dteta, comp + 2, m.dy};
+    if (fl) use_flags(sy, fl->fy, bit, g.nx);
     return fn(sy, g.ny + 2, g.nx, g.nz, std::true_type(), "line_y_face");
   }
   if (dir == 1) {
     SysPeriodicFace<double, TC> sz{LineAxis{g, 2}, f,    cv,   r.rdz, r.rdz12, beta,
                                    dteta,          comp + 2, g.nx, jmax,  m.dz};
+    if (fl) use_flags(sz, fl->fz, bit, g.nx);
     return fn(sz, g.nz, g.nx, jmax, std::true_type(), "line_z_face");
   }
   SysPeriodicFace<double, TC> sx{LineAxis{g, 0}, f,    cv,   r.rdx, r.rdx12, beta,
                                  dteta,          comp + 2, jmax, g.nz,  m.dx};
+  if (fl) use_flags(sx, fl->fx, bit, g.ny);
   return fn(sx, g.nx, jmax, g.nz, std::false_type(), "line_x_face");
 }
 
@@ -500,18 +548,22 @@ int with_velocity_system(const Geom& g, const Metrics& m, const RMetrics& r, dou
 template <typename TT, typename Fn>
 int with_temperature_system(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
                             const TT* cv, const TT* rc_, const TT* thc, int dir, double beta,
-                            double dteta, Fn&& fn, TT* out = nullptr) {
+                            double dteta, Fn&& fn, TT* out = nullptr,
+                            const LineFlags* fl = nullptr) {
   if (dir == 0) {
     SysYT<TT, TT> sy{g, f, cv, rc_, thc, m, r, beta, dteta, out};
+    if (fl) use_flags(sy, fl->fy, 1u, g.nx);
     return fn(sy, g.ny + 2, g.nx, g.nz, std::true_type(), "line_y_t");
   }
   if (dir == 1) {
     SysPeriodicT<TT, TT> sx{LineAxis{g, 0}, f, cv, rc_, thc, m.dx, r.rdx, r.rdx12, beta, dteta,
                             g.ny, g.nz};
+    if (fl) use_flags(sx, fl->fx, 1u, g.ny);
     return fn(sx, g.nx, g.ny, g.nz, std::false_type(), "line_x_t");
   }
   SysPeriodicT<TT, TT> sz{LineAxis{g, 2}, f, cv, rc_, thc, m.dz, r.rdz, r.rdz12, beta, dteta,
                           g.nx, g.ny};
+  if (fl) use_flags(sz, fl->fz, 1u, g.nx);
   return fn(sz, g.nz, g.nx, g.ny, std::true_type(), "line_z_t");
 }
 
@@ -540,24 +592,26 @@ struct SolveAdj {
 template <typename TC>
 int launch_velocity_sweep(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
                           const TC* cv, int comp, int dir, double beta, double dteta,
-                          hipStream_t s) {
-  return with_velocity_system(g, m, r, f, cv, comp, dir, beta, dteta, SolveFwd{s});
+                          hipStream_t s, const LineFlags* fl) {
+  return with_velocity_system(g, m, r, f, cv, comp, dir, beta, dteta, SolveFwd{s}, fl);
 }
 
 template <typename TT>
 int launch_temperature_sweep(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
                              const TT* cv, const TT* rc_, const TT* thc, int dir, double beta,
-                             double dteta, hipStream_t s, TT* out) {
-  return with_temperature_system(g, m, r, f, cv, rc_, thc, dir, beta, dteta, SolveFwd{s}, out);
+                             double dteta, hipStream_t s, TT* out, const LineFlags* fl) {
+  return with_temperature_system(g, m, r, f, cv, rc_, thc, dir, beta, dteta, SolveFwd{s}, out,
+                                 fl);
 }
 
 // solve_u_temp body after the penalisation divide: y, z, x sweeps of one
 // component
 template <typename TC>
 int launch_adi_velocity(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
-                        const TC* cv, int comp, double beta, double dteta, hipStream_t s) {
+                        const TC* cv, int comp, double beta, double dteta, hipStream_t s,
+                        const LineFlags* fl) {
   for (int dir = 0; dir < 3; dir++) {
-    int rc = launch_velocity_sweep<TC>(g, m, r, f, cv, comp, dir, beta, dteta, s);
+    int rc = launch_velocity_sweep<TC>(g, m, r, f, cv, comp, dir, beta, dteta, s, fl);
     if (rc) return rc;
   }
   return VK_OK;
@@ -568,11 +622,11 @@ int launch_adi_velocity(const Geom& g, const Metrics& m, const RMetrics& r, doub
 template <typename TT>
 int launch_adi_temperature(const Geom& g, const Metrics& m, const RMetrics& r, TT* f,
                            const TT* cv, const TT* rc_, const TT* thc, double beta, double dteta,
-                           hipStream_t s, TT* out) {
+                           hipStream_t s, TT* out, const LineFlags* fl) {
   for (int dir = 0; dir < 3; dir++) {
     TT* src = (dir == 0 || !out) ? f : out;
     int rc = launch_temperature_sweep<TT>(g, m, r, src, cv, rc_, thc, dir, beta, dteta, s,
-                                          dir == 0 ? out : nullptr);
+                                          dir == 0 ? out : nullptr, fl);
     if (rc) return rc;
   }
   return VK_OK;

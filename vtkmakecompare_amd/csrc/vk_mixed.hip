@@ -80,6 +80,7 @@ struct ExplicitMixedArgs {
   float* ut;
   float dt, eta, a1, a2, b1, c0, cd;
   double dtd, etad;
+  LineFlags lf;  // filled when non-null
 };
 
 // x / (1 + eta cv dt): exactly x where cv is 0 (the divisor is exactly 1)
@@ -101,10 +102,13 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
   F3<float> T{x.t, g}, RC{x.rc, g}, K{x.thc, g};
   const long long o = g.at(i, j, k);
   const float dt = x.dt, eta = x.eta;
+  const float cv1 = x.cv[g.cv(i, j, k, 1)], cv2 = x.cv[g.cv(i, j, k, 2)],
+              cv3 = x.cv[g.cv(i, j, k, 3)], cv4 = x.cv[g.cv(i, j, k, 4)];
+  const float rc0 = RC(i, j, k);
+  const unsigned mk = x.lf.fx ? dirty_bits(cv1, cv2, cv3, cv4, rc0, K(i, j, k)) : 0u;
   {
     const float uv = conv_u_r<float>(U, V, W, f, f, i, j, k, dt);
     const float ud = cond_u_r<float>(U, f, i, j, k, x.c0);
-    const float cv2 = x.cv[g.cv(i, j, k, 2)];
     const float ul = eta * cv2 * x.u_ta[o] * dt;
     const float ua = x.a1 * uv + x.a2 * x.u_pre[o] + x.b1 * ud + x.b1 * ul;
     x.u_pre[o] = uv;
@@ -113,7 +117,6 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
   {
     const float vv = conv_v_r<float>(U, V, W, f, f, i, j, k, dt);
     const float vd = cond_v_r<float>(V, f, i, j, k, x.c0);
-    const float cv3 = x.cv[g.cv(i, j, k, 3)];
     const float vl = eta * cv3 * x.v_ta[o] * dt;
     const float va = x.a1 * vv + x.a2 * x.v_pre[o] + x.b1 * vd + x.b1 * vl;
     x.v_pre[o] = vv;
@@ -125,20 +128,20 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
   {
     const float wv = conv_w_r<float>(U, V, W, f, f, i, j, k, dt);
     const float wd = cond_w_r<float>(W, f, i, j, k, x.c0);
-    const float cv4 = x.cv[g.cv(i, j, k, 4)];
     const float wl = eta * cv4 * x.w_ta[o] * dt;
     const float wa = x.a1 * wv + x.a2 * x.w_pre[o] + x.b1 * wd + x.b1 * wl;
     x.w_pre[o] = wv;
     x.wa[o] = pen_div_m<double>(x.w[o] + (double)wa, x.etad, (double)cv4, x.dtd);
   }
   {
-    const float rrc = 1.0f / RC(i, j, k);
+    const float rrc = 1.0f / rc0;
     const float vt = conv_scalar_r<float>(U, V, W, T, RC, f, f, i, j, k, dt, rrc);
     const float dtt = diff_scalar_r<float>(T, K, f, f, i, j, k, x.cd, rrc);
     const float ut = x.a1 * vt + x.a2 * x.t_pre[o] + x.b1 * dtt;
     x.t_pre[o] = vt;
-    x.ut[o] = pen_div_m<float>(x.t[o] + ut, eta, x.cv[g.cv(i, j, k, 1)], dt);
+    x.ut[o] = pen_div_m<float>(x.t[o] + ut, eta, cv1, dt);
   }
+  if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
 }
 
 }  // namespace vk
@@ -199,7 +202,11 @@ extern "C" int vk_time_step_mixed(double* u, double* v, double* w, double* p, fl
   RMetrics r;
   rc = make_rmetrics(g, m, r, s);
   if (rc) return rc;
-  ExplicitMixedArgs x;
+  LineFlags lf{};
+  rc = line_flags_clear(g, lf, s);
+  if (rc) return rc;
+  ExplicitMixedArgs x{};
+  x.lf = lf;
   x.u = du; x.v = dv; x.w = dw; x.t = dtt; x.rc = drc; x.thc = dk;
   x.u_ta = duta; x.v_ta = dvta; x.w_ta = dwta; x.cv = dcv;
   x.u_pre = dup; x.v_pre = dvp; x.w_pre = dwp; x.t_pre = dtp;
@@ -218,11 +225,11 @@ extern "C" int vk_time_step_mixed(double* u, double* v, double* w, double* p, fl
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
   double* comps[3] = {dut, dvt, dwt};
   for (int c = 0; c < 3; c++) {
-    rc = launch_adi_velocity<float>(g, m, r, comps[c], dcv, c, beta_u, dteta, s);
+    rc = launch_adi_velocity<float>(g, m, r, comps[c], dcv, c, beta_u, dteta, s, &lf);
     if (rc) return rc;
   }
   // T = T work after the sweeps: the y sweep writes T's interior
-  rc = launch_adi_temperature<float>(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s, dtt);
+  rc = launch_adi_temperature<float>(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s, dtt, &lf);
   if (rc) return rc;
   launch_set_bc<float>(g, dut, dvt, dwt, dp, dtt, s);
   MkpOutMixed o{dp, duta, dvta, dwta, ddiv, du, dv, dw};

@@ -393,6 +393,11 @@ struct LineQueue {
 //                                               kNb, its neighbours'
 //   void coef_ref(r, a, b, c) const;            row r of the reference line
 //   void store(la, lb, r, V x) const;           solution of row r
+//   const unsigned* lflag; unsigned lbit;       when lflag is set, line
+//   int lstride;                                (la, lb) has a point whose
+//                                               pt_ref is false only if bit
+//                                               lbit of lflag[la + lstride lb]
+//                                               is set
 // Block (bx, by) holds lines la = bx * LPB + (line within block), lb = by.
 // Every load of a thread is issued before any value is examined (a thread
 // past the last line reads the block's first line, and rows past a short
@@ -601,10 +606,19 @@ __global__ __launch_bounds__(NT) void k_line_fast(Sys sys, const double* __restr
     for (int r = 0; r < MMAX; r++) {
       const int q = lo + (r < m ? r : m - 1);
       d[r] = sys.rhs(lc, lb, q);
-      clean &= int(sys.pt_ref(sys.pt(lc, lb, q)));
     }
-    if (Sys::kNb) clean &= int(sys.pt_ref(sys.pt(lc, lb, lo - 1))) &
-                           int(sys.pt_ref(sys.pt(lc, lb, lo + m)));
+    if (sys.lflag) {
+      // the step's dirty-line mask stands for the per-point test
+      clean = int((sys.lflag[lc + sys.lstride * lb] & sys.lbit) == 0u);
+    } else {
+#pragma unroll
+      for (int r = 0; r < MMAX; r++) {
+        const int q = lo + (r < m ? r : m - 1);
+        clean &= int(sys.pt_ref(sys.pt(lc, lb, q)));
+      }
+      if (Sys::kNb) clean &= int(sys.pt_ref(sys.pt(lc, lb, lo - 1))) &
+                             int(sys.pt_ref(sys.pt(lc, lb, lo + m)));
+    }
     if (gi == 0) {
       // stage the table while the first group's rows are in flight: KT
       // loads in flight per thread before the first wait

@@ -46,6 +46,7 @@ enum WorkSlot {
   kWorkPhi2,
   kWorkRecipF,
   kWorkRhs,
+  kWorkFlags,
   kWorkSlots
 };
 // `grown` (optional) reports whether this call allocated the slot anew.

@@ -38,11 +38,16 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
   F3<double> U{x.u, g}, V{x.v, g}, W{x.w, g}, T{x.t, g}, RC{x.rc, g}, K{x.thc, g};
   const long long o = g.at(i, j, k);
   const double dt = x.dt, eta = x.eta;
+  // the point's cellvalue slots, rhoCp and thc, read before any store (the
+  // outputs may alias them as far as the compiler knows)
+  const double cv1 = x.cv[g.cv(i, j, k, 1)], cv2 = x.cv[g.cv(i, j, k, 2)],
+               cv3 = x.cv[g.cv(i, j, k, 3)], cv4 = x.cv[g.cv(i, j, k, 4)];
+  const double rc0 = RC(i, j, k);
+  const unsigned mk = x.lf.fx ? dirty_bits(cv1, cv2, cv3, cv4, rc0, K(i, j, k)) : 0u;
   // u: SET_CONVECT, SET_CONDUCTION, u_l, SET_EXPLICIT, solve_u_temp divide
   {
     double uv = conv_u_r(U, V, W, m, r, i, j, k, dt);
     double ud = cond_u_r(U, r, i, j, k, x.c0);
-    double cv2 = x.cv[g.cv(i, j, k, 2)];
     double ul = eta * cv2 * x.u_ta[o] * dt;
     double ua = x.a1 * uv + x.a2 * x.u_pre[o] + x.b1 * ud + x.b1 * ul;
     x.u_pre[o] = uv;
@@ -51,7 +56,6 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
   {
     double vv = conv_v_r(U, V, W, m, r, i, j, k, dt);
     double vd = cond_v_r(V, r, i, j, k, x.c0);
-    double cv3 = x.cv[g.cv(i, j, k, 3)];
     double vl = eta * cv3 * x.v_ta[o] * dt;
     double va = x.a1 * vv + x.a2 * x.v_pre[o] + x.b1 * vd + x.b1 * vl;
     x.v_pre[o] = vv;
@@ -63,7 +67,6 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
   {
     double wv = conv_w_r(U, V, W, m, r, i, j, k, dt);
     double wd = cond_w_r(W, r, i, j, k, x.c0);
-    double cv4 = x.cv[g.cv(i, j, k, 4)];
     double wl = eta * cv4 * x.w_ta[o] * dt;
     double wa = x.a1 * wv + x.a2 * x.w_pre[o] + x.b1 * wd + x.b1 * wl;
     x.w_pre[o] = wv;
@@ -71,13 +74,34 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
   }
   // T: convection, diffusion, T_l = 0, combine, history, main-loop divide
   {
-    double rrc = 1.0 / RC(i, j, k);
+    double rrc = 1.0 / rc0;
     double vt = conv_scalar_r(U, V, W, T, RC, m, r, i, j, k, dt, rrc);
     double dtt = diff_scalar_r(T, K, m, r, i, j, k, x.cd, rrc);
     double ut = x.a1 * vt + x.a2 * x.t_pre[o] + x.b1 * dtt;
     x.t_pre[o] = vt;
-    x.ut[o] = pen_div(x.t[o] + ut, eta, x.cv[g.cv(i, j, k, 1)], dt);
+    x.ut[o] = pen_div(x.t[o] + ut, eta, cv1, dt);
   }
+  if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
+}
+
+__global__ void k_fill_u32(unsigned* p, long long n) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x)
+    p[q] = 0u;
+}
+
+int line_flags_clear(const Geom& g, LineFlags& lf, hipStream_t s) {
+  const long long nxl = (long long)g.ny * g.nz, nyl = (long long)g.nx * g.nz,
+                  nzl = (long long)g.nx * g.ny, n = nxl + nyl + nzl;
+  unsigned* w = static_cast<unsigned*>(workspace(kWorkFlags, (size_t)n * sizeof(unsigned)));
+  if (!w) return VK_ERR_HIP;
+  lf.fx = w;
+  lf.fy = w + nxl;
+  lf.fz = w + nxl + nyl;
+  ProfScope ps("line_flags", s);
+  const long long nb = (n + 255) / 256;
+  k_fill_u32<<<(unsigned)(nb < 1024 ? nb : 1024), 256, 0, s>>>(w, n);
+  return VK_OK;
 }
 
 void launch_explicit_fused(const Geom& g, const Metrics& m, const RMetrics& r,
@@ -167,18 +191,25 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
   RMetrics r;
   int rrc = make_rmetrics(g, m, r, s);
   if (rrc) return rrc;
+  // the explicit phase also marks the lines with solid, rhoCp or thc
+  // points, which the sweeps then send to their general path
+  LineFlags lf{};
+  rrc = line_flags_clear(g, lf, s);
+  if (rrc) return rrc;
+  x.lf = lf;
   launch_explicit_fused(g, m, r, x, s);
 
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
   double* comps[3] = {dut, dvt, dwt};
   for (int c = 0; c < 3; c++) {
-    int lrc = launch_adi_velocity(g, m, r, comps[c], dcv, c, beta_u, dteta, s);
+    int lrc = launch_adi_velocity(g, m, r, comps[c], dcv, c, beta_u, dteta, s, &lf);
     if (lrc) return lrc;
   }
   {
     // T = T work after the sweeps (nse_dominik.f:429-437): the y sweep
     // writes T's interior and the x and z sweeps run there
-    int lrc = launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s, dtt);
+    int lrc =
+        launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s, dtt, &lf);
     if (lrc) return lrc;
   }
   launch_set_bc(g, dut, dvt, dwt, dp, dtt, s);

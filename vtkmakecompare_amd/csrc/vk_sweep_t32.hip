@@ -5,9 +5,10 @@ namespace vk {
 
 template int launch_temperature_sweep<float>(const Geom&, const Metrics&, const RMetrics&,
                                              float*, const float*, const float*, const float*,
-                                             int, double, double, hipStream_t, float*);
+                                             int, double, double, hipStream_t, float*,
+                                             const LineFlags*);
 template int launch_adi_temperature<float>(const Geom&, const Metrics&, const RMetrics&, float*,
                                            const float*, const float*, const float*, double,
-                                           double, hipStream_t, float*);
+                                           double, hipStream_t, float*, const LineFlags*);
 
 }  // namespace vk

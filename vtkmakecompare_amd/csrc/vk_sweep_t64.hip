@@ -6,10 +6,10 @@ namespace vk {
 template int launch_temperature_sweep<double>(const Geom&, const Metrics&, const RMetrics&,
                                               double*, const double*, const double*,
                                               const double*, int, double, double, hipStream_t,
-                                              double*);
+                                              double*, const LineFlags*);
 template int launch_adi_temperature<double>(const Geom&, const Metrics&, const RMetrics&,
                                             double*, const double*, const double*,
                                             const double*, double, double, hipStream_t,
-                                            double*);
+                                            double*, const LineFlags*);
 
 }  // namespace vk

@@ -4,8 +4,10 @@
 namespace vk {
 
 template int launch_velocity_sweep<double>(const Geom&, const Metrics&, const RMetrics&, double*,
-                                           const double*, int, int, double, double, hipStream_t);
+                                           const double*, int, int, double, double, hipStream_t,
+                                           const LineFlags*);
 template int launch_adi_velocity<double>(const Geom&, const Metrics&, const RMetrics&, double*,
-                                         const double*, int, double, double, hipStream_t);
+                                         const double*, int, double, double, hipStream_t,
+                                         const LineFlags*);
 
 }  // namespace vk
