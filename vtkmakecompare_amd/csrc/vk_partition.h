@@ -581,7 +581,7 @@ __global__ __launch_bounds__(NT) void k_line_fast(Sys sys, const double* __restr
   // a block takes GPB consecutive groups of lines and stages the table once
   // (x lines: four lines per block would otherwise stage a table larger
   // than their data; y and z blocks hold 16-32 lines and take one group)
-  constexpr int GPB = LANE_LINES ? 1 : 4;
+  constexpr int GPB = LANE_LINES ? 1 : 8;
   const unsigned first = xcd_linear(blockIdx.x, gridDim.x) * (unsigned)GPB;
   const M mp(threadIdx.x);
   const int seg = mp.seg;
@@ -775,7 +775,7 @@ inline int line_sweep(const Sys& sys, int n, int nla, int nlb, hipStream_t s) {
     const unsigned nla_pad = (unsigned)nbx * (unsigned)LPB_F;
     LineQueue Q = line_queue(nla_pad * (unsigned)nlb, s);
     if (!Q.list) return -2;
-    constexpr unsigned GPB = LANE_LINES ? 1u : 4u;  // groups per block, as in k_line_fast
+    constexpr unsigned GPB = LANE_LINES ? 1u : 8u;  // groups per block, as in k_line_fast
     const unsigned ngb = (nb + GPB - 1) / GPB;
     k_line_factor<SF, MF, Sys><<<1, SF, 0, s>>>(sys, tab, L, fresh ? 1 : 0);
     k_line_fast<SF, MF, NT, LANE_LINES, Sys><<<ngb, NT, lds, s>>>(sys, tab, L, Q, nbx, nb);
