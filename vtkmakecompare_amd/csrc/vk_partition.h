@@ -369,12 +369,12 @@ __device__ inline void pcr_apply(LdsVec<V> ld, LSlot<S> sl, int seg, bool cyclic
 }
 
 // Lines the fast kernel leaves to the general one: `count` entries of
-// `list`, each a line number la + nla_pad * lb, and `done`, which the
-// general kernel's blocks count up so that the last one can clear both
-// counters for the next sweep on the stream.
+// `list`, each the first line la + nla_pad * lb of a group of adjacent
+// lines (one general-kernel block).  `next` is the counter of the next
+// queued sweep on the stream, which this sweep's general kernel clears.
 struct LineQueue {
   unsigned* count;
-  unsigned* done;
+  unsigned* next;
   unsigned* list;
 };
 
@@ -426,9 +426,15 @@ __device__ inline unsigned xcd_linear(unsigned p, unsigned nb) {
 // thread's line is (la, lb), a valid line; `ok` says whether its solution
 // is stored (threads past the last line load a valid line and store
 // nothing).
-template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
+//
+// With ONLY_DIRTY the lines whose rows all have the reference inputs (the
+// ones the recorded-multiplier path has stored) are solved but not
+// stored: the test is the one k_line_fast makes (the step's dirty-line
+// mask when the system has one, else every row's inputs).  `lcl` holds
+// one int per line of the block.
+template <int S, int MMAX, int NT, bool LANE_LINES, bool ONLY_DIRTY, typename Sys>
 __device__ inline void line_block_general(const Sys& sys, int la, int lb, bool ok, double* s_a,
-                                          double* s_b, double* s_c, double* s_d) {
+                                          double* s_b, double* s_c, double* s_d, int* lcl) {
   using V = typename Sys::V;
   using Pt = typename Sys::Pt;
   using M = LineMap<S, NT, LANE_LINES>;
@@ -447,8 +453,11 @@ __device__ inline void line_block_general(const Sys& sys, int la, int lb, bool o
   // through a three-row window (previous, current, next row)
   V d[MMAX];
   double a[MMAX], b[MMAX], c[MMAX];
+  if (ONLY_DIRTY && mp.seg == 0) lcl[mp.line] = 1;
+  int clean = 1;
   Pt prev = Sys::kNb ? sys.pt(la, lb, lo - 1) : Pt();
   Pt cur = sys.pt(la, lb, lo);
+  if (ONLY_DIRTY && Sys::kNb) clean &= int(sys.pt_ref(prev));
 #pragma unroll
   for (int r = 0; r < MMAX; r++) {
     const int q = lo + (r < m ? r : m - 1);
@@ -456,11 +465,16 @@ __device__ inline void line_block_general(const Sys& sys, int la, int lb, bool o
     Pt next = Pt();
     if (Sys::kNb) next = sys.pt(la, lb, r < m - 1 ? q + 1 : lo + m);
     sys.coef(la, lb, q, cur, prev, next, a[r], b[r], c[r]);
+    if (ONLY_DIRTY) {
+      clean &= int(sys.pt_ref(cur));
+      if (Sys::kNb && r == m - 1) clean &= int(sys.pt_ref(next));
+    }
     if (r + 1 < MMAX) {
       prev = cur;
       cur = Sys::kNb ? next : sys.pt(la, lb, lo + (r + 1 < m ? r + 1 : m - 1));
     }
   }
+  if (ONLY_DIRTY && sys.lflag) clean = int((sys.lflag[la + sys.lstride * lb] & sys.lbit) == 0u);
   TabLayout none{};
   segment_eliminate<MMAX, false, V>(lo, m, mp.seg, a, b, c, d, nullptr, none);
   const LSlot<S> sl = LANE_LINES ? LSlot<S>{mp.line, M::LPB} : LSlot<S>{mp.line * P, 1};
@@ -485,8 +499,13 @@ __device__ inline void line_block_general(const Sys& sys, int la, int lb, bool o
   s_c[i1] = clast;
   ld.set(i1, dlast);
   __syncthreads();
+  if (ONLY_DIRTY && !clean) lcl[mp.line] = 0;
   V xf, xl;
   pcr_pair<S, false, V>(s_a, s_b, s_c, ld, sl, mp.seg, Sys::kCyclic, xf, xl, nullptr, none);
+  if (ONLY_DIRTY) {
+    __syncthreads();
+    ok = ok && lcl[mp.line] == 0;
+  }
   if (ok) {
 #pragma unroll
     for (int r = 0; r < MMAX; r++) {
@@ -520,13 +539,16 @@ __global__ __launch_bounds__(NT) void k_line_general(Sys sys, int nbx, int nby) 
   const M mp(threadIdx.x);
   const int la = bx * LPB + mp.line;
   const bool ok = sys.valid(la, by);
-  line_block_general<S, MMAX, NT, LANE_LINES, Sys>(sys, ok ? la : bx * LPB, by, ok, s_a, s_b,
-                                                   s_c, s_d);
+  line_block_general<S, MMAX, NT, LANE_LINES, false, Sys>(sys, ok ? la : bx * LPB, by, ok, s_a,
+                                                          s_b, s_c, s_d, nullptr);
 }
 
-// The general solve of the lines the fast kernel queued, LPB at a time; the
-// grid strides over the queue and the last block to finish clears the
-// queue's counters.
+// The general solve of the line groups the fast kernel queued: each entry
+// is the first line (la0, lb) of LPB adjacent lines, which a block loads as
+// the fast kernel does (coalesced) and of which it stores the lines the
+// fast kernel left.  The grid strides over the queue; block 0 clears the
+// next sweep's counter (the previous sweep's general kernel, which read it,
+// has finished, and the next sweep's fast kernel has not started).
 template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
 __global__ __launch_bounds__(NT) void k_line_general_q(Sys sys, LineQueue Q, unsigned nla_pad) {
   using V = typename Sys::V;
@@ -535,25 +557,18 @@ __global__ __launch_bounds__(NT) void k_line_general_q(Sys sys, LineQueue Q, uns
   constexpr int P = 2 * S;
   __shared__ double s_a[LPB * P], s_b[LPB * P], s_c[LPB * P];
   __shared__ double s_d[LPB * P * vwords<V>()];
+  __shared__ int lcl[LPB];
   const unsigned count = __atomic_load_n(Q.count, __ATOMIC_RELAXED);
-  const unsigned groups = (count + LPB - 1) / LPB;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *Q.next = 0u;
   const M mp(threadIdx.x);
-  for (unsigned gi = blockIdx.x; gi < groups; gi += gridDim.x) {
-    const unsigned e = gi * LPB + (unsigned)mp.line;
-    const bool ok = e < count;
-    const unsigned id = Q.list[ok ? e : gi * LPB];
-    line_block_general<S, MMAX, NT, LANE_LINES, Sys>(sys, (int)(id % nla_pad),
-                                                     (int)(id / nla_pad), ok, s_a, s_b, s_c,
-                                                     s_d);
+  for (unsigned gi = blockIdx.x; gi < count; gi += gridDim.x) {
+    const unsigned id = Q.list[gi];
+    const int la0 = (int)(id % nla_pad), lb = (int)(id / nla_pad);
+    const int la = la0 + mp.line;
+    const bool ok = sys.valid(la, lb);
+    line_block_general<S, MMAX, NT, LANE_LINES, true, Sys>(sys, ok ? la : la0, lb, ok, s_a, s_b,
+                                                           s_c, s_d, lcl);
     __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned prev = atomicAdd(Q.done, 1u);
-    if (prev == gridDim.x - 1) {
-      atomicExch(Q.count, 0u);
-      atomicExch(Q.done, 0u);
-    }
   }
 }
 
@@ -565,7 +580,7 @@ __global__ __launch_bounds__(NT) void k_line_general_q(Sys sys, LineQueue Q, uns
 // reference coefficients bit for bit, so it performs the same operations on
 // d as the general path.  Any other line is queued for k_line_general_q and
 // not written here.
-template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
+template <int S, int MMAX, int NT, bool LANE_LINES, int LPBG, typename Sys>
 __global__ __launch_bounds__(NT) void k_line_fast(Sys sys, const double* __restrict__ tab,
                                                   TabLayout L, LineQueue Q, int nbx,
                                                   unsigned ngrp) {
@@ -665,8 +680,17 @@ __global__ __launch_bounds__(NT) void k_line_fast(Sys sys, const double* __restr
     ld.set(sl.at(2 * seg + 1), dlast);
     __syncthreads();
     const bool line_clean = lclean[mp.line] != 0;
-    if (!line_clean && seg == 0 && ok)
-      Q.list[atomicAdd(Q.count, 1u)] = (unsigned)la + (unsigned)(nbx * M::LPB) * (unsigned)lb;
+    // queue each group of LPBG adjacent lines (the general kernel's block)
+    // that holds a valid line left unsolved here
+    static_assert(M::LPB % LPBG == 0, "general groups must tile the fast block");
+    if ((int)threadIdx.x < M::LPB / LPBG) {
+      const int l0 = (int)threadIdx.x * LPBG, la0 = bx * M::LPB + l0;
+      int dirty = 0;
+#pragma unroll
+      for (int q = 0; q < LPBG; q++) dirty |= int(lclean[l0 + q] == 0 && sys.valid(la0 + q, lb));
+      if (dirty)
+        Q.list[atomicAdd(Q.count, 1u)] = (unsigned)la0 + (unsigned)(nbx * M::LPB) * (unsigned)lb;
+    }
     V xf, xl;
     pcr_apply<S, MMAX, V>(ld, sl, seg, Sys::kCyclic, xf, xl, st, T);
     if (ok && line_clean) {
@@ -748,10 +772,10 @@ __global__ __launch_bounds__(S) void k_line_factor(Sys sys, double* tab, TabLayo
 // least `nlines` entries.
 LineQueue line_queue(unsigned nlines, hipStream_t s);
 
-// Host cache of reference-line tables, keyed by the system's description.
-// A stale table is harmless: blocks whose rows do not match its recorded
-// coefficients take the general path.
-double* line_table(const void* key, size_t keylen, long total, bool* fresh);
+// Host cache of reference-line tables, keyed by the system's description;
+// *validated is set when the table was validated earlier in the same
+// entry-point call (whose metric arrays and scalars do not change).
+double* line_table(const void* key, size_t keylen, long total, bool* fresh, bool* validated);
 
 // One sweep: the general kernel uses SG segments of at most MG rows; the
 // fast kernel SF segments of at most MF rows (fewer, longer segments: its
@@ -761,13 +785,13 @@ template <bool LANE_LINES, int SG, int MG, int SF, int MF, int NT, typename Sys>
 inline int line_sweep(const Sys& sys, int n, int nla, int nlb, hipStream_t s) {
   constexpr int LPB_G = NT / SG, LPB_F = NT / SF;
   double* tab = nullptr;
-  bool fresh = false;
+  bool fresh = false, validated = false;
   TabLayout L = tab_layout(n, SF, Sys::kCyclic);
   const size_t lds = sizeof(double) * ((size_t)LdsTab<SF, MF>(L.steps).total +
                                        (size_t)LPB_F * 2 * SF * vwords<typename Sys::V>());
   if (Sys::kHasRef && lds <= 65536 && (n + SF - 1) / SF <= MF && n >= 2 * SF) {
     auto key = sys.ref_key(n, SF);
-    tab = line_table(&key, sizeof(key), L.total, &fresh);  // null: general path only
+    tab = line_table(&key, sizeof(key), L.total, &fresh, &validated);  // null: general only
   }
   if (tab) {
     const int nbx = (nla + LPB_F - 1) / LPB_F;
@@ -777,8 +801,8 @@ inline int line_sweep(const Sys& sys, int n, int nla, int nlb, hipStream_t s) {
     if (!Q.list) return -2;
     constexpr unsigned GPB = LANE_LINES ? 1u : 8u;  // groups per block, as in k_line_fast
     const unsigned ngb = (nb + GPB - 1) / GPB;
-    k_line_factor<SF, MF, Sys><<<1, SF, 0, s>>>(sys, tab, L, fresh ? 1 : 0);
-    k_line_fast<SF, MF, NT, LANE_LINES, Sys><<<ngb, NT, lds, s>>>(sys, tab, L, Q, nbx, nb);
+    if (!validated) k_line_factor<SF, MF, Sys><<<1, SF, 0, s>>>(sys, tab, L, fresh ? 1 : 0);
+    k_line_fast<SF, MF, NT, LANE_LINES, LPB_G, Sys><<<ngb, NT, lds, s>>>(sys, tab, L, Q, nbx, nb);
     const unsigned ngen = nb < 1024u ? nb : 1024u;
     k_line_general_q<SG, MG, NT, LANE_LINES, Sys><<<ngen, NT, 0, s>>>(sys, Q, nla_pad);
   } else {

@@ -52,6 +52,11 @@ int fail(int code, const std::string& msg) {
 
 int last_status() { return g_last_status; }
 
+unsigned long long& call_epoch() {
+  static unsigned long long e = 0;
+  return e;
+}
+
 int hip_check(hipError_t e, const char* what) {
   if (e == hipSuccess) return VK_OK;
   return fail(VK_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));

@@ -23,6 +23,9 @@ Runtime& rt();
 int fail(int code, const std::string& msg);
 // the code of the last fail(): what resolve() refused an array with
 int last_status();
+// counts entry-point calls (every C-ABI entry point bumps it on entry):
+// state derived from a call's arguments can be reused within one call
+unsigned long long& call_epoch();
 // Checks a HIP status; records and returns VK_ERR_HIP on failure.
 int hip_check(hipError_t e, const char* what);
 
@@ -69,6 +72,7 @@ struct ProfScope {
   do {                                                                      \
     if (!vk::rt().initialised)                                              \
       return vk::fail(VK_ERR_NOT_INITIALISED, "vk_init has not been called"); \
+    ++vk::call_epoch();                                                     \
   } while (0)
 
 #define VK_RESOLVE(var, type, ptr, bytes)                                   \

@@ -15,26 +15,41 @@ namespace vk {
 // the fast kernel's queue: two counters, each on its own 128-byte line,
 // then the list; cleared when allocated, afterwards by the general kernel
 LineQueue line_queue(unsigned nlines, hipStream_t s) {
+  // two counters, used by alternate queued sweeps: the general kernel of
+  // one sweep clears the other's, which the next sweep then fills
+  static unsigned parity = 0;
   bool grown = false;
   unsigned* p = static_cast<unsigned*>(
       workspace(kWorkQueue, sizeof(unsigned) * (64 + (size_t)nlines), &grown));
   if (!p) return LineQueue{nullptr, nullptr, nullptr};
   if (grown && hipMemsetAsync(p, 0, 64 * sizeof(unsigned), s) != hipSuccess)
     return LineQueue{nullptr, nullptr, nullptr};
-  return LineQueue{p, p + 32, p + 64};
+  parity ^= 1u;
+  return LineQueue{p + 32 * parity, p + 32 * (parity ^ 1u), p + 64};
 }
 
-// reference-line tables, created on first use of a key
-double* line_table(const void* key, size_t keylen, long total, bool* fresh) {
-  static std::map<std::string, double*> cache;
+// reference-line tables, created on first use of a key; *validated says
+// whether the table was already checked in this entry-point call
+double* line_table(const void* key, size_t keylen, long total, bool* fresh, bool* validated) {
+  struct Entry {
+    double* p;
+    unsigned long long epoch;
+  };
+  static std::map<std::string, Entry> cache;
   std::string k(static_cast<const char*>(key), keylen);
   auto it = cache.find(k);
   *fresh = false;
-  if (it != cache.end()) return it->second;
+  const unsigned long long now = call_epoch();
+  if (it != cache.end()) {
+    *validated = it->second.epoch == now;
+    it->second.epoch = now;
+    return it->second.p;
+  }
   double* p = nullptr;
   if (hipMalloc(&p, sizeof(double) * (size_t)total) != hipSuccess) return nullptr;
-  cache[k] = p;
+  cache[k] = Entry{p, now};
   *fresh = true;
+  *validated = false;
   return p;
 }
 
