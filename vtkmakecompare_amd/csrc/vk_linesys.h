@@ -111,8 +111,9 @@ struct SysYFace {
     coef(q, p.cv, a, b, c);
   }
   __device__ void coef_ref(int q, double& a, double& b, double& c) const { coef(q, 0.0, a, b, c); }
-  // the reference rows do not depend on the cellvalue slot: u and w share
-  RefKey ref_key(int nn, int S) const { return RefKey{1, nn, S, 0, beta, dteta, mkey}; }
+  // one table per component: the components' sweeps run on streams of
+  // their own, and each validates its table in its own stream order
+  RefKey ref_key(int nn, int S) const { return RefKey{1, nn, S, ncv, beta, dteta, mkey}; }
   // element of row q, and whether the solve reads row q of its input
   __device__ unsigned addr(int la, int lb, int q) const { return g.at32(la + 1, q, lb + 1); }
   __device__ bool in_row(int q) const { return q >= 1 && q <= g.ny; }
@@ -294,8 +295,8 @@ struct SysPeriodicFace {
     coef(r, p.cv, a, b, c);
   }
   __device__ void coef_ref(int r, double& a, double& b, double& c) const { coef(r, 0.0, a, b, c); }
-  // shared by u, v and w (the reference rows do not depend on the slot)
-  RefKey ref_key(int nn, int S) const { return RefKey{4 + ax.dir, nn, S, 0, beta, dteta, mkey}; }
+  // one table per component (see SysYFace)
+  RefKey ref_key(int nn, int S) const { return RefKey{4 + ax.dir, nn, S, ncv, beta, dteta, mkey}; }
   __device__ unsigned addr(int la, int lb, int r) const { return ax.at(la, lb, r + 1); }
   __device__ bool in_row(int) const { return true; }
   __device__ void store(int la, int lb, int r, double x) const {
@@ -356,7 +357,7 @@ struct SysPeriodicT {
   __device__ void coef_ref(int r, double& a, double& b, double& c) const {
     coef(r, 0.0, 1.0, 1.0, 1.0, 1.0, a, b, c);
   }
-  RefKey ref_key(int nn, int S) const { return RefKey{6 + ax.dir, nn, S, 1, beta, dteta, d}; }
+  RefKey ref_key(int nn, int S) const { return RefKey{10 + ax.dir, nn, S, 1, beta, dteta, d}; }
   __device__ unsigned addr(int la, int lb, int r) const { return ax.at(la, lb, r + 1); }
   __device__ bool in_row(int) const { return true; }
   __device__ void store(int la, int lb, int r, double x) const {

@@ -224,12 +224,17 @@ extern "C" int vk_time_step_mixed(double* u, double* v, double* w, double* p, fl
   }
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
   double* comps[3] = {dut, dvt, dwt};
-  for (int c = 0; c < 3; c++) {
-    rc = launch_adi_velocity<float>(g, m, r, comps[c], dcv, c, beta_u, dteta, s, &lf);
-    if (rc) return rc;
-  }
-  // T = T work after the sweeps: the y sweep writes T's interior
-  rc = launch_adi_temperature<float>(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s, dtt, &lf);
+  // u, v, w on the auxiliary streams, T here; T = T work after the sweeps:
+  // the y sweep writes T's interior
+  rc = fork_join(
+      s,
+      [&](int c, hipStream_t sc) {
+        return launch_adi_velocity<float>(g, m, r, comps[c], dcv, c, beta_u, dteta, sc, &lf);
+      },
+      [&]() {
+        return launch_adi_temperature<float>(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s,
+                                             dtt, &lf);
+      });
   if (rc) return rc;
   launch_set_bc<float>(g, dut, dvt, dwt, dp, dtt, s);
   MkpOutMixed o{dp, duta, dvta, dwta, ddiv, du, dv, dw};

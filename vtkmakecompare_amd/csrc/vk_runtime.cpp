@@ -202,6 +202,15 @@ int vk_init(const int* device) {
   if (rc) return rc;
   rc = hip_check(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking), "hipStreamCreate");
   if (rc) return rc;
+  for (int q = 0; q < 3; q++) {
+    rc = hip_check(hipStreamCreateWithFlags(&r.aux[q], hipStreamNonBlocking), "hipStreamCreate");
+    if (rc) return rc;
+    rc = hip_check(hipEventCreateWithFlags(&r.ev_join[q], hipEventDisableTiming),
+                   "hipEventCreate");
+    if (rc) return rc;
+  }
+  rc = hip_check(hipEventCreateWithFlags(&r.ev_fork, hipEventDisableTiming), "hipEventCreate");
+  if (rc) return rc;
   r.device = dev;
   r.initialised = true;
   return VK_OK;
@@ -219,6 +228,15 @@ int vk_finalize(void) {
       w.ptr = nullptr;
       w.bytes = 0;
     }
+  for (int q = 0; q < 3; q++) {
+    (void)hipStreamSynchronize(r.aux[q]);
+    (void)hipStreamDestroy(r.aux[q]);
+    (void)hipEventDestroy(r.ev_join[q]);
+    r.aux[q] = nullptr;
+    r.ev_join[q] = nullptr;
+  }
+  (void)hipEventDestroy(r.ev_fork);
+  r.ev_fork = nullptr;
   (void)hipStreamDestroy(r.stream);
   r.stream = nullptr;
   r.initialised = false;

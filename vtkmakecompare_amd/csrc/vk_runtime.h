@@ -16,6 +16,13 @@ struct Runtime {
   bool initialised = false;
   int device = 0;
   hipStream_t stream = nullptr;
+  // three more streams (default priority) and the events that fork work
+  // from `stream` onto them and join it back: the fused steps solve the
+  // velocity components' ADI sweeps on these while the temperature's run
+  // on `stream`
+  hipStream_t aux[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
 };
 Runtime& rt();
 
@@ -28,6 +35,32 @@ int last_status();
 unsigned long long& call_epoch();
 // Checks a HIP status; records and returns VK_ERR_HIP on failure.
 int hip_check(hipError_t e, const char* what);
+
+// Runs vel(c, stream) for c = 0, 1, 2 on the auxiliary streams and then
+// temp() on `s`, with the auxiliary streams starting after everything
+// enqueued on `s` so far and `s` continuing after all of them (also when a
+// launch fails).  Returns the first error.
+template <typename VelFn, typename TFn>
+int fork_join(hipStream_t s, VelFn&& vel, TFn&& temp) {
+  Runtime& R = rt();
+  int err = hip_check(hipEventRecord(R.ev_fork, s), "fork");
+  if (err) return err;
+  int forked = 0;
+  for (int c = 0; c < 3 && !err; c++) {
+    err = hip_check(hipStreamWaitEvent(R.aux[c], R.ev_fork, 0), "fork wait");
+    if (!err) err = vel(c, R.aux[c]);
+    const int e2 = hip_check(hipEventRecord(R.ev_join[c], R.aux[c]), "join");
+    forked = c + 1;
+    if (!err) err = e2;
+  }
+  if (!err) err = temp();
+  for (int c = 0; c < forked; c++) {
+    const int e2 = hip_check(hipStreamWaitEvent(s, R.ev_join[c], 0), "join wait");
+    if (!err) err = e2;
+  }
+  return err;
+}
+
 
 // Resolves a caller pointer to device memory holding at least `bytes`:
 // a registered host array (or an address inside one) maps to its mirror,

@@ -201,17 +201,18 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
 
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
   double* comps[3] = {dut, dvt, dwt};
-  for (int c = 0; c < 3; c++) {
-    int lrc = launch_adi_velocity(g, m, r, comps[c], dcv, c, beta_u, dteta, s, &lf);
-    if (lrc) return lrc;
-  }
-  {
-    // T = T work after the sweeps (nse_dominik.f:429-437): the y sweep
-    // writes T's interior and the x and z sweeps run there
-    int lrc =
-        launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s, dtt, &lf);
-    if (lrc) return lrc;
-  }
+  // the four components' sweeps are independent: u, v, w on the auxiliary
+  // streams, T here; T = T work after the sweeps (nse_dominik.f:429-437):
+  // the y sweep writes T's interior and the x and z sweeps run there
+  rrc = fork_join(
+      s,
+      [&](int c, hipStream_t sc) {
+        return launch_adi_velocity(g, m, r, comps[c], dcv, c, beta_u, dteta, sc, &lf);
+      },
+      [&]() {
+        return launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s, dtt, &lf);
+      });
+  if (rrc) return rrc;
   launch_set_bc(g, dut, dvt, dwt, dp, dtt, s);
   MkpOut o{dp, duta, dvta, dwta, de1, de2, de3, ddiv, du, dv, dw};
   int rc = launch_mkp(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s);

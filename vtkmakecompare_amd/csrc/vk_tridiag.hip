@@ -15,17 +15,35 @@ namespace vk {
 // the fast kernel's queue: two counters, each on its own 128-byte line,
 // then the list; cleared when allocated, afterwards by the general kernel
 LineQueue line_queue(unsigned nlines, hipStream_t s) {
-  // two counters, used by alternate queued sweeps: the general kernel of
-  // one sweep clears the other's, which the next sweep then fills
-  static unsigned parity = 0;
-  bool grown = false;
-  unsigned* p = static_cast<unsigned*>(
-      workspace(kWorkQueue, sizeof(unsigned) * (64 + (size_t)nlines), &grown));
-  if (!p) return LineQueue{nullptr, nullptr, nullptr};
-  if (grown && hipMemsetAsync(p, 0, 64 * sizeof(unsigned), s) != hipSuccess)
-    return LineQueue{nullptr, nullptr, nullptr};
-  parity ^= 1u;
-  return LineQueue{p + 32 * parity, p + 32 * (parity ^ 1u), p + 64};
+  // one queue per stream (sweeps on different streams run concurrently),
+  // each with two counters used by alternate queued sweeps on that stream:
+  // the general kernel of one sweep clears the other's, which the next
+  // sweep then fills
+  struct Q {
+    unsigned* p = nullptr;
+    size_t cap = 0;
+    unsigned parity = 0;
+  };
+  static std::map<hipStream_t, Q> queues;
+  Q& q = queues[s];
+  if (q.cap < nlines || !q.p) {
+    if (q.p) {
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(q.p);
+      q.p = nullptr;
+      q.cap = 0;
+    }
+    void* p = nullptr;
+    if (hip_check(hipMalloc(&p, sizeof(unsigned) * (64 + (size_t)nlines)), "line queue")) {
+      return LineQueue{nullptr, nullptr, nullptr};
+    }
+    q.p = static_cast<unsigned*>(p);
+    q.cap = nlines;
+    if (hip_check(hipMemsetAsync(q.p, 0, 64 * sizeof(unsigned), s), "line queue"))
+      return LineQueue{nullptr, nullptr, nullptr};
+  }
+  q.parity ^= 1u;
+  return LineQueue{q.p + 32 * q.parity, q.p + 32 * (q.parity ^ 1u), q.p + 64};
 }
 
 // reference-line tables, created on first use of a key; *validated says
