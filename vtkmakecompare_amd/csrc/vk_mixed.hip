@@ -223,22 +223,31 @@ extern "C" int vk_time_step_mixed(double* u, double* v, double* w, double* p, fl
     k_explicit_mixed<<<interior_grid(g), interior_block(), 0, s>>>(g, f, x);
   }
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
-  double* comps[3] = {dut, dvt, dwt};
-  // u, v, w on the auxiliary streams, T here; T = T work after the sweeps:
-  // the y sweep writes T's interior
-  rc = fork_join(
-      s,
-      [&](int c, hipStream_t sc) {
-        return launch_adi_velocity<float>(g, m, r, comps[c], dcv, c, beta_u, dteta, sc, &lf);
-      },
-      [&]() {
-        return launch_adi_temperature<float>(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s,
-                                             dtt, &lf);
-      });
+  // u and v on auxiliary streams 0 and 1, w here, T (its sweeps, then its
+  // halo) on stream 2 beside MKP until the end of the step; the y sweep of
+  // T writes T's interior
+  AuxFork fk(s, 3);
+  if (fk.err) return fk.err;
+  for (int c = 0; c < 2; c++) {
+    rc = launch_adi_velocity<float>(g, m, r, c == 0 ? dut : dvt, dcv, c, beta_u, dteta,
+                                    fk.aux(c), &lf);
+    if (rc) return rc;
+  }
+  rc = launch_adi_temperature<float>(g, m, r, twork, dcv, drc, dk, beta_t, dteta, fk.aux(2),
+                                     dtt, &lf);
   if (rc) return rc;
-  launch_set_bc<float>(g, dut, dvt, dwt, dp, dtt, s);
+  launch_set_bc<float>(g, nullptr, nullptr, nullptr, nullptr, dtt, fk.aux(2));
+  rc = launch_adi_velocity<float>(g, m, r, dwt, dcv, 2, beta_u, dteta, s, &lf);
+  if (rc) return rc;
+  for (int c = 0; c < 2; c++) {
+    rc = fk.join(c);
+    if (rc) return rc;
+  }
+  launch_set_bc<float>(g, dut, dvt, dwt, dp, nullptr, s);
   MkpOutMixed o{dp, duta, dvta, dwta, ddiv, du, dv, dw};
   rc = launch_mkp_mixed(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s);
+  if (rc) return rc;
+  rc = fk.join(2);
   if (rc) return rc;
   VK_LAUNCH_CHECK("time_step_mixed");
   return VK_OK;

@@ -125,7 +125,8 @@ void* workspace(WorkSlot slot, size_t bytes, bool* grown) {
   if (grown) *grown = false;
   if (w.bytes >= bytes && w.ptr) return w.ptr;
   if (w.ptr) {
-    (void)hipStreamSynchronize(rt().stream);
+    // every stream of the library may still read the old buffer
+    (void)hipDeviceSynchronize();
     (void)hipFree(w.ptr);
     w.ptr = nullptr;
     w.bytes = 0;

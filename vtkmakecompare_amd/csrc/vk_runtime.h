@@ -36,30 +36,33 @@ unsigned long long& call_epoch();
 // Checks a HIP status; records and returns VK_ERR_HIP on failure.
 int hip_check(hipError_t e, const char* what);
 
-// Runs vel(c, stream) for c = 0, 1, 2 on the auxiliary streams and then
-// temp() on `s`, with the auxiliary streams starting after everything
-// enqueued on `s` so far and `s` continuing after all of them (also when a
-// launch fails).  Returns the first error.
-template <typename VelFn, typename TFn>
-int fork_join(hipStream_t s, VelFn&& vel, TFn&& temp) {
-  Runtime& R = rt();
-  int err = hip_check(hipEventRecord(R.ev_fork, s), "fork");
-  if (err) return err;
-  int forked = 0;
-  for (int c = 0; c < 3 && !err; c++) {
-    err = hip_check(hipStreamWaitEvent(R.aux[c], R.ev_fork, 0), "fork wait");
-    if (!err) err = vel(c, R.aux[c]);
-    const int e2 = hip_check(hipEventRecord(R.ev_join[c], R.aux[c]), "join");
-    forked = c + 1;
-    if (!err) err = e2;
+// Forks work from `s` onto the first `naux` auxiliary streams: they start
+// after everything enqueued on `s` so far.  join(c) makes `s` continue only
+// after aux[c]'s work; the destructor joins every stream not yet joined, so
+// `s` waits for all of them on every path out of the caller.
+struct AuxFork {
+  hipStream_t s;
+  bool pending[3] = {false, false, false};
+  int err = 0;
+  AuxFork(hipStream_t s_, int naux) : s(s_) {
+    err = hip_check(hipEventRecord(rt().ev_fork, s), "fork");
+    for (int c = 0; c < naux && c < 3 && !err; c++) {
+      err = hip_check(hipStreamWaitEvent(rt().aux[c], rt().ev_fork, 0), "fork wait");
+      pending[c] = !err;
+    }
   }
-  if (!err) err = temp();
-  for (int c = 0; c < forked; c++) {
-    const int e2 = hip_check(hipStreamWaitEvent(s, R.ev_join[c], 0), "join wait");
-    if (!err) err = e2;
+  hipStream_t aux(int c) const { return rt().aux[c]; }
+  int join(int c) {
+    if (!pending[c]) return 0;
+    pending[c] = false;
+    int e = hip_check(hipEventRecord(rt().ev_join[c], rt().aux[c]), "join");
+    if (!e) e = hip_check(hipStreamWaitEvent(s, rt().ev_join[c], 0), "join wait");
+    return e;
   }
-  return err;
-}
+  ~AuxFork() {
+    for (int c = 0; c < 3; c++) (void)join(c);
+  }
+};
 
 
 // Resolves a caller pointer to device memory holding at least `bytes`:

@@ -388,9 +388,30 @@ int make_rmetrics(const Geom& g, const Metrics& m, RMetrics& r, hipStream_t s) {
   int ax = g.nx + 2, ay = g.ny + 2, az = g.nz + 2;
   double* w = static_cast<double*>(workspace(kWorkRecip, sizeof(double) * 2 * (ax + ay + az)));
   if (!w) return VK_ERR_HIP;
-  int nmax = ax > ay ? ax : ay;
-  nmax = nmax > az ? nmax : az;
-  k_rmetrics<<<(nmax + 255) / 256, 256, 0, s>>>(m, g.nx, g.ny, g.nz, w);
+  // within one entry-point call the metric arrays do not change: the
+  // reciprocals made earlier in the call are reused (and are not rewritten
+  // while kernels on other streams read them)
+  static struct {
+    unsigned long long epoch = 0;
+    Metrics m{};
+    int nx = 0, ny = 0, nz = 0;
+    double* w = nullptr;
+  } made;
+  const bool same = made.epoch == call_epoch() && made.w == w && made.nx == g.nx &&
+                    made.ny == g.ny && made.nz == g.nz && made.m.dx == m.dx &&
+                    made.m.dy == m.dy && made.m.dz == m.dz && made.m.dx12 == m.dx12 &&
+                    made.m.dy12 == m.dy12 && made.m.dz12 == m.dz12;
+  if (!same) {
+    int nmax = ax > ay ? ax : ay;
+    nmax = nmax > az ? nmax : az;
+    k_rmetrics<<<(nmax + 255) / 256, 256, 0, s>>>(m, g.nx, g.ny, g.nz, w);
+    made.epoch = call_epoch();
+    made.m = m;
+    made.nx = g.nx;
+    made.ny = g.ny;
+    made.nz = g.nz;
+    made.w = w;
+  }
   r.rdx = w;
   r.rdx12 = w + ax;
   r.rdy = w + 2 * ax;

@@ -200,22 +200,32 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
   launch_explicit_fused(g, m, r, x, s);
 
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
-  double* comps[3] = {dut, dvt, dwt};
-  // the four components' sweeps are independent: u, v, w on the auxiliary
-  // streams, T here; T = T work after the sweeps (nse_dominik.f:429-437):
-  // the y sweep writes T's interior and the x and z sweeps run there
-  rrc = fork_join(
-      s,
-      [&](int c, hipStream_t sc) {
-        return launch_adi_velocity(g, m, r, comps[c], dcv, c, beta_u, dteta, sc, &lf);
-      },
-      [&]() {
-        return launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, s, dtt, &lf);
-      });
+  // The four components' sweeps are independent, and the temperature's
+  // are not needed by MKP: u and v run on auxiliary streams 0 and 1, w
+  // here, and T (its sweeps, then its halo) on stream 2 until the end of
+  // the step, beside MKP.  T = T work after the sweeps
+  // (nse_dominik.f:429-437): the y sweep writes T's interior and the x and
+  // z sweeps run there.
+  AuxFork fk(s, 3);
+  if (fk.err) return fk.err;
+  for (int c = 0; c < 2; c++) {
+    rrc = launch_adi_velocity(g, m, r, c == 0 ? dut : dvt, dcv, c, beta_u, dteta, fk.aux(c), &lf);
+    if (rrc) return rrc;
+  }
+  rrc = launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, fk.aux(2), dtt, &lf);
   if (rrc) return rrc;
-  launch_set_bc(g, dut, dvt, dwt, dp, dtt, s);
+  launch_set_bc<double>(g, nullptr, nullptr, nullptr, nullptr, dtt, fk.aux(2));
+  rrc = launch_adi_velocity(g, m, r, dwt, dcv, 2, beta_u, dteta, s, &lf);
+  if (rrc) return rrc;
+  for (int c = 0; c < 2; c++) {
+    rrc = fk.join(c);
+    if (rrc) return rrc;
+  }
+  launch_set_bc<double>(g, dut, dvt, dwt, dp, nullptr, s);
   MkpOut o{dp, duta, dvta, dwta, de1, de2, de3, ddiv, du, dv, dw};
   int rc = launch_mkp(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s);
+  if (rc) return rc;
+  rc = fk.join(2);
   if (rc) return rc;
   VK_LAUNCH_CHECK("time_step");
   return VK_OK;
