@@ -61,8 +61,11 @@ int vk_extent(const void *p, size_t *bytes);
 /* Kernel timing on the library stream with HIP events, for benchmarks:
    between vk_profile_begin and vk_profile_end every kernel launch is
    bracketed by an event pair under its kernel's name; vk_profile_entry
-   returns the summed duration and the launch count per name.  No
-   reference counterpart (the reference prints system_clock deltas). */
+   returns the summed duration and the launch count per name.  While
+   timing, the fused steps issue the work they otherwise spread over
+   several streams on the library stream, one kernel after another, so
+   each duration is the kernel's own.  No reference counterpart (the
+   reference prints system_clock deltas). */
 int vk_profile_begin(void);
 int vk_profile_end(void);
 int vk_profile_count(int *n);

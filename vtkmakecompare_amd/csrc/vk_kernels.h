@@ -152,8 +152,9 @@ struct ExplicitArgs {
 };
 // the scalar part of ExplicitArgs for step l (AB2 weights, CN factors)
 ExplicitArgs explicit_args(double dt, double re, double sc, double eta, int l);
+// part 1: the velocity components, part 2: the temperature, 0: both
 void launch_explicit_fused(const Geom& g, const Metrics& m, const RMetrics& r,
-                           const ExplicitArgs& x, hipStream_t s);
+                           const ExplicitArgs& x, hipStream_t s, int part = 0);
 // the spectrum (double) of f(i,j,k) / dy(j), index 0 of x and z read as
 // the periodic image n (for the transposed Poisson solve)
 int launch_p2s_wscaled(const Geom& g, const double* f, const double* rdy, double* phit,

@@ -91,6 +91,9 @@ __device__ inline R pen_div_m(R x, R eta, R cv, R dt) {
 
 // The explicit phase of one step: the same terms as k_explicit_fused
 // (vk_step.hip), from the FP64 velocity read as float.
+// PART 1: u, v, w; PART 2: T (disjoint outputs, so the two may run
+// concurrently)
+template <int PART>
 __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, ExplicitMixedArgs x) {
   int bx, by, bz;
   xcd_block(bx, by, bz);
@@ -102,46 +105,50 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
   F3<float> T{x.t, g}, RC{x.rc, g}, K{x.thc, g};
   const long long o = g.at(i, j, k);
   const float dt = x.dt, eta = x.eta;
-  const float cv1 = x.cv[g.cv(i, j, k, 1)], cv2 = x.cv[g.cv(i, j, k, 2)],
-              cv3 = x.cv[g.cv(i, j, k, 3)], cv4 = x.cv[g.cv(i, j, k, 4)];
-  const float rc0 = RC(i, j, k);
-  const unsigned mk = x.lf.fx ? dirty_bits(cv1, cv2, cv3, cv4, rc0, K(i, j, k)) : 0u;
-  {
-    const float uv = conv_u_r<float>(U, V, W, f, f, i, j, k, dt);
-    const float ud = cond_u_r<float>(U, f, i, j, k, x.c0);
-    const float ul = eta * cv2 * x.u_ta[o] * dt;
-    const float ua = x.a1 * uv + x.a2 * x.u_pre[o] + x.b1 * ud + x.b1 * ul;
-    x.u_pre[o] = uv;
-    x.ua[o] = pen_div_m<double>(x.u[o] + (double)ua, x.etad, (double)cv2, x.dtd);
-  }
-  {
-    const float vv = conv_v_r<float>(U, V, W, f, f, i, j, k, dt);
-    const float vd = cond_v_r<float>(V, f, i, j, k, x.c0);
-    const float vl = eta * cv3 * x.v_ta[o] * dt;
-    const float va = x.a1 * vv + x.a2 * x.v_pre[o] + x.b1 * vd + x.b1 * vl;
-    x.v_pre[o] = vv;
-    x.va[o] = pen_div_m<double>(x.v[o] + (double)va, x.etad, (double)cv3, x.dtd);
-    // SET_EXPLICIT's reflection v_a(i,ny+1,k) = -v_a(i,ny,k) of the
-    // increment (nse_dominik.f:1132-1286)
-    if (j == g.ny) x.va[g.at(i, g.ny + 1, k)] = -(double)va;
-  }
-  {
-    const float wv = conv_w_r<float>(U, V, W, f, f, i, j, k, dt);
-    const float wd = cond_w_r<float>(W, f, i, j, k, x.c0);
-    const float wl = eta * cv4 * x.w_ta[o] * dt;
-    const float wa = x.a1 * wv + x.a2 * x.w_pre[o] + x.b1 * wd + x.b1 * wl;
-    x.w_pre[o] = wv;
-    x.wa[o] = pen_div_m<double>(x.w[o] + (double)wa, x.etad, (double)cv4, x.dtd);
-  }
-  {
+  if (PART == 1) {
+    const float cv2 = x.cv[g.cv(i, j, k, 2)], cv3 = x.cv[g.cv(i, j, k, 3)],
+                cv4 = x.cv[g.cv(i, j, k, 4)];
+    const unsigned mk = x.lf.fx ? dirty_bits(0.0f, cv2, cv3, cv4, 1.0f, 1.0f) : 0u;
+    {
+      const float uv = conv_u_r<float>(U, V, W, f, f, i, j, k, dt);
+      const float ud = cond_u_r<float>(U, f, i, j, k, x.c0);
+      const float ul = eta * cv2 * x.u_ta[o] * dt;
+      const float ua = x.a1 * uv + x.a2 * x.u_pre[o] + x.b1 * ud + x.b1 * ul;
+      x.u_pre[o] = uv;
+      x.ua[o] = pen_div_m<double>(x.u[o] + (double)ua, x.etad, (double)cv2, x.dtd);
+    }
+    {
+      const float vv = conv_v_r<float>(U, V, W, f, f, i, j, k, dt);
+      const float vd = cond_v_r<float>(V, f, i, j, k, x.c0);
+      const float vl = eta * cv3 * x.v_ta[o] * dt;
+      const float va = x.a1 * vv + x.a2 * x.v_pre[o] + x.b1 * vd + x.b1 * vl;
+      x.v_pre[o] = vv;
+      x.va[o] = pen_div_m<double>(x.v[o] + (double)va, x.etad, (double)cv3, x.dtd);
+      // SET_EXPLICIT's reflection v_a(i,ny+1,k) = -v_a(i,ny,k) of the
+      // increment (nse_dominik.f:1132-1286)
+      if (j == g.ny) x.va[g.at(i, g.ny + 1, k)] = -(double)va;
+    }
+    {
+      const float wv = conv_w_r<float>(U, V, W, f, f, i, j, k, dt);
+      const float wd = cond_w_r<float>(W, f, i, j, k, x.c0);
+      const float wl = eta * cv4 * x.w_ta[o] * dt;
+      const float wa = x.a1 * wv + x.a2 * x.w_pre[o] + x.b1 * wd + x.b1 * wl;
+      x.w_pre[o] = wv;
+      x.wa[o] = pen_div_m<double>(x.w[o] + (double)wa, x.etad, (double)cv4, x.dtd);
+    }
+    if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
+  } else {
+    const float cv1 = x.cv[g.cv(i, j, k, 1)];
+    const float rc0 = RC(i, j, k);
+    const unsigned mk = x.lf.fx ? dirty_bits(cv1, 0.0f, 0.0f, 0.0f, rc0, K(i, j, k)) : 0u;
     const float rrc = 1.0f / rc0;
     const float vt = conv_scalar_r<float>(U, V, W, T, RC, f, f, i, j, k, dt, rrc);
     const float dtt = diff_scalar_r<float>(T, K, f, f, i, j, k, x.cd, rrc);
     const float ut = x.a1 * vt + x.a2 * x.t_pre[o] + x.b1 * dtt;
     x.t_pre[o] = vt;
     x.ut[o] = pen_div_m<float>(x.t[o] + ut, eta, cv1, dt);
+    if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
   }
-  if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
 }
 
 }  // namespace vk
@@ -218,36 +225,45 @@ extern "C" int vk_time_step_mixed(double* u, double* v, double* w, double* p, fl
   x.b1 = 1.0f;
   x.c0 = (float)(*dt / *re / 2.0);
   x.cd = (float)(*dt / (2.0 * *re * *sc));
-  {
-    ProfScope ps("explicit_mixed", s);
-    k_explicit_mixed<<<interior_grid(g), interior_block(), 0, s>>>(g, f, x);
-  }
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
-  // u and v on auxiliary streams 0 and 1, w here, T (its sweeps, then its
-  // halo) on stream 2 beside MKP until the end of the step; the y sweep of
-  // T writes T's interior
-  AuxFork fk(s, 3);
-  if (fk.err) return fk.err;
-  for (int c = 0; c < 2; c++) {
-    rc = launch_adi_velocity<float>(g, m, r, c == 0 ? dut : dvt, dcv, c, beta_u, dteta,
-                                    fk.aux(c), &lf);
-    if (rc) return rc;
+  // T's explicit part, sweeps and halo on auxiliary stream 2 beside the
+  // velocity's work and MKP (which waits until the T part has read u, v,
+  // w); the y sweep of T writes T's interior
+  AuxFork ft(s, 4u);
+  if (ft.err) return ft.err;
+  {
+    ProfScope ps("explicit_mixed_t", ft.aux(2));
+    k_explicit_mixed<2><<<interior_grid(g), interior_block(), 0, ft.aux(2)>>>(g, f, x);
   }
-  rc = launch_adi_temperature<float>(g, m, r, twork, dcv, drc, dk, beta_t, dteta, fk.aux(2),
+  rc = hip_check(hipEventRecord(rt().ev_mid, ft.aux(2)), "event");
+  if (rc) return rc;
+  rc = launch_adi_temperature<float>(g, m, r, twork, dcv, drc, dk, beta_t, dteta, ft.aux(2),
                                      dtt, &lf);
   if (rc) return rc;
-  launch_set_bc<float>(g, nullptr, nullptr, nullptr, nullptr, dtt, fk.aux(2));
-  rc = launch_adi_velocity<float>(g, m, r, dwt, dcv, 2, beta_u, dteta, s, &lf);
-  if (rc) return rc;
-  for (int c = 0; c < 2; c++) {
-    rc = fk.join(c);
+  launch_set_bc<float>(g, nullptr, nullptr, nullptr, nullptr, dtt, ft.aux(2));
+  {
+    ProfScope ps("explicit_mixed", s);
+    k_explicit_mixed<1><<<interior_grid(g), interior_block(), 0, s>>>(g, f, x);
+  }
+  // u and v on auxiliary streams 0 and 1, w here
+  {
+    AuxFork fv(s, 3u);
+    if (fv.err) return fv.err;
+    for (int c = 0; c < 2; c++) {
+      rc = launch_adi_velocity<float>(g, m, r, c == 0 ? dut : dvt, dcv, c, beta_u, dteta,
+                                      fv.aux(c), &lf);
+      if (rc) return rc;
+    }
+    rc = launch_adi_velocity<float>(g, m, r, dwt, dcv, 2, beta_u, dteta, s, &lf);
     if (rc) return rc;
   }
   launch_set_bc<float>(g, dut, dvt, dwt, dp, nullptr, s);
+  rc = hip_check(hipStreamWaitEvent(s, rt().ev_mid, 0), "event wait");
+  if (rc) return rc;
   MkpOutMixed o{dp, duta, dvta, dwta, ddiv, du, dv, dw};
   rc = launch_mkp_mixed(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s);
   if (rc) return rc;
-  rc = fk.join(2);
+  rc = ft.join(2);
   if (rc) return rc;
   VK_LAUNCH_CHECK("time_step_mixed");
   return VK_OK;

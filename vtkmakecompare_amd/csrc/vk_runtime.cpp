@@ -212,6 +212,8 @@ int vk_init(const int* device) {
   }
   rc = hip_check(hipEventCreateWithFlags(&r.ev_fork, hipEventDisableTiming), "hipEventCreate");
   if (rc) return rc;
+  rc = hip_check(hipEventCreateWithFlags(&r.ev_mid, hipEventDisableTiming), "hipEventCreate");
+  if (rc) return rc;
   r.device = dev;
   r.initialised = true;
   return VK_OK;
@@ -237,7 +239,9 @@ int vk_finalize(void) {
     r.ev_join[q] = nullptr;
   }
   (void)hipEventDestroy(r.ev_fork);
+  (void)hipEventDestroy(r.ev_mid);
   r.ev_fork = nullptr;
+  r.ev_mid = nullptr;
   (void)hipStreamDestroy(r.stream);
   r.stream = nullptr;
   r.initialised = false;
@@ -341,12 +345,14 @@ int vk_profile_begin(void) {
   if (rc) return rc;
   for (auto& ps : prof_slots()) ps.used = 0;
   g_prof_on = true;
+  rt().serial = true;
   return VK_OK;
 }
 
 int vk_profile_end(void) {
   VK_REQUIRE_INIT();
   g_prof_on = false;
+  rt().serial = false;
   return hip_check(hipStreamSynchronize(rt().stream), "vk_profile_end");
 }
 

@@ -23,6 +23,11 @@ struct Runtime {
   hipStream_t aux[3] = {nullptr, nullptr, nullptr};
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_mid = nullptr;  // a point within auxiliary stream 2's work
+  // set while kernels are being timed: the work meant for the auxiliary
+  // streams is issued on the forking stream instead, one kernel after
+  // another, so that each kernel's events time it alone
+  bool serial = false;
 };
 Runtime& rt();
 
@@ -36,22 +41,25 @@ unsigned long long& call_epoch();
 // Checks a HIP status; records and returns VK_ERR_HIP on failure.
 int hip_check(hipError_t e, const char* what);
 
-// Forks work from `s` onto the first `naux` auxiliary streams: they start
-// after everything enqueued on `s` so far.  join(c) makes `s` continue only
-// after aux[c]'s work; the destructor joins every stream not yet joined, so
-// `s` waits for all of them on every path out of the caller.
+// Forks work from `s` onto the auxiliary streams in `mask` (bit c for
+// aux[c]): they start after everything enqueued on `s` so far.  join(c)
+// makes `s` continue only after aux[c]'s work; the destructor joins every
+// stream not yet joined, so `s` waits for all of them on every path out of
+// the caller.
 struct AuxFork {
   hipStream_t s;
   bool pending[3] = {false, false, false};
   int err = 0;
-  AuxFork(hipStream_t s_, int naux) : s(s_) {
+  AuxFork(hipStream_t s_, unsigned mask) : s(s_) {
+    if (rt().serial) return;
     err = hip_check(hipEventRecord(rt().ev_fork, s), "fork");
-    for (int c = 0; c < naux && c < 3 && !err; c++) {
+    for (int c = 0; c < 3 && !err; c++) {
+      if (!(mask & (1u << c))) continue;
       err = hip_check(hipStreamWaitEvent(rt().aux[c], rt().ev_fork, 0), "fork wait");
       pending[c] = !err;
     }
   }
-  hipStream_t aux(int c) const { return rt().aux[c]; }
+  hipStream_t aux(int c) const { return rt().serial ? s : rt().aux[c]; }
   int join(int c) {
     if (!pending[c]) return 0;
     pending[c] = false;

@@ -27,6 +27,9 @@ __device__ inline double pen_div(double x, double eta, double cv, double dt) {
   return x / (1.0 + eta * cv * dt);
 }
 
+// PART 1: u, v, w; PART 2: T.  The two parts read the same velocity and
+// write disjoint arrays, so they may run concurrently.
+template <int PART>
 __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetrics r,
                                                         ExplicitArgs x) {
   int bx, by, bz;
@@ -38,50 +41,54 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
   F3<double> U{x.u, g}, V{x.v, g}, W{x.w, g}, T{x.t, g}, RC{x.rc, g}, K{x.thc, g};
   const long long o = g.at(i, j, k);
   const double dt = x.dt, eta = x.eta;
-  // the point's cellvalue slots, rhoCp and thc, read before any store (the
-  // outputs may alias them as far as the compiler knows)
-  const double cv1 = x.cv[g.cv(i, j, k, 1)], cv2 = x.cv[g.cv(i, j, k, 2)],
-               cv3 = x.cv[g.cv(i, j, k, 3)], cv4 = x.cv[g.cv(i, j, k, 4)];
-  const double rc0 = RC(i, j, k);
-  const unsigned mk = x.lf.fx ? dirty_bits(cv1, cv2, cv3, cv4, rc0, K(i, j, k)) : 0u;
-  // u: SET_CONVECT, SET_CONDUCTION, u_l, SET_EXPLICIT, solve_u_temp divide
-  {
-    double uv = conv_u_r(U, V, W, m, r, i, j, k, dt);
-    double ud = cond_u_r(U, r, i, j, k, x.c0);
-    double ul = eta * cv2 * x.u_ta[o] * dt;
-    double ua = x.a1 * uv + x.a2 * x.u_pre[o] + x.b1 * ud + x.b1 * ul;
-    x.u_pre[o] = uv;
-    x.ua[o] = pen_div(x.u[o] + ua, eta, cv2, dt);
-  }
-  {
-    double vv = conv_v_r(U, V, W, m, r, i, j, k, dt);
-    double vd = cond_v_r(V, r, i, j, k, x.c0);
-    double vl = eta * cv3 * x.v_ta[o] * dt;
-    double va = x.a1 * vv + x.a2 * x.v_pre[o] + x.b1 * vd + x.b1 * vl;
-    x.v_pre[o] = vv;
-    x.va[o] = pen_div(x.v[o] + va, eta, cv3, dt);
-    // SET_EXPLICIT's reflection v_a(i,ny+1,k) = -v_a(i,ny,k) survives into
-    // v_temp (no later routine writes that plane of v)
-    if (j == g.ny) x.va[g.at(i, g.ny + 1, k)] = -1.0 * va;
-  }
-  {
-    double wv = conv_w_r(U, V, W, m, r, i, j, k, dt);
-    double wd = cond_w_r(W, r, i, j, k, x.c0);
-    double wl = eta * cv4 * x.w_ta[o] * dt;
-    double wa = x.a1 * wv + x.a2 * x.w_pre[o] + x.b1 * wd + x.b1 * wl;
-    x.w_pre[o] = wv;
-    x.wa[o] = pen_div(x.w[o] + wa, eta, cv4, dt);
-  }
-  // T: convection, diffusion, T_l = 0, combine, history, main-loop divide
-  {
+  if (PART == 1) {
+    // the point's cellvalue slots, read before any store (the outputs may
+    // alias them as far as the compiler knows)
+    const double cv2 = x.cv[g.cv(i, j, k, 2)], cv3 = x.cv[g.cv(i, j, k, 3)],
+                 cv4 = x.cv[g.cv(i, j, k, 4)];
+    const unsigned mk = x.lf.fx ? dirty_bits(0.0, cv2, cv3, cv4, 1.0, 1.0) : 0u;
+    // u: SET_CONVECT, SET_CONDUCTION, u_l, SET_EXPLICIT, solve_u_temp divide
+    {
+      double uv = conv_u_r(U, V, W, m, r, i, j, k, dt);
+      double ud = cond_u_r(U, r, i, j, k, x.c0);
+      double ul = eta * cv2 * x.u_ta[o] * dt;
+      double ua = x.a1 * uv + x.a2 * x.u_pre[o] + x.b1 * ud + x.b1 * ul;
+      x.u_pre[o] = uv;
+      x.ua[o] = pen_div(x.u[o] + ua, eta, cv2, dt);
+    }
+    {
+      double vv = conv_v_r(U, V, W, m, r, i, j, k, dt);
+      double vd = cond_v_r(V, r, i, j, k, x.c0);
+      double vl = eta * cv3 * x.v_ta[o] * dt;
+      double va = x.a1 * vv + x.a2 * x.v_pre[o] + x.b1 * vd + x.b1 * vl;
+      x.v_pre[o] = vv;
+      x.va[o] = pen_div(x.v[o] + va, eta, cv3, dt);
+      // SET_EXPLICIT's reflection v_a(i,ny+1,k) = -v_a(i,ny,k) survives into
+      // v_temp (no later routine writes that plane of v)
+      if (j == g.ny) x.va[g.at(i, g.ny + 1, k)] = -1.0 * va;
+    }
+    {
+      double wv = conv_w_r(U, V, W, m, r, i, j, k, dt);
+      double wd = cond_w_r(W, r, i, j, k, x.c0);
+      double wl = eta * cv4 * x.w_ta[o] * dt;
+      double wa = x.a1 * wv + x.a2 * x.w_pre[o] + x.b1 * wd + x.b1 * wl;
+      x.w_pre[o] = wv;
+      x.wa[o] = pen_div(x.w[o] + wa, eta, cv4, dt);
+    }
+    if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
+  } else {
+    // T: convection, diffusion, T_l = 0, combine, history, main-loop divide
+    const double cv1 = x.cv[g.cv(i, j, k, 1)];
+    const double rc0 = RC(i, j, k);
+    const unsigned mk = x.lf.fx ? dirty_bits(cv1, 0.0, 0.0, 0.0, rc0, K(i, j, k)) : 0u;
     double rrc = 1.0 / rc0;
     double vt = conv_scalar_r(U, V, W, T, RC, m, r, i, j, k, dt, rrc);
     double dtt = diff_scalar_r(T, K, m, r, i, j, k, x.cd, rrc);
     double ut = x.a1 * vt + x.a2 * x.t_pre[o] + x.b1 * dtt;
     x.t_pre[o] = vt;
     x.ut[o] = pen_div(x.t[o] + ut, eta, cv1, dt);
+    if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
   }
-  if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
 }
 
 __global__ void k_fill_u32(unsigned* p, long long n) {
@@ -105,9 +112,15 @@ int line_flags_clear(const Geom& g, LineFlags& lf, hipStream_t s) {
 }
 
 void launch_explicit_fused(const Geom& g, const Metrics& m, const RMetrics& r,
-                           const ExplicitArgs& x, hipStream_t s) {
-  ProfScope ps("explicit_fused", s);
-  k_explicit_fused<<<interior_grid(g), interior_block(), 0, s>>>(g, m, r, x);
+                           const ExplicitArgs& x, hipStream_t s, int part) {
+  if (part != 2) {
+    ProfScope ps("explicit_vel", s);
+    k_explicit_fused<1><<<interior_grid(g), interior_block(), 0, s>>>(g, m, r, x);
+  }
+  if (part != 1) {
+    ProfScope ps("explicit_t", s);
+    k_explicit_fused<2><<<interior_grid(g), interior_block(), 0, s>>>(g, m, r, x);
+  }
 }
 
 ExplicitArgs explicit_args(double dt, double re, double sc, double eta, int l) {
@@ -197,35 +210,41 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
   rrc = line_flags_clear(g, lf, s);
   if (rrc) return rrc;
   x.lf = lf;
-  launch_explicit_fused(g, m, r, x, s);
-
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
-  // The four components' sweeps are independent, and the temperature's
-  // are not needed by MKP: u and v run on auxiliary streams 0 and 1, w
-  // here, and T (its sweeps, then its halo) on stream 2 until the end of
-  // the step, beside MKP.  T = T work after the sweeps
+  // The temperature's explicit part, sweeps and halo run on auxiliary
+  // stream 2 beside the velocity's work and MKP (which waits only until the
+  // T part has read u, v, w).  T = T work after the sweeps
   // (nse_dominik.f:429-437): the y sweep writes T's interior and the x and
   // z sweeps run there.
-  AuxFork fk(s, 3);
-  if (fk.err) return fk.err;
-  for (int c = 0; c < 2; c++) {
-    rrc = launch_adi_velocity(g, m, r, c == 0 ? dut : dvt, dcv, c, beta_u, dteta, fk.aux(c), &lf);
-    if (rrc) return rrc;
-  }
-  rrc = launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, fk.aux(2), dtt, &lf);
+  AuxFork ft(s, 4u);
+  if (ft.err) return ft.err;
+  launch_explicit_fused(g, m, r, x, ft.aux(2), 2);
+  rrc = hip_check(hipEventRecord(rt().ev_mid, ft.aux(2)), "event");
   if (rrc) return rrc;
-  launch_set_bc<double>(g, nullptr, nullptr, nullptr, nullptr, dtt, fk.aux(2));
-  rrc = launch_adi_velocity(g, m, r, dwt, dcv, 2, beta_u, dteta, s, &lf);
+  rrc = launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, ft.aux(2), dtt, &lf);
   if (rrc) return rrc;
-  for (int c = 0; c < 2; c++) {
-    rrc = fk.join(c);
+  launch_set_bc<double>(g, nullptr, nullptr, nullptr, nullptr, dtt, ft.aux(2));
+  launch_explicit_fused(g, m, r, x, s, 1);
+  // the velocity components' sweeps are independent: u and v on auxiliary
+  // streams 0 and 1, w here
+  {
+    AuxFork fv(s, 3u);
+    if (fv.err) return fv.err;
+    for (int c = 0; c < 2; c++) {
+      rrc = launch_adi_velocity(g, m, r, c == 0 ? dut : dvt, dcv, c, beta_u, dteta, fv.aux(c),
+                                &lf);
+      if (rrc) return rrc;
+    }
+    rrc = launch_adi_velocity(g, m, r, dwt, dcv, 2, beta_u, dteta, s, &lf);
     if (rrc) return rrc;
   }
   launch_set_bc<double>(g, dut, dvt, dwt, dp, nullptr, s);
+  rrc = hip_check(hipStreamWaitEvent(s, rt().ev_mid, 0), "event wait");
+  if (rrc) return rrc;
   MkpOut o{dp, duta, dvta, dwta, de1, de2, de3, ddiv, du, dv, dw};
   int rc = launch_mkp(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s);
   if (rc) return rc;
-  rc = fk.join(2);
+  rc = ft.join(2);
   if (rc) return rc;
   VK_LAUNCH_CHECK("time_step");
   return VK_OK;
