@@ -163,11 +163,13 @@ def timed_run(lib, run, steps: int, warmup: int):
     run.step(steps)
     api.synchronize()
     t1 = time.perf_counter()
+    # --steps 0 (for kernel traces): the profiled steps only, no throughput
+    el = (t1 - t0) if steps > 0 else None
     _lib.check(lib.vk_profile_begin(), "profile")
     run.step(PROFILED_STEPS)
     api.synchronize()
     _lib.check(lib.vk_profile_end(), "profile")
-    return t1 - t0, profile_entries(lib)
+    return el, profile_entries(lib)
 
 
 def kernel_table(prof, g, with_ep2, mixed):
@@ -198,7 +200,8 @@ def roofline(prof, g, with_ep2, mixed, traffic_file):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=200,
+                    help="timed steps (0: only the profiled steps, for kernel traces)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="step384")
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -233,7 +236,8 @@ def main():
     if args.only in ("both", "fp64"):
         el, prof = timed_run(lib, r64, args.steps, args.warmup)
         res["fp64"] = (el, prof, r64)
-        print(f"[bench] fp64: {1e3 * el / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+        if el:
+            print(f"[bench] fp64: {1e3 * el / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
     if args.only in ("both", "mixed"):
         if args.only == "mixed":
             r64 = None
@@ -241,21 +245,25 @@ def main():
         rmx = api.ChannelRun(cfg, g, st, cellvalue=cv, precision="mixed")
         el, prof = timed_run(lib, rmx, args.steps, args.warmup)
         res["mixed"] = (el, prof, rmx)
-        print(f"[bench] mixed: {1e3 * el / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+        if el:
+            print(f"[bench] mixed: {1e3 * el / args.steps:.3f} ms/step", file=sys.stderr,
+                  flush=True)
 
     forms = {}
     for name, (el, prof, run) in res.items():
         mixed = name == "mixed"
         with_ep2 = args.diagnostics and not mixed
-        ms = 1e3 * el / args.steps
+        ms = 1e3 * el / args.steps if el else None
         sb = step_bytes(g, with_ep2, mixed)
         tfile = os.path.join(ROOT, "profiles", f"{cfg.name}_traffic.txt")
         forms[name] = {
-            "value": round(npts * args.steps / el / 1e6, 2), "ms_per_step": round(ms, 4),
+            "value": round(npts * args.steps / el / 1e6, 2) if el else None,
+            "ms_per_step": round(ms, 4) if ms else None,
             "roofline": roofline(prof, g, with_ep2, mixed, tfile),
             "step_roofline": {"alg_bytes_per_step": sb,
-                              "alg_GBs": round(sb / (ms / 1e3) / 1e9, 1),
-                              "frac": round(sb / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                              "alg_GBs": round(sb / (ms / 1e3) / 1e9, 1) if ms else None,
+                              "frac": round(sb / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+                              if ms else None,
                               # the kernels one after another (profiled steps);
                               # ms_per_step is less where streams overlap them
                               "kernel_time_ms_per_step":
@@ -309,7 +317,7 @@ def main():
         "forms": forms,
         "cpu_baseline": cpu,
     }
-    if "fp64" in forms and "mixed" in forms:
+    if "fp64" in forms and "mixed" in forms and forms["fp64"]["value"]:
         out["mixed_vs_fp64"] = round(forms["mixed"]["value"] / forms["fp64"]["value"], 4)
     print(json.dumps(out))
 
