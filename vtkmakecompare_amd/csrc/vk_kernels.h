@@ -126,9 +126,10 @@ int fft_sizes_ok(const Geom& g);
 struct MkpOut {
   double *p, *u_ta, *v_ta, *w_ta, *u_ep2, *v_ep2, *w_ep2, *divu, *u, *v, *w;
 };
+// before_project: an event the projection (which writes u, v, w) waits for
 int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const double* v_temp,
                const double* w_temp, double dt, double xl, double zl, const MkpOut& o,
-               hipStream_t s);
+               hipStream_t s, hipEvent_t before_project = nullptr);
 
 // the mixed-precision projection: FP64 velocity, p and divu, float u_ta
 struct MkpOutMixed {
@@ -138,7 +139,7 @@ struct MkpOutMixed {
 };
 int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
                      const double* v_temp, const double* w_temp, double dt, double xl, double zl,
-                     const MkpOutMixed& o, hipStream_t s);
+                     const MkpOutMixed& o, hipStream_t s, hipEvent_t before_project = nullptr);
 
 // explicit phase of the FP64 fused step (vk_step.hip): all explicit terms,
 // penalisation, AB2 and both divides; writes u_temp.. (ua, va, wa), the

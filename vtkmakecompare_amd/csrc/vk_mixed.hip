@@ -226,9 +226,13 @@ extern "C" int vk_time_step_mixed(double* u, double* v, double* w, double* p, fl
   x.c0 = (float)(*dt / *re / 2.0);
   x.cd = (float)(*dt / (2.0 * *re * *sc));
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
+  {
+    ProfScope ps("explicit_mixed", s);
+    k_explicit_mixed<1><<<interior_grid(g), interior_block(), 0, s>>>(g, f, x);
+  }
   // T's explicit part, sweeps and halo on auxiliary stream 2 beside the
-  // velocity's work and MKP (which waits until the T part has read u, v,
-  // w); the y sweep of T writes T's interior
+  // velocity sweeps and MKP, whose projection waits until the T part has
+  // read u, v, w; the y sweep of T writes T's interior
   AuxFork ft(s, 4u);
   if (ft.err) return ft.err;
   {
@@ -241,10 +245,6 @@ extern "C" int vk_time_step_mixed(double* u, double* v, double* w, double* p, fl
                                      dtt, &lf);
   if (rc) return rc;
   launch_set_bc<float>(g, nullptr, nullptr, nullptr, nullptr, dtt, ft.aux(2));
-  {
-    ProfScope ps("explicit_mixed", s);
-    k_explicit_mixed<1><<<interior_grid(g), interior_block(), 0, s>>>(g, f, x);
-  }
   // u and v on auxiliary streams 0 and 1, w here
   {
     AuxFork fv(s, 3u);
@@ -258,10 +258,9 @@ extern "C" int vk_time_step_mixed(double* u, double* v, double* w, double* p, fl
     if (rc) return rc;
   }
   launch_set_bc<float>(g, dut, dvt, dwt, dp, nullptr, s);
-  rc = hip_check(hipStreamWaitEvent(s, rt().ev_mid, 0), "event wait");
-  if (rc) return rc;
   MkpOutMixed o{dp, duta, dvta, dwta, ddiv, du, dv, dw};
-  rc = launch_mkp_mixed(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s);
+  rc = launch_mkp_mixed(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s,
+                        rt().serial ? nullptr : rt().ev_mid);
   if (rc) return rc;
   rc = ft.join(2);
   if (rc) return rc;

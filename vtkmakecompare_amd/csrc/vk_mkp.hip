@@ -110,7 +110,7 @@ __global__ void k_project_faces(Geom g, RMetrics r, const double* phi, const dou
 
 int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const double* v_temp,
                const double* w_temp, double dt, double xl, double zl, const MkpOut& o,
-               hipStream_t s) {
+               hipStream_t s, hipEvent_t before_project) {
   if (int frc = fft_sizes_ok(g)) return frc;
   const int nh = g.nx / 2;
   size_t nk = (size_t)(nh + 1) * (g.nz + 1) + 3 * (size_t)g.ny;
@@ -136,6 +136,10 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
   rc = launch_s2p_inplace<double>(g, phit, phi, s, "cfft_z_inv", "irfft_x");
   if (rc) return rc;
   launch_halo_neumann(g, phi, s);
+  if (before_project) {
+    rc = hip_check(hipStreamWaitEvent(s, before_project, 0), "mkp wait");
+    if (rc) return rc;
+  }
   ProfScope ps("project", s);
   k_project<<<interior_grid(g), interior_block(), 0, s>>>(g, r, phi, u_temp, v_temp, w_temp, dt,
                                                          o);
@@ -206,7 +210,7 @@ __global__ void k_project_mixed_faces(Geom g, RMetrics r, const float* phi1, con
 // solved the same way for delta.  The projection uses phi1 + delta in FP64.
 int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
                      const double* v_temp, const double* w_temp, double dt, double xl, double zl,
-                     const MkpOutMixed& o, hipStream_t s) {
+                     const MkpOutMixed& o, hipStream_t s, hipEvent_t before_project) {
   if (int frc = fft_sizes_ok(g)) return frc;
   const int nh = g.nx / 2;
   size_t nk = (size_t)(nh + 1) * (g.nz + 1) + 3 * (size_t)g.ny;
@@ -243,6 +247,10 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
   rc = launch_s2p_inplace<float>(g, phit, dphi, s, "cfft_z_inv2", "irfft_x2");
   if (rc) return rc;
   launch_halo_neumann<float>(g, dphi, s);
+  if (before_project) {
+    rc = hip_check(hipStreamWaitEvent(s, before_project, 0), "mkp wait");
+    if (rc) return rc;
+  }
   ProfScope ps("project_mixed", s);
   k_project_mixed<<<interior_grid(g), interior_block(), 0, s>>>(g, r, phi1, dphi, u_temp, v_temp,
                                                                w_temp, dt, o);

@@ -211,11 +211,12 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
   if (rrc) return rrc;
   x.lf = lf;
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
+  launch_explicit_fused(g, m, r, x, s, 1);
   // The temperature's explicit part, sweeps and halo run on auxiliary
-  // stream 2 beside the velocity's work and MKP (which waits only until the
-  // T part has read u, v, w).  T = T work after the sweeps
-  // (nse_dominik.f:429-437): the y sweep writes T's interior and the x and
-  // z sweeps run there.
+  // stream 2 beside the velocity sweeps and MKP, whose projection (which
+  // writes u, v, w) waits only until the T part has read them.  T = T work
+  // after the sweeps (nse_dominik.f:429-437): the y sweep writes T's
+  // interior and the x and z sweeps run there.
   AuxFork ft(s, 4u);
   if (ft.err) return ft.err;
   launch_explicit_fused(g, m, r, x, ft.aux(2), 2);
@@ -224,7 +225,6 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
   rrc = launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, ft.aux(2), dtt, &lf);
   if (rrc) return rrc;
   launch_set_bc<double>(g, nullptr, nullptr, nullptr, nullptr, dtt, ft.aux(2));
-  launch_explicit_fused(g, m, r, x, s, 1);
   // the velocity components' sweeps are independent: u and v on auxiliary
   // streams 0 and 1, w here
   {
@@ -239,10 +239,8 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
     if (rrc) return rrc;
   }
   launch_set_bc<double>(g, dut, dvt, dwt, dp, nullptr, s);
-  rrc = hip_check(hipStreamWaitEvent(s, rt().ev_mid, 0), "event wait");
-  if (rrc) return rrc;
   MkpOut o{dp, duta, dvta, dwta, de1, de2, de3, ddiv, du, dv, dw};
-  int rc = launch_mkp(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s);
+  int rc = launch_mkp(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s, rt().serial ? nullptr : rt().ev_mid);
   if (rc) return rc;
   rc = ft.join(2);
   if (rc) return rc;
