@@ -571,6 +571,11 @@ int taped_step(const Problem& P, Work& w, int l, double* const* tape, double* co
   x.u_ta = w.st[SUTA]; x.v_ta = w.st[SVTA]; x.w_ta = w.st[SWTA]; x.cv = P.cv;
   x.u_pre = w.st[SUP]; x.v_pre = w.st[SVP]; x.w_pre = w.st[SWP]; x.t_pre = w.st[STP];
   x.ua = w.ut; x.va = w.vt; x.wa = w.wt; x.ut = twork;
+  // the explicit kernels mark the lines next to the solid for the sweeps
+  LineFlags lf{};
+  rc = line_flags_clear(g, lf, s);
+  if (rc) return rc;
+  x.lf = lf;
   launch_explicit_fused(g, P.m, r, x, s);
   const double beta_u = P.dt / P.re / 2.0, beta_t = P.dt / P.re / P.sc / 2.0;
   const double dteta = P.dt * P.eta;
@@ -579,9 +584,9 @@ int taped_step(const Problem& P, Work& w, int l, double* const* tape, double* co
     copy(g, comps[c], tape[c * 4]);
     for (int d = 0; d < 3; d++) {
       rc = (c < 3) ? launch_velocity_sweep<double>(g, P.m, r, comps[c], P.cv, c, d, beta_u,
-                                                   dteta, s)
+                                                   dteta, s, &lf)
                    : launch_temperature_sweep<double>(g, P.m, r, twork, P.cv, P.rc, P.thc, d,
-                                                      beta_t, dteta, s);
+                                                      beta_t, dteta, s, nullptr, &lf);
       if (rc) return rc;
       copy(g, comps[c], tape[c * 4 + d + 1]);
     }
