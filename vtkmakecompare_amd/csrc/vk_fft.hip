@@ -154,14 +154,15 @@ __device__ typename Cx<R>::T* fft_lds(typename Cx<R>::T* a, typename Cx<R>::T* b
     const int nb = n / Rd;
     const int twstep = n / (ns * Rd);
     for (int j = t; j < nb; j += T) {
-      const int kk = j % ns;
+      const int jq = j / ns, kk = j - jq * ns;
       C v[4];
 #pragma unroll
       for (int r = 0; r < 4; r++)
         if (r < Rd) {
           v[r] = src[j + r * nb];
           if (r > 0) {
-            C w = p.tw[(r * kk * twstep) % n];
+            // r kk twstep < Rd ns n / (ns Rd) = n: no wrap
+            C w = p.tw[r * kk * twstep];
             if (sign > 0) w = conjc(w);
             v[r] = cmul(v[r], w);
           }
@@ -186,7 +187,7 @@ __device__ typename Cx<R>::T* fft_lds(typename Cx<R>::T* a, typename Cx<R>::T* b
         o[0] = cadd(v[0], v[1]);
         o[1] = csub(v[0], v[1]);
       }
-      const int base = (j / ns) * ns * Rd + kk;
+      const int base = jq * ns * Rd + kk;
 #pragma unroll
       for (int r = 0; r < 4; r++)
         if (r < Rd) dst[base + r * ns] = o[r];
