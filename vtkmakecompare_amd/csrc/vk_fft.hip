@@ -158,15 +158,19 @@ __device__ typename Cx<R>::T* fft_lds(typename Cx<R>::T* a, typename Cx<R>::T* b
       C v[4];
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        if (r < Rd) {
-          v[r] = src[j + r * nb];
-          if (r > 0) {
-            // r kk twstep < Rd ns n / (ns Rd) = n: no wrap
-            C w = p.tw[r * kk * twstep];
-            if (sign > 0) w = conjc(w);
-            v[r] = cmul(v[r], w);
-          }
+        if (r < Rd) v[r] = src[j + r * nb];
+      if (kk > 0) {
+        // one table read per butterfly: w^2 and w^3 by products (kk twstep
+        // < n / Rd, so the index never wraps)
+        C w1 = p.tw[kk * twstep];
+        if (sign > 0) w1 = conjc(w1);
+        v[1] = cmul(v[1], w1);
+        if (Rd > 2) {
+          const C w2 = cmul(w1, w1);
+          v[2] = cmul(v[2], w2);
+          if (Rd > 3) v[3] = cmul(v[3], cmul(w2, w1));
         }
+      }
       C o[4];
       if (Rd == 4) {
         C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
