@@ -141,6 +141,51 @@ __device__ inline C mul_si(C a, int sign) {
 // group.  sign = -1 forward, +1 backward, unnormalised.  Returns the
 // buffer holding the result (a or b).  All threads of the block must call
 // it together (it contains block barriers).
+// One Stockham butterfly in place on v0..v3 (radix Rd = 4, 3 or 2; v3, or
+// v2 and v3, unused below radix 4): the inputs times w^r for w =
+// tw[kk twstep] (conjugated for sign > 0; one table read, the higher
+// powers by products; kk twstep < n / Rd, so the index never wraps), then
+// the radix-Rd DFT.  Scalars rather than arrays, so that nothing is
+// addressed through memory.
+template <typename R>
+__device__ __forceinline__ void stockham_bfly(typename Cx<R>::T& v0, typename Cx<R>::T& v1,
+                                              typename Cx<R>::T& v2, typename Cx<R>::T& v3,
+                                              int Rd, int kk, int twstep,
+                                              const typename Cx<R>::T* tw, int sign) {
+  using C = typename Cx<R>::T;
+  if (kk > 0) {
+    C w1 = tw[kk * twstep];
+    if (sign > 0) w1 = conjc(w1);
+    v1 = cmul(v1, w1);
+    if (Rd > 2) {
+      const C w2 = cmul(w1, w1);
+      v2 = cmul(v2, w2);
+      if (Rd > 3) v3 = cmul(v3, cmul(w2, w1));
+    }
+  }
+  if (Rd == 4) {
+    const C s02 = cadd(v0, v2), d02 = csub(v0, v2);
+    const C s13 = cadd(v1, v3), d13 = mul_si(csub(v1, v3), sign);
+    v0 = cadd(s02, s13);
+    v2 = csub(s02, s13);
+    v1 = cadd(d02, d13);
+    v3 = csub(d02, d13);
+  } else if (Rd == 3) {
+    const R h = (R)0.86602540378443864676;  // sqrt(3)/2
+    const C sm = cadd(v1, v2), df = csub(v1, v2);
+    const C a0 = v0;
+    v0 = cadd(a0, sm);
+    const C tt = mkc<C>(a0.x - (R)0.5 * sm.x, a0.y - (R)0.5 * sm.y);
+    const C rot = mul_si(cscale(h, df), sign);
+    v1 = cadd(tt, rot);
+    v2 = csub(tt, rot);
+  } else {
+    const C a0 = v0;
+    v0 = cadd(a0, v1);
+    v1 = csub(a0, v1);
+  }
+}
+
 template <typename R>
 __device__ typename Cx<R>::T* fft_lds(typename Cx<R>::T* a, typename Cx<R>::T* b, int t, int T,
                                       const FftPlan<R>& p, int sign) {
@@ -155,46 +200,15 @@ __device__ typename Cx<R>::T* fft_lds(typename Cx<R>::T* a, typename Cx<R>::T* b
     const int twstep = n / (ns * Rd);
     for (int j = t; j < nb; j += T) {
       const int jq = j / ns, kk = j - jq * ns;
-      C v[4];
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-        if (r < Rd) v[r] = src[j + r * nb];
-      if (kk > 0) {
-        // one table read per butterfly: w^2 and w^3 by products (kk twstep
-        // < n / Rd, so the index never wraps)
-        C w1 = p.tw[kk * twstep];
-        if (sign > 0) w1 = conjc(w1);
-        v[1] = cmul(v[1], w1);
-        if (Rd > 2) {
-          const C w2 = cmul(w1, w1);
-          v[2] = cmul(v[2], w2);
-          if (Rd > 3) v[3] = cmul(v[3], cmul(w2, w1));
-        }
-      }
-      C o[4];
-      if (Rd == 4) {
-        C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-        C s13 = cadd(v[1], v[3]), d13 = mul_si(csub(v[1], v[3]), sign);
-        o[0] = cadd(s02, s13);
-        o[2] = csub(s02, s13);
-        o[1] = cadd(d02, d13);
-        o[3] = csub(d02, d13);
-      } else if (Rd == 3) {
-        const R h = (R)0.86602540378443864676;  // sqrt(3)/2
-        C sm = cadd(v[1], v[2]), df = csub(v[1], v[2]);
-        o[0] = cadd(v[0], sm);
-        C tt = mkc<C>(v[0].x - (R)0.5 * sm.x, v[0].y - (R)0.5 * sm.y);
-        C rot = mul_si(cscale(h, df), sign);
-        o[1] = cadd(tt, rot);
-        o[2] = csub(tt, rot);
-      } else {
-        o[0] = cadd(v[0], v[1]);
-        o[1] = csub(v[0], v[1]);
-      }
+      C v0 = src[j], v1 = src[j + nb], v2 = v0, v3 = v0;
+      if (Rd > 2) v2 = src[j + 2 * nb];
+      if (Rd > 3) v3 = src[j + 3 * nb];
+      stockham_bfly<R>(v0, v1, v2, v3, Rd, kk, twstep, p.tw, sign);
       const int base = jq * ns * Rd + kk;
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-        if (r < Rd) dst[base + r * ns] = o[r];
+      dst[base] = v0;
+      dst[base + ns] = v1;
+      if (Rd > 2) dst[base + 2 * ns] = v2;
+      if (Rd > 3) dst[base + 3 * ns] = v3;
     }
     __syncthreads();
     C* tmp = src;
@@ -313,6 +327,14 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_rfft_x(
 // Forward divides by nx*nz (p2s_fftw2d.f:101).  CC columns per block, so
 // that a block's loads cover whole 128-byte lines where the LDS allows
 // (8 float or 4 double columns at nz = 384); 256 / CC threads per column.
+// The z pass on CC adjacent columns (I0..I0+CC-1, j) of the spectrum.  The
+// lanes take the columns in turn (lane = c + CC t), so every global access
+// of a wave covers CC adjacent complex numbers per k.  The first Stockham
+// pass reads its inputs straight from the spectrum and the last writes its
+// outputs straight back (scaled by 1/(nx nz) on the forward pass); the
+// passes between ping-pong in LDS, one padded buffer pair per column.  The
+// first pass's reads all precede, across a barrier, the last pass's
+// writes, except with one pass, which is a single butterfly per column.
 template <typename R, int CC>
 __global__ __launch_bounds__(256) void k_cfft_z(Geom g, R* phit, FftPlan<R> p, int sign,
                                                 int scale) {
@@ -324,34 +346,68 @@ __global__ __launch_bounds__(256) void k_cfft_z(Geom g, R* phit, FftPlan<R> p, i
   const int nh1 = g.nx / 2 + 1;
   int bx, by, bz;
   xcd_block(bx, by, bz);
-  const int I0 = bx * CC, j = 1 + by;
-  // load: element e = c + CC*k
-  for (int e = threadIdx.x; e < CC * N; e += blockDim.x) {
-    int c = e % CC, k = e / CC;
-    int I = I0 + c;
-    C v = mkc<C>((R)0, (R)0);
-    if (I < nh1) {
-      long long off = 2 * g.spec_at(I, j, k);
-      v = mkc<C>(phit[off], phit[off + 1]);
+  const int c = threadIdx.x % CC, t = threadIdx.x / CC;
+  const int I = bx * CC + c, jrow = 1 + by;
+  const bool okc = I < nh1;
+  const long long o0 = 2 * g.spec_at(okc ? I : 0, jrow, 0);
+  const long long ks = 2 * g.spec_at(0, 0, 1);  // k stride of the spectrum
+  const R rsc = scale ? (R)(1.0 / ((double)g.nx * (double)g.nz)) : (R)1;
+  C* src = sh + c * (2 * N + 1);
+  C* dst = src + N;
+  int ns = 1;
+  for (int s = 0; s < p.nst; s++) {
+    const int Rd = p.r[s];
+    const int nb = N / Rd;
+    const int twstep = N / (ns * Rd);
+    const bool first = s == 0, last = s == p.nst - 1;
+    for (int jj = t; jj < nb; jj += TPC) {
+      const int jq = jj / ns, kk = jj - jq * ns;
+      C v0, v1, v2, v3;
+      if (first) {
+        const C z = mkc<C>((R)0, (R)0);
+        const long long off = o0 + ks * jj, st = ks * nb;
+        v0 = okc ? mkc<C>(phit[off], phit[off + 1]) : z;
+        v1 = okc ? mkc<C>(phit[off + st], phit[off + st + 1]) : z;
+        v2 = (okc && Rd > 2) ? mkc<C>(phit[off + 2 * st], phit[off + 2 * st + 1]) : z;
+        v3 = (okc && Rd > 3) ? mkc<C>(phit[off + 3 * st], phit[off + 3 * st + 1]) : z;
+      } else {
+        v0 = src[jj];
+        v1 = src[jj + nb];
+        v2 = Rd > 2 ? src[jj + 2 * nb] : v0;
+        v3 = Rd > 3 ? src[jj + 3 * nb] : v0;
+      }
+      stockham_bfly<R>(v0, v1, v2, v3, Rd, kk, twstep, p.tw, sign);
+      const int base = jq * ns * Rd + kk;
+      if (last) {
+        if (okc) {
+          const long long off = o0 + ks * base, st = ks * ns;
+          phit[off] = v0.x * rsc;
+          phit[off + 1] = v0.y * rsc;
+          phit[off + st] = v1.x * rsc;
+          phit[off + st + 1] = v1.y * rsc;
+          if (Rd > 2) {
+            phit[off + 2 * st] = v2.x * rsc;
+            phit[off + 2 * st + 1] = v2.y * rsc;
+          }
+          if (Rd > 3) {
+            phit[off + 3 * st] = v3.x * rsc;
+            phit[off + 3 * st + 1] = v3.y * rsc;
+          }
+        }
+      } else {
+        dst[base] = v0;
+        dst[base + ns] = v1;
+        if (Rd > 2) dst[base + 2 * ns] = v2;
+        if (Rd > 3) dst[base + 3 * ns] = v3;
+      }
     }
-    sh[c * 2 * N + k] = v;
-  }
-  __syncthreads();
-  const int c = threadIdx.x / TPC, t = threadIdx.x % TPC;
-  C* res = fft_lds<R>(sh + c * 2 * N, sh + c * 2 * N + N, t, TPC, p, sign);
-  // each column's result buffer is the same half for every column
-  const int half = (res == sh + c * 2 * N) ? 0 : N;
-  __syncthreads();
-  const R rsc = (R)(1.0 / ((double)g.nx * (double)g.nz));
-  for (int e = threadIdx.x; e < CC * N; e += blockDim.x) {
-    int cc = e % CC, k = e / CC;
-    int I = I0 + cc;
-    if (I >= nh1) continue;
-    C v = sh[cc * 2 * N + half + k];
-    if (scale) v = mkc<C>(v.x * rsc, v.y * rsc);
-    long long off = 2 * g.spec_at(I, j, k);
-    phit[off] = v.x;
-    phit[off + 1] = v.y;
+    if (!last) {
+      __syncthreads();
+      C* tmp = src;
+      src = dst;
+      dst = tmp;
+    }
+    ns *= Rd;
   }
 }
 
@@ -360,7 +416,7 @@ template <typename R>
 static void launch_cfft_z(const Geom& g, R* phit, const FftPlan<R>& pz, int sign, int scale,
                           hipStream_t s) {
   using C = typename Cx<R>::T;
-  const size_t per_col = sizeof(C) * 2 * (size_t)g.nz;
+  const size_t per_col = sizeof(C) * (2 * (size_t)g.nz + 1);
   const int nh1 = g.nx / 2 + 1;
   if (per_col * 8 <= 65536) {
     k_cfft_z<R, 8><<<dim3((nh1 + 7) / 8, g.ny), 256, per_col * 8, s>>>(g, phit, pz, sign, scale);
@@ -419,7 +475,7 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_irfft_x(
 }
 
 // The sizes the spectral solve takes: nx even, nx/2 and nz products of 2
-// and 3, one FP64 column (2 nz complex) and four FP64 x rows (2 nx complex
+// and 3, one FP64 column (2 nz + 1 complex) and four FP64 x rows (2 nx complex
 // each) within 64 KB of LDS.  Checked before a step launches anything, so
 // a refused size leaves every field as it was.
 int fft_sizes_ok(const Geom& g) {
@@ -432,7 +488,7 @@ int fft_sizes_ok(const Geom& g) {
   if (g.nx % 2 != 0) return fail(VK_ERR_BAD_SIZE, "FFT: nx must be even");
   if (!smooth(g.nx / 2) || !smooth(g.nz))
     return fail(VK_ERR_BAD_SIZE, "FFT: nx/2 and nz must be products of 2 and 3");
-  if (2 * (size_t)g.nz * 16 > 65536 || 4 * 2 * (size_t)(g.nx / 2) * 16 > 65536)
+  if ((2 * (size_t)g.nz + 1) * 16 > 65536 || 4 * 2 * (size_t)(g.nx / 2) * 16 > 65536)
     return fail(VK_ERR_BAD_SIZE, "FFT: size above the LDS plan");
   return VK_OK;
 }
@@ -446,7 +502,7 @@ static int plans_for(const Geom& g, FftPlan<R>& px, FftPlan<R>& pz,
   twl = twiddle_table<R>(g.nx);
   if (!twl) return fail(VK_ERR_HIP, "FFT: twiddle table");
   // one column (2 nz complex) and four x rows (2 nx complex each) in 64 KB
-  if (2 * (size_t)g.nz * sizeof(typename Cx<R>::T) > 65536 ||
+  if ((2 * (size_t)g.nz + 1) * sizeof(typename Cx<R>::T) > 65536 ||
       4 * 2 * (size_t)(g.nx / 2) * sizeof(typename Cx<R>::T) > 65536)
     return fail(VK_ERR_BAD_SIZE, "FFT: size above the LDS plan");
   return VK_OK;
