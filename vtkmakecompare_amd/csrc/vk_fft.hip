@@ -186,15 +186,20 @@ __device__ __forceinline__ void stockham_bfly(typename Cx<R>::T& v0, typename Cx
   }
 }
 
+// Passes s_begin .. s_end-1 (all by default) from a into the pair (a, b);
+// returns the buffer that holds the result.  A barrier follows each pass.
 template <typename R>
 __device__ typename Cx<R>::T* fft_lds(typename Cx<R>::T* a, typename Cx<R>::T* b, int t, int T,
-                                      const FftPlan<R>& p, int sign) {
+                                      const FftPlan<R>& p, int sign, int s_begin = 0,
+                                      int s_end = -1) {
   using C = typename Cx<R>::T;
   const int n = p.n;
+  if (s_end < 0) s_end = p.nst;
   int ns = 1;
+  for (int s = 0; s < s_begin; s++) ns *= p.r[s];
   C* src = a;
   C* dst = b;
-  for (int s = 0; s < p.nst; s++) {
+  for (int s = s_begin; s < s_end; s++) {
     const int Rd = p.r[s];
     const int nb = n / Rd;
     const int twstep = n / (ns * Rd);
@@ -323,10 +328,7 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_rfft_x(
   }
 }
 
-// z pass in place over PHIT columns (I, j) for I = bx*CC + c, j = 1 + by.
-// Forward divides by nx*nz (p2s_fftw2d.f:101).  CC columns per block, so
-// that a block's loads cover whole 128-byte lines where the LDS allows
-// (8 float or 4 double columns at nz = 384); 256 / CC threads per column.
+// (forward: p2s_fftw2d.f:101 divides by nx nz)
 // The z pass on CC adjacent columns (I0..I0+CC-1, j) of the spectrum.  The
 // lanes take the columns in turn (lane = c + CC t), so every global access
 // of a wave covers CC adjacent complex numbers per k.  The first Stockham
@@ -447,8 +449,9 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_irfft_x(
   C* b = a + M;
   const bool ok = j <= g.ny;
   // Z_k = (X_k + conj X_{M-k}) + i e^{2 pi i k/nx} (X_k - conj X_{M-k}),
-  // with the imaginary parts of X_0 and X_M dropped as a c2r transform does
-  for (int q = t; q < M; q += kRowThreads) {
+  // with the imaginary parts of X_0 and X_M dropped as a c2r transform does;
+  // the first pass reads its inputs this way straight from the spectrum
+  auto zin = [&](int q) {
     C xk = mkc<C>((R)0, (R)0), xc = mkc<C>((R)0, (R)0);
     if (ok) {
       long long o1 = 2 * g.spec_at(q, j, k), o2 = 2 * g.spec_at(M - q, j, k);
@@ -457,19 +460,74 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_irfft_x(
     }
     C sum = cadd(xk, xc), dif = csub(xk, xc);
     C r = cmul(conjc(twl[q]), dif);
-    a[q] = mkc<C>(sum.x - r.y, sum.y + r.x);
-  }
-  __syncthreads();
-  C* z = fft_lds<R>(a, b, t, kRowThreads, p, +1);
-  if (!ok) return;
-  const R* zr = reinterpret_cast<const R*>(z);
-  for (int i = t; i < 2 * M; i += kRowThreads) {
-    R x = zr[i];
-    fp[g.at(i, j, k)] = x;
-    if (i == 0) fp[g.at(g.nx, j, k)] = x;
+    return mkc<C>(sum.x - r.y, sum.y + r.x);
+  };
+  // the last pass writes FP(0..nx-1, j, k) straight to the field, with the
+  // wrap FP(nx, j, k) = FP(0, j, k) and, from row k = 0, the plane k = nz
+  // (p2s_fftw2d.f:205-216)
+  auto out = [&](int m, C v) {
+    const int i = 2 * m;
+    fp[g.at(i, j, k)] = v.x;
+    fp[g.at(i + 1, j, k)] = v.y;
+    if (i == 0) fp[g.at(g.nx, j, k)] = v.x;
     if (k == 0) {
-      fp[g.at(i, j, g.nz)] = x;
-      if (i == 0) fp[g.at(g.nx, j, g.nz)] = x;
+      fp[g.at(i, j, g.nz)] = v.x;
+      fp[g.at(i + 1, j, g.nz)] = v.y;
+      if (i == 0) fp[g.at(g.nx, j, g.nz)] = v.x;
+    }
+  };
+  const int last = p.nst - 1;
+  int ns = 1;
+  for (int s = 0; s < last; s++) ns *= p.r[s];
+  if (last == 0) {
+    // one pass (nx <= 8): one butterfly per row, read and written by one thread
+    const int Rd = p.r[0], nb = M / Rd;
+    for (int jj = t; jj < nb; jj += kRowThreads) {
+      C v0 = zin(jj), v1 = zin(jj + nb), v2 = v0, v3 = v0;
+      if (Rd > 2) v2 = zin(jj + 2 * nb);
+      if (Rd > 3) v3 = zin(jj + 3 * nb);
+      stockham_bfly<R>(v0, v1, v2, v3, Rd, 0, 0, p.tw, +1);
+      if (ok) {
+        out(jj * Rd, v0);
+        out(jj * Rd + 1, v1);
+        if (Rd > 2) out(jj * Rd + 2, v2);
+        if (Rd > 3) out(jj * Rd + 3, v3);
+      }
+    }
+    return;
+  }
+  {
+    const int Rd = p.r[0], nb = M / Rd;
+    for (int jj = t; jj < nb; jj += kRowThreads) {
+      C v0 = zin(jj), v1 = zin(jj + nb), v2 = v0, v3 = v0;
+      if (Rd > 2) v2 = zin(jj + 2 * nb);
+      if (Rd > 3) v3 = zin(jj + 3 * nb);
+      stockham_bfly<R>(v0, v1, v2, v3, Rd, 0, 0, p.tw, +1);
+      const int base = jj * Rd;
+      b[base] = v0;
+      b[base + 1] = v1;
+      if (Rd > 2) b[base + 2] = v2;
+      if (Rd > 3) b[base + 3] = v3;
+    }
+    __syncthreads();
+  }
+  const C* z = fft_lds<R>(b, a, t, kRowThreads, p, +1, 1, last);
+  {
+    const int Rd = p.r[last], nb = M / Rd;
+    const int twstep = M / (ns * Rd);
+    for (int jj = t; jj < nb; jj += kRowThreads) {
+      const int jq = jj / ns, kk = jj - jq * ns;
+      C v0 = z[jj], v1 = z[jj + nb], v2 = v0, v3 = v0;
+      if (Rd > 2) v2 = z[jj + 2 * nb];
+      if (Rd > 3) v3 = z[jj + 3 * nb];
+      stockham_bfly<R>(v0, v1, v2, v3, Rd, kk, twstep, p.tw, +1);
+      const int base = jq * ns * Rd + kk;
+      if (ok) {
+        out(base, v0);
+        out(base + ns, v1);
+        if (Rd > 2) out(base + 2 * ns, v2);
+        if (Rd > 3) out(base + 3 * ns, v3);
+      }
     }
   }
 }
