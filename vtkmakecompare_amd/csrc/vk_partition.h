@@ -826,6 +826,22 @@ inline int launch_line_solve(const Sys& sys, int n, int nla, int nlb, hipStream_
   const int mmax = (n + S - 1) / S;
   if (n < 2 || mmax > 16 || n < 2 * S) return -1;
   const bool f12 = S > 1 && (n + S / 2 - 1) / (S / 2) <= 12;
+  if constexpr (!Sys::kHasRef && LANE_LINES) {
+    // systems with no reference line (the spectral solve, the transposed
+    // sweeps) run the general kernel alone, whose registers allow 3 waves
+    // per SIMD: 4-wave blocks fill the CUs better than 8-wave ones
+    if (mmax <= 8) {
+      switch (S) {
+        case 64: return line_sweep<LANE_LINES, 64, 8, 64, 8, 256>(sys, n, nla, nlb, s);
+        case 32: return line_sweep<LANE_LINES, 32, 8, 32, 8, 256>(sys, n, nla, nlb, s);
+        case 16: return line_sweep<LANE_LINES, 16, 8, 16, 8, 256>(sys, n, nla, nlb, s);
+        case 8: return line_sweep<LANE_LINES, 8, 8, 8, 8, 256>(sys, n, nla, nlb, s);
+        case 4: return line_sweep<LANE_LINES, 4, 8, 4, 8, 256>(sys, n, nla, nlb, s);
+        case 2: return line_sweep<LANE_LINES, 2, 8, 2, 8, 256>(sys, n, nla, nlb, s);
+        default: return line_sweep<LANE_LINES, 1, 8, 1, 8, 256>(sys, n, nla, nlb, s);
+      }
+    }
+  }
 #define VK_SW(SG, MG, SF, MF) \
   return line_sweep<LANE_LINES, SG, MG, SF, MF, 512>(sys, n, nla, nlb, s)
   if (mmax > 8) {
