@@ -781,9 +781,9 @@ double* line_table(const void* key, size_t keylen, long total, bool* fresh, bool
 // fast kernel SF segments of at most MF rows (fewer, longer segments: its
 // rows need one register each, and fewer segments mean a smaller staged
 // table and fewer PCR steps).  Both use NT-thread blocks.
-template <bool LANE_LINES, int SG, int MG, int SF, int MF, int NT, typename Sys>
+template <bool LANE_LINES, int SG, int MG, int SF, int MF, int NT, typename Sys, int NTG = NT>
 inline int line_sweep(const Sys& sys, int n, int nla, int nlb, hipStream_t s) {
-  constexpr int LPB_G = NT / SG, LPB_F = NT / SF;
+  constexpr int LPB_G = NTG / SG, LPB_F = NT / SF;
   double* tab = nullptr;
   bool fresh = false, validated = false;
   TabLayout L = tab_layout(n, SF, Sys::kCyclic);
@@ -804,10 +804,10 @@ inline int line_sweep(const Sys& sys, int n, int nla, int nlb, hipStream_t s) {
     if (!validated) k_line_factor<SF, MF, Sys><<<1, SF, 0, s>>>(sys, tab, L, fresh ? 1 : 0);
     k_line_fast<SF, MF, NT, LANE_LINES, LPB_G, Sys><<<ngb, NT, lds, s>>>(sys, tab, L, Q, nbx, nb);
     const unsigned ngen = nb < 1024u ? nb : 1024u;
-    k_line_general_q<SG, MG, NT, LANE_LINES, Sys><<<ngen, NT, 0, s>>>(sys, Q, nla_pad);
+    k_line_general_q<SG, MG, NTG, LANE_LINES, Sys><<<ngen, NTG, 0, s>>>(sys, Q, nla_pad);
   } else {
     const int nbx = (nla + LPB_G - 1) / LPB_G;
-    k_line_general<SG, MG, NT, LANE_LINES, Sys><<<(unsigned)nbx * (unsigned)nlb, NT, 0, s>>>(
+    k_line_general<SG, MG, NTG, LANE_LINES, Sys><<<(unsigned)nbx * (unsigned)nlb, NTG, 0, s>>>(
         sys, nbx, nlb);
   }
   return 0;
@@ -842,8 +842,10 @@ inline int launch_line_solve(const Sys& sys, int n, int nla, int nlb, hipStream_
       }
     }
   }
+  // the general kernels hold 116-200 VGPRs (2-4 waves per SIMD): 4-wave
+  // blocks for them, 8-wave blocks for the fast kernels
 #define VK_SW(SG, MG, SF, MF) \
-  return line_sweep<LANE_LINES, SG, MG, SF, MF, 512>(sys, n, nla, nlb, s)
+  return line_sweep<LANE_LINES, SG, MG, SF, MF, 512, Sys, 256>(sys, n, nla, nlb, s)
   if (mmax > 8) {
     // long lines: 64 segments of up to 12 rows keep the fast kernel's
     // staged table within 64 KB of LDS (768-row lines), 16 rows beyond
