@@ -42,7 +42,7 @@ KERNEL_SYMBOL = {
     "explicit_t": "vk::k_explicit_fused<2>",
     "explicit_mixed": "vk::k_explicit_mixed<1>",
     "explicit_mixed_t": "vk::k_explicit_mixed<2>",
-    "project": "vk::k_project",
+    "project": "vk::k_project<false>",
     "project_mixed": "vk::k_project_mixed",
     "rfft_x_div": "vk::k_rfft_x<vk::SampleDiv,double>",
     "irfft_x": "vk::k_irfft_x<double>",

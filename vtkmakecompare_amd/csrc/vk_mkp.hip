@@ -35,35 +35,49 @@ __global__ void k_mkp_coefficients(Geom g, Metrics m, double xl, double zl, doub
 // corrected velocity, and on the interior p = PHI, the diagnostic
 // u_ep2 = -dt grad(p) + u_ta (read before u_ta is replaced; p's halo at
 // nx+1, ny+1, nz+1 is the caller's, as in the reference), u_ta = dt grad,
-// and divu of the corrected velocity.
+// and divu of the corrected velocity.  Every value is computed before the
+// first store, so all the point's loads can be in flight together (a
+// store would otherwise fence the loads after it: the outputs may alias
+// the inputs as far as the compiler knows).  INTERIOR: i, j, k >= 1 known
+// at compile time; EP2: the u_ep2 diagnostic is produced.
+template <bool INTERIOR, bool EP2>
 __device__ inline void project_point(const Geom& g, const RMetrics& r, const double* phi,
                                      const double* ut, const double* vt, const double* wt,
                                      double dt, const MkpOut& o, int i, int j, int k) {
   long long c = g.at(i, j, k);
   const long long sy = g.sy, sz = g.sz;
-  double ph = phi[c];
-  double gx = (phi[c + 1] - ph) * r.rdx12[i];
-  double gy = (phi[c + sy] - ph) * r.rdy12[j];
-  double gz = (phi[c + sz] - ph) * r.rdz12[k];
-  double un = ut[c] - dt * gx, vn = vt[c] - dt * gy, wn = wt[c] - dt * gz;
+  const double ph = phi[c], phx = phi[c + 1], phy = phi[c + sy], phz = phi[c + sz];
+  const double gx = (phx - ph) * r.rdx12[i];
+  const double gy = (phy - ph) * r.rdy12[j];
+  const double gz = (phz - ph) * r.rdz12[k];
+  const double un = ut[c] - dt * gx, vn = vt[c] - dt * gy, wn = wt[c] - dt * gz;
+  const bool interior = INTERIOR || (i >= 1 && j >= 1 && k >= 1);
+  double div = 0.0, ex = 0.0, ey = 0.0, ez = 0.0;
+  if (interior) {
+    if (EP2) {
+      const double px = (i < g.nx) ? phx : o.p[c + 1];
+      const double py = (j < g.ny) ? phy : o.p[c + sy];
+      const double pz = (k < g.nz) ? phz : o.p[c + sz];
+      ex = -dt * (px - ph) * r.rdx12[i] + o.u_ta[c];
+      ey = -dt * (py - ph) * r.rdy12[j] + o.v_ta[c];
+      ez = -dt * (pz - ph) * r.rdz12[k] + o.w_ta[c];
+    }
+    const double um = ut[c - 1] - dt * ((ph - phi[c - 1]) * r.rdx12[i - 1]);
+    const double vm = vt[c - sy] - dt * ((ph - phi[c - sy]) * r.rdy12[j - 1]);
+    const double wm = wt[c - sz] - dt * ((ph - phi[c - sz]) * r.rdz12[k - 1]);
+    div = (un - um) * r.rdx[i] + (vn - vm) * r.rdy[j] + (wn - wm) * r.rdz[k];
+  }
   o.u[c] = un;
   o.v[c] = vn;
   o.w[c] = wn;
-  bool interior = i >= 1 && j >= 1 && k >= 1;
   if (interior) {
     o.p[c] = ph;
-    if (o.u_ep2) {
-      double px = (i < g.nx) ? phi[c + 1] : o.p[c + 1];
-      double py = (j < g.ny) ? phi[c + sy] : o.p[c + sy];
-      double pz = (k < g.nz) ? phi[c + sz] : o.p[c + sz];
-      o.u_ep2[c] = -dt * (px - ph) * r.rdx12[i] + o.u_ta[c];
-      o.v_ep2[c] = -dt * (py - ph) * r.rdy12[j] + o.v_ta[c];
-      o.w_ep2[c] = -dt * (pz - ph) * r.rdz12[k] + o.w_ta[c];
+    if (EP2) {
+      o.u_ep2[c] = ex;
+      o.v_ep2[c] = ey;
+      o.w_ep2[c] = ez;
     }
-    double um = ut[c - 1] - dt * ((ph - phi[c - 1]) * r.rdx12[i - 1]);
-    double vm = vt[c - sy] - dt * ((ph - phi[c - sy]) * r.rdy12[j - 1]);
-    double wm = wt[c - sz] - dt * ((ph - phi[c - sz]) * r.rdz12[k - 1]);
-    o.divu[c] = (un - um) * r.rdx[i] + (vn - vm) * r.rdy[j] + (wn - wm) * r.rdz[k];
+    o.divu[c] = div;
   }
   o.u_ta[c] = dt * gx;
   o.v_ta[c] = dt * gy;
@@ -71,6 +85,7 @@ __device__ inline void project_point(const Geom& g, const RMetrics& r, const dou
 }
 
 // the interior points, in full 64 x 4 tiles along (i, j)
+template <bool EP2>
 __global__ void k_project(Geom g, RMetrics r, const double* phi, const double* ut,
                           const double* vt, const double* wt, double dt, MkpOut o) {
   int bx, by, bz;
@@ -78,7 +93,7 @@ __global__ void k_project(Geom g, RMetrics r, const double* phi, const double* u
   const int i = 1 + bx * blockDim.x + threadIdx.x;
   const int j = 1 + by * blockDim.y + threadIdx.y;
   if (i > g.nx || j > g.ny) return;
-  project_point(g, r, phi, ut, vt, wt, dt, o, i, j, 1 + bz);
+  project_point<true, EP2>(g, r, phi, ut, vt, wt, dt, o, i, j, 1 + bz);
 }
 
 // the points of 0..n with an index 0: the faces i = 0, j = 0, k = 0
@@ -102,10 +117,13 @@ inline long long zero_face_count(const Geom& g) {
          (long long)g.nx * g.ny;
 }
 
+template <bool EP2>
 __global__ void k_project_faces(Geom g, RMetrics r, const double* phi, const double* ut,
                                 const double* vt, const double* wt, double dt, MkpOut o) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  zero_faces(g, q, [&](int i, int j, int k) { project_point(g, r, phi, ut, vt, wt, dt, o, i, j, k); });
+  zero_faces(g, q, [&](int i, int j, int k) {
+    project_point<false, EP2>(g, r, phi, ut, vt, wt, dt, o, i, j, k);
+  });
 }
 
 int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const double* v_temp,
@@ -147,11 +165,17 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
     if (rc) return rc;
   }
   ProfScope ps("project", s);
-  k_project<<<interior_grid(g), interior_block(), 0, s>>>(g, r, phi, u_temp, v_temp, w_temp, dt,
-                                                         o);
   const long long nf = zero_face_count(g);
-  k_project_faces<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(g, r, phi, u_temp, v_temp,
-                                                               w_temp, dt, o);
+  const unsigned nfb = (unsigned)((nf + 255) / 256);
+  if (o.u_ep2) {
+    k_project<true><<<interior_grid(g), interior_block(), 0, s>>>(g, r, phi, u_temp, v_temp,
+                                                                  w_temp, dt, o);
+    k_project_faces<true><<<nfb, 256, 0, s>>>(g, r, phi, u_temp, v_temp, w_temp, dt, o);
+  } else {
+    k_project<false><<<interior_grid(g), interior_block(), 0, s>>>(g, r, phi, u_temp, v_temp,
+                                                                   w_temp, dt, o);
+    k_project_faces<false><<<nfb, 256, 0, s>>>(g, r, phi, u_temp, v_temp, w_temp, dt, o);
+  }
   return VK_OK;
 }
 
@@ -160,6 +184,7 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
 // p and divu are FP64, u_ta is stored in float.  Same index ranges and
 // formulas as project_point (mkp.f:141-217); the u_ep2 diagnostic is not
 // produced.
+template <bool INTERIOR>
 __device__ inline void project_mixed_point(const Geom& g, const RMetrics& r, const float* phi1,
                                            const float* dphi, const double* ut,
                                            const double* vt, const double* wt, double dt,
@@ -172,15 +197,21 @@ __device__ inline void project_mixed_point(const Geom& g, const RMetrics& r, con
   const double gy = (phi(c + sy) - ph) * r.rdy12[j];
   const double gz = (phi(c + sz) - ph) * r.rdz12[k];
   const double un = ut[c] - dt * gx, vn = vt[c] - dt * gy, wn = wt[c] - dt * gz;
-  o.u[c] = un;
-  o.v[c] = vn;
-  o.w[c] = wn;
-  if (i >= 1 && j >= 1 && k >= 1) {
-    o.p[c] = ph;
+  const bool interior = INTERIOR || (i >= 1 && j >= 1 && k >= 1);
+  // every value computed before the first store, as in project_point
+  double div = 0.0;
+  if (interior) {
     const double um = ut[c - 1] - dt * ((ph - phi(c - 1)) * r.rdx12[i - 1]);
     const double vm = vt[c - sy] - dt * ((ph - phi(c - sy)) * r.rdy12[j - 1]);
     const double wm = wt[c - sz] - dt * ((ph - phi(c - sz)) * r.rdz12[k - 1]);
-    o.divu[c] = (un - um) * r.rdx[i] + (vn - vm) * r.rdy[j] + (wn - wm) * r.rdz[k];
+    div = (un - um) * r.rdx[i] + (vn - vm) * r.rdy[j] + (wn - wm) * r.rdz[k];
+  }
+  o.u[c] = un;
+  o.v[c] = vn;
+  o.w[c] = wn;
+  if (interior) {
+    o.p[c] = ph;
+    o.divu[c] = div;
   }
   o.u_ta[c] = (float)(dt * gx);
   o.v_ta[c] = (float)(dt * gy);
@@ -195,7 +226,7 @@ __global__ void k_project_mixed(Geom g, RMetrics r, const float* phi1, const flo
   const int i = 1 + bx * blockDim.x + threadIdx.x;
   const int j = 1 + by * blockDim.y + threadIdx.y;
   if (i > g.nx || j > g.ny) return;
-  project_mixed_point(g, r, phi1, dphi, ut, vt, wt, dt, o, i, j, 1 + bz);
+  project_mixed_point<true>(g, r, phi1, dphi, ut, vt, wt, dt, o, i, j, 1 + bz);
 }
 
 __global__ void k_project_mixed_faces(Geom g, RMetrics r, const float* phi1, const float* dphi,
@@ -203,7 +234,7 @@ __global__ void k_project_mixed_faces(Geom g, RMetrics r, const float* phi1, con
                                       double dt, MkpOutMixed o) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   zero_faces(g, q, [&](int i, int j, int k) {
-    project_mixed_point(g, r, phi1, dphi, ut, vt, wt, dt, o, i, j, k);
+    project_mixed_point<false>(g, r, phi1, dphi, ut, vt, wt, dt, o, i, j, k);
   });
 }
 

@@ -47,34 +47,35 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
     const double cv2 = x.cv[g.cv(i, j, k, 2)], cv3 = x.cv[g.cv(i, j, k, 3)],
                  cv4 = x.cv[g.cv(i, j, k, 4)];
     const unsigned mk = x.lf.fx ? dirty_bits(0.0, cv2, cv3, cv4, 1.0, 1.0) : 0u;
-    // u: SET_CONVECT, SET_CONDUCTION, u_l, SET_EXPLICIT, solve_u_temp divide
-    {
-      double uv = conv_u_r(U, V, W, m, r, i, j, k, dt);
-      double ud = cond_u_r(U, r, i, j, k, x.c0);
-      double ul = eta * cv2 * x.u_ta[o] * dt;
-      double ua = x.a1 * uv + x.a2 * x.u_pre[o] + x.b1 * ud + x.b1 * ul;
-      x.u_pre[o] = uv;
-      x.ua[o] = pen_div(x.u[o] + ua, eta, cv2, dt);
-    }
-    {
-      double vv = conv_v_r(U, V, W, m, r, i, j, k, dt);
-      double vd = cond_v_r(V, r, i, j, k, x.c0);
-      double vl = eta * cv3 * x.v_ta[o] * dt;
-      double va = x.a1 * vv + x.a2 * x.v_pre[o] + x.b1 * vd + x.b1 * vl;
-      x.v_pre[o] = vv;
-      x.va[o] = pen_div(x.v[o] + va, eta, cv3, dt);
-      // SET_EXPLICIT's reflection v_a(i,ny+1,k) = -v_a(i,ny,k) survives into
-      // v_temp (no later routine writes that plane of v)
-      if (j == g.ny) x.va[g.at(i, g.ny + 1, k)] = -1.0 * va;
-    }
-    {
-      double wv = conv_w_r(U, V, W, m, r, i, j, k, dt);
-      double wd = cond_w_r(W, r, i, j, k, x.c0);
-      double wl = eta * cv4 * x.w_ta[o] * dt;
-      double wa = x.a1 * wv + x.a2 * x.w_pre[o] + x.b1 * wd + x.b1 * wl;
-      x.w_pre[o] = wv;
-      x.wa[o] = pen_div(x.w[o] + wa, eta, cv4, dt);
-    }
+    // u, v, w: SET_CONVECT, SET_CONDUCTION, u_l, SET_EXPLICIT and
+    // solve_u_temp's divide.  Every value is computed before the first
+    // store, so the loads of all three stencils can be in flight together
+    // (a store would otherwise fence the later loads: the outputs may
+    // alias the inputs as far as the compiler knows).
+    const double uv = conv_u_r(U, V, W, m, r, i, j, k, dt);
+    const double ud = cond_u_r(U, r, i, j, k, x.c0);
+    const double ul = eta * cv2 * x.u_ta[o] * dt;
+    const double ua = x.a1 * uv + x.a2 * x.u_pre[o] + x.b1 * ud + x.b1 * ul;
+    const double vv = conv_v_r(U, V, W, m, r, i, j, k, dt);
+    const double vd = cond_v_r(V, r, i, j, k, x.c0);
+    const double vl = eta * cv3 * x.v_ta[o] * dt;
+    const double va = x.a1 * vv + x.a2 * x.v_pre[o] + x.b1 * vd + x.b1 * vl;
+    const double wv = conv_w_r(U, V, W, m, r, i, j, k, dt);
+    const double wd = cond_w_r(W, r, i, j, k, x.c0);
+    const double wl = eta * cv4 * x.w_ta[o] * dt;
+    const double wa = x.a1 * wv + x.a2 * x.w_pre[o] + x.b1 * wd + x.b1 * wl;
+    const double un = pen_div(x.u[o] + ua, eta, cv2, dt);
+    const double vn = pen_div(x.v[o] + va, eta, cv3, dt);
+    const double wn = pen_div(x.w[o] + wa, eta, cv4, dt);
+    x.u_pre[o] = uv;
+    x.ua[o] = un;
+    x.v_pre[o] = vv;
+    x.va[o] = vn;
+    // SET_EXPLICIT's reflection v_a(i,ny+1,k) = -v_a(i,ny,k) survives into
+    // v_temp (no later routine writes that plane of v)
+    if (j == g.ny) x.va[g.at(i, g.ny + 1, k)] = -1.0 * va;
+    x.w_pre[o] = wv;
+    x.wa[o] = wn;
     if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
   } else {
     // T: convection, diffusion, T_l = 0, combine, history, main-loop divide
