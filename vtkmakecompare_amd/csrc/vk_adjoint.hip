@@ -141,12 +141,16 @@ struct CLam3 {
 __global__ void k_adj_project_in(Geom g, CLam3 lu, CLam3 luta, Lam3 lut, Lam3 lg, double dt) {
   ALL_CELLS;
   const bool in = is_interior(g, qi, qj, qk);
-  lut.u[q_] = in ? lu.u[q_] : 0.0;
-  lut.v[q_] = in ? lu.v[q_] : 0.0;
-  lut.w[q_] = in ? lu.w[q_] : 0.0;
-  lg.u[q_] = in ? dt * (luta.u[q_] - lu.u[q_]) : 0.0;
-  lg.v[q_] = in ? dt * (luta.v[q_] - lu.v[q_]) : 0.0;
-  lg.w[q_] = in ? dt * (luta.w[q_] - lu.w[q_]) : 0.0;
+  // all reads before the first store (distinct buffers; a store would
+  // fence the later loads otherwise)
+  const double u = lu.u[q_], v = lu.v[q_], w = lu.w[q_];
+  const double ut = luta.u[q_], vt = luta.v[q_], wt = luta.w[q_];
+  lut.u[q_] = in ? u : 0.0;
+  lut.v[q_] = in ? v : 0.0;
+  lut.w[q_] = in ? w : 0.0;
+  lg.u[q_] = in ? dt * (ut - u) : 0.0;
+  lg.v[q_] = in ? dt * (vt - v) : 0.0;
+  lg.w[q_] = in ? dt * (wt - w) : 0.0;
 }
 
 // G^T: g_x(p) = (phi(p+ex) - phi(p)) / dx12(i_p) on interior p
@@ -260,46 +264,45 @@ __global__ void k_adj_explicit_weights(Geom g, StepIn in, AdjIn a, AdjOut out, W
   const long long o = q_;
   const int i = qi, j = qj, k = qk;
   const double ed = eta * dt;
-  // velocity components: x_out = (x + a1 N + a2 N_pre + b1 D + b1 eta cv x_ta dt) / alpha
-  {
-    const long long c = g.cv(i, j, k, 2);
-    const double cvv = in.cv[c], s = 1.0 / (1.0 + ed * cvv), mu = a.lu[o] * s;
-    out.lu[o] = mu;
-    out.lup[o] = a2 * mu;
-    out.luta[o] = b1 * eta * cvv * dt * mu;
-    gcv[c] += a.lu[o] * ed * s * (b1 * in.u_ta[o] - in.su[o]);
-    wt.uv[o] = a1 * mu + a.lup[o];
-    wt.ud[o] = b1 * mu;
-  }
-  {
-    const long long c = g.cv(i, j, k, 3);
-    const double cvv = in.cv[c], s = 1.0 / (1.0 + ed * cvv), mu = a.lv[o] * s;
-    out.lv[o] = mu;
-    out.lvp[o] = a2 * mu;
-    out.lvta[o] = b1 * eta * cvv * dt * mu;
-    gcv[c] += a.lv[o] * ed * s * (b1 * in.v_ta[o] - in.sv[o]);
-    wt.vv[o] = a1 * mu + a.lvp[o];
-    wt.vd[o] = b1 * mu;
-  }
-  {
-    const long long c = g.cv(i, j, k, 4);
-    const double cvv = in.cv[c], s = 1.0 / (1.0 + ed * cvv), mu = a.lw[o] * s;
-    out.lw[o] = mu;
-    out.lwp[o] = a2 * mu;
-    out.lwta[o] = b1 * eta * cvv * dt * mu;
-    gcv[c] += a.lw[o] * ed * s * (b1 * in.w_ta[o] - in.sw[o]);
-    wt.wv[o] = a1 * mu + a.lwp[o];
-    wt.wd[o] = b1 * mu;
-  }
-  {
-    const long long c = g.cv(i, j, k, 1);
-    const double cvv = in.cv[c], s = 1.0 / (1.0 + ed * cvv), mu = a.lt[o] * s;
-    out.lt[o] = mu;
-    out.ltp[o] = a2 * mu;
-    gcv[c] += -a.lt[o] * ed * s * in.st[o];
-    wt.vt[o] = a1 * mu + a.ltp[o];
-    wt.dtt[o] = b1 * mu;
-  }
+  // velocity components: x_out = (x + a1 N + a2 N_pre + b1 D + b1 eta cv x_ta dt) / alpha.
+  // Every input is read before the first store (a store would fence the
+  // loads after it: the outputs may alias the inputs as far as the
+  // compiler knows; they are distinct buffers).
+  const long long c2 = g.cv(i, j, k, 2), c3 = g.cv(i, j, k, 3), c4 = g.cv(i, j, k, 4),
+                  c1 = g.cv(i, j, k, 1);
+  const double cu = in.cv[c2], cv_ = in.cv[c3], cw = in.cv[c4], ct = in.cv[c1];
+  const double alu = a.lu[o], alv = a.lv[o], alw = a.lw[o], alt = a.lt[o];
+  const double alup = a.lup[o], alvp = a.lvp[o], alwp = a.lwp[o], altp = a.ltp[o];
+  const double uta = in.u_ta[o], vta = in.v_ta[o], wta = in.w_ta[o];
+  const double su = in.su[o], sv = in.sv[o], sw = in.sw[o], st = in.st[o];
+  const double gu = gcv[c2], gv = gcv[c3], gw = gcv[c4], gt = gcv[c1];
+  const double s_u = 1.0 / (1.0 + ed * cu), mu_u = alu * s_u;
+  const double s_v = 1.0 / (1.0 + ed * cv_), mu_v = alv * s_v;
+  const double s_w = 1.0 / (1.0 + ed * cw), mu_w = alw * s_w;
+  const double s_t = 1.0 / (1.0 + ed * ct), mu_t = alt * s_t;
+  out.lu[o] = mu_u;
+  out.lup[o] = a2 * mu_u;
+  out.luta[o] = b1 * eta * cu * dt * mu_u;
+  gcv[c2] = gu + alu * ed * s_u * (b1 * uta - su);
+  wt.uv[o] = a1 * mu_u + alup;
+  wt.ud[o] = b1 * mu_u;
+  out.lv[o] = mu_v;
+  out.lvp[o] = a2 * mu_v;
+  out.lvta[o] = b1 * eta * cv_ * dt * mu_v;
+  gcv[c3] = gv + alv * ed * s_v * (b1 * vta - sv);
+  wt.vv[o] = a1 * mu_v + alvp;
+  wt.vd[o] = b1 * mu_v;
+  out.lw[o] = mu_w;
+  out.lwp[o] = a2 * mu_w;
+  out.lwta[o] = b1 * eta * cw * dt * mu_w;
+  gcv[c4] = gw + alw * ed * s_w * (b1 * wta - sw);
+  wt.wv[o] = a1 * mu_w + alwp;
+  wt.wd[o] = b1 * mu_w;
+  out.lt[o] = mu_t;
+  out.ltp[o] = a2 * mu_t;
+  gcv[c1] = gt + -alt * ed * s_t * st;
+  wt.vt[o] = a1 * mu_t + altp;
+  wt.dtt[o] = b1 * mu_t;
 }
 
 // Where each explicit term at p reads field F (0, 1, 2, 3 = u, v, w, T):

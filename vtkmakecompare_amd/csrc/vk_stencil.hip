@@ -100,9 +100,14 @@ __global__ void k_convect(Geom g, Metrics m, const double* u, const double* v, c
   INTERIOR_INDEX;
   F3<double> U{u, g}, V{v, g}, W{w, g};
   long long o = g.at(i, j, k);
-  u_v[o] = conv_u(U, V, W, m, i, j, k, dt);
-  v_v[o] = conv_v(U, V, W, m, i, j, k, dt);
-  w_v[o] = conv_w(U, V, W, m, i, j, k, dt);
+  // the three values before the first store (a store would fence the later
+  // loads: the outputs may alias the inputs as far as the compiler knows)
+  const double a = conv_u(U, V, W, m, i, j, k, dt);
+  const double b = conv_v(U, V, W, m, i, j, k, dt);
+  const double c = conv_w(U, V, W, m, i, j, k, dt);
+  u_v[o] = a;
+  v_v[o] = b;
+  w_v[o] = c;
 }
 
 __global__ void k_conv_scalar(Geom g, Metrics m, const double* u, const double* v,
@@ -119,9 +124,12 @@ __global__ void k_conduction(Geom g, Metrics m, const double* u, const double* v
   INTERIOR_INDEX;
   F3<double> U{u, g}, V{v, g}, W{w, g};
   long long o = g.at(i, j, k);
-  u_d[o] = cond_u(U, m, i, j, k, c0);
-  v_d[o] = cond_v(V, m, i, j, k, c0);
-  w_d[o] = cond_w(W, m, i, j, k, c0);
+  const double a = cond_u(U, m, i, j, k, c0);
+  const double b = cond_v(V, m, i, j, k, c0);
+  const double c = cond_w(W, m, i, j, k, c0);
+  u_d[o] = a;
+  v_d[o] = b;
+  w_d[o] = c;
 }
 
 __global__ void k_diffusion(Geom g, Metrics m, const double* t, const double* thc,
@@ -252,7 +260,6 @@ __global__ void k_adj_convect(Geom g, Metrics m, const double* u_, const double*
                   (-w(i + 1, j, k) - w(i + 1, j, k - 1) + w(i, j, k) + v(i, j, k - 1)) /
                   dx12[i] / 2.0;
   long long o = g.at(i, j, k);
-  u_v[o] = 0.25 * (-u1 - u2 - u3) * dt + 1.0 * dt;
 
   double v1 = ((uf(i, j + 1, k) * dy[j + 1] + uf(i, j, k) * dy[j]) *
                    (v(i + 1, j, k) + v(i, j, k)) -
@@ -278,7 +285,6 @@ __global__ void k_adj_convect(Geom g, Metrics m, const double* u_, const double*
                   dy12[j] *
                   (-w(i, j + 1, k) - w(i, j + 1, k - 1) + w(i, j, k) + w(i, j, k - 1)) /
                   dy12[j] / 2.0;
-  v_v[o] = 0.25 * (-v1 - v2 - v3) * dt;
 
   double w1 = ((uf(i, j, k + 1) * dz[k + 1] + uf(i, j, k) * dz[k]) *
                    (w(i + 1, j, k) + w(i, j, k)) -
@@ -304,6 +310,9 @@ __global__ void k_adj_convect(Geom g, Metrics m, const double* u_, const double*
                (wf(i, j, k) + wf(i, j, k - 1)) * (w(i, j, k) + w(i, j, k - 1))) /
                   dz12[k] -
               wf(i, j, k) * (w(i, j, k - 1) - w(i, j, k + 1)) * 2.0 / dz12[k];
+  // stored together at the end (a store would fence the later loads)
+  u_v[o] = 0.25 * (-u1 - u2 - u3) * dt + 1.0 * dt;
+  v_v[o] = 0.25 * (-v1 - v2 - v3) * dt;
   w_v[o] = 0.25 * (-w1 - w2 - w3) * dt;
 }
 
