@@ -382,7 +382,10 @@ int sum_objective(const Geom& g, const double* jacc, double* out) {
 // outputs and every sweep's result kept.  tape[c*4 + d] for c = u, v, w, T:
 // d = 0 the explicit output, 1..3 the results of the three sweeps.
 // in4 receives u, v, w, T after the first SET_BC.
-int taped_step(const Problem& P, Work& w, int l, double* const* tape, double* const* in4) {
+// lf_out: the step's dirty-line masks (in the workspace until the next
+// step clears them), which the transposed sweeps of its adjoint reuse
+int taped_step(const Problem& P, Work& w, int l, double* const* tape, double* const* in4,
+               LineFlags* lf_out) {
   const Geom& g = P.g;
   hipStream_t s = S();
   RMetrics r;
@@ -403,6 +406,7 @@ int taped_step(const Problem& P, Work& w, int l, double* const* tape, double* co
   rc = line_flags_clear(g, lf, s);
   if (rc) return rc;
   x.lf = lf;
+  *lf_out = lf;
   launch_explicit_fused(g, P.m, r, x, s);
   const double beta_u = P.dt / P.re / 2.0, beta_t = P.dt / P.re / P.sc / 2.0;
   const double dteta = P.dt * P.eta;
@@ -466,7 +470,7 @@ __global__ void k_adj_coefficients(Geom g, Metrics m, double xl, double zl, doub
 // state after step l; on return B.lam holds the adjoint of the state
 // before it, and the cellvalue gradient of step l is added to B.gcv.
 int adjoint_step(const Problem& P, AdjBufs& B, int l, double* const* tape, double* const* in4,
-                 const double* uta_in[3]) {
+                 const double* uta_in[3], const LineFlags* fl) {
   const Geom& g = P.g;
   hipStream_t s = S();
   RMetrics r;
@@ -507,9 +511,9 @@ int adjoint_step(const Problem& P, AdjBufs& B, int l, double* const* tape, doubl
     const int jmax = (c == 1) ? g.ny - 1 : g.ny;
     const int slot = (c < 3) ? c + 2 : 1;
     for (int d = 2; d >= 0; d--) {
-      rc = (c < 3) ? launch_velocity_sweep_adj(g, P.m, r, lam, P.cv, c, d, beta_u, dteta, s)
+      rc = (c < 3) ? launch_velocity_sweep_adj(g, P.m, r, lam, P.cv, c, d, beta_u, dteta, s, fl)
                    : launch_temperature_sweep_adj(g, P.m, r, lam, P.cv, P.rc, P.thc, d, beta_t,
-                                                  dteta, s);
+                                                  dteta, s, fl);
       if (rc) return rc;
       k_adj_sweep_grad<<<interior_grid(g), interior_block(), 0, s>>>(
           g, lam, tape[c * 4 + d + 1], tape[c * 4 + d], P.cv, slot, B.gcv, dteta, jmax);
@@ -758,9 +762,10 @@ int vk_adjoint_gradient(const double* u, const double* v, const double* w, const
       const double* uta_in[3] = {seg[(size_t)(l - l0) * kState + SUTA],
                                  seg[(size_t)(l - l0) * kState + SVTA],
                                  seg[(size_t)(l - l0) * kState + SWTA]};
-      rc = taped_step(P, wk, l, tape, in4);
+      LineFlags lf{};
+      rc = taped_step(P, wk, l, tape, in4, &lf);
       if (rc) return rc;
-      rc = adjoint_step(P, B, l, tape, in4, uta_in);
+      rc = adjoint_step(P, B, l, tape, in4, uta_in, &lf);
       if (rc) return rc;
       if (l > 1) k_objective_src<<<(nJ + 255) / 256, 256, 0, S()>>>(g, P.m, P.dt, B.lam[ST]);
     }

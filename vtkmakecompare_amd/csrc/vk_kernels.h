@@ -92,13 +92,14 @@ int launch_temperature_sweep(const Geom& g, const Metrics& m, const RMetrics& r,
                              double dteta, hipStream_t s, TT* out = nullptr,
                              const LineFlags* fl = nullptr);
 // the transposed sweeps (vk_sweep_adj.hip), in place on the adjoint field
+// (fl: the dirty-line masks of the forward step being transposed)
 int launch_velocity_sweep_adj(const Geom& g, const Metrics& m, const RMetrics& r, double* lam,
                               const double* cv, int comp, int dir, double beta, double dteta,
-                              hipStream_t s);
+                              hipStream_t s, const LineFlags* fl = nullptr);
 int launch_temperature_sweep_adj(const Geom& g, const Metrics& m, const RMetrics& r,
                                  double* lam, const double* cv, const double* rc,
                                  const double* thc, int dir, double beta, double dteta,
-                                 hipStream_t s);
+                                 hipStream_t s, const LineFlags* fl = nullptr);
 // reciprocal metric arrays in workspace (recomputed each call, 6(n+2) doubles)
 int make_rmetrics(const Geom& g, const Metrics& m, RMetrics& r, hipStream_t s);
 

@@ -445,13 +445,19 @@ int launch_spn(const Geom& g, TS* phit, const double* ac, const double* kxz, hip
 // otherwise).  The right-hand side is the adjoint of the solve's output,
 // read from `lam` on the rows the forward solve reads from its input (the
 // others are constant rows of the forward system, zero here); the solution
-// overwrites `lam` on those rows.  General path only (no reference-line
-// table): the adjoint sweeps run once per step of the backward pass.
+// overwrites `lam` on those rows.
+//
+// A line whose forward rows all equal the reference line's has the
+// transposed reference rows, so the recorded-multiplier path applies to
+// the transpose too, with its own table (coef_ref below).  Which lines are
+// clean comes from the forward sweep's dirty-line mask (lflag); without
+// one every line is taken as dirty (pt_ref is false) and solved by the
+// general kernel.
 template <typename S>
 struct SysT {
   using V = double;
   using Pt = PtNone;
-  static constexpr bool kCyclic = S::kCyclic, kHasRef = false, kNb = false;
+  static constexpr bool kCyclic = S::kCyclic, kHasRef = S::kHasRef, kNb = false;
   S s;
   double* lam;
   __device__ int n() const { return s.n(); }
@@ -492,12 +498,36 @@ struct SysT {
     }
     b = b0;
   }
-  __device__ void coef_ref(int, double& a, double& b, double& c) const {
-    a = 0.0;
-    b = 1.0;
-    c = 0.0;
+  // row q of the transposed reference line, as coef transposes row()
+  __device__ void coef_ref(int q, double& a, double& b, double& c) const {
+    const int nn = s.n();
+    double a0, b0, c0, am, bm, cm, ap, bp, cp;
+    s.coef_ref(q, a0, b0, c0);
+    if (kCyclic) {
+      s.coef_ref(q == 0 ? nn - 1 : q - 1, am, bm, cm);
+      s.coef_ref(q == nn - 1 ? 0 : q + 1, ap, bp, cp);
+      a = cm;
+      c = ap;
+    } else {
+      a = 0.0;
+      c = 0.0;
+      if (q >= 1) {
+        s.coef_ref(q - 1, am, bm, cm);
+        a = cm;
+      }
+      if (q <= nn - 2) {
+        s.coef_ref(q + 1, ap, bp, cp);
+        c = ap;
+      }
+    }
+    b = b0;
   }
-  RefKey ref_key(int nn, int S_) const { return RefKey{-1, nn, S_, 0, 0.0, 0.0, nullptr}; }
+  // the forward system's key, marked as its transpose
+  RefKey ref_key(int nn, int S_) const {
+    RefKey k = s.ref_key(nn, S_);
+    k.kind += 100;
+    return k;
+  }
   __device__ void store(int la, int lb, int q, double x) const {
     if (s.in_row(q)) lam[s.addr(la, lb, q)] = x;
   }
@@ -587,6 +617,8 @@ struct SolveAdj {
   template <typename Sys, typename Lane>
   int operator()(const Sys& sys, int n, int nla, int nlb, Lane, const char* name) const {
     SysT<Sys> st{sys, lam};
+    // the forward sweep's dirty-line mask, when the system carries one
+    use_flags(st, sys.lflag, sys.lbit, sys.lstride);
     ProfScope ps("line_adjoint", s);
     return line_status(launch_line_solve<Lane::value>(st, n, nla, nlb, s), name);
   }
