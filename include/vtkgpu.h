@@ -319,6 +319,44 @@ int vk_field_copy(const double *src, double *dst, const int *nx,
 int vk_field_absmax(const double *f, const int *nx, const int *ny,
                     const int *nz, double *result);
 
+/* ------------------------------------------------------- diagnostics
+   The reference's run-time checks as device reductions over resident
+   fields (metrics resident too); the scalars are host variables, as in the
+   reference's calls.  Maxima and positions are exact; sums agree with the
+   reference's serial loops to rounding. */
+
+/* CHECK_COURANT(DT,DX,DY,DZ,NX,NY,NZ,U,V,W) (nse_dominik.f:2519-2601,
+   called at :467).  The reference prints the largest signed u dt/dx,
+   v dt/dy, w dt/dz (from 0); here it is returned in *courant. */
+int vk_check_courant(const double *dt, const double *dx, const double *dy,
+                     const double *dz, const int *nx, const int *ny,
+                     const int *nz, const double *u, const double *v,
+                     const double *w, double *courant);
+
+/* MONITOR(...) (nse_dominik.f:2797-2872, called at :493), same arguments
+   and updates: the velocity integrals and their relative changes, T_all
+   over the plane k = nz+1 (the reference's loop index after its k loop),
+   dif, T_b, and the largest |T - T_before|.  With *int_report == 1 it
+   prints what the reference prints.  loc (may be null) receives the
+   position the reference prints as xx, yy, zz (0s when T == T_before). */
+int vk_monitor(const int *nx, const int *ny, const int *nz, const int *l,
+               const double *t, const double *dx, const double *dy,
+               const double *dz, double *dif, double *t_all, double *t_dif,
+               const double *t_before, const double *u, const double *v,
+               const double *w, double *vel_dif, double *u_dif, double *v_dif,
+               double *w_dif, double *u_int_before, double *v_int_before,
+               double *w_int_before, double *t_b, const double *dt,
+               const int *int_report, int *loc);
+
+/* CHECK_CONVERGENCE(Y,YL,U,L,DT,RE,NX,NY,NZ) (nse_dominik.f:2875-2892,
+   called at :483): the sum over the plane i = nx of |u - (-Re/2 y (y-yl))|
+   that the reference appends to timely_convergence.dat with l*dt; here it
+   is returned in *error_sum and the caller writes the record. */
+int vk_check_convergence(const double *y, const double *yl, const double *u,
+                         const int *l, const double *dt, const double *re,
+                         const int *nx, const int *ny, const int *nz,
+                         double *error_sum);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1527,3 +1527,103 @@ void o_time_step(o_state *s, int l) {
         s->w_ta, s->u_ep2, s->v_ep2, s->w_ep2, s->divu, s->u, s->v, s->w,
         s->xl, s->zl);
 }
+
+/* ------------------------------------------------------- diagnostics */
+
+/* nse_dominik.f:2519-2601.  Each ratio is (x dt) / d as the reference
+   forms it; the three maxima start at 0 and take a value only when it is
+   larger (a negative or NaN ratio never enters), and the result is their
+   maximum. */
+void o_check_courant(double dt, const double *dx, const double *dy, const double *dz, int nx,
+                     int ny, int nz, const double *u, const double *v, const double *w,
+                     double *courant) {
+  GEOM;
+  double mx = 0.0, my = 0.0, mz = 0.0;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        const double cx = FIELD(u, i, j, k) * dt / dx[i];
+        const double cy = FIELD(v, i, j, k) * dt / dy[j];
+        const double cz = FIELD(w, i, j, k) * dt / dz[k];
+        if (mx < cx) mx = cx;
+        if (my < cy) my = cy;
+        if (mz < cz) mz = cz;
+      }
+  /* max(courant_max_x, courant_max_y, courant_max_z) (:2591) */
+  double c = mx;
+  if (my > c) c = my;
+  if (mz > c) c = mz;
+  *courant = c;
+}
+
+/* nse_dominik.f:2797-2872.  The three velocity integrals accumulate in one
+   k, j, i loop (:2814-2822); vel_dif leaves out w (:2829).  T_all is summed
+   over j, i after the k loop has ended, so k = nz + 1 there, the top halo
+   plane (:2838-2843).  The largest |T - T_before| keeps its first position
+   in k, j, i order (strict >, :2854-2864). */
+void o_monitor(int nx, int ny, int nz, int l, const double *t, const double *dx,
+               const double *dy, const double *dz, double *dif, double *t_all, double *t_dif,
+               const double *t_before, const double *u, const double *v, const double *w,
+               double *vel_dif, double *u_dif, double *v_dif, double *w_dif,
+               double *u_int_before, double *v_int_before, double *w_int_before, double *t_b,
+               double dt, int *loc) {
+  GEOM;
+  (void)dt;
+  if (l == 1) {
+    *t_b = 0.0;
+    *u_int_before = 0.0;
+    *v_int_before = 0.0;
+    *w_int_before = 0.0;
+  }
+  double u_int = 0.0, v_int = 0.0, w_int = 0.0;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        u_int = u_int + fabs(FIELD(u, i, j, k)) * dx[i];
+        v_int = v_int + fabs(FIELD(v, i, j, k)) * dy[j];
+        w_int = w_int + fabs(FIELD(w, i, j, k)) * dz[k];
+      }
+  *u_dif = fabs((u_int - *u_int_before) / u_int);
+  *v_dif = fabs((v_int - *v_int_before) / v_int);
+  *w_dif = fabs((w_int - *w_int_before) / w_int);
+  *vel_dif = (*u_dif >= *v_dif) ? *u_dif : *v_dif;
+  *u_int_before = u_int;
+  *v_int_before = v_int;
+  *w_int_before = w_int;
+  const int kt = nz + 1;
+  double ta = 0.0;
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) ta = ta + FIELD(t, i, j, kt) * dx[i] * dy[j] * dz[kt];
+  *t_all = ta;
+  *dif = fabs((ta - *t_b) / ta);
+  *t_b = ta;
+  double td = 0.0;
+  loc[0] = loc[1] = loc[2] = 0;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        const double e = fabs(FIELD(t, i, j, k) - FIELD(t_before, i, j, k));
+        if (e > td) {
+          td = e;
+          loc[0] = i;
+          loc[1] = j;
+          loc[2] = k;
+        }
+      }
+  *t_dif = td;
+}
+
+/* nse_dominik.f:2875-2892: sum over k, j of |u(nx,j,k) - (-Re/2 y (y - yl))|,
+   the analytic value formed as -(((Re/2) y(j)) (y(j) - yl)). */
+void o_check_convergence(const double *y, double yl, const double *u, double re, int nx,
+                         int ny, int nz, double *error_sum) {
+  GEOM;
+  double e = 0.0;
+  const int i = nx;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++) {
+      const double a = -(re / 2.0 * y[j] * (y[j] - yl));
+      e = e + fabs(FIELD(u, i, j, k) - a);
+    }
+  *error_sum = e;
+}

@@ -209,6 +209,27 @@ typedef struct {
 } o_state;
 void o_time_step(o_state *s, int l);
 
+/* ------------------------------------------------------- diagnostics */
+/* CHECK_COURANT nse_dominik.f:2519-2601: the largest of u dt/dx(i),
+   v dt/dy(j), w dt/dz(k) over the interior (signed, from 0), which the
+   reference prints; here returned in *courant. */
+void o_check_courant(double dt, const double *dx, const double *dy, const double *dz, int nx,
+                     int ny, int nz, const double *u, const double *v, const double *w,
+                     double *courant);
+/* MONITOR nse_dominik.f:2797-2872, same argument order (the scalars by
+   pointer as in the reference); loc[3] receives the (i, j, k) of the
+   largest |T - T_before| that the reference prints as xx, yy, zz. */
+void o_monitor(int nx, int ny, int nz, int l, const double *t, const double *dx,
+               const double *dy, const double *dz, double *dif, double *t_all, double *t_dif,
+               const double *t_before, const double *u, const double *v, const double *w,
+               double *vel_dif, double *u_dif, double *v_dif, double *w_dif,
+               double *u_int_before, double *v_int_before, double *w_int_before, double *t_b,
+               double dt, int *loc);
+/* CHECK_CONVERGENCE nse_dominik.f:2875-2892: the error sum that the
+   reference appends to timely_convergence.dat. */
+void o_check_convergence(const double *y, double yl, const double *u, double re, int nx,
+                         int ny, int nz, double *error_sum);
+
 #ifdef __cplusplus
 }
 #endif

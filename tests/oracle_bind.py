@@ -94,6 +94,10 @@ def _declare_oracle(lib):
         "o_set_adj_convection_for_scalar": [vp, vp, vp, cd, cd, ci, ci, ci] + [vp] * 12,
         "o_set_adj_ns_scalar": [cd, ci, ci, ci] + [vp] * 11,
         "o_time_step": [vp, ci],
+        "o_check_courant": [cd, vp, vp, vp, ci, ci, ci, vp, vp, vp, vp],
+        "o_monitor": [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                      vp, vp, vp, vp, cd, vp],
+        "o_check_convergence": [vp, cd, vp, cd, ci, ci, ci, vp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)

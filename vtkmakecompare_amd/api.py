@@ -157,6 +157,61 @@ def field_absmax(f, nx, ny, nz) -> float:
     return r.value
 
 
+def check_courant(dt, dx, dy, dz, nx, ny, nz, u, v, w) -> float:
+    """CHECK_COURANT (nse_dominik.f:2519): the largest signed u dt/dx,
+    v dt/dy, w dt/dz over the interior, from 0 (the reference prints it)."""
+    import ctypes
+    r = ctypes.c_double(0.0)
+    check(lib().vk_check_courant(D(dt), dx, dy, dz, I(nx), I(ny), I(nz), u, v, w,
+                                 ctypes.byref(r)), "check_courant")
+    return r.value
+
+
+class MonitorState:
+    """MONITOR's variables kept between calls by program main (u_int_before,
+    v_int_before, w_int_before, T_b) and the ones it sets."""
+
+    NAMES = ("dif", "t_all", "t_dif", "vel_dif", "u_dif", "v_dif", "w_dif", "u_int_before",
+             "v_int_before", "w_int_before", "t_b")
+
+    def __init__(self):
+        import ctypes
+        self._c = {n: ctypes.c_double(0.0) for n in self.NAMES}
+        self.loc = (0, 0, 0)
+
+    def __getitem__(self, name):
+        return self._c[name].value
+
+
+def monitor(nx, ny, nz, l, t, dx, dy, dz, t_before, u, v, w, dt, state: MonitorState,
+            int_report=0) -> MonitorState:
+    """MONITOR (nse_dominik.f:2797) on device fields; ``state`` carries the
+    scalars between calls as the reference's host variables do."""
+    import ctypes
+    c = state._c
+    loc = (ctypes.c_int * 3)()
+    check(lib().vk_monitor(I(nx), I(ny), I(nz), I(l), t, dx, dy, dz, ctypes.byref(c["dif"]),
+                           ctypes.byref(c["t_all"]), ctypes.byref(c["t_dif"]), t_before, u, v,
+                           w, ctypes.byref(c["vel_dif"]), ctypes.byref(c["u_dif"]),
+                           ctypes.byref(c["v_dif"]), ctypes.byref(c["w_dif"]),
+                           ctypes.byref(c["u_int_before"]), ctypes.byref(c["v_int_before"]),
+                           ctypes.byref(c["w_int_before"]), ctypes.byref(c["t_b"]), D(dt),
+                           I(int_report), loc), "monitor")
+    state.loc = tuple(loc)
+    return state
+
+
+def check_convergence(y, yl, u, l, dt, re, nx, ny, nz) -> float:
+    """CHECK_CONVERGENCE (nse_dominik.f:2875): the error sum over the plane
+    i = nx against the Poiseuille profile (the reference appends it to
+    timely_convergence.dat with l*dt)."""
+    import ctypes
+    r = ctypes.c_double(0.0)
+    check(lib().vk_check_convergence(y, D(yl), u, I(l), D(dt), D(re), I(nx), I(ny), I(nz),
+                                     ctypes.byref(r)), "check_convergence")
+    return r.value
+
+
 def levelset_cellvalue(g: G.Grid, phi0: np.ndarray) -> np.ndarray:
     """cellvalue (4, nz, ny, nx) from set_levelset on the device (slot 1
     filled, slots 2-4 zero until a time step derives them)."""

@@ -94,6 +94,7 @@ enum WorkSlot {
   kWorkRecipF,
   kWorkRhs,
   kWorkFlags,
+  kWorkDiag,
   kWorkSlots
 };
 // `grown` (optional) reports whether this call allocated the slot anew.

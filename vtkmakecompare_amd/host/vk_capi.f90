@@ -188,6 +188,29 @@ module vk_capi
       real(c_double) :: dt, re, sc, eta, xl, zl, objective, grad(*)
       integer(c_int) :: nx, ny, nz, nsteps, every
     end function
+    ! the run-time diagnostics (nse_dominik.f:2519, 2797, 2875); the
+    ! results come back in the trailing arguments
+    integer(c_int) function vk_check_courant(dt, dx, dy, dz, nx, ny, nz, u, v, w, courant) &
+        bind(C, name="vk_check_courant")
+      import :: c_int, c_double
+      real(c_double) :: dt, dx(*), dy(*), dz(*), u(*), v(*), w(*), courant
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_monitor(nx, ny, nz, l, t, dx, dy, dz, dif, t_all, t_dif, &
+        t_before, u, v, w, vel_dif, u_dif, v_dif, w_dif, u_int_before, v_int_before, &
+        w_int_before, t_b, dt, int_report, loc) bind(C, name="vk_monitor")
+      import :: c_int, c_double
+      integer(c_int) :: nx, ny, nz, l, int_report, loc(3)
+      real(c_double) :: t(*), dx(*), dy(*), dz(*), dif, t_all, t_dif, t_before(*)
+      real(c_double) :: u(*), v(*), w(*), vel_dif, u_dif, v_dif, w_dif
+      real(c_double) :: u_int_before, v_int_before, w_int_before, t_b, dt
+    end function
+    integer(c_int) function vk_check_convergence(y, yl, u, l, dt, re, nx, ny, nz, error_sum) &
+        bind(C, name="vk_check_convergence")
+      import :: c_int, c_double
+      real(c_double) :: y(*), yl, u(*), dt, re, error_sum
+      integer(c_int) :: l, nx, ny, nz
+    end function
   end interface
 
 contains

@@ -178,3 +178,52 @@ subroutine set_levelset(nx, ny, nz, dx, dx12, dy12, dz12, phi0, phi, x, y, z, ce
   call vk_check(vk_set_levelset(nx, ny, nz, dx, dx12, dy12, dz12, phi0, phi, x, y, z, &
                                 cellvalue), 'set_levelset')
 end subroutine
+
+! CHECK_COURANT (nse_dominik.f:2519-2601): prints the largest Courant
+! number as the reference does
+subroutine check_courant(dt, dx, dy, dz, nx, ny, nz, u, v, w)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: dt, dx(0:nx+1), dy(0:ny+1), dz(0:nz+1)
+  double precision :: u(0:nx+1,0:ny+1,0:nz+1), v(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: w(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: courant
+  call vk_check(vk_check_courant(dt, dx, dy, dz, nx, ny, nz, u, v, w, courant), &
+                'CHECK_COURANT')
+  write(6,*) 'courant max =', courant
+end subroutine
+
+! MONITOR (nse_dominik.f:2797-2872), the reference's argument list
+subroutine monitor(nx, ny, nz, l, t, dx, dy, dz, dif, t_all, t_dif, t_before, u, v, w, &
+                   vel_dif, u_dif, v_dif, w_dif, u_int_before, v_int_before, w_int_before, &
+                   t_b, dt, int_report)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz, l, int_report
+  double precision :: t(0:nx+1,0:ny+1,0:nz+1), t_before(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: u(0:nx+1,0:ny+1,0:nz+1), v(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: w(0:nx+1,0:ny+1,0:nz+1)
+  double precision :: dx(0:nx+1), dy(0:ny+1), dz(0:nz+1)
+  double precision :: dif, t_all, t_dif, vel_dif, u_dif, v_dif, w_dif
+  double precision :: u_int_before, v_int_before, w_int_before, t_b, dt
+  integer :: loc(3)
+  call vk_check(vk_monitor(nx, ny, nz, l, t, dx, dy, dz, dif, t_all, t_dif, t_before, u, v, &
+                           w, vel_dif, u_dif, v_dif, w_dif, u_int_before, v_int_before, &
+                           w_int_before, t_b, dt, int_report, loc), 'MONITOR')
+end subroutine
+
+! CHECK_CONVERGENCE (nse_dominik.f:2875-2892): the error sum appended to
+! timely_convergence.dat with l*dt, as the reference does
+subroutine check_convergence(y, yl, u, l, dt, re, nx, ny, nz)
+  use vk_capi
+  implicit none
+  integer :: l, nx, ny, nz
+  double precision :: y(0:ny+1), yl, u(0:nx+1,0:ny+1,0:nz+1), dt, re
+  double precision :: error_sum
+  call vk_check(vk_check_convergence(y, yl, u, l, dt, re, nx, ny, nz, error_sum), &
+                'CHECK_CONVERGENCE')
+  open(100, file='timely_convergence.dat', form='formatted', position='APPEND')
+  write(100,*) l*dt, error_sum
+  close(100)
+end subroutine
