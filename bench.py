@@ -172,6 +172,38 @@ def timed_run(lib, run, steps: int, warmup: int):
     return el, profile_entries(lib)
 
 
+def diagnostics_timing(lib, run, cfg, g, reps: int = 10):
+    """CHECK_COURANT, MONITOR and CHECK_CONVERGENCE on the FP64 run's
+    resident fields (T_pre standing in for T_before), each timed with HIP
+    events around its two passes.  Algorithmic bytes: u, v, w for the
+    Courant maximum; u, v, w, T, T_before and the plane k = nz+1 of T for
+    MONITOR; the plane i = nx of u for the convergence sum."""
+    from vtkmakecompare_amd import _lib, api
+    d = run.d
+    st = api.MonitorState()
+    _lib.check(lib.vk_profile_begin(), "profile")
+    for r in range(reps):
+        api.check_courant(cfg.dt, d["dx"], d["dy"], d["dz"], g.nx, g.ny, g.nz, d["u"], d["v"],
+                          d["w"])
+        api.monitor(g.nx, g.ny, g.nz, r + 1, d["t"], d["dx"], d["dy"], d["dz"], d["t_pre"],
+                    d["u"], d["v"], d["w"], cfg.dt, st)
+        api.check_convergence(d["y"], g.yl, d["u"], r + 1, cfg.dt, cfg.re, g.nx, g.ny, g.nz)
+    api.synchronize()
+    _lib.check(lib.vk_profile_end(), "profile")
+    prof = profile_entries(lib)
+    n = g.npoints
+    alg = {"check_courant": 3 * 8 * n, "monitor": 5 * 8 * n + 8 * g.nx * g.ny,
+           "check_convergence": 8 * g.ny * g.nz}
+    out = {}
+    for k, b in alg.items():
+        if k in prof:
+            ms, cnt = prof[k]
+            avg_s = ms / cnt / 1e3
+            out[k] = {"avg_us": round(avg_s * 1e6, 2), "launches": cnt, "alg_bytes": b,
+                      "alg_GBs": round(b / avg_s / 1e9, 1)}
+    return out
+
+
 def kernel_table(prof, g, with_ep2, mixed):
     return {k: {"total_ms": round(v[0], 3), "launches": v[1],
                 "avg_us": round(1e3 * v[0] / v[1], 2),
@@ -233,9 +265,11 @@ def main():
     api.synchronize()
     cv = r64.get("cellvalue")
     res = {}
+    diag = None
     if args.only in ("both", "fp64"):
         el, prof = timed_run(lib, r64, args.steps, args.warmup)
         res["fp64"] = (el, prof, r64)
+        diag = diagnostics_timing(lib, r64, cfg, g)
         if el:
             print(f"[bench] fp64: {1e3 * el / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
     if args.only in ("both", "mixed"):
@@ -315,6 +349,7 @@ def main():
         "mixed_vs_fp64": None,
         "errors_mixed_vs_fp64": errors,
         "forms": forms,
+        "diagnostics": diag,
         "cpu_baseline": cpu,
     }
     if "fp64" in forms and "mixed" in forms and forms["fp64"]["value"]:

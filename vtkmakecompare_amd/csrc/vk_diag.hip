@@ -198,17 +198,27 @@ __global__ __launch_bounds__(1024) void k_monitor_final(Geom g, const double* pa
   }
 }
 
-// CHECK_CONVERGENCE: one block over the plane i = nx, (j, k) in linear order
-__global__ __launch_bounds__(1024) void k_convergence(Geom g, const double* u, const double* y,
-                                                      double yl, double re, double* out) {
-  __shared__ double sh[16];
-  const long long np = (long long)g.ny * g.nz;
+// CHECK_CONVERGENCE, pass 1: block b sums the planes k = 1 + b, 1 + b +
+// gridDim.x, ... of the line i = nx (the threads over j)
+__global__ __launch_bounds__(kDiagThreads) void k_convergence(Geom g, const double* u,
+                                                              const double* y, double yl,
+                                                              double re, double* part) {
+  __shared__ double sh[kDiagThreads / 64];
   double s = 0.0;
-  for (long long q = threadIdx.x; q < np; q += blockDim.x) {
-    const int j = 1 + (int)(q % g.ny), k = 1 + (int)(q / g.ny);
-    const double a = -(re / 2.0 * y[j] * (y[j] - yl));
-    s = s + fabs(u[g.at(g.nx, j, k)] - a);
-  }
+  for (int k = 1 + blockIdx.x; k <= g.nz; k += gridDim.x)
+    for (int j = 1 + threadIdx.x; j <= g.ny; j += blockDim.x) {
+      const double a = -(re / 2.0 * y[j] * (y[j] - yl));
+      s = s + fabs(u[g.at(g.nx, j, k)] - a);
+    }
+  s = block_combine(s, Sum{}, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// pass 2: the partials in block order
+__global__ __launch_bounds__(1024) void k_sum_final(const double* part, int nb, double* out) {
+  __shared__ double sh[16];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s = s + part[b];
   s = block_combine(s, Sum{}, sh);
   if (threadIdx.x == 0) out[0] = s;
 }
@@ -355,7 +365,9 @@ int vk_check_convergence(const double* y, const double* yl, const double* u, con
   hipStream_t s = rt().stream;
   {
     ProfScope ps("check_convergence", s);
-    k_convergence<<<1, 1024, 0, s>>>(g, du, dy_, *yl, *re, ws);
+    const int nb = g.nz < kDiagBlocks ? g.nz : kDiagBlocks;
+    k_convergence<<<nb, kDiagThreads, 0, s>>>(g, du, dy_, *yl, *re, ws + 8);
+    k_sum_final<<<1, 1024, 0, s>>>(ws + 8, nb, ws);
   }
   VK_LAUNCH_CHECK("check_convergence");
   return fetch(ws, error_sum, 1, s);
