@@ -54,7 +54,16 @@ def test_courant_exact(api, n):
     m = metrics(g)
     u, v, w, _, _ = fields(g, 21)
     dt = 1e-3
-    for a, b, c in ((u, v, w), (-np.abs(u), -np.abs(v), -np.abs(w)), (u, 3.0 * v, w)):
+    # quotients within a few ulp of 0.7 everywhere: the device skips the
+    # divides that cannot raise its running maximum, and a skip that is too
+    # eager shows here as a last-bit difference
+    rng = np.random.default_rng(5)
+    ulps = 1.0 + rng.integers(-6, 7, size=g.shape) * 2.0 ** -52
+    un = 0.7 * (g.dx / dt)[None, None, :] * ulps
+    vn = 0.7 * (g.dy / dt)[None, :, None] * ulps[:, ::-1, :]
+    wn = 0.7 * (g.dz / dt)[:, None, None] * ulps[::-1, :, :]
+    for a, b, c in ((u, v, w), (-np.abs(u), -np.abs(v), -np.abs(w)), (u, 3.0 * v, w),
+                    (un, vn, wn)):
         got = api.check_courant(dt, m["dx"], m["dy"], m["dz"], g.nx, g.ny, g.nz, dev(a),
                                 dev(b), dev(c))
         assert got == o_courant(g, dt, a, b, c)
