@@ -24,7 +24,8 @@ OUT = ("u", "v", "w", "p", "t", "u_ta", "v_ta", "w_ta", "divu", "u_temp", "v_tem
 def write_input(path, cfg, g, st, cv, nsteps, mode, every=0):
     with open(path, "wb") as f:
         np.array([g.nx, g.ny, g.nz, nsteps, mode, 0, every, 0], dtype=np.int32).tofile(f)
-        np.array([cfg.dt, cfg.re, cfg.sc, cfg.eta, g.xl, g.zl, 0, 0], dtype=np.float64).tofile(f)
+        np.array([cfg.dt, cfg.re, cfg.sc, cfg.eta, g.xl, g.zl, g.yl, 0],
+                 dtype=np.float64).tofile(f)
         for n in ("dx", "dy", "dz", "dx12", "dy12", "dz12", "x", "y", "z"):
             np.ascontiguousarray(getattr(g, n), dtype=np.float64).tofile(f)
         for n in ("u", "v", "w", "t", "phi0"):
@@ -57,7 +58,9 @@ def test_fortran_driver_matches_oracle(tmp_path, mode):
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     write_input(fin, cfg, g, st, cv, nsteps, mode)
     print(f"vk_driver mode {mode} 16x17x16 {nsteps} steps", flush=True)
-    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    # in tmp_path: the CHECK_CONVERGENCE wrapper appends to timely_convergence.dat
+    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=120, cwd=tmp_path)
     assert r.returncode == 0, r.stderr + r.stdout
     got = read_output(fout, g)
     st_o = dict(st)
@@ -69,6 +72,38 @@ def test_fortran_driver_matches_oracle(tmp_path, mode):
     for n in ("u_temp", "v_temp", "w_temp"):
         assert nrel(got[n], ora[n]) < 1e-12, n
     np.testing.assert_array_equal(got["cellvalue"], ora["cellvalue"])
+    check_diagnostics(r.stdout, tmp_path, cfg, g, ora, nsteps)
+
+
+def check_diagnostics(stdout, tmp_path, cfg, g, ora, nsteps):
+    """The reference-named diagnostics wrappers on the final state against
+    the oracle's (the fields agree to 1e-12, the sums' order differs)."""
+    import ctypes
+    from oracle_bind import P, oracle
+    lines = stdout.splitlines()
+    cour = float(next(x for x in lines if "courant max =" in x).split()[-1])
+    c = ctypes.c_double(0.0)
+    oracle().o_check_courant(cfg.dt, P(g.dx), P(g.dy), P(g.dz), g.nx, g.ny, g.nz,
+                             P(ora["u"]), P(ora["v"]), P(ora["w"]), ctypes.byref(c))
+    assert cour == pytest.approx(c.value, rel=1e-10)
+    mon = [float(x) for x in next(x for x in lines if "vk_driver: monitor" in x).split()[2:]]
+    o = {k: ctypes.c_double(0.0) for k in ("dif", "ta", "td", "vd", "ud", "vdd", "wd", "ub",
+                                          "vb", "wb", "tb")}
+    loc = (ctypes.c_int * 3)()
+    oracle().o_monitor(g.nx, g.ny, g.nz, 1, P(ora["t"]), P(g.dx), P(g.dy), P(g.dz),
+                       *[ctypes.byref(o[k]) for k in ("dif", "ta", "td")], P(ora["t_pre"]),
+                       P(ora["u"]), P(ora["v"]), P(ora["w"]),
+                       *[ctypes.byref(o[k]) for k in ("vd", "ud", "vdd", "wd", "ub", "vb", "wb",
+                                                     "tb")], cfg.dt, loc)
+    assert mon[0] == pytest.approx(o["ta"].value, rel=1e-10)
+    assert mon[1] == pytest.approx(o["td"].value, rel=1e-9, abs=1e-11)
+    assert mon[2] == pytest.approx(o["ub"].value, rel=1e-10)
+    rec = (tmp_path / "timely_convergence.dat").read_text().split()
+    e = ctypes.c_double(0.0)
+    oracle().o_check_convergence(P(g.y), g.yl, P(ora["u"]), cfg.re, g.nx, g.ny, g.nz,
+                                 ctypes.byref(e))
+    assert float(rec[0]) == pytest.approx(nsteps * cfg.dt, rel=1e-12)
+    assert float(rec[1]) == pytest.approx(e.value, rel=1e-10)
 
 
 @pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran host not built")

@@ -17,7 +17,7 @@
 !           2 = mixed-precision step, 3 = objective and adjoint gradient),
 !           levelset (1 = set_levelset from phi0, 0 = cellvalue given),
 !           checkpoint interval (mode 3), 0
-!   float64 dt, Re, Sc, eta, xl, zl, 0, 0
+!   float64 dt, Re, Sc, eta, xl, zl, yl, 0
 !   float64 dx, dy, dz, dx12, dy12, dz12, x, y, z   (n+2 each)
 !   float64 u, v, w, T, phi0                          (halo fields)
 !   float64 cellvalue                                 (nx*ny*nz*4)
@@ -28,7 +28,9 @@ program vk_driver
   use vk_capi
   implicit none
   integer :: hdr(8), nx, ny, nz, nsteps, mode, lsflag, l, ompnthreads, one, zero
-  double precision :: prm(8), dt, re, sc, eta, xl, zl, cons_cell
+  double precision :: prm(8), dt, re, sc, eta, xl, zl, yl, cons_cell
+  double precision :: dif, t_all, t_dif, vel_dif, u_dif, v_dif, w_dif, t_b
+  double precision :: u_int_before, v_int_before, w_int_before
   double precision, allocatable, dimension(:) :: dx, dy, dz, dx12, dy12, dz12, x, y, z
   double precision, allocatable, dimension(:,:,:) :: u, v, w, p, t, u_v, v_v, w_v, u_d, v_d, &
       w_d, u_a, v_a, w_a, u_temp, v_temp, w_temp, u_pre, v_pre, w_pre, u_ta, v_ta, w_ta, &
@@ -52,6 +54,7 @@ program vk_driver
   read (10) prm
   nx = hdr(1); ny = hdr(2); nz = hdr(3); nsteps = hdr(4); mode = hdr(5); lsflag = hdr(6)
   dt = prm(1); re = prm(2); sc = prm(3); eta = prm(4); xl = prm(5); zl = prm(6)
+  yl = prm(7)
   ompnthreads = 1
   one = 1
   zero = 0
@@ -197,6 +200,18 @@ program vk_driver
     call mkp(p, u_temp, v_temp, w_temp, dt, dx, dy, dz, dx12, dy12, dz12, re, nx, ny, nz, &
              ompnthreads, u_ta, v_ta, w_ta, u_ep2, v_ep2, w_ep2, divu, u, v, w, xl, zl)
   end do
+
+  if (mode /= 2) then
+    ! the report-interval checks of program main (nse_dominik.f:467-493) on
+    ! the final state, reduced on the device: CHECK_COURANT prints, MONITOR
+    ! (l = 1, T_pre standing in for T_before) sets its scalars,
+    ! CHECK_CONVERGENCE appends to timely_convergence.dat
+    call check_courant(dt, dx, dy, dz, nx, ny, nz, u, v, w)
+    call monitor(nx, ny, nz, one, t, dx, dy, dz, dif, t_all, t_dif, t_pre, u, v, w, vel_dif, &
+                 u_dif, v_dif, w_dif, u_int_before, v_int_before, w_int_before, t_b, dt, zero)
+    write (6, '(a,3es25.16)') 'vk_driver: monitor ', t_all, t_dif, u_int_before
+    call check_convergence(y, yl, u, nsteps, dt, re, nx, ny, nz)
+  end if
 
   call vk_check(vk_download(u), 'download'); call vk_check(vk_download(v), 'download')
   call vk_check(vk_download(w), 'download'); call vk_check(vk_download(p), 'download')
