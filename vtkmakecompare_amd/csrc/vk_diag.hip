@@ -129,9 +129,20 @@ __global__ __launch_bounds__(kDiagThreads) void k_monitor(Geom g, const double* 
       qe = q;
     }
   }
+  // T_all's share of this block: the plane k = nz + 1 (the k loop's index
+  // after it ends, nse_dominik.f:2838-2843), in the same linear split
+  const int kt = g.nz + 1;
+  const long long np = (long long)g.nx * g.ny;
+  double ta = 0.0;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < np;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int i = 1 + (int)(q % g.nx), j = 1 + (int)(q / g.nx);
+    ta = ta + t[g.at(i, j, kt)] * dx[i] * dy[j] * dz[kt];
+  }
   su = block_combine(su, Sum{}, sh);
   sv = block_combine(sv, Sum{}, sh);
   sw = block_combine(sw, Sum{}, sh);
+  ta = block_combine(ta, Sum{}, sh);
   for (int off = 32; off > 0; off >>= 1)
     combine_pair(e, qe, __shfl_xor(e, off), __shfl_xor(qe, off));
   if (threadIdx.x % 64 == 0) {
@@ -141,40 +152,32 @@ __global__ __launch_bounds__(kDiagThreads) void k_monitor(Geom g, const double* 
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int q = 1; q < kDiagThreads / 64; q++) combine_pair(e, qe, she[q], shq[q]);
-    double* p = part + 5 * blockIdx.x;
+    double* p = part + 6 * blockIdx.x;
     p[0] = su;
     p[1] = sv;
     p[2] = sw;
     p[3] = e;
     p[4] = (double)qe;  // exact below 2^53
+    p[5] = ta;
   }
 }
 
-// MONITOR, pass 2: the partials in block order, and T_all over the plane
-// k = nz + 1 (the k loop's index after it ends, nse_dominik.f:2838-2843)
+// MONITOR, pass 2: the partials in block order
 __global__ __launch_bounds__(1024) void k_monitor_final(Geom g, const double* part, int nb,
-                                                        const double* t, const double* dx,
-                                                        const double* dy, const double* dz,
                                                         double* out) {
   __shared__ double sh[16];
   __shared__ double she[16];
   __shared__ long long shq[16];
   const long long n = (long long)g.interior();
-  double su = 0.0, sv = 0.0, sw = 0.0, e = 0.0;
+  double su = 0.0, sv = 0.0, sw = 0.0, e = 0.0, ta = 0.0;
   long long qe = n;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-    const double* p = part + 5 * b;
+    const double* p = part + 6 * b;
     su = su + p[0];
     sv = sv + p[1];
     sw = sw + p[2];
     combine_pair(e, qe, p[3], (long long)p[4]);
-  }
-  const int kt = g.nz + 1;
-  double ta = 0.0;
-  const long long np = (long long)g.nx * g.ny;
-  for (long long q = threadIdx.x; q < np; q += blockDim.x) {
-    const int i = 1 + (int)(q % g.nx), j = 1 + (int)(q / g.nx);
-    ta = ta + t[g.at(i, j, kt)] * dx[i] * dy[j] * dz[kt];
+    ta = ta + p[5];
   }
   su = block_combine(su, Sum{}, sh);
   sv = block_combine(sv, Sum{}, sh);
@@ -262,7 +265,7 @@ int vk_check_courant(const double* dt, const double* dx, const double* dy, const
   VK_RESOLVE(ddz, const double, dz, (g.nz + 2) * sizeof(double));
   if (!courant) return fail(VK_ERR_ARGUMENT, "check_courant: courant is null");
   const int nb = diag_blocks(g);
-  double* ws = static_cast<double*>(workspace(kWorkDiag, (size_t)(5 * kDiagBlocks + 8) *
+  double* ws = static_cast<double*>(workspace(kWorkDiag, (size_t)(6 * kDiagBlocks + 8) *
                                                              sizeof(double)));
   if (!ws) return VK_ERR_HIP;
   hipStream_t s = rt().stream;
@@ -296,14 +299,14 @@ int vk_monitor(const int* nx, const int* ny, const int* nz, const int* l, const 
       !v_int_before || !w_int_before || !t_b)
     return fail(VK_ERR_ARGUMENT, "monitor: a scalar output is null");
   const int nb = diag_blocks(g);
-  double* ws = static_cast<double*>(workspace(kWorkDiag, (size_t)(5 * kDiagBlocks + 8) *
+  double* ws = static_cast<double*>(workspace(kWorkDiag, (size_t)(6 * kDiagBlocks + 8) *
                                                              sizeof(double)));
   if (!ws) return VK_ERR_HIP;
   hipStream_t s = rt().stream;
   {
     ProfScope ps("monitor", s);
     k_monitor<<<nb, kDiagThreads, 0, s>>>(g, du, dv, dw, dtt, dtb, ddx, ddy, ddz, ws + 8);
-    k_monitor_final<<<1, 1024, 0, s>>>(g, ws + 8, nb, dtt, ddx, ddy, ddz, ws);
+    k_monitor_final<<<1, 1024, 0, s>>>(g, ws + 8, nb, ws);
   }
   VK_LAUNCH_CHECK("monitor");
   double r[6];
@@ -359,7 +362,7 @@ int vk_check_convergence(const double* y, const double* yl, const double* u, con
   DIAG_FIELD(du, u);
   VK_RESOLVE(dy_, const double, y, (g.ny + 2) * sizeof(double));
   if (!error_sum) return fail(VK_ERR_ARGUMENT, "check_convergence: error_sum is null");
-  double* ws = static_cast<double*>(workspace(kWorkDiag, (size_t)(5 * kDiagBlocks + 8) *
+  double* ws = static_cast<double*>(workspace(kWorkDiag, (size_t)(6 * kDiagBlocks + 8) *
                                                              sizeof(double)));
   if (!ws) return VK_ERR_HIP;
   hipStream_t s = rt().stream;
