@@ -357,6 +357,15 @@ int vk_check_convergence(const double *y, const double *yl, const double *u,
                          const int *nx, const int *ny, const int *nz,
                          double *error_sum);
 
+/* READGRID3D(NX,NY,NZ,PHI0) (set_levelset.f:202-214, called at
+   nse_dominik.f:203): the interior of phi0 from the list-directed records
+   "a b c phi0" of structure.dat in the working directory, k, j, i order.
+   phi0 is host memory as in the reference; a registered array's device
+   mirror is refreshed.  Repeat counts, null values and '/' are refused
+   (VK_ERR_ARGUMENT), as are a missing or short file.  Needs no GPU for an
+   unregistered array. */
+int vk_readgrid3d(const int *nx, const int *ny, const int *nz, double *phi0);
+
 #ifdef __cplusplus
 }
 #endif

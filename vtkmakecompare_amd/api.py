@@ -212,6 +212,19 @@ def check_convergence(y, yl, u, l, dt, re, nx, ny, nz) -> float:
     return r.value
 
 
+def readgrid3d(nx, ny, nz, phi0: np.ndarray) -> np.ndarray:
+    """READGRID3D (set_levelset.f:202-214): the interior of the halo array
+    phi0 (float64, C-contiguous, shape (nz+2, ny+2, nx+2)) from
+    structure.dat in the working directory, in place."""
+    import ctypes
+    if phi0.dtype != np.float64 or not phi0.flags.c_contiguous or \
+            phi0.shape != (nz + 2, ny + 2, nx + 2):
+        raise ValueError("phi0 must be a C-contiguous float64 (nz+2, ny+2, nx+2) array")
+    check(lib().vk_readgrid3d(I(nx), I(ny), I(nz), phi0.ctypes.data_as(ctypes.c_void_p)),
+          "readgrid3d")
+    return phi0
+
+
 def levelset_cellvalue(g: G.Grid, phi0: np.ndarray) -> np.ndarray:
     """cellvalue (4, nz, ny, nx) from set_levelset on the device (slot 1
     filled, slots 2-4 zero until a time step derives them)."""

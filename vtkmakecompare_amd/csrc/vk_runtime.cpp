@@ -3,6 +3,7 @@
 #include "vk_runtime.h"
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -408,6 +409,79 @@ int vk_extent(const void* p, size_t* bytes) {
 int vk_memset(void* dst, const int* value, const size_t* bytes) {
   VK_REQUIRE_INIT();
   return hip_check(hipMemsetAsync(dst, *value, *bytes, rt().stream), "vk_memset");
+}
+
+
+// readgrid3d (set_levelset.f:202-214): nx*ny*nz list-directed records
+// "a b c phi0" of structure.dat in k, j, i order into the interior of
+// phi0 (the halo is left as it is).  The list-directed rules used: values
+// are separated by blanks or a comma, an item may continue on the next
+// records, the rest of the record after an item's fourth value is
+// skipped, and the exponent letter may be E or D.  Repeat counts (r*c),
+// null values and the slash terminator are refused.  phi0 is host memory,
+// as in the reference; when it is registered its device mirror is
+// refreshed.
+int vk_readgrid3d(const int* nx, const int* ny, const int* nz, double* phi0) {
+  if (*nx < 1 || *ny < 1 || *nz < 1) return fail(VK_ERR_BAD_SIZE, "readgrid3d: nx, ny, nz >= 1");
+  if (!phi0) return fail(VK_ERR_ARGUMENT, "readgrid3d: phi0 is null");
+  std::FILE* f = std::fopen("structure.dat", "rb");
+  if (!f) return fail(VK_ERR_ARGUMENT, "readgrid3d: cannot open structure.dat");
+  std::string text;
+  {
+    std::vector<char> buf(1 << 20);
+    size_t got;
+    while ((got = std::fread(buf.data(), 1, buf.size(), f)) > 0) text.append(buf.data(), got);
+    std::fclose(f);
+  }
+  const char* p = text.data();
+  const char* const end = p + text.size();
+  const size_t sy = (size_t)*nx + 2, sz = sy * ((size_t)*ny + 2);
+  char tok[128];
+  for (int k = 1; k <= *nz; k++)
+    for (int j = 1; j <= *ny; j++)
+      for (int i = 1; i <= *nx; i++) {
+        double val[4];
+        int nv = 0;
+        bool sep_comma = false;
+        while (nv < 4) {
+          while (p < end && (*p == ' ' || *p == '\t' || *p == '\r' || *p == ',')) {
+            if (*p == ',') {
+              if (sep_comma) return fail(VK_ERR_ARGUMENT, "readgrid3d: null value");
+              sep_comma = true;
+            }
+            p++;
+          }
+          if (p >= end) return fail(VK_ERR_ARGUMENT, "readgrid3d: structure.dat ends early");
+          if (*p == '\n') {  // the item continues on the next record
+            p++;
+            continue;
+          }
+          const char* t0 = p;
+          while (p < end && *p != ' ' && *p != '\t' && *p != '\r' && *p != ',' && *p != '\n') p++;
+          const size_t n = (size_t)(p - t0);
+          if (n == 0 || n >= sizeof(tok))
+            return fail(VK_ERR_ARGUMENT, "readgrid3d: bad value in structure.dat");
+          for (size_t q = 0; q < n; q++) {
+            const char c = t0[q];
+            if (c == '*' || c == '/')
+              return fail(VK_ERR_ARGUMENT, "readgrid3d: repeat counts and '/' not supported");
+            tok[q] = (c == 'd' || c == 'D') ? 'e' : c;
+          }
+          tok[n] = '\0';
+          char* e = nullptr;
+          const double v = std::strtod(tok, &e);
+          if (e != tok + n) return fail(VK_ERR_ARGUMENT, "readgrid3d: bad value in structure.dat");
+          val[nv++] = v;
+          sep_comma = false;
+        }
+        phi0[(size_t)i + sy * (size_t)j + sz * (size_t)k] = val[3];
+        while (p < end && *p != '\n') p++;  // the rest of the record
+        if (p < end) p++;
+      }
+  if (rt().initialised &&
+      registry().count(reinterpret_cast<uintptr_t>(static_cast<const void*>(phi0))))
+    return vk_upload(phi0);
+  return VK_OK;
 }
 
 }  // extern "C"

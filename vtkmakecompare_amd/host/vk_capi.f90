@@ -211,6 +211,12 @@ module vk_capi
       real(c_double) :: y(*), yl, u(*), dt, re, error_sum
       integer(c_int) :: l, nx, ny, nz
     end function
+    ! set_levelset.f:202-214: structure.dat into the interior of phi0
+    integer(c_int) function vk_readgrid3d(nx, ny, nz, phi0) bind(C, name="vk_readgrid3d")
+      import :: c_int, c_double
+      integer(c_int) :: nx, ny, nz
+      real(c_double) :: phi0(*)
+    end function
   end interface
 
 contains

@@ -227,3 +227,12 @@ subroutine check_convergence(y, yl, u, l, dt, re, nx, ny, nz)
   write(100,*) l*dt, error_sum
   close(100)
 end subroutine
+
+! READGRID3D (set_levelset.f:202-214): phi0's interior from structure.dat
+subroutine readgrid3d(nx, ny, nz, phi0)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: phi0(0:nx+1,0:ny+1,0:nz+1)
+  call vk_check(vk_readgrid3d(nx, ny, nz, phi0), 'READGRID3D')
+end subroutine
