@@ -265,8 +265,21 @@ def main():
     npts = g.npoints
     print(f"[bench] {cfg.name} {g.nx}x{g.ny}x{g.nz}: level set on device", file=sys.stderr,
           flush=True)
+    # the level-set setup (set_levelset's 2001 reinitialisation iterations)
+    # timed on its own: HIP events around the iteration loop
+    _lib.check(lib.vk_profile_begin(), "profile")
     r64 = api.ChannelRun(cfg, g, st, diagnostics=args.diagnostics)
     api.synchronize()
+    _lib.check(lib.vk_profile_end(), "profile")
+    lsp = profile_entries(lib).get("levelset_reinit")
+    levelset = None
+    if lsp:
+        per_it = lsp[0] / 2001 / 1e3
+        # per iteration: phi (read with its neighbours) and phi0 in, phi out
+        lb = 3 * 8 * npts
+        levelset = {"total_ms": round(lsp[0], 2), "iterations": 2001,
+                    "us_per_iteration": round(per_it * 1e6, 2), "alg_bytes_per_iteration": lb,
+                    "alg_GBs": round(lb / per_it / 1e9, 1)}
     cv = r64.get("cellvalue")
     res = {}
     diag = None
@@ -354,6 +367,7 @@ def main():
         "errors_mixed_vs_fp64": errors,
         "forms": forms,
         "diagnostics": diag,
+        "levelset": levelset,
         "cpu_baseline": cpu,
     }
     if "fp64" in forms and "mixed" in forms and forms["fp64"]["value"]:
