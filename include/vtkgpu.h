@@ -366,6 +366,28 @@ int vk_check_convergence(const double *y, const double *yl, const double *u,
    unregistered array. */
 int vk_readgrid3d(const int *nx, const int *ny, const int *nz, double *phi0);
 
+/* OUTFIELD(X,Y,Z,U,V,W,P,NX,NY,NZ,TID,T,PHI,U_TA,V_TA,W_TA)
+   (nse_dominik.f:2726-2766, called at :489 and :523): field.u (the record
+   tid, then one record of u, v, w, p, T, phi, u_ta, v_ta, w_ta per halo
+   point, k, j, i order) and ./nseoutput2/grid.u (x, y, z of the interior,
+   one record each), Fortran sequential unformatted with 4-byte record
+   markers, byte for byte as the reference writes them.  The fields are
+   host arrays; registered ones are downloaded first. */
+int vk_outfield(const double *x, const double *y, const double *z,
+                const double *u, const double *v, const double *w,
+                const double *p, const int *nx, const int *ny, const int *nz,
+                const double *tid, const double *t, const double *phi,
+                const double *u_ta, const double *v_ta, const double *w_ta);
+
+/* READFIELD(U,V,W,P,NX,NY,NZ,T,PHI,U_TA,V_TA,W_TA) (nse_dominik.f:2769-2793,
+   the restart at :240): ./nseoutput2/field.u, one record of 9 values per
+   halo point.  Unlike the reference's read, a leading 8-byte record (the
+   tid OUTFIELD writes) is skipped, so OUTFIELD's file reads back.
+   Registered arrays are uploaded after the read. */
+int vk_readfield(double *u, double *v, double *w, double *p, const int *nx,
+                 const int *ny, const int *nz, double *t, double *phi,
+                 double *u_ta, double *v_ta, double *w_ta);
+
 #ifdef __cplusplus
 }
 #endif

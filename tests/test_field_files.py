@@ -1,0 +1,92 @@
+"""OUTFIELD and READFIELD (nse_dominik.f:2726-2793), the reference's field
+files, against the reference's own compiled routines: the written files
+byte for byte, and the read fields bit for bit.  The reference's READFIELD
+expects no tid record, which OUTFIELD writes first; the library skips a
+leading 8-byte record, so its own file reads back (the round trip below).
+Host arrays, so no GPU is touched.
+"""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from oracle_bind import D, I, P, ref
+from vtkmakecompare_amd import api as A
+from vtkmakecompare_amd import grid as G
+
+NAMES = ("u", "v", "w", "p", "t", "phi", "u_ta", "v_ta", "w_ta")
+needs_ref = pytest.mark.skipif(ref() is None, reason="reference library not built")
+
+
+def state(g, seed):
+    rng = np.random.default_rng(seed)
+    return {k: rng.normal(size=g.shape) for k in NAMES}
+
+
+class chdir:
+    def __init__(self, d):
+        self.d = d
+
+    def __enter__(self):
+        self.old = os.getcwd()
+        os.chdir(self.d)
+
+    def __exit__(self, *a):
+        os.chdir(self.old)
+
+
+@needs_ref
+@pytest.mark.parametrize("n", [(6, 5, 4), (9, 4, 7)])
+def test_outfield_bytes_match_reference(tmp_path, n):
+    g = G.channel_grid(*n)
+    st = state(g, 1)
+    tid = 12.375
+    for d in ("ours", "ref"):
+        (tmp_path / d / "nseoutput2").mkdir(parents=True)
+    with chdir(tmp_path / "ours"):
+        A.outfield(g, tid, st)
+    with chdir(tmp_path / "ref"):
+        s = st
+        ref().outfield_(P(g.x), P(g.y), P(g.z), P(s["u"]), P(s["v"]), P(s["w"]), P(s["p"]),
+                        I(g.nx), I(g.ny), I(g.nz), D(tid), P(s["t"]), P(s["phi"]),
+                        P(s["u_ta"]), P(s["v_ta"]), P(s["w_ta"]))
+    for f in ("field.u", "nseoutput2/grid.u"):
+        assert (tmp_path / "ours" / f).read_bytes() == (tmp_path / "ref" / f).read_bytes(), f
+
+
+@needs_ref
+def test_readfield_matches_reference(tmp_path):
+    g = G.channel_grid(7, 5, 6)
+    st = state(g, 2)
+    (tmp_path / "nseoutput2").mkdir()
+    with chdir(tmp_path):
+        A.outfield(g, 1.5, st)
+        # the file the reference's READFIELD takes: the per-point records only
+        raw = (tmp_path / "field.u").read_bytes()
+        (tmp_path / "nseoutput2" / "field.u").write_bytes(raw[16:])
+        ours = A.readfield(g, {k: np.zeros(g.shape) for k in NAMES})
+        theirs = {k: np.zeros(g.shape) for k in NAMES}
+        t = theirs
+        ref().readfield_(P(t["u"]), P(t["v"]), P(t["w"]), P(t["p"]), I(g.nx), I(g.ny), I(g.nz),
+                         P(t["t"]), P(t["phi"]), P(t["u_ta"]), P(t["v_ta"]), P(t["w_ta"]))
+    for k in NAMES:
+        np.testing.assert_array_equal(ours[k], theirs[k], err_msg=k)
+        np.testing.assert_array_equal(ours[k], st[k], err_msg=k)
+
+
+def test_outfield_readfield_round_trip(tmp_path):
+    g = G.channel_grid(8, 6, 5)
+    st = state(g, 3)
+    (tmp_path / "nseoutput2").mkdir()
+    with chdir(tmp_path):
+        A.outfield(g, 3.0, st)
+        shutil.copy("field.u", "nseoutput2/field.u")  # with its tid record
+        back = A.readfield(g, {k: np.zeros(g.shape) for k in NAMES})
+        # a truncated file is refused
+        raw = open("nseoutput2/field.u", "rb").read()
+        open("nseoutput2/field.u", "wb").write(raw[:-40])
+        with pytest.raises(A.VKError):
+            A.readfield(g, {k: np.zeros(g.shape) for k in NAMES})
+    for k in NAMES:
+        np.testing.assert_array_equal(back[k], st[k], err_msg=k)

@@ -225,6 +225,31 @@ def readgrid3d(nx, ny, nz, phi0: np.ndarray) -> np.ndarray:
     return phi0
 
 
+def _host(a):
+    import ctypes
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise ValueError("field files take C-contiguous float64 host arrays")
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def outfield(g: G.Grid, tid: float, fields: dict) -> None:
+    """OUTFIELD (nse_dominik.f:2726): field.u and ./nseoutput2/grid.u in the
+    working directory from host arrays u, v, w, p, t, phi, u_ta, v_ta, w_ta."""
+    f = [_host(fields[k]) for k in ("u", "v", "w", "p")]
+    r = [_host(fields[k]) for k in ("t", "phi", "u_ta", "v_ta", "w_ta")]
+    check(lib().vk_outfield(_host(g.x), _host(g.y), _host(g.z), *f, I(g.nx), I(g.ny), I(g.nz),
+                            D(tid), *r), "outfield")
+
+
+def readfield(g: G.Grid, fields: dict) -> dict:
+    """READFIELD (nse_dominik.f:2769): ./nseoutput2/field.u into the host
+    arrays u, v, w, p, t, phi, u_ta, v_ta, w_ta (halo shape), in place."""
+    f = [_host(fields[k]) for k in ("u", "v", "w", "p")]
+    r = [_host(fields[k]) for k in ("t", "phi", "u_ta", "v_ta", "w_ta")]
+    check(lib().vk_readfield(*f, I(g.nx), I(g.ny), I(g.nz), *r), "readfield")
+    return fields
+
+
 def levelset_cellvalue(g: G.Grid, phi0: np.ndarray) -> np.ndarray:
     """cellvalue (4, nz, ny, nx) from set_levelset on the device (slot 1
     filled, slots 2-4 zero until a time step derives them)."""

@@ -2,6 +2,7 @@
 // registry (host array -> device mirror), workspace and error reporting.
 #include "vk_runtime.h"
 
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -481,6 +482,123 @@ int vk_readgrid3d(const int* nx, const int* ny, const int* nz, double* phi0) {
   if (rt().initialised &&
       registry().count(reinterpret_cast<uintptr_t>(static_cast<const void*>(phi0))))
     return vk_upload(phi0);
+  return VK_OK;
+}
+
+
+// ------------------------------------------------------------ field files
+// Fortran sequential unformatted records: a 4-byte length, the data, the
+// length again (the layout flang and gfortran write for records under
+// 2 GiB).
+
+// a registered host array's device copy brought back before it is written
+// to a file, or refreshed after it is read from one
+static int host_sync(const void* host, bool down) {
+  if (!rt().initialised || !registry().count(reinterpret_cast<uintptr_t>(host))) return VK_OK;
+  return down ? vk_download(const_cast<void*>(host)) : vk_upload(host);
+}
+
+struct RecordWriter {
+  std::FILE* f;
+  std::vector<char> buf;
+  bool ok = true;
+  explicit RecordWriter(std::FILE* f_) : f(f_) { buf.reserve(1 << 22); }
+  void record(const double* v, int n) {
+    const int32_t len = (int32_t)(n * sizeof(double));
+    const char* l = reinterpret_cast<const char*>(&len);
+    buf.insert(buf.end(), l, l + 4);
+    const char* d = reinterpret_cast<const char*>(v);
+    buf.insert(buf.end(), d, d + len);
+    buf.insert(buf.end(), l, l + 4);
+    if (buf.size() >= (1u << 22)) flush();
+  }
+  void flush() {
+    if (!buf.empty() && std::fwrite(buf.data(), 1, buf.size(), f) != buf.size()) ok = false;
+    buf.clear();
+  }
+};
+
+// OUTFIELD (nse_dominik.f:2726-2766): field.u holds the record tid, then
+// one record (u, v, w, p, T, phi, u_ta, v_ta, w_ta) per halo point in k, j,
+// i order; ./nseoutput2/grid.u holds x(1..nx), y(1..ny), z(1..nz), one
+// record each.
+int vk_outfield(const double* x, const double* y, const double* z, const double* u,
+                const double* v, const double* w, const double* p, const int* nx,
+                const int* ny, const int* nz, const double* tid, const double* t,
+                const double* phi, const double* u_ta, const double* v_ta, const double* w_ta) {
+  if (*nx < 1 || *ny < 1 || *nz < 1) return fail(VK_ERR_BAD_SIZE, "outfield: nx, ny, nz >= 1");
+  const double* f9[9] = {u, v, w, p, t, phi, u_ta, v_ta, w_ta};
+  for (const double* a : f9) {
+    if (!a) return fail(VK_ERR_ARGUMENT, "outfield: a field is null");
+    if (int rc = host_sync(a, true)) return rc;
+  }
+  const size_t n = ((size_t)*nx + 2) * ((size_t)*ny + 2) * ((size_t)*nz + 2);
+  std::FILE* f = std::fopen("field.u", "wb");
+  if (!f) return fail(VK_ERR_ARGUMENT, "outfield: cannot open field.u");
+  RecordWriter wr(f);
+  wr.record(tid, 1);
+  double rec[9];
+  for (size_t q = 0; q < n; q++) {
+    for (int c = 0; c < 9; c++) rec[c] = f9[c][q];
+    wr.record(rec, 9);
+  }
+  wr.flush();
+  const bool ok = wr.ok && std::fclose(f) == 0;
+  if (!ok) return fail(VK_ERR_ARGUMENT, "outfield: writing field.u failed");
+  std::FILE* gfile = std::fopen("./nseoutput2/grid.u", "wb");
+  if (!gfile) return fail(VK_ERR_ARGUMENT, "outfield: cannot open ./nseoutput2/grid.u");
+  RecordWriter gw(gfile);
+  for (int i = 1; i <= *nx; i++) gw.record(x + i, 1);
+  for (int j = 1; j <= *ny; j++) gw.record(y + j, 1);
+  for (int k = 1; k <= *nz; k++) gw.record(z + k, 1);
+  gw.flush();
+  if (!gw.ok || std::fclose(gfile) != 0) return fail(VK_ERR_ARGUMENT, "outfield: writing grid.u failed");
+  return VK_OK;
+}
+
+// READFIELD (nse_dominik.f:2769-2793): ./nseoutput2/field.u, one record of
+// 9 values per halo point in k, j, i order.  A leading 8-byte record (the
+// tid that OUTFIELD writes first, which the reference's read does not
+// expect) is skipped, so that a file OUTFIELD wrote reads back.
+int vk_readfield(double* u, double* v, double* w, double* p, const int* nx, const int* ny,
+                 const int* nz, double* t, double* phi, double* u_ta, double* v_ta,
+                 double* w_ta) {
+  if (*nx < 1 || *ny < 1 || *nz < 1) return fail(VK_ERR_BAD_SIZE, "readfield: nx, ny, nz >= 1");
+  double* f9[9] = {u, v, w, p, t, phi, u_ta, v_ta, w_ta};
+  for (double* a : f9)
+    if (!a) return fail(VK_ERR_ARGUMENT, "readfield: a field is null");
+  std::FILE* f = std::fopen("./nseoutput2/field.u", "rb");
+  if (!f) return fail(VK_ERR_ARGUMENT, "readfield: cannot open ./nseoutput2/field.u");
+  std::string text;
+  {
+    std::vector<char> buf(1 << 22);
+    size_t got;
+    while ((got = std::fread(buf.data(), 1, buf.size(), f)) > 0) text.append(buf.data(), got);
+    std::fclose(f);
+  }
+  const size_t n = ((size_t)*nx + 2) * ((size_t)*ny + 2) * ((size_t)*nz + 2);
+  const char* c = text.data();
+  size_t left = text.size();
+  auto len_at = [&](const char* q) {
+    int32_t l;
+    std::memcpy(&l, q, 4);
+    return l;
+  };
+  if (left >= 16 && len_at(c) == 8 && len_at(c + 12) == 8) {  // the tid record
+    c += 16;
+    left -= 16;
+  }
+  const size_t recb = 4 + 72 + 4;
+  if (left < n * recb) return fail(VK_ERR_ARGUMENT, "readfield: field.u is too short");
+  for (size_t q = 0; q < n; q++, c += recb) {
+    if (len_at(c) != 72 || len_at(c + 76) != 72)
+      return fail(VK_ERR_ARGUMENT, "readfield: field.u record is not 9 doubles");
+    double rec[9];
+    std::memcpy(rec, c + 4, 72);
+    for (int k = 0; k < 9; k++) f9[k][q] = rec[k];
+  }
+  for (double* a : f9)
+    if (int rc = host_sync(a, false)) return rc;
   return VK_OK;
 }
 

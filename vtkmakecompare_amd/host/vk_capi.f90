@@ -217,6 +217,20 @@ module vk_capi
       integer(c_int) :: nx, ny, nz
       real(c_double) :: phi0(*)
     end function
+    ! nse_dominik.f:2726-2793: the field files
+    integer(c_int) function vk_outfield(x, y, z, u, v, w, p, nx, ny, nz, tid, t, phi, u_ta, &
+        v_ta, w_ta) bind(C, name="vk_outfield")
+      import :: c_int, c_double
+      real(c_double) :: x(*), y(*), z(*), u(*), v(*), w(*), p(*), tid, t(*), phi(*)
+      real(c_double) :: u_ta(*), v_ta(*), w_ta(*)
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_readfield(u, v, w, p, nx, ny, nz, t, phi, u_ta, v_ta, w_ta) &
+        bind(C, name="vk_readfield")
+      import :: c_int, c_double
+      real(c_double) :: u(*), v(*), w(*), p(*), t(*), phi(*), u_ta(*), v_ta(*), w_ta(*)
+      integer(c_int) :: nx, ny, nz
+    end function
   end interface
 
 contains

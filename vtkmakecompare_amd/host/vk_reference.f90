@@ -236,3 +236,23 @@ subroutine readgrid3d(nx, ny, nz, phi0)
   double precision :: phi0(0:nx+1,0:ny+1,0:nz+1)
   call vk_check(vk_readgrid3d(nx, ny, nz, phi0), 'READGRID3D')
 end subroutine
+
+! OUTFIELD (nse_dominik.f:2726-2766): field.u and ./nseoutput2/grid.u
+subroutine outfield(x, y, z, u, v, w, p, nx, ny, nz, tid, t, phi, u_ta, v_ta, w_ta)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: x(0:nx+1), y(0:ny+1), z(0:nz+1), tid
+  double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: u, v, w, p, t, phi, u_ta, v_ta, w_ta
+  call vk_check(vk_outfield(x, y, z, u, v, w, p, nx, ny, nz, tid, t, phi, u_ta, v_ta, w_ta), &
+                'OUTFIELD')
+end subroutine
+
+! READFIELD (nse_dominik.f:2769-2793): the restart from ./nseoutput2/field.u
+subroutine readfield(u, v, w, p, nx, ny, nz, t, phi, u_ta, v_ta, w_ta)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: u, v, w, p, t, phi, u_ta, v_ta, w_ta
+  call vk_check(vk_readfield(u, v, w, p, nx, ny, nz, t, phi, u_ta, v_ta, w_ta), 'READFIELD')
+end subroutine
