@@ -381,12 +381,27 @@ int vk_outfield(const double *x, const double *y, const double *z,
 
 /* READFIELD(U,V,W,P,NX,NY,NZ,T,PHI,U_TA,V_TA,W_TA) (nse_dominik.f:2769-2793,
    the restart at :240): ./nseoutput2/field.u, one record of 9 values per
-   halo point.  Unlike the reference's read, a leading 8-byte record (the
-   tid OUTFIELD writes) is skipped, so OUTFIELD's file reads back.
+   halo point, as OUTPUT writes it.  Unlike the reference's read, a leading
+   8-byte record (the tid OUTFIELD writes first) is skipped, so OUTFIELD's
+   field.u reads back too.
    Registered arrays are uploaded after the read. */
 int vk_readfield(double *u, double *v, double *w, double *p, const int *nx,
                  const int *ny, const int *nz, double *t, double *phi,
                  double *u_ta, double *v_ta, double *w_ta);
+
+/* OUTPUT(U,V,W,P,X,Y,Z,NX,NY,NZ,T,CELLVALUE,U_TA,V_TA,W_TA,PHI,DIVU)
+   (nse_dominik.f:2606-2723, called at :480 and :527): creates nseoutput2/
+   when missing and writes its twelve unformatted per-point files, gridnumber.dat,
+   field.u (no tid record: the file READFIELD reads) and grid.u, byte for
+   byte as the reference; temperature_check.dat holds the same numbers in
+   list-directed text (17 significant digits).  Registered arrays are
+   downloaded first. */
+int vk_output(const double *u, const double *v, const double *w,
+              const double *p, const double *x, const double *y,
+              const double *z, const int *nx, const int *ny, const int *nz,
+              const double *t, const double *cellvalue, const double *u_ta,
+              const double *v_ta, const double *w_ta, const double *phi,
+              const double *divu);
 
 #ifdef __cplusplus
 }

@@ -1,9 +1,10 @@
-"""OUTFIELD and READFIELD (nse_dominik.f:2726-2793), the reference's field
-files, against the reference's own compiled routines: the written files
-byte for byte, and the read fields bit for bit.  The reference's READFIELD
-expects no tid record, which OUTFIELD writes first; the library skips a
-leading 8-byte record, so its own file reads back (the round trip below).
-Host arrays, so no GPU is touched.
+"""OUTPUT, OUTFIELD and READFIELD (nse_dominik.f:2606-2793), the
+reference's field files, against the reference's own compiled routines:
+the written files byte for byte (temperature_check.dat, list-directed
+text, number for number), and the read fields bit for bit.  READFIELD
+reads the field.u that OUTPUT writes; OUTFIELD writes a tid record first,
+which the library's reader skips, so that file reads back too (the round
+trip below).  Host arrays, so no GPU is touched.
 """
 import os
 import shutil
@@ -90,3 +91,37 @@ def test_outfield_readfield_round_trip(tmp_path):
             A.readfield(g, {k: np.zeros(g.shape) for k in NAMES})
     for k in NAMES:
         np.testing.assert_array_equal(back[k], st[k], err_msg=k)
+
+
+@needs_ref
+def test_output_matches_reference(tmp_path):
+    g = G.channel_grid(7, 5, 6)
+    rng = np.random.default_rng(9)
+    names = ("u", "v", "w", "p", "t", "u_ta", "v_ta", "w_ta", "phi", "divu")
+    f = {k: rng.normal(size=g.shape) for k in names}
+    f["u"][2, 3, 3] = -f["u"][2, 3, 2]  # a face average of exactly 0
+    f["v"][4, 1, 5] = 1e-31 - f["v"][4, 0, 5]  # below the 1e-30 cut
+    cv = rng.uniform(size=(4, g.nz, g.ny, g.nx))
+    for d in ("ours", "ref"):
+        (tmp_path / d).mkdir()
+    with chdir(tmp_path / "ours"):
+        A.output(g, f, cv)
+    with chdir(tmp_path / "ref"):
+        ref().output_(P(f["u"]), P(f["v"]), P(f["w"]), P(f["p"]), P(g.x), P(g.y), P(g.z),
+                      I(g.nx), I(g.ny), I(g.nz), P(f["t"]), P(cv), P(f["u_ta"]), P(f["v_ta"]),
+                      P(f["w_ta"]), P(f["phi"]), P(f["divu"]))
+    files = sorted(os.listdir(tmp_path / "ref" / "nseoutput2"))
+    assert files == sorted(os.listdir(tmp_path / "ours" / "nseoutput2"))
+    assert len(files) == 15
+    for name in files:
+        a = (tmp_path / "ours" / "nseoutput2" / name).read_bytes()
+        b = (tmp_path / "ref" / "nseoutput2" / name).read_bytes()
+        assert a == b, name
+    ta = [float(x) for x in (tmp_path / "ours" / "temperature_check.dat").read_text().split()]
+    tb = [float(x) for x in (tmp_path / "ref" / "temperature_check.dat").read_text().split()]
+    assert ta == tb and len(ta) == 2 * (g.ny + 2)
+    # OUTPUT's field.u is the restart READFIELD reads
+    with chdir(tmp_path / "ours"):
+        back = A.readfield(g, {k: np.zeros(g.shape) for k in NAMES})
+    for k in NAMES:
+        np.testing.assert_array_equal(back[k], f[k], err_msg=k)

@@ -241,6 +241,17 @@ def outfield(g: G.Grid, tid: float, fields: dict) -> None:
                             D(tid), *r), "outfield")
 
 
+def output(g: G.Grid, fields: dict, cellvalue: np.ndarray) -> None:
+    """OUTPUT (nse_dominik.f:2606): the nseoutput2/ files and
+    temperature_check.dat in the working directory, from host arrays u, v,
+    w, p, t, u_ta, v_ta, w_ta, phi, divu and cellvalue (4, nz, ny, nx)."""
+    f = {k: _host(fields[k]) for k in ("u", "v", "w", "p", "t", "u_ta", "v_ta", "w_ta", "phi",
+                                       "divu")}
+    check(lib().vk_output(f["u"], f["v"], f["w"], f["p"], _host(g.x), _host(g.y), _host(g.z),
+                          I(g.nx), I(g.ny), I(g.nz), f["t"], _host(cellvalue), f["u_ta"],
+                          f["v_ta"], f["w_ta"], f["phi"], f["divu"]), "output")
+
+
 def readfield(g: G.Grid, fields: dict) -> dict:
     """READFIELD (nse_dominik.f:2769): ./nseoutput2/field.u into the host
     arrays u, v, w, p, t, phi, u_ta, v_ta, w_ta (halo shape), in place."""

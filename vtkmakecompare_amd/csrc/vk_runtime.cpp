@@ -2,6 +2,7 @@
 // registry (host array -> device mirror), workspace and error reporting.
 #include "vk_runtime.h"
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -11,6 +12,9 @@
 #include <string>
 #include <utility>
 #include <vector>
+
+#include <sys/stat.h>
+#include <sys/types.h>
 
 namespace vk {
 
@@ -557,9 +561,9 @@ int vk_outfield(const double* x, const double* y, const double* z, const double*
 }
 
 // READFIELD (nse_dominik.f:2769-2793): ./nseoutput2/field.u, one record of
-// 9 values per halo point in k, j, i order.  A leading 8-byte record (the
-// tid that OUTFIELD writes first, which the reference's read does not
-// expect) is skipped, so that a file OUTFIELD wrote reads back.
+// 9 values per halo point in k, j, i order, as OUTPUT writes it.  A leading
+// 8-byte record (the tid that OUTFIELD writes first, which the reference's
+// read does not expect) is skipped, so OUTFIELD's field.u reads back too.
 int vk_readfield(double* u, double* v, double* w, double* p, const int* nx, const int* ny,
                  const int* nz, double* t, double* phi, double* u_ta, double* v_ta,
                  double* w_ta) {
@@ -599,6 +603,120 @@ int vk_readfield(double* u, double* v, double* w, double* p, const int* nx, cons
   }
   for (double* a : f9)
     if (int rc = host_sync(a, false)) return rc;
+  return VK_OK;
+}
+
+
+// OUTPUT (nse_dominik.f:2606-2723): creates nseoutput2/ when missing, then
+// writes, one record per interior point in k, j, i order, velocity.dat
+// (the face averages ua, va, wa, each set to 0 below 1e-30 in magnitude),
+// pressure, temperature, divergence, phi and cellvalue1..4; gridx/y/z.dat
+// and gridnumber.dat (nx, ny, nz as 4-byte integers); field.u and grid.u as
+// OUTFIELD writes them but without the tid record (the file READFIELD
+// reads); and temperature_check.dat, y(j) and T(nx, j, nz/2) for j =
+// 0..ny+1 in list-directed text (the values to 17 significant digits).
+int vk_output(const double* u, const double* v, const double* w, const double* p,
+              const double* x, const double* y, const double* z, const int* nx, const int* ny,
+              const int* nz, const double* t, const double* cellvalue, const double* u_ta,
+              const double* v_ta, const double* w_ta, const double* phi, const double* divu) {
+  if (*nx < 1 || *ny < 1 || *nz < 1) return fail(VK_ERR_BAD_SIZE, "output: nx, ny, nz >= 1");
+  const double* fl[11] = {u, v, w, p, t, cellvalue, u_ta, v_ta, w_ta, phi, divu};
+  for (const double* a : fl) {
+    if (!a) return fail(VK_ERR_ARGUMENT, "output: an array is null");
+    if (int rc = host_sync(a, true)) return rc;
+  }
+  struct stat sb;
+  if (stat("nseoutput2", &sb) != 0) {
+    std::printf("Create nseoutput2 directory\n");
+    std::fflush(stdout);
+    if (mkdir("nseoutput2", 0777) != 0)
+      return fail(VK_ERR_ARGUMENT, "output: cannot create nseoutput2");
+  }
+  static const char* names[12] = {"velocity", "temperature", "pressure", "divergence",
+                                  "phi", "cellvalue1", "cellvalue2", "cellvalue3",
+                                  "cellvalue4", "gridx", "gridy", "gridz"};
+  std::FILE* f[12];
+  for (int q = 0; q < 12; q++) {
+    const std::string path = std::string("./nseoutput2/") + names[q] + ".dat";
+    f[q] = std::fopen(path.c_str(), "wb");
+    if (!f[q]) {
+      for (int r = 0; r < q; r++) std::fclose(f[r]);
+      return fail(VK_ERR_ARGUMENT, "output: cannot open a file in nseoutput2");
+    }
+  }
+  std::vector<RecordWriter> wr;
+  wr.reserve(12);
+  for (int q = 0; q < 12; q++) wr.emplace_back(f[q]);
+  const size_t sy = (size_t)*nx + 2, sz = sy * ((size_t)*ny + 2);
+  const size_t ni = (size_t)*nx * *ny * *nz;
+  auto at = [&](int i, int j, int k) { return (size_t)i + sy * (size_t)j + sz * (size_t)k; };
+  for (int k = 1; k <= *nz; k++)
+    for (int j = 1; j <= *ny; j++)
+      for (int i = 1; i <= *nx; i++) {
+        const size_t o = at(i, j, k);
+        double ua = (u[at(i - 1, j, k)] + u[o]) * 0.50;
+        double va = (v[at(i, j - 1, k)] + v[o]) * 0.50;
+        double wa = (w[at(i, j, k - 1)] + w[o]) * 0.50;
+        if (std::fabs(ua) < 1.0e-30) ua = 0.0;
+        if (std::fabs(va) < 1.0e-30) va = 0.0;
+        if (std::fabs(wa) < 1.0e-30) wa = 0.0;
+        const double vel[3] = {ua, va, wa};
+        wr[0].record(vel, 3);
+        wr[1].record(t + o, 1);
+        wr[2].record(p + o, 1);
+        wr[3].record(divu + o, 1);
+        wr[4].record(phi + o, 1);
+        const size_t c = (size_t)(i - 1) + (size_t)*nx * ((size_t)(j - 1) + (size_t)*ny * (k - 1));
+        for (int n = 0; n < 4; n++) wr[5 + n].record(cellvalue + c + ni * n, 1);
+      }
+  for (int i = 1; i <= *nx; i++) wr[9].record(x + i, 1);
+  for (int j = 1; j <= *ny; j++) wr[10].record(y + j, 1);
+  for (int k = 1; k <= *nz; k++) wr[11].record(z + k, 1);
+  bool ok = true;
+  for (int q = 0; q < 12; q++) {
+    wr[q].flush();
+    ok = ok && wr[q].ok;
+    ok = (std::fclose(f[q]) == 0) && ok;
+  }
+  {
+    std::FILE* gn = std::fopen("./nseoutput2/gridnumber.dat", "wb");
+    if (!gn) return fail(VK_ERR_ARGUMENT, "output: cannot open gridnumber.dat");
+    const int32_t len = 12, dims[3] = {*nx, *ny, *nz};
+    ok = std::fwrite(&len, 4, 1, gn) == 1 && ok;
+    ok = std::fwrite(dims, 4, 3, gn) == 3 && ok;
+    ok = std::fwrite(&len, 4, 1, gn) == 1 && ok;
+    ok = (std::fclose(gn) == 0) && ok;
+  }
+  {
+    std::FILE* tc = std::fopen("temperature_check.dat", "w");
+    if (!tc) return fail(VK_ERR_ARGUMENT, "output: cannot open temperature_check.dat");
+    const int i = *nx, k = *nz / 2;
+    for (int j = 0; j <= *ny + 1; j++)
+      ok = std::fprintf(tc, " %.17g %.17g\n", y[j], t[at(i, j, k)]) > 0 && ok;
+    ok = (std::fclose(tc) == 0) && ok;
+  }
+  if (!ok) return fail(VK_ERR_ARGUMENT, "output: writing failed");
+  // the restart pair: field.u without the tid record, and grid.u
+  std::FILE* ff = std::fopen("./nseoutput2/field.u", "wb");
+  if (!ff) return fail(VK_ERR_ARGUMENT, "output: cannot open nseoutput2/field.u");
+  RecordWriter fw(ff);
+  const double* f9[9] = {u, v, w, p, t, phi, u_ta, v_ta, w_ta};
+  const size_t n = sz * ((size_t)*nz + 2);
+  double rec[9];
+  for (size_t q = 0; q < n; q++) {
+    for (int c = 0; c < 9; c++) rec[c] = f9[c][q];
+    fw.record(rec, 9);
+  }
+  fw.flush();
+  if (!fw.ok || std::fclose(ff) != 0) return fail(VK_ERR_ARGUMENT, "output: writing field.u failed");
+  std::FILE* gf = std::fopen("./nseoutput2/grid.u", "wb");
+  if (!gf) return fail(VK_ERR_ARGUMENT, "output: cannot open nseoutput2/grid.u");
+  RecordWriter gw(gf);
+  for (int i = 1; i <= *nx; i++) gw.record(x + i, 1);
+  for (int j = 1; j <= *ny; j++) gw.record(y + j, 1);
+  for (int k = 1; k <= *nz; k++) gw.record(z + k, 1);
+  gw.flush();
+  if (!gw.ok || std::fclose(gf) != 0) return fail(VK_ERR_ARGUMENT, "output: writing grid.u failed");
   return VK_OK;
 }
 

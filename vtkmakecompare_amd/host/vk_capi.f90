@@ -231,6 +231,13 @@ module vk_capi
       real(c_double) :: u(*), v(*), w(*), p(*), t(*), phi(*), u_ta(*), v_ta(*), w_ta(*)
       integer(c_int) :: nx, ny, nz
     end function
+    integer(c_int) function vk_output(u, v, w, p, x, y, z, nx, ny, nz, t, cellvalue, u_ta, &
+        v_ta, w_ta, phi, divu) bind(C, name="vk_output")
+      import :: c_int, c_double
+      real(c_double) :: u(*), v(*), w(*), p(*), x(*), y(*), z(*), t(*), cellvalue(*)
+      real(c_double) :: u_ta(*), v_ta(*), w_ta(*), phi(*), divu(*)
+      integer(c_int) :: nx, ny, nz
+    end function
   end interface
 
 contains

@@ -256,3 +256,14 @@ subroutine readfield(u, v, w, p, nx, ny, nz, t, phi, u_ta, v_ta, w_ta)
   double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: u, v, w, p, t, phi, u_ta, v_ta, w_ta
   call vk_check(vk_readfield(u, v, w, p, nx, ny, nz, t, phi, u_ta, v_ta, w_ta), 'READFIELD')
 end subroutine
+
+! OUTPUT (nse_dominik.f:2606-2723): the nseoutput2/ files
+subroutine output(u, v, w, p, x, y, z, nx, ny, nz, t, cellvalue, u_ta, v_ta, w_ta, phi, divu)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: x(0:nx+1), y(0:ny+1), z(0:nz+1), cellvalue(1:nx,1:ny,1:nz,1:4)
+  double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: u, v, w, p, t, u_ta, v_ta, w_ta, phi, divu
+  call vk_check(vk_output(u, v, w, p, x, y, z, nx, ny, nz, t, cellvalue, u_ta, v_ta, w_ta, &
+                          phi, divu), 'OUTPUT')
+end subroutine
