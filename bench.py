@@ -100,6 +100,74 @@ def kernel_bytes(name: str, g, with_ep2: bool = False, mixed: bool = False) -> f
     return table.get(name, 0.0)
 
 
+# Algorithmic bytes per interior point, each kernel's share of its phase:
+# every array counted once per global-synchronisation phase that must read
+# or write it (perfect fusion inside a phase, halos ignored, cellvalue(..,1)
+# read once per phase that needs it).  FP64 form: 8 B per value.  Mixed
+# form: u, v, w in FP64, every other array (history, u_ta, cellvalue, the
+# ADI right-hand sides, u_temp, the spectra, the Poisson solutions) in FP32.
+#   A  explicit: u,v,w,T, 4 history, 3 u_ta, cv1 in; u*,v*,w*,T*, 4 history
+#      out (20 values).  Velocity kernel: u,v,w, 3 history, 3 u_ta, cv1 in,
+#      6 out (16 values, FP64 128 B, mixed 3*8 + 13*4 = 76 B); temperature
+#      kernel: T, T history in, T*, T history out (4 values; u, v, w and cv1
+#      are counted with the velocity kernel).
+#   B, C, D  the y, z, x ADI sweeps: 4 right-hand sides and cv1 in, 4 out
+#      per phase; each component's sweep 2 values plus a quarter of cv1.
+#   E+F  divergence and x real FFT: u_temp, v_temp, w_temp in, the half
+#      spectrum (about one field) out (4 values).
+#   G+H  z FFT, spectral Thomas forward and backward, inverse z FFT: 5
+#      values, split evenly over the three kernels that do them here.
+#   I1  inverse x FFT: 2 values.
+#   R   (mixed) the FP64 residual, fused with the second x FFT: u_temp x3
+#      and phi1 in, the residual spectrum out (5 + 2 FP32 values), then the
+#      second G+H (5) and I1 (2).
+#   I2  gradient, projection, u_ta, divu: phi, u_temp x3 in; u, v, w,
+#      3 u_ta, divu out (11 values; mixed 72 B).
+# Totals: FP64 552 B per point per step, mixed 372 B.
+PHASE_BYTES_PER_POINT = {
+    False: {"explicit_vel": 128.0, "explicit_t": 32.0,
+            "line_y_face": 18.0, "line_y_v": 18.0, "line_z_face": 18.0, "line_x_face": 18.0,
+            "line_y_t": 18.0, "line_x_t": 18.0, "line_z_t": 18.0,
+            "rfft_x_div": 32.0, "cfft_z_fwd": 40.0 / 3, "line_spectral": 40.0 / 3,
+            "cfft_z_inv": 40.0 / 3, "irfft_x": 16.0, "project": 88.0},
+    True: {"explicit_mixed": 76.0, "explicit_mixed_t": 16.0,
+           "line_y_face": 9.0, "line_y_v": 9.0, "line_z_face": 9.0, "line_x_face": 9.0,
+           "line_y_t": 9.0, "line_x_t": 9.0, "line_z_t": 9.0,
+           "rfft_x_div": 16.0, "cfft_z_fwd": 20.0 / 3, "line_spectral": 20.0 / 3,
+           "cfft_z_inv": 20.0 / 3, "irfft_x": 8.0, "rfft_x_resid": 28.0,
+           "cfft_z_fwd2": 20.0 / 3, "cfft_z_inv2": 20.0 / 3, "irfft_x2": 8.0,
+           "project_mixed": 72.0},
+}
+STEP_PHASE_BYTES_PER_POINT = {False: 552.0, True: 372.0}
+
+
+def phase_bytes(name: str, g, mixed: bool = False) -> float:
+    """Algorithmic bytes of one launch of a kernel (its phase share, above)."""
+    return PHASE_BYTES_PER_POINT[mixed].get(name, 0.0) * g.nx * g.ny * g.nz
+
+
+# Algorithmic FP64 operations per unknown of the line solves, counted on
+# the reference's algorithms: a tridiagonal line (TDMA, or the dense
+# elimination of implicit_uy/vy/Ty restricted to its three bands) takes
+# 10 -- 4 to build the row from the metrics and beta/alpha, 4 for the
+# forward elimination (one of them a division), 2 for the back
+# substitution; a periodic line (cyclic: two TDMA solves and the
+# Sherman-Morrison correction) 24; the spectral systems (real matrix,
+# complex right-hand side, CTRIDIAG) 14.
+LINE_FLOPS_PER_UNKNOWN = {"line_y_face": 10, "line_y_v": 10, "line_y_t": 10,
+                          "line_z_face": 24, "line_x_face": 24, "line_x_t": 24,
+                          "line_z_t": 24, "line_spectral": 14}
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+
+
+def line_unknowns(name: str, g) -> int:
+    """Unknowns one launch of a line sweep solves (all components' lines of
+    that sweep for one field; spectral: every (kx, kz) mode's y line)."""
+    if name == "line_spectral":
+        return (g.nx // 2 + 1) * g.nz * g.ny
+    return g.nx * g.ny * g.nz
+
+
 STEP_LAUNCHES = {"explicit_vel": 1, "explicit_t": 1, "explicit_mixed": 1, "explicit_mixed_t": 1, "line_y_face": 2, "line_y_v": 1,
                  "line_z_face": 3, "line_x_face": 3, "line_y_t": 1, "line_x_t": 1,
                  "line_z_t": 1, "rfft_x_div": 1, "rfft_x_resid": 1, "cfft_z_fwd": 1,
@@ -129,23 +197,79 @@ def profile_entries(lib):
     return out
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(cfg, g, st, cellvalue, steps: int):
-    """The oracle (C restatement, OpenMP) on the same workload for `steps`
-    steps from the same initial state."""
+    """The oracle (the reference's loop body restated in C, OpenMP) on the
+    same workload for `steps` steps from the same initial state, on every
+    host core this process may use (its affinity mask, capped by
+    OMP_NUM_THREADS where that is set: the CPU share of the job), with
+    OMP_PROC_BIND=close.  Returns the baseline record and the oracle run
+    (its state after `steps` steps)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_bind import OracleRun
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    avail = len(os.sched_getaffinity(0))
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(avail, env) if env > 0 else avail
     os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    from oracle_bind import OracleRun
     st_o = dict(st)
     st_o["cellvalue"] = cellvalue
     run = OracleRun(cfg, g, st_o)
     t0 = time.perf_counter()
     run.step(steps)
     dt = time.perf_counter() - t0
-    return {"value": g.npoints * steps / dt / 1e6, "unit": "M grid-pt-steps/s",
-            "cores": threads, "kind": "port",
-            "sample": f"{steps} full time steps of {cfg.name} "
-                      f"({g.nx}x{g.ny}x{g.nz}), oracle C/OpenMP, {dt:.1f} s"}
+    rec = {"value": g.npoints * steps / dt / 1e6, "unit": "M grid-pt-steps/s",
+           "cores": threads, "kind": "port",
+           "sample": f"{steps} full time steps of {cfg.name} ({g.nx}x{g.ny}x{g.nz}) from its "
+                     f"initial state, oracle C/OpenMP (O(n) y solves), {dt:.1f} s",
+           "nproc": os.cpu_count(), "affinity_cpus": avail, "cpu_model": _cpu_model(),
+           "omp_num_threads": threads, "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
+    return rec, run
+
+
+def copy_bandwidth(lib, reps: int = 5, shape=(1024, 510, 1022)):
+    """The library's interior copy (vk_field_copy with interior_only) of a
+    field far larger than the L2 and the 256 MB MALL, timed with HIP
+    events on the library stream: the practical HBM ceiling for a kernel
+    with this layout (reads and writes nx*ny*nz doubles per launch)."""
+    from vtkmakecompare_amd import _lib, api
+    from vtkmakecompare_amd.device import DeviceArray
+    nx, ny, nz = shape
+    cells = (nx + 2) * (ny + 2) * (nz + 2)
+    a = DeviceArray((cells,))
+    b = DeviceArray((cells,))
+    a.fill_zero()
+    b.fill_zero()
+    api.synchronize()
+    for r in range(reps + 1):
+        if r == 1:  # the first launch loads the code object
+            api.synchronize()
+            _lib.check(lib.vk_profile_begin(), "profile")
+        src, dst = (a, b) if r % 2 == 0 else (b, a)
+        _lib.check(lib.vk_field_copy(src, dst, _lib.I(nx), _lib.I(ny), _lib.I(nz), _lib.I(1)),
+                   "field_copy")
+    api.synchronize()
+    _lib.check(lib.vk_profile_end(), "profile")
+    prof = profile_entries(lib)
+    a.free()
+    b.free()
+    if "field_copy" not in prof:
+        return None
+    ms, cnt = prof["field_copy"]
+    moved = 2 * 8 * nx * ny * nz
+    return {"GBs": round(moved / (ms / cnt / 1e3) / 1e9, 1), "bytes_per_launch": moved,
+            "avg_us": round(1e3 * ms / cnt, 2), "launches": cnt,
+            "what": f"interior copy of a {nx}x{ny}x{nz} FP64 field"}
 
 
 PROFILED_STEPS = 10
@@ -209,28 +333,66 @@ def diagnostics_timing(lib, run, cfg, g, reps: int = 10):
 
 
 def kernel_table(prof, g, with_ep2, mixed):
-    return {k: {"total_ms": round(v[0], 3), "launches": v[1],
-                "avg_us": round(1e3 * v[0] / v[1], 2),
-                "alg_GBs": round(kernel_bytes(k, g, with_ep2, mixed) / (v[0] / v[1] / 1e3) / 1e9, 1)
-                if kernel_bytes(k, g, with_ep2, mixed) else None}
-            for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+    """Per kernel: time, launches, and two rates -- the algorithmic bytes
+    (its phase share) and the bytes the kernel itself reads and writes, each
+    over its average launch."""
+    out = {}
+    for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0]):
+        avg_s = v[0] / v[1] / 1e3
+        pb, kb = phase_bytes(k, g, mixed), kernel_bytes(k, g, with_ep2, mixed)
+        out[k] = {"total_ms": round(v[0], 3), "launches": v[1], "avg_us": round(avg_s * 1e6, 2),
+                  "alg_GBs": round(pb / avg_s / 1e9, 1) if pb else None,
+                  "kernel_GBs": round(kb / avg_s / 1e9, 1) if kb else None}
+    return out
+
+
+def line_solves(prof, g):
+    """Algorithmic FP64 rate of each line sweep (flops per unknown above,
+    times the unknowns of one launch, over its average launch)."""
+    out = {}
+    for k, per in LINE_FLOPS_PER_UNKNOWN.items():
+        if k not in prof:
+            continue
+        ms, cnt = prof[k]
+        avg_s = ms / cnt / 1e3
+        fl = per * line_unknowns(k, g)
+        out[k] = {"avg_us": round(avg_s * 1e6, 2), "flops_per_launch": fl,
+                  "GFLOPs": round(fl / avg_s / 1e9, 1),
+                  "frac_fp64_peak": round(fl / avg_s / 1e12 / FP64_VECTOR_PEAK_TFLOPS, 5)}
+    return out
 
 
 def roofline(prof, g, with_ep2, mixed, traffic_file):
-    """The dominant kernel (by total time): algorithmic bytes per launch
-    over its average launch duration, against the 8 TB/s HBM peak."""
+    """The dominant kernel (by total time): its algorithmic bytes per launch
+    (phase share) over its average launch duration, against the 8 TB/s HBM
+    peak; beside it the rate on the bytes the kernel itself moves and its
+    counted traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)."""
     from vtkmakecompare_amd.profsummary import read_traffic
     dname, (dms, dcount) = max(prof.items(), key=lambda kv: kv[1][0])
     avg_s = dms / dcount / 1e3
-    alg = kernel_bytes(dname, g, with_ep2, mixed)
+    alg = phase_bytes(dname, g, mixed)
+    kb = kernel_bytes(dname, g, with_ep2, mixed)
     traffic = None
     if dname in KERNEL_SYMBOL and not with_ep2:
         traffic = read_traffic(traffic_file, KERNEL_SYMBOL[dname])
     achieved = alg / avg_s / 1e9 if alg else 0.0
     return {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "alg_bytes_per_launch": alg, "avg_launch_us": round(avg_s * 1e6, 2),
-            "traffic": traffic}
+            "traffic": traffic, "alg_bytes_per_launch": alg,
+            "avg_launch_us": round(avg_s * 1e6, 2), "kernel_bytes_per_launch": kb,
+            "kernel_GBs": round(kb / avg_s / 1e9, 1) if kb else None}
+
+
+def field_errors(run, ref, names):
+    """max|run - ref| / max|ref| over the interior, per field."""
+    import numpy as np
+    out = {}
+    for k in names:
+        x = run.get(k)[1:-1, 1:-1, 1:-1]
+        y = ref[k][1:-1, 1:-1, 1:-1]
+        out[k] = float(np.max(np.abs(x - y)) / max(float(np.max(np.abs(y))), 1e-300))
+        del x, y
+    return out
 
 
 def main():
@@ -240,7 +402,7 @@ def main():
                     help="timed steps (0: only the profiled steps, for kernel traces)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="step384")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=("both", "fp64", "mixed"), default="both",
                     help="run one form only (for profiling)")
@@ -300,50 +462,78 @@ def main():
             print(f"[bench] mixed: {1e3 * el / args.steps:.3f} ms/step", file=sys.stderr,
                   flush=True)
 
+    copy = copy_bandwidth(lib)
     forms = {}
     for name, (el, prof, run) in res.items():
         mixed = name == "mixed"
         with_ep2 = args.diagnostics and not mixed
         ms = 1e3 * el / args.steps if el else None
-        sb = step_bytes(g, with_ep2, mixed)
+        sb = STEP_PHASE_BYTES_PER_POINT[mixed] * npts
+        kb = step_bytes(g, with_ep2, mixed)
         tfile = os.path.join(ROOT, "profiles", f"{cfg.name}_traffic.txt")
         forms[name] = {
             "value": round(npts * args.steps / el / 1e6, 2) if el else None,
             "ms_per_step": round(ms, 4) if ms else None,
             "roofline": roofline(prof, g, with_ep2, mixed, tfile),
-            "step_roofline": {"alg_bytes_per_step": sb,
+            "step_roofline": {"alg_bytes_per_point": STEP_PHASE_BYTES_PER_POINT[mixed],
+                              "alg_bytes_per_step": sb,
                               "alg_GBs": round(sb / (ms / 1e3) / 1e9, 1) if ms else None,
                               "frac": round(sb / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
                               if ms else None,
+                              "kernel_bytes_per_step": kb,
+                              "kernel_GBs": round(kb / (ms / 1e3) / 1e9, 1) if ms else None,
                               # the kernels one after another (profiled steps);
                               # ms_per_step is less where streams overlap them
                               "kernel_time_ms_per_step":
                                   round(sum(v[0] for v in prof.values()) / PROFILED_STEPS, 4)},
             "kernels": kernel_table(prof, g, with_ep2, mixed),
+            "line_solves": line_solves(prof, g),
             "max_abs_divu": api.field_absmax(run.d["divu"], g.nx, g.ny, g.nz),
         }
 
     # error of the mixed form against the FP64 form after the same steps
     errors = None
     if "fp64" in res and "mixed" in res:
-        a, b = res["mixed"][2], res["fp64"][2]
-        errors = {}
-        for k in ("u", "v", "w", "p", "t"):
-            x = a.get(k)[1:-1, 1:-1, 1:-1]
-            y = b.get(k)[1:-1, 1:-1, 1:-1]
-            errors[k] = float(np.max(np.abs(x - y)) / max(float(np.max(np.abs(y))), 1e-300))
-            del x, y
+        errors = field_errors(res["mixed"][2], {k: res["fp64"][2].get(k)
+                                                for k in ("u", "v", "w", "p", "t")},
+                              ("u", "v", "w", "p", "t"))
         errors["velocity"] = max(errors["u"], errors["v"], errors["w"])
         errors["steps_compared"] = args.warmup + args.steps + PROFILED_STEPS
+    for _, _, run in res.values():
+        run.free()
 
     cpu = None
+    vs_oracle = None
     if not args.no_cpu_baseline:
         print(f"[bench] CPU baseline {args.cpu_steps} steps", file=sys.stderr, flush=True)
-        st2 = G.initial_state(cfg, g)
-        cpu = cpu_baseline(cfg, g, st2, cv, args.cpu_steps)
+        cpu, ora = cpu_baseline(cfg, g, st, cv, args.cpu_steps)
+        # both GPU forms over the same steps from the same state, against
+        # the oracle's state after its sample
+        vs_oracle = {"steps": args.cpu_steps}
+        for name in res:
+            run = api.ChannelRun(cfg, g, st, cellvalue=cv, precision=name)
+            run.step(args.cpu_steps)
+            e = field_errors(run, ora, ("u", "v", "w", "p", "t"))
+            e["velocity"] = max(e["u"], e["v"], e["w"])
+            e["max_abs_divu"] = api.field_absmax(run.d["divu"], g.nx, g.ny, g.nz)
+            vs_oracle[name] = e
+            run.free()
+        del ora
 
     head = forms.get("mixed") or forms["fp64"]
     is_mixed = "mixed" in forms
+    within = None
+    if errors is not None:
+        within = {"velocity_le_1e-6": errors["velocity"] <= 1e-6,
+                  "mixed_divu_le_1e-10": forms["mixed"]["max_abs_divu"] <= 1e-10}
+        if vs_oracle and "mixed" in vs_oracle:
+            within["mixed_velocity_vs_oracle_le_1e-6"] = vs_oracle["mixed"]["velocity"] <= 1e-6
+        if vs_oracle and "fp64" in vs_oracle:
+            within["fp64_vs_oracle_le_1e-12"] = max(
+                vs_oracle["fp64"][k] for k in ("u", "v", "w", "p", "t")) <= 1e-12
+        if not all(within.values()):
+            print(f"[bench] WARNING: a precision target failed: {within}", file=sys.stderr,
+                  flush=True)
     out = {
         "metric": "M grid-pt-steps/s @384x193x384: mixed vs FP64 at <=1e-6 rel err; HBM GB/s",
         "value": head["value"],
@@ -361,10 +551,15 @@ def main():
                                f"Re={cfg.re:g}, 8 spheres, tanh-stretched y; value is the "
                                f"{'mixed-precision' if is_mixed else 'FP64'} step",
                    "precision": "mixed" if is_mixed else "fp64", "grid": [g.nx, g.ny, g.nz],
-                   "mkp_diagnostics": args.diagnostics},
+                   "mkp_diagnostics": args.diagnostics,
+                   "mkp_u_ep2": "computed" if args.diagnostics else
+                                "skipped in both forms (MKP's splitting-error diagnostic)"},
         "roofline": head["roofline"],
         "mixed_vs_fp64": None,
         "errors_mixed_vs_fp64": errors,
+        "errors_vs_oracle": vs_oracle,
+        "within_target": within,
+        "copy_bandwidth": copy,
         "forms": forms,
         "diagnostics": diag,
         "levelset": levelset,

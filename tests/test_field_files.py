@@ -125,3 +125,30 @@ def test_output_matches_reference(tmp_path):
         back = A.readfield(g, {k: np.zeros(g.shape) for k in NAMES})
     for k in NAMES:
         np.testing.assert_array_equal(back[k], f[k], err_msg=k)
+
+
+def test_field_file_wrappers_refuse_wrong_shapes(tmp_path):
+    """The wrappers check every array's shape before the library touches
+    it: the library moves (nx+2)(ny+2)(nz+2) doubles per field and
+    4 nx ny nz of cellvalue, so a short array would be over-read or
+    overwritten.  Nothing is written on a refusal."""
+    g = G.channel_grid(6, 5, 4)
+    good = state(g, 3)
+    cv = np.zeros((4, g.nz, g.ny, g.nx))
+    with chdir(tmp_path):
+        for bad_name in ("u", "w_ta"):
+            bad = dict(good)
+            bad[bad_name] = np.zeros((g.nz + 2, g.ny + 2, g.nx + 1))
+            with pytest.raises(ValueError, match=bad_name):
+                A.outfield(g, 1.0, bad)
+            with pytest.raises(ValueError, match=bad_name):
+                A.readfield(g, bad)
+        with pytest.raises(ValueError, match="cellvalue"):
+            A.output(g, dict(good, divu=np.zeros(g.shape)), cv[:3])
+        with pytest.raises(ValueError, match="float64"):
+            A.output(g, dict(good, divu=np.zeros(g.shape, dtype=np.float32)), cv)
+        short = G.channel_grid(6, 5, 4)
+        short.y = short.y[:-1]
+        with pytest.raises(ValueError, match="y"):
+            A.outfield(short, 1.0, good)
+    assert not any(tmp_path.iterdir())

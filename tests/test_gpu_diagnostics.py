@@ -54,9 +54,9 @@ def test_courant_exact(api, n):
     m = metrics(g)
     u, v, w, _, _ = fields(g, 21)
     dt = 1e-3
-    # quotients within a few ulp of 0.7 everywhere: the device skips the
-    # divides that cannot raise its running maximum, and a skip that is too
-    # eager shows here as a last-bit difference
+    # quotients within a few ulp of 0.7 everywhere: the maximum must be the
+    # exact largest quotient, so a reduction that loses or reorders a
+    # near-tie shows here as a last-bit difference
     rng = np.random.default_rng(5)
     ulps = 1.0 + rng.integers(-6, 7, size=g.shape) * 2.0 ** -52
     un = 0.7 * (g.dx / dt)[None, None, :] * ulps

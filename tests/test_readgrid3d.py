@@ -77,5 +77,13 @@ def test_readgrid3d_refusals(tmp_path):
         (tmp_path / "structure.dat").write_text("1 2 3 2*4\n" * 24)
         with pytest.raises(A.VKError):  # repeat count
             A.readgrid3d(g.nx, g.ny, g.nz, np.zeros(g.shape))
+        # a leading comma is a null value for a (list-directed input would
+        # then give phi0 = 3 from ',1,2,3,4'), refused like any other null
+        (tmp_path / "structure.dat").write_text("1 2 3 4\n" * 23 + ",1,2,3,4\n")
+        with pytest.raises(A.VKError):
+            A.readgrid3d(g.nx, g.ny, g.nz, np.zeros(g.shape))
+        (tmp_path / "structure.dat").write_text("1 2 3 4\n" * 23 + "  , 1 2 3 4\n")
+        with pytest.raises(A.VKError):
+            A.readgrid3d(g.nx, g.ny, g.nz, np.zeros(g.shape))
     finally:
         os.chdir(cwd)

@@ -225,38 +225,51 @@ def readgrid3d(nx, ny, nz, phi0: np.ndarray) -> np.ndarray:
     return phi0
 
 
-def _host(a):
+def _host(a, shape, name):
+    """Pointer to a host array the field-file routines read or write in
+    full: the library moves shape's product of doubles, so a wrongly sized
+    array would be over-read or overwritten."""
     import ctypes
-    if a.dtype != np.float64 or not a.flags.c_contiguous:
-        raise ValueError("field files take C-contiguous float64 host arrays")
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise ValueError(f"{name}: field files take C-contiguous float64 host arrays")
+    if a.shape != tuple(shape):
+        raise ValueError(f"{name}: shape {a.shape}, need {tuple(shape)}")
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _coords(g: G.Grid):
+    return (_host(g.x, (g.nx + 2,), "x"), _host(g.y, (g.ny + 2,), "y"),
+            _host(g.z, (g.nz + 2,), "z"))
 
 
 def outfield(g: G.Grid, tid: float, fields: dict) -> None:
     """OUTFIELD (nse_dominik.f:2726): field.u and ./nseoutput2/grid.u in the
-    working directory from host arrays u, v, w, p, t, phi, u_ta, v_ta, w_ta."""
-    f = [_host(fields[k]) for k in ("u", "v", "w", "p")]
-    r = [_host(fields[k]) for k in ("t", "phi", "u_ta", "v_ta", "w_ta")]
-    check(lib().vk_outfield(_host(g.x), _host(g.y), _host(g.z), *f, I(g.nx), I(g.ny), I(g.nz),
-                            D(tid), *r), "outfield")
+    working directory from host arrays u, v, w, p, t, phi, u_ta, v_ta, w_ta
+    (halo shape (nz+2, ny+2, nx+2))."""
+    f = [_host(fields[k], g.shape, k) for k in ("u", "v", "w", "p")]
+    r = [_host(fields[k], g.shape, k) for k in ("t", "phi", "u_ta", "v_ta", "w_ta")]
+    check(lib().vk_outfield(*_coords(g), *f, I(g.nx), I(g.ny), I(g.nz), D(tid), *r),
+          "outfield")
 
 
 def output(g: G.Grid, fields: dict, cellvalue: np.ndarray) -> None:
     """OUTPUT (nse_dominik.f:2606): the nseoutput2/ files and
     temperature_check.dat in the working directory, from host arrays u, v,
-    w, p, t, u_ta, v_ta, w_ta, phi, divu and cellvalue (4, nz, ny, nx)."""
-    f = {k: _host(fields[k]) for k in ("u", "v", "w", "p", "t", "u_ta", "v_ta", "w_ta", "phi",
-                                       "divu")}
-    check(lib().vk_output(f["u"], f["v"], f["w"], f["p"], _host(g.x), _host(g.y), _host(g.z),
-                          I(g.nx), I(g.ny), I(g.nz), f["t"], _host(cellvalue), f["u_ta"],
-                          f["v_ta"], f["w_ta"], f["phi"], f["divu"]), "output")
+    w, p, t, u_ta, v_ta, w_ta, phi, divu (halo shape) and cellvalue
+    (4, nz, ny, nx)."""
+    f = {k: _host(fields[k], g.shape, k) for k in ("u", "v", "w", "p", "t", "u_ta", "v_ta",
+                                                   "w_ta", "phi", "divu")}
+    cv = _host(cellvalue, (4, g.nz, g.ny, g.nx), "cellvalue")
+    check(lib().vk_output(f["u"], f["v"], f["w"], f["p"], *_coords(g), I(g.nx), I(g.ny),
+                          I(g.nz), f["t"], cv, f["u_ta"], f["v_ta"], f["w_ta"], f["phi"],
+                          f["divu"]), "output")
 
 
 def readfield(g: G.Grid, fields: dict) -> dict:
     """READFIELD (nse_dominik.f:2769): ./nseoutput2/field.u into the host
     arrays u, v, w, p, t, phi, u_ta, v_ta, w_ta (halo shape), in place."""
-    f = [_host(fields[k]) for k in ("u", "v", "w", "p")]
-    r = [_host(fields[k]) for k in ("t", "phi", "u_ta", "v_ta", "w_ta")]
+    f = [_host(fields[k], g.shape, k) for k in ("u", "v", "w", "p")]
+    r = [_host(fields[k], g.shape, k) for k in ("t", "phi", "u_ta", "v_ta", "w_ta")]
     check(lib().vk_readfield(*f, I(g.nx), I(g.ny), I(g.nz), *r), "readfield")
     return fields
 
@@ -408,3 +421,10 @@ class ChannelRun:
     def get(self, name: str) -> np.ndarray:
         """A field downloaded to the host, as float64 whatever its storage."""
         return self.d[name].download().astype(np.float64, copy=False)
+
+    def free(self) -> None:
+        """Releases every device array of the run now (rather than when
+        the Python objects are collected)."""
+        for a in self.d.values():
+            a.free()
+        self.d = {}

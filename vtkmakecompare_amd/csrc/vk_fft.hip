@@ -50,6 +50,16 @@ std::map<int, const typename Cx<R>::T*>& tables() {
   static std::map<int, const typename Cx<R>::T*> t;
   return t;
 }
+// vk_finalize hook: the twiddle tables are device memory; the plans point
+// into them
+void release_fft_tables() {
+  for (auto& kv : tables<double>()) (void)hipFree(const_cast<double2*>(kv.second));
+  for (auto& kv : tables<float>()) (void)hipFree(const_cast<float2*>(kv.second));
+  tables<double>().clear();
+  tables<float>().clear();
+  plans<double>().clear();
+  plans<float>().clear();
+}
 }  // namespace
 
 // e^{-2 pi i m / n} for m = 0..n-1 in R, rounded once from extended
@@ -57,6 +67,7 @@ std::map<int, const typename Cx<R>::T*>& tables() {
 template <typename R>
 const typename Cx<R>::T* twiddle_table(int n) {
   using C = typename Cx<R>::T;
+  on_finalize(release_fft_tables);
   auto it = tables<R>().find(n);
   if (it != tables<R>().end()) return it->second;
   std::vector<C> h(n);

@@ -126,6 +126,17 @@ bool device_extent(const void* p, size_t* bytes) {
   return true;
 }
 
+std::vector<void (*)()>& finalize_hooks() {
+  static std::vector<void (*)()> v;
+  return v;
+}
+
+void on_finalize(void (*hook)()) {
+  for (auto h : finalize_hooks())
+    if (h == hook) return;
+  finalize_hooks().push_back(hook);
+}
+
 void* workspace(WorkSlot slot, size_t bytes, bool* grown) {
   Work& w = g_work[slot];
   if (grown) *grown = false;
@@ -228,7 +239,8 @@ int vk_init(const int* device) {
 int vk_finalize(void) {
   Runtime& r = rt();
   if (!r.initialised) return VK_OK;
-  (void)hipStreamSynchronize(r.stream);
+  (void)hipDeviceSynchronize();
+  for (auto h : finalize_hooks()) h();
   for (auto& kv : registry()) (void)hipFree(kv.second.dev);
   registry().clear();
   for (auto& w : g_work)
@@ -447,7 +459,10 @@ int vk_readgrid3d(const int* nx, const int* ny, const int* nz, double* phi0) {
       for (int i = 1; i <= *nx; i++) {
         double val[4];
         int nv = 0;
-        bool sep_comma = false;
+        // a comma before the first value of an item is a null value, as
+        // is a comma after another comma (blanks and record ends between
+        // them do not count)
+        bool sep_comma = true;
         while (nv < 4) {
           while (p < end && (*p == ' ' || *p == '\t' || *p == '\r' || *p == ',')) {
             if (*p == ',') {

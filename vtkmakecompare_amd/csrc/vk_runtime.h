@@ -81,6 +81,12 @@ void* resolve(const void* p, size_t bytes, const char* what);
 // bytes from p to the end of its device allocation, when the runtime knows
 bool device_extent(const void* p, size_t* bytes);
 
+// Registers a function that vk_finalize calls, after every stream has
+// drained and before the streams are destroyed, to free device memory a
+// translation unit keeps in caches of its own.  A hook is registered once
+// per process and stays registered across finalize/init cycles.
+void on_finalize(void (*hook)());
+
 // Grow-only device workspace, one per slot.
 enum WorkSlot {
   kWorkPhi = 0,

@@ -750,6 +750,7 @@ int vk_field_copy(const double* src, double* dst, const int* nx, const int* ny, 
   RESOLVE_FIELD(ds, src);
   RESOLVE_FIELD(dd, dst);
   if (*interior_only) {
+    ProfScope ps("field_copy", rt().stream);
     k_copy_interior<<<interior_grid(g), interior_block(), 0, rt().stream>>>(g, ds, dd);
     VK_LAUNCH_CHECK("field_copy");
     return VK_OK;

@@ -14,18 +14,40 @@ namespace vk {
 
 // the fast kernel's queue: two counters, each on its own 128-byte line,
 // then the list; cleared when allocated, afterwards by the general kernel
+namespace {
+// one queue per stream (sweeps on different streams run concurrently),
+// each with two counters used by alternate queued sweeps on that stream:
+// the general kernel of one sweep clears the other's, which the next
+// sweep then fills
+struct QueueBuf {
+  unsigned* p = nullptr;
+  size_t cap = 0;
+  unsigned parity = 0;
+};
+std::map<hipStream_t, QueueBuf> g_queues;
+
+struct TableEntry {
+  double* p;
+  unsigned long long epoch;
+};
+std::map<std::string, TableEntry> g_tables;
+
+// vk_finalize hook: the queues and tables are device memory of this
+// device, and their keys (stream handles, metric addresses) mean nothing
+// after the streams are destroyed
+void release_line_caches() {
+  for (auto& kv : g_queues)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  g_queues.clear();
+  for (auto& kv : g_tables)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  g_tables.clear();
+}
+}  // namespace
+
 LineQueue line_queue(unsigned nlines, hipStream_t s) {
-  // one queue per stream (sweeps on different streams run concurrently),
-  // each with two counters used by alternate queued sweeps on that stream:
-  // the general kernel of one sweep clears the other's, which the next
-  // sweep then fills
-  struct Q {
-    unsigned* p = nullptr;
-    size_t cap = 0;
-    unsigned parity = 0;
-  };
-  static std::map<hipStream_t, Q> queues;
-  Q& q = queues[s];
+  on_finalize(release_line_caches);
+  QueueBuf& q = g_queues[s];
   if (q.cap < nlines || !q.p) {
     if (q.p) {
       (void)hipStreamSynchronize(s);
@@ -49,11 +71,8 @@ LineQueue line_queue(unsigned nlines, hipStream_t s) {
 // reference-line tables, created on first use of a key; *validated says
 // whether the table was already checked in this entry-point call
 double* line_table(const void* key, size_t keylen, long total, bool* fresh, bool* validated) {
-  struct Entry {
-    double* p;
-    unsigned long long epoch;
-  };
-  static std::map<std::string, Entry> cache;
+  on_finalize(release_line_caches);
+  auto& cache = g_tables;
   std::string k(static_cast<const char*>(key), keylen);
   auto it = cache.find(k);
   *fresh = false;
@@ -65,7 +84,7 @@ double* line_table(const void* key, size_t keylen, long total, bool* fresh, bool
   }
   double* p = nullptr;
   if (hipMalloc(&p, sizeof(double) * (size_t)total) != hipSuccess) return nullptr;
-  cache[k] = Entry{p, now};
+  cache[k] = TableEntry{p, now};
   *fresh = true;
   *validated = false;
   return p;
