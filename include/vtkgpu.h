@@ -193,6 +193,24 @@ int vk_set_levelset(const int *nx, const int *ny, const int *nz,
                     const double *x, const double *y, const double *z,
                     double *cellvalue);
 
+/* The analytic geometry front-end: phi0 of set_levelset from a union of
+   ellipsoids, on the device.  No reference counterpart: the reference
+   reads phi0 from structure.dat (READGRID3D, set_levelset.f:202-214), and
+   this writes the same array (interior; the halo is set to 0, as a
+   zeroed phi0 read by READGRID3D has it).  shapes is host memory,
+   nshapes records of 10 doubles: centre (cx, cy, cz), semi-axes (a, b, c)
+   along the body axes, and the body rotation as a quaternion (w, x, y, z)
+   (normalised here; (1,0,0,0) for axis-aligned).  At each cell centre
+   (x(i), y(j), z(k)),
+     phi0 = max over shapes of min(a,b,c) (1 - |R^T d / (a,b,c)|),
+   d the offset from the centre with the minimum image in the periodic x
+   (period xl) and z (period zl): positive inside (the solid), exactly the
+   signed distance for a sphere, and -1 everywhere with no shape.  The
+   reinitialisation of set_levelset then makes it a signed distance. */
+int vk_levelset_shapes(const int *nx, const int *ny, const int *nz, const double *x,
+                       const double *y, const double *z, const double *xl, const double *zl,
+                       const int *nshapes, const double *shapes, double *phi0);
+
 /* ------------------------------------------------------ adjoint sweeps */
 /* set_adj_convect, adjoint.f:1-126 */
 int vk_set_adj_convect(const double *u, double *u_v, const double *v,
@@ -247,7 +265,13 @@ int vk_time_step(double *u, double *v, double *w, double *p, double *t,
    double.  The explicit phase and the FFTs compute in FP32, the line
    solves in FP64 on the mixed storage; the projection adds one FP64
    iterative-refinement pass, so the post-projection divergence is at
-   FP64 round-off.  MKP's u_ep2 diagnostic is not produced. */
+   FP64 round-off.  When the divergence after that pass is still above
+   5e-11 (steps where the penalisation cuts the velocity sharply at the
+   solid's surface, such as the first steps from a non-solenoidal state),
+   one more refinement pass on the post-projection divergence follows and
+   the projection is redone; deciding this reads one word back to the
+   host, so the call returns after the projection has run.  MKP's u_ep2
+   diagnostic is not produced. */
 int vk_time_step_mixed(double *u, double *v, double *w, double *p, float *t,
                        float *u_pre, float *v_pre, float *w_pre, float *t_pre,
                        float *u_ta, float *v_ta, float *w_ta, double *u_temp,

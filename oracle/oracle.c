@@ -1627,3 +1627,46 @@ void o_check_convergence(const double *y, double yl, const double *u, double re,
     }
   *error_sum = e;
 }
+
+/* The analytic shape union (no reference routine; see oracle.h). */
+void o_levelset_shapes(int nx, int ny, int nz, const double *x, const double *y,
+                       const double *z, double xl, double zl, int nshapes,
+                       const double *shapes, double *phi0) {
+  const long sy = nx + 2, sz = (long)(nx + 2) * (ny + 2);
+  for (int k = 0; k <= nz + 1; k++)
+    for (int j = 0; j <= ny + 1; j++)
+      for (int i = 0; i <= nx + 1; i++) {
+        const long o = i + sy * j + sz * k;
+        if (i == 0 || j == 0 || k == 0 || i == nx + 1 || j == ny + 1 || k == nz + 1) {
+          phi0[o] = 0.0;
+          continue;
+        }
+        double best = -1.0;
+        for (int q = 0; q < nshapes; q++) {
+          const double *p = shapes + 10 * (long)q;
+          double w = p[6], qx = p[7], qy = p[8], qz = p[9];
+          const double nq = sqrt(w * w + qx * qx + qy * qy + qz * qz);
+          w /= nq;
+          qx /= nq;
+          qy /= nq;
+          qz /= nq;
+          const double r0 = 1.0 - 2.0 * (qy * qy + qz * qz), r1 = 2.0 * (qx * qy - qz * w),
+                       r2 = 2.0 * (qx * qz + qy * w), r3 = 2.0 * (qx * qy + qz * w),
+                       r4 = 1.0 - 2.0 * (qx * qx + qz * qz), r5 = 2.0 * (qy * qz - qx * w),
+                       r6 = 2.0 * (qx * qz - qy * w), r7 = 2.0 * (qy * qz + qx * w),
+                       r8 = 1.0 - 2.0 * (qx * qx + qy * qy);
+          double dx = x[i] - p[0], dy = y[j] - p[1], dz = z[k] - p[2];
+          if (dx > 0.5 * xl) dx -= xl;
+          if (dx < -0.5 * xl) dx += xl;
+          if (dz > 0.5 * zl) dz -= zl;
+          if (dz < -0.5 * zl) dz += zl;
+          const double b0 = (r0 * dx + r3 * dy + r6 * dz) * (1.0 / p[3]);
+          const double b1 = (r1 * dx + r4 * dy + r7 * dz) * (1.0 / p[4]);
+          const double b2 = (r2 * dx + r5 * dy + r8 * dz) * (1.0 / p[5]);
+          const double amin = fmin(p[3], fmin(p[4], p[5]));
+          const double v = amin * (1.0 - sqrt(b0 * b0 + b1 * b1 + b2 * b2));
+          best = q == 0 ? v : fmax(best, v);
+        }
+        phi0[o] = best;
+      }
+}

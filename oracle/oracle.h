@@ -14,8 +14,8 @@
  * (0:nx/2,0:ny+1,0:nz+1) stored as interleaved (re,im) doubles.
  * Scalars are passed by value.
  *
- * Defined semantics where the reference is undefined (documented in
- * DESIGN.md's parity table):
+ * Defined semantics where the reference is undefined (the library defines
+ * them the same way):
  *  - harmonic-mean conductivity ghosts of the T line solves
  *    (nse_dominik.f:1755,1760,1814,1819 read llx(0), llx(n+1), lly(0),
  *    lly(ny+1), which parallel_Tx/Ty/Tz never fill): x and z wrap
@@ -30,6 +30,19 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* No reference routine: phi0 from a union of ellipsoids, the analytic
+ * stand-in for the structure.dat that READGRID3D reads
+ * (set_levelset.f:202-214).  shapes holds nshapes records (cx, cy, cz, a,
+ * b, c, qw, qx, qy, qz); phi0 = max over shapes of
+ * min(a,b,c) (1 - |R^T d / (a,b,c)|), d the minimum-image offset in x and
+ * z, R the rotation of the normalised quaternion; -1 with no shape; halo 0.
+ * Pinned by closed forms (tests/test_oracle_math.py): a sphere gives
+ * r - |d|, surface points of a rotated ellipsoid give 0, and the sign
+ * tells inside from outside. */
+void o_levelset_shapes(int nx, int ny, int nz, const double *x, const double *y,
+                       const double *z, double xl, double zl, int nshapes,
+                       const double *shapes, double *phi0);
 
 /* nse_dominik.f:553-607 */
 void o_set_c(int nx, int ny, int nz, double *x, double *x12, double *y,

@@ -90,6 +90,7 @@ def _declare_oracle(lib):
         "o_mkp_coefficients": [ci, ci, ci] + [vp] * 4 + [cd, cd, vp, vp],
         "o_fft": [vp, ci, ci, ci],
         "o_set_levelset": [ci, ci, ci] + [vp] * 10 + [ci],
+        "o_levelset_shapes": [ci, ci, ci, vp, vp, vp, cd, cd, ci, vp, vp],
         "o_set_adj_convect": [vp] * 6 + [cd, cd, ci, ci, ci] + [vp] * 9,
         "o_set_adj_convection_for_scalar": [vp, vp, vp, cd, cd, ci, ci, ci] + [vp] * 12,
         "o_set_adj_ns_scalar": [cd, ci, ci, ci] + [vp] * 11,
@@ -162,3 +163,12 @@ def oracle_levelset(grid, phi0, iterations=2001):
                        P(grid.dy12), P(grid.dz12), P(np.ascontiguousarray(phi0)),
                        P(phi), P(grid.x), P(grid.y), P(grid.z), P(cv), iterations)
     return phi, cv
+
+
+def oracle_shapes(grid, shapes):
+    """phi0 of the analytic shape union from the oracle."""
+    sh = np.ascontiguousarray(np.asarray(shapes, dtype=np.float64).reshape(-1, 10))
+    phi0 = np.zeros(grid.shape)
+    oracle().o_levelset_shapes(grid.nx, grid.ny, grid.nz, P(grid.x), P(grid.y), P(grid.z),
+                               grid.xl, grid.zl, len(sh), P(sh), P(phi0))
+    return phi0

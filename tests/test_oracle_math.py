@@ -5,6 +5,8 @@ uninitialised local).  Each check is independent of the oracle's own
 formulas: numpy's pocketfft, a dense least-squares Poisson solve, the
 post-projection divergence, and closed forms.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -170,3 +172,73 @@ def test_adj_ns_scalar_closed_form():
     np.testing.assert_allclose(vs[1:-1, 1:-1, 1:-1],
                                np.broadcast_to(expect[None, :, None], (nz, ny, nx)),
                                rtol=1e-12)
+
+
+# ------------------------------------------------ analytic shape union
+
+def _shapes_grid():
+    return G.channel_grid(24, 17, 20)
+
+
+def test_shapes_sphere_is_signed_distance():
+    """An axis-aligned sphere gives r - |d| (the exact signed distance,
+    positive inside), with the minimum image in x and z: a centre at
+    x = 0.1 is seen across the periodic boundary at x = xl - 0.1."""
+    from oracle_bind import oracle_shapes
+    g = _shapes_grid()
+    spheres = [(0.1, 0.2, 0.5 * g.zl, 0.45), (0.6 * g.xl, -0.4, 0.05, 0.3)]
+    recs = [(cx, cy, cz, r, r, r, 1.0, 0.0, 0.0, 0.0) for cx, cy, cz, r in spheres]
+    got = oracle_shapes(g, recs)
+    want = G.sphere_levelset(g, spheres)
+    assert np.max(np.abs(got - want)) < 1e-15
+    assert np.all(got[0] == 0.0) and np.all(got[:, :, -1] == 0.0)
+
+
+def test_shapes_rotated_ellipsoids_against_scipy_rotation():
+    """Rotated ellipsoids against body coordinates formed with scipy's
+    rotation of the same quaternion (scalar-last there): catches a
+    transposed rotation, a wrong quaternion order or a wrong axis."""
+    from scipy.spatial.transform import Rotation
+
+    from oracle_bind import oracle_shapes
+    g = _shapes_grid()
+    recs = G.random_ellipsoids(0x5EED0FFF, 5, g.xl, g.zl, 0.2, 0.7)
+    got = oracle_shapes(g, recs)
+    X = g.x[1:-1][None, None, :]
+    Y = g.y[1:-1][None, :, None]
+    Z = g.z[1:-1][:, None, None]
+    best = None
+    for rec in recs:
+        c, ax, q = rec[:3], rec[3:6], rec[6:]
+        R = Rotation.from_quat([q[1], q[2], q[3], q[0]]).as_matrix()
+        dx = X - c[0]
+        dx = np.where(dx > 0.5 * g.xl, dx - g.xl, np.where(dx < -0.5 * g.xl, dx + g.xl, dx))
+        dz = Z - c[2]
+        dz = np.where(dz > 0.5 * g.zl, dz - g.zl, np.where(dz < -0.5 * g.zl, dz + g.zl, dz))
+        d = np.stack(np.broadcast_arrays(dx, Y - c[1], dz), axis=-1)
+        body = d @ R  # R^T d for each point
+        v = ax.min() * (1.0 - np.sqrt(np.sum((body / ax) ** 2, axis=-1)))
+        best = v if best is None else np.maximum(best, v)
+    assert np.max(np.abs(got[1:-1, 1:-1, 1:-1] - best)) < 1e-14
+
+
+def test_shapes_surface_and_sign():
+    """A cell centre on the surface of a rotated ellipsoid gives 0: the
+    quaternion of a quarter turn about z maps the body x axis onto world
+    y, so the point a along world y from the centre is on the surface."""
+    from oracle_bind import oracle_shapes
+    g = G.set_c(16, 16, 16, 4.0, 4.0, 4.0)
+    i, j, k = 8, 8, 8
+    a = g.y[j + 3] - g.y[j]
+    h = math.sqrt(0.5)
+    rec = (g.x[i], g.y[j], g.z[k], a, 0.5 * a, 0.4 * a, h, 0.0, 0.0, h)
+    phi = oracle_shapes(g, [rec])
+    assert abs(phi[k, j + 3, i]) < 1e-15
+    assert phi[k, j, i] == pytest.approx(0.4 * a, rel=1e-15)  # the centre: min(a,b,c)
+    assert phi[k, j + 4, i] < 0.0 < phi[k, j + 2, i]
+    # world x is the body y axis (semi-axis 0.5 a = 0.375): one cell (0.25)
+    # is inside, two (0.5) outside
+    assert phi[k, j, i + 2] < 0.0 < phi[k, j, i + 1]
+    # no shape: -1 everywhere inside, 0 on the halo
+    none = oracle_shapes(g, np.zeros((0, 10)))
+    assert np.all(none[1:-1, 1:-1, 1:-1] == -1.0) and np.all(none[0] == 0.0)

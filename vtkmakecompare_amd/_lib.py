@@ -22,7 +22,7 @@ EXPORTS = [
     "vk_set_immersed_boundary", "vk_set_explicit",
     "vk_set_explicit_for_tempreture", "vk_t_divide", "vk_solve_u_temp",
     "vk_solve_t", "vk_mkp", "vk_p2s", "vk_s2p", "vk_spn", "vk_sp0b",
-    "vk_phibc", "vk_set_levelset", "vk_set_adj_convect",
+    "vk_phibc", "vk_set_levelset", "vk_levelset_shapes", "vk_set_adj_convect",
     "vk_set_adj_convection_for_scalar", "vk_set_adj_ns_scalar",
     "vk_time_step", "vk_time_step_mixed", "vk_objective", "vk_adjoint_gradient",
     "vk_field_absmax", "vk_field_copy", "vk_check_courant", "vk_monitor",

@@ -119,6 +119,32 @@ def set_levelset(nx, ny, nz, dx, dx12, dy12, dz12, phi0, phi, x, y, z, cellvalue
                                 cellvalue), "set_levelset")
 
 
+def levelset_shapes(nx, ny, nz, x, y, z, xl, zl, shapes, phi0):
+    """phi0 (device) of the union of ellipsoid records `shapes` ((n, 10):
+    centre, semi-axes, quaternion), positive inside, halo 0
+    (vk_levelset_shapes)."""
+    import ctypes
+    sh = np.ascontiguousarray(np.asarray(shapes, dtype=np.float64).reshape(-1, 10))
+    check(lib().vk_levelset_shapes(I(nx), I(ny), I(nz), x, y, z, D(xl), D(zl), I(len(sh)),
+                                   sh.ctypes.data_as(ctypes.c_void_p), phi0),
+          "levelset_shapes")
+
+
+def device_phi0(cfg: G.Config, g: G.Grid, state: dict | None = None) -> DeviceArray:
+    """phi0 of a configuration on the device: generated there from its
+    ellipsoids, or uploaded from the host state's phi0."""
+    if cfg.ellipsoids is not None:
+        phi0 = DeviceArray(g.shape)
+        xyz = [DeviceArray.from_host(getattr(g, n)) for n in ("x", "y", "z")]
+        levelset_shapes(g.nx, g.ny, g.nz, *xyz, g.xl, g.zl, cfg.ellipsoids, phi0)
+        synchronize()
+        for a in xyz:
+            a.free()
+        return phi0
+    st = state if state is not None else G.initial_state(cfg, g)
+    return DeviceArray.from_host(st["phi0"])
+
+
 def set_adj_convect(u, u_v, v, v_v, w, w_v, re, dt, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12,
                     u_final, v_final, w_final):
     check(lib().vk_set_adj_convect(u, u_v, v, v_v, w, w_v, D(re), D(dt), I(nx), I(ny), I(nz),
@@ -279,7 +305,7 @@ def levelset_cellvalue(g: G.Grid, phi0: np.ndarray) -> np.ndarray:
     filled, slots 2-4 zero until a time step derives them)."""
     d = {n: DeviceArray.from_host(getattr(g, n)) for n in ("dx", "dx12", "dy12", "dz12",
                                                            "x", "y", "z")}
-    p0 = DeviceArray.from_host(phi0)
+    p0 = phi0 if isinstance(phi0, DeviceArray) else DeviceArray.from_host(phi0)
     phi = DeviceArray(g.shape)
     cv = DeviceArray((4, g.nz, g.ny, g.nx))
     cv.fill_zero()
@@ -371,14 +397,17 @@ class ChannelRun:
         for n in STEP_FIELDS:
             if mixed and n in ("u_ep2", "v_ep2", "w_ep2"):
                 continue
-            a = state[n]
-            if mixed and n in MIXED_F32:
-                a = np.asarray(a, dtype=np.float32)
-            self.d[n] = DeviceArray.from_host(a)
+            dtype = np.float32 if mixed and n in MIXED_F32 else np.float64
+            a = state.get(n)
+            if a is None:  # a lean state: the field starts at 0
+                self.d[n] = DeviceArray(g.shape, dtype)
+                self.d[n].fill_zero()
+            else:
+                self.d[n] = DeviceArray.from_host(np.asarray(a, dtype=dtype))
         for n in ("dx", "dy", "dz", "dx12", "dy12", "dz12", "x", "y", "z"):
             self.d[n] = DeviceArray.from_host(getattr(g, n))
         if cellvalue is None:
-            self.d["phi0"] = DeviceArray.from_host(state["phi0"])
+            self.d["phi0"] = device_phi0(cfg, g, state)
             self.d["phi"] = DeviceArray(g.shape)
             self.d["cellvalue"] = DeviceArray((4, g.nz, g.ny, g.nx))
             d = self.d

@@ -610,6 +610,13 @@ int launch_p2s_div(const Geom& g, const RMetrics& r, const double* u, const doub
 template int launch_p2s_div<double>(const Geom&, const RMetrics&, const double*, const double*,
                                     const double*, double, double*, hipStream_t);
 
+int launch_p2s_div_f32(const Geom& g, const RMetrics& r, const double* u, const double* v,
+                       const double* w, double dt, float* phit, hipStream_t s, const char* xname,
+                       const char* zname) {
+  return launch_forward<float>(g, SampleDiv{u, v, w, r.rdx, r.rdy, r.rdz, 1.0 / dt}, phit, xname,
+                               zname, s);
+}
+
 int launch_p2s_div_store(const Geom& g, const RMetrics& r, const double* u, const double* v,
                          const double* w, double dt, double* rhs, float* phit, hipStream_t s) {
   return launch_forward<float>(

@@ -110,6 +110,10 @@ int launch_p2s(const Geom& g, const double* fp, double* phit, hipStream_t s);
 template <typename R>
 int launch_p2s_div(const Geom& g, const RMetrics& r, const double* u, const double* v,
                    const double* w, double dt, R* phit, hipStream_t s);
+// the same into a float spectrum, timed under the given names
+int launch_p2s_div_f32(const Geom& g, const RMetrics& r, const double* u, const double* v,
+                       const double* w, double dt, float* phit, hipStream_t s, const char* xname,
+                       const char* zname);
 // float spectrum of div(u_temp)/dt, which is also stored in FP64 in `rhs`
 int launch_p2s_div_store(const Geom& g, const RMetrics& r, const double* u, const double* v,
                          const double* w, double dt, double* rhs, float* phit, hipStream_t s);
@@ -137,7 +141,15 @@ struct MkpOutMixed {
   double* p;
   float *u_ta, *v_ta, *w_ta;
   double *divu, *u, *v, *w;
+  // set by launch_mkp_mixed: slots of the largest |divu| of the projection
+  // (bits of non-negative doubles, atomicMax) and the correction pass's
+  // solution
+  unsigned long long* divmax = nullptr;
+  const float* psi = nullptr;
 };
+// post-projection divergence above which launch_mkp_mixed adds a further
+// correction pass (half the 1e-10 bar the mixed form is held to)
+constexpr double kMixedDivTol = 5e-11;
 int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
                      const double* v_temp, const double* w_temp, double dt, double xl, double zl,
                      const MkpOutMixed& o, hipStream_t s, hipEvent_t before_project = nullptr);

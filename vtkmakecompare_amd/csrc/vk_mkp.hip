@@ -2,6 +2,8 @@
 // div(u_temp)/dt fused into the forward FFT, spectral y solves (SPN,
 // SP0B), inverse FFT, halo fill, then gradient, velocity correction,
 // pressure, splitting-error diagnostic, u_ta and divergence in one pass.
+#include <cstring>
+
 #include "vk_kernels.h"
 #include "vk_runtime.h"
 
@@ -179,19 +181,24 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
   return VK_OK;
 }
 
-// Mixed-precision projection: phi = phi1 + delta from two float Poisson
-// solutions, summed in FP64 at every point read; the corrected velocity,
-// p and divu are FP64, u_ta is stored in float.  Same index ranges and
-// formulas as project_point (mkp.f:141-217); the u_ep2 diagnostic is not
-// produced.
+// Mixed-precision projection: phi = phi1 + delta (+ psi after a correction
+// pass) from float Poisson solutions, summed in FP64 at every point read;
+// the corrected velocity, p and divu are FP64, u_ta is stored in float.
+// Same index ranges and formulas as project_point (mkp.f:141-217); the
+// u_ep2 diagnostic is not produced.  Returns |divu| at the point (0 on the
+// index-0 faces, where divu is not defined).
 template <bool INTERIOR>
-__device__ inline void project_mixed_point(const Geom& g, const RMetrics& r, const float* phi1,
-                                           const float* dphi, const double* ut,
-                                           const double* vt, const double* wt, double dt,
-                                           const MkpOutMixed& o, int i, int j, int k) {
+__device__ inline double project_mixed_point(const Geom& g, const RMetrics& r, const float* phi1,
+                                             const float* dphi, const double* ut,
+                                             const double* vt, const double* wt, double dt,
+                                             const MkpOutMixed& o, int i, int j, int k) {
   const long long c = g.at(i, j, k);
   const long long sy = g.sy, sz = g.sz;
-  auto phi = [&](long long q) { return (double)phi1[q] + (double)dphi[q]; };
+  const float* psi = o.psi;
+  auto phi = [&](long long q) {
+    const double a = (double)phi1[q] + (double)dphi[q];
+    return psi ? a + (double)psi[q] : a;
+  };
   const double ph = phi(c);
   const double gx = (phi(c + 1) - ph) * r.rdx12[i];
   const double gy = (phi(c + sy) - ph) * r.rdy12[j];
@@ -216,6 +223,34 @@ __device__ inline void project_mixed_point(const Geom& g, const RMetrics& r, con
   o.u_ta[c] = (float)(dt * gx);
   o.v_ta[c] = (float)(dt * gy);
   o.w_ta[c] = (float)(dt * gz);
+  return fabs(div);
+}
+
+// Slots of the post-projection |divu| maximum: each block folds its
+// largest value into slot (block index mod kDivSlots), so no address takes
+// more than a few hundred atomics per projection; the host takes the
+// maximum over the slots.
+constexpr int kDivSlots = 1024;
+
+// the block's largest |divu| into its slot: non-negative doubles order as
+// their bit patterns do, so an integer maximum over the bits is the
+// floating-point maximum.  Every lane of the block must call it.
+__device__ inline void block_divmax(double a, unsigned long long* slots) {
+  __shared__ unsigned long long wmax[kBX * kBY / 64];
+  unsigned long long b = (unsigned long long)__double_as_longlong(a);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(b, off);
+    b = o > b ? o : b;
+  }
+  const int t = threadIdx.x + blockDim.x * threadIdx.y;
+  if ((t & 63) == 0) wmax[t >> 6] = b;
+  __syncthreads();
+  if (t == 0) {
+    for (int q = 1; q < kBX * kBY / 64; q++) b = wmax[q] > b ? wmax[q] : b;
+    const unsigned blk = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    atomicMax(&slots[blk % kDivSlots], b);
+  }
 }
 
 __global__ void k_project_mixed(Geom g, RMetrics r, const float* phi1, const float* dphi,
@@ -225,8 +260,16 @@ __global__ void k_project_mixed(Geom g, RMetrics r, const float* phi1, const flo
   xcd_block(bx, by, bz);
   const int i = 1 + bx * blockDim.x + threadIdx.x;
   const int j = 1 + by * blockDim.y + threadIdx.y;
-  if (i > g.nx || j > g.ny) return;
-  project_mixed_point<true>(g, r, phi1, dphi, ut, vt, wt, dt, o, i, j, 1 + bz);
+  // lanes past the interior take part in the block maximum with 0
+  const double a = (i > g.nx || j > g.ny)
+                       ? 0.0
+                       : project_mixed_point<true>(g, r, phi1, dphi, ut, vt, wt, dt, o, i, j,
+                                                   1 + bz);
+  block_divmax(a, o.divmax);
+}
+
+__global__ void k_clear_slots(unsigned long long* p) {
+  p[blockIdx.x * blockDim.x + threadIdx.x] = 0ull;
 }
 
 __global__ void k_project_mixed_faces(Geom g, RMetrics r, const float* phi1, const float* dphi,
@@ -244,7 +287,15 @@ __global__ void k_project_mixed_faces(Geom g, RMetrics r, const float* phi1, con
 // march of the zero mode included, runs in FP64 arithmetic).  Pass 2 is
 // one iterative-refinement step: the residual div(u_temp)/dt - L phi1 is
 // formed in FP64 from the kept right-hand side and the stored phi1, and
-// solved the same way for delta.  The projection uses phi1 + delta in FP64.
+// solved the same way for delta.  The projection uses phi1 + delta in FP64
+// and reports the largest |divu|.  Where that is above kMixedDivTol (the
+// first steps from a state far from solenoidal, or while the penalisation
+// still cuts the velocity sharply at the solid's surface), the
+// refinement continues once more: the residual is the post-projection
+// divergence itself, div(u)/dt from the FP64 velocity, solved the same way
+// for psi, and the projection is redone from u_temp with phi1 + delta +
+// psi.  Deciding that reads one 8-byte word back to the host, so the host
+// waits for the stream here.
 int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
                      const double* v_temp, const double* w_temp, double dt, double xl, double zl,
                      const MkpOutMixed& o, hipStream_t s, hipEvent_t before_project) {
@@ -296,12 +347,49 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
     rc = hip_check(hipStreamWaitEvent(s, before_project, 0), "mkp wait");
     if (rc) return rc;
   }
-  ProfScope ps("project_mixed", s);
-  k_project_mixed<<<interior_grid(g), interior_block(), 0, s>>>(g, r, phi1, dphi, u_temp, v_temp,
-                                                               w_temp, dt, o);
+  unsigned long long* divmax = static_cast<unsigned long long*>(
+      workspace(kWorkDivMax, kDivSlots * sizeof(unsigned long long)));
+  if (!divmax) return VK_ERR_HIP;
+  MkpOutMixed op = o;
+  op.divmax = divmax;
+  op.psi = nullptr;
   const long long nf = zero_face_count(g);
-  k_project_mixed_faces<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(g, r, phi1, dphi, u_temp,
-                                                                     v_temp, w_temp, dt, o);
+  auto project = [&](const char* name) {
+    ProfScope ps(name, s);
+    k_clear_slots<<<kDivSlots / 256, 256, 0, s>>>(divmax);
+    k_project_mixed<<<interior_grid(g), interior_block(), 0, s>>>(g, r, phi1, dphi, u_temp,
+                                                                 v_temp, w_temp, dt, op);
+    k_project_mixed_faces<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(g, r, phi1, dphi, u_temp,
+                                                                       v_temp, w_temp, dt, op);
+  };
+  project("project_mixed");
+  unsigned long long slots[kDivSlots];
+  rc = hip_check(hipMemcpyAsync(slots, divmax, sizeof(slots), hipMemcpyDeviceToHost, s),
+                 "mkp divmax");
+  if (!rc) rc = hip_check(hipStreamSynchronize(s), "mkp divmax");
+  if (rc) return rc;
+  unsigned long long bits = 0;
+  for (int q = 0; q < kDivSlots; q++) bits = slots[q] > bits ? slots[q] : bits;
+  double dmax;
+  std::memcpy(&dmax, &bits, sizeof(dmax));
+  if (!(dmax > kMixedDivTol)) return VK_OK;
+  // the correction pass: L psi = div(u)/dt of the projected velocity
+  float* psi = static_cast<float*>(workspace(kWorkPhi3, g.cells() * sizeof(float)));
+  if (!psi) return VK_ERR_HIP;
+  rc = launch_p2s_div_f32(g, r, o.u, o.v, o.w, dt, phit, s, "rfft_x_corr", "cfft_z_corr");
+  if (rc) return rc;
+  {
+    AuxFork f0(s, 1u);
+    if (f0.err) return f0.err;
+    launch_sp0b<float>(g, phit, ac, f0.aux(0));
+    rc = launch_spn<float>(g, phit, ac, kxz, s);
+    if (rc) return rc;
+  }
+  rc = launch_s2p_inplace<float>(g, phit, psi, s, "cfft_z_inv_corr", "irfft_x_corr");
+  if (rc) return rc;
+  launch_halo_neumann<float>(g, psi, s);
+  op.psi = psi;
+  project("project_mixed_corr");
   return VK_OK;
 }
 

@@ -101,6 +101,8 @@ enum WorkSlot {
   kWorkRhs,
   kWorkFlags,
   kWorkDiag,
+  kWorkPhi3,
+  kWorkDivMax,
   kWorkSlots
 };
 // `grown` (optional) reports whether this call allocated the slot anew.

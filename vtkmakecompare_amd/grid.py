@@ -154,6 +154,10 @@ class Config:
     seed: int
     noise: float
     spheres: list = field(default_factory=list)  # (cx, cy, cz, r)
+    # (count, 10) records (cx, cy, cz, a, b, c, qw, qx, qy, qz) of ellipsoids;
+    # when set, phi0 is generated on the device (vk_levelset_shapes) and
+    # `spheres` is unused
+    ellipsoids: np.ndarray | None = None
     xl: float = 2.0 * math.pi
     zl: float = math.pi
     beta: float = 1.8
@@ -171,6 +175,26 @@ def random_spheres(seed: int, count: int, xl: float, zl: float,
         r = rmin + (rmax - rmin) * d
         cy = -1.0 + r + (2.0 - 2.0 * r) * b
         out.append((a * xl, cy, c * zl, r))
+    return out
+
+
+def random_ellipsoids(seed: int, count: int, xl: float, zl: float, amin: float,
+                      amax: float) -> np.ndarray:
+    """Seeded ellipsoid records (count, 10): centre, semi-axes uniform in
+    [amin, amax], a uniformly random orientation (the unit quaternion of
+    Shoemake's method from three uniforms), the centre kept inside the
+    walls by the largest semi-axis."""
+    u = uniform01(seed, 41, 9 * count).reshape(count, 9)
+    out = np.zeros((count, 10))
+    for q in range(count):
+        a, b, c, d, e, f, u1, u2, u3 = u[q]
+        ax = amin + (amax - amin) * np.array([d, e, f])
+        rmax = float(ax.max())
+        cy = -1.0 + rmax + (2.0 - 2.0 * rmax) * b
+        s1, s2 = math.sqrt(1.0 - u1), math.sqrt(u1)
+        quat = (s1 * math.sin(2.0 * math.pi * u2), s1 * math.cos(2.0 * math.pi * u2),
+                s2 * math.sin(2.0 * math.pi * u3), s2 * math.cos(2.0 * math.pi * u3))
+        out[q] = (a * xl, cy, c * zl, ax[0], ax[1], ax[2]) + quat
     return out
 
 
@@ -192,9 +216,9 @@ def make_config(name: str) -> Config:
         return Config(name, 256, 129, 256, 2800.0, 0.71, 1e-3, 1e4, 200,
                       0x5EED0004, 0.05, sp)
     if name == "large768":
-        sp = random_spheres(0x5EED0005, 64, 2.0 * math.pi, math.pi, 0.05, 0.2)
+        el = random_ellipsoids(0x5EED0005, 64, 2.0 * math.pi, math.pi, 0.05, 0.2)
         return Config(name, 768, 385, 768, 5600.0, 0.71, 1e-3, 1e4, 400,
-                      0x5EED0005, 0.05, sp)
+                      0x5EED0005, 0.05, [], ellipsoids=el)
     raise ValueError(f"unknown configuration {name!r}")
 
 
@@ -238,17 +262,27 @@ FIELDS = ("u", "v", "w", "p", "t",
           "rhocp", "thc", "u_ep2", "v_ep2", "w_ep2", "divu")
 
 
-def initial_state(cfg: Config, g: Grid | None = None) -> dict:
+# the fields a fused time step needs that initial_state fills with
+# non-zero values; the rest of the step's fields start at 0
+NONZERO_STEP_FIELDS = ("u", "v", "w", "t", "u_pre", "v_pre", "w_pre", "t_pre", "rhocp", "thc")
+
+
+def initial_state(cfg: Config, g: Grid | None = None, lean: bool = False) -> dict:
     """All halo fields of program main (nse_dominik.f:27-95) initialised
     for a configuration: laminar u = 1.5 (1 - y^2) plus uniform noise,
     v and w noise (v zero on the wall faces), T linear (1 at y=-1, 0 at
     y=+1), rhoCp = thc = 1, history arrays copied from the fields as the
     reference does before its time loop (nse_dominik.f:246-257).
-    cellvalue(...,1) is not filled here: it comes from set_levelset."""
+    cellvalue(...,1) is not filled here: it comes from set_levelset.
+
+    With `lean`, only NONZERO_STEP_FIELDS and phi0 are held (the zero
+    fields are left to the device, for grids whose full host state would
+    not fit).  phi0 is None for a configuration of ellipsoids: it is
+    generated on the device."""
     g = g or cfg.grid()
     nx, ny, nz = g.nx, g.ny, g.nz
     n = g.npoints
-    st = {k: np.zeros(g.shape) for k in FIELDS}
+    st = {k: np.zeros(g.shape) for k in (NONZERO_STEP_FIELDS if lean else FIELDS)}
     noise = cfg.noise
 
     def rnd(fid):
@@ -266,7 +300,8 @@ def initial_state(cfg: Config, g: Grid | None = None) -> dict:
     st["thc"][...] = 1.0
     for a, b in (("u_pre", "u"), ("v_pre", "v"), ("w_pre", "w"), ("t_pre", "t")):
         st[a][...] = st[b]
-    st["phi0"] = sphere_levelset(g, cfg.spheres)
-    st["phi"] = np.zeros(g.shape)
-    st["cellvalue"] = np.zeros((4, nz, ny, nx))
+    st["phi0"] = None if cfg.ellipsoids is not None else sphere_levelset(g, cfg.spheres)
+    if not lean:
+        st["phi"] = np.zeros(g.shape)
+        st["cellvalue"] = np.zeros((4, nz, ny, nx))
     return st

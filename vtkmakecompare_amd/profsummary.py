@@ -3,6 +3,8 @@
     python -m vtkmakecompare_amd.profsummary trace  <results.db> [--steps K]
     python -m vtkmakecompare_amd.profsummary traffic <fetch.db> <write.db> \
         --points N [--calib-fetch F.db --calib-write W.db --calib-bytes B]
+    python -m vtkmakecompare_amd.profsummary counters <pmc.db> COUNTER... \
+        [--trace-db T.db] [--match NAME]
 
 `trace` lists per-kernel call counts and average/total durations of a
 --kernel-trace run; `traffic` lists FETCH_SIZE and WRITE_SIZE per dispatch
@@ -72,6 +74,59 @@ def traffic(fetch_db: str, write_db: str, points: int, fetch_scale: float = 1.0,
     return "\n".join(out)
 
 
+def _durations(db: str):
+    """Average duration (ns) and count per kernel of a --kernel-trace run."""
+    c = sqlite3.connect(db)
+    return {_short(n): (avg, cnt) for n, cnt, avg in
+            c.execute("select name, count(*), avg(duration) from kernels group by name")}
+
+
+FP64_COUNTERS = ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64",
+                 "SQ_INSTS_VALU_TRANS_F64")
+
+
+def counters(db: str, names, trace_db: str | None = None, match: str = "") -> str:
+    """Per-kernel averages per dispatch of the counters `names` (one
+    --pmc run).  With the FP64 instruction counters and a kernel trace of
+    the same command, also the FP64 operations per dispatch (64 lanes per
+    wave instruction, an FMA counted as two) and their rate over the traced
+    average duration: an upper bound where lanes are masked."""
+    per = {n: _per_kernel(db, n) for n in names}
+    dur = _durations(trace_db) if trace_db else {}
+    kernels = sorted({k for d in per.values() for k in d if match in k},
+                     key=lambda k: -max(per[n].get(k, (0, 0))[0] for n in names))
+    f64 = all(n in names for n in FP64_COUNTERS)
+    lds = "SQ_LDS_BANK_CONFLICT" in names and "SQ_INSTS_LDS" in names
+    wait = "SQ_WAIT_INST_ANY" in names and "SQ_WAVE_CYCLES" in names
+    head = f"{'kernel':64s} " + " ".join(f"{n.replace('SQ_', ''):>18s}" for n in names)
+    if f64:
+        head += f" {'fp64_ops':>14s} {'avg_us':>9s} {'GFLOP/s':>9s}"
+    if lds:
+        head += f" {'confl/lds':>9s}"
+    if wait:
+        head += f" {'wait_frac':>9s}"
+    out = [head]
+    for k in kernels:
+        vals = [per[n].get(k, (0.0, 0))[0] for n in names]
+        row = f"{k.replace(' ', '')[:64]:64s} " + " ".join(f"{v:18.0f}" for v in vals)
+        if f64:
+            v = dict(zip(names, vals))
+            ops = 64.0 * (v["SQ_INSTS_VALU_ADD_F64"] + v["SQ_INSTS_VALU_MUL_F64"] +
+                          v["SQ_INSTS_VALU_TRANS_F64"] + 2.0 * v["SQ_INSTS_VALU_FMA_F64"])
+            d = dur.get(k, (0.0, 0))[0]
+            row += f" {ops:14.4g} {d / 1e3:9.2f} " + (f"{ops / d:9.1f}" if d else f"{'-':>9s}")
+        if lds:
+            v = dict(zip(names, vals))
+            n_lds = v["SQ_INSTS_LDS"]
+            row += f" {v['SQ_LDS_BANK_CONFLICT'] / n_lds:9.2f}" if n_lds else f" {'-':>9s}"
+        if wait:
+            v = dict(zip(names, vals))
+            cyc = v["SQ_WAVE_CYCLES"]
+            row += f" {v['SQ_WAIT_INST_ANY'] / cyc:9.3f}" if cyc else f" {'-':>9s}"
+        out.append(row)
+    return "\n".join(out)
+
+
 def read_traffic(path: str, kernel: str):
     """HBM bytes per dispatch (fetch + write) of `kernel` from a summary
     written by `traffic`, or None."""
@@ -101,9 +156,16 @@ def main():
     m.add_argument("--calib-kernel", default="vk::k_copy_interior")
     m.add_argument("--calib-bytes", type=float,
                    help="bytes the calibration copy reads (and writes) per launch")
+    c = sub.add_parser("counters")
+    c.add_argument("db")
+    c.add_argument("names", nargs="+")
+    c.add_argument("--trace-db", help="kernel trace of the same command, for durations")
+    c.add_argument("--match", default="", help="only kernels whose name contains this")
     a = ap.parse_args()
     if a.cmd == "trace":
         print(trace(a.db, a.steps))
+    elif a.cmd == "counters":
+        print(counters(a.db, a.names, a.trace_db, a.match))
     else:
         fs = ws = 1.0
         if a.calib_fetch and a.calib_write and a.calib_bytes:
