@@ -357,6 +357,25 @@ int vk_check_courant(const double *dt, const double *dx, const double *dy,
                      const int *nz, const double *u, const double *v,
                      const double *w, double *courant);
 
+/* vk_check_courant that allocates nothing.  No counterpart in the
+   reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  Same arguments, kernels and result as vk_check_courant
+   (bit for bit), with the per-block partial maxima in a static device
+   array of the library instead of workspace: 8 + 3 x 1024 doubles
+   (24.6 KB) on each device, held until the process ends.  The pass's
+   grid is capped at 1024 blocks, so that serves every grid the
+   diagnostics take (fewer than 2^53 interior points); a larger need would
+   be refused with VK_ERR_BAD_SIZE.  Every call on a device shares that
+   device's array and nothing in it needs clearing between calls; calls
+   that overlap on one device (from several host threads) race.  The
+   device current on the calling thread must be the library's.  Like
+   vk_check_courant it waits for the result. */
+int vk_check_courant_noalloc(const double *dt, const double *dx, const double *dy,
+                             const double *dz, const int *nx, const int *ny,
+                             const int *nz, const double *u, const double *v,
+                             const double *w, double *courant);
+
 /* MONITOR(...) (nse_dominik.f:2797-2872, called at :493), same arguments
    and updates: the velocity integrals and their relative changes, T_all
    over the plane k = nz+1 (the reference's loop index after its k loop),

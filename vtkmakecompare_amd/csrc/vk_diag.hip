@@ -10,6 +10,7 @@
 // agree with it to rounding (a few ulp of the sum).
 #include <cmath>
 #include <cstdio>
+#include <string>
 
 #include "vk_kernels.h"
 #include "vk_runtime.h"
@@ -238,6 +239,36 @@ static int diag_blocks(const Geom& g) {
   return (int)(nb < kDiagBlocks ? nb : kDiagBlocks);
 }
 
+// The scratch of vk_check_courant_noalloc: the result (slot 0, then 7
+// spare slots) and three partial maxima per block of the Courant pass.
+// The pass's grid is capped at kDiagBlocks blocks, so 24.6 KB serves every
+// grid.  Each device holds its own copy until the process ends.
+constexpr int kCourantScratch = 8 + 3 * kDiagBlocks;
+static __device__ __align__(256) double scratch[kCourantScratch];
+
+// `scratch` is a host-side stand-in in host code, not a device address:
+// its address on each device, looked up once per device (with that device
+// current) and used for every kernel argument and copy
+constexpr int kMaxDevices = 64;
+static double* g_scratch[kMaxDevices] = {};
+
+static double* courant_scratch() {
+  int cur = -1;
+  if (hip_check(hipGetDevice(&cur), "check_courant_noalloc: hipGetDevice")) return nullptr;
+  if (cur != rt().device || cur < 0 || cur >= kMaxDevices) {
+    fail(VK_ERR_ARGUMENT, "check_courant_noalloc: the current device is not the library's");
+    return nullptr;
+  }
+  if (!g_scratch[cur]) {
+    void* p = nullptr;
+    if (hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                  "check_courant_noalloc: hipGetSymbolAddress"))
+      return nullptr;
+    g_scratch[cur] = static_cast<double*>(p);
+  }
+  return g_scratch[cur];
+}
+
 }  // namespace vk
 
 using namespace vk;
@@ -275,6 +306,35 @@ int vk_check_courant(const double* dt, const double* dx, const double* dy, const
     k_courant_final<<<1, 1024, 0, s>>>(ws + 8, nb, ws);
   }
   VK_LAUNCH_CHECK("check_courant");
+  return fetch(ws, courant, 1, s);
+}
+
+int vk_check_courant_noalloc(const double* dt, const double* dx, const double* dy,
+                             const double* dz, const int* nx, const int* ny, const int* nz,
+                             const double* u, const double* v, const double* w,
+                             double* courant) {
+  VK_REQUIRE_INIT();
+  DIAG_GRID();
+  DIAG_FIELD(du, u);
+  DIAG_FIELD(dv, v);
+  DIAG_FIELD(dw, w);
+  VK_RESOLVE(ddx, const double, dx, (g.nx + 2) * sizeof(double));
+  VK_RESOLVE(ddy, const double, dy, (g.ny + 2) * sizeof(double));
+  VK_RESOLVE(ddz, const double, dz, (g.nz + 2) * sizeof(double));
+  if (!courant) return fail(VK_ERR_ARGUMENT, "check_courant_noalloc: courant is null");
+  const int nb = diag_blocks(g);
+  if (8 + 3 * nb > kCourantScratch)
+    return fail(VK_ERR_BAD_SIZE, "check_courant_noalloc: " + std::to_string(nb) +
+                                     " partials exceed the scratch of " +
+                                     std::to_string(kCourantScratch) + " doubles");
+  double* ws = courant_scratch();
+  if (!ws) return last_status();
+  // every partial the final pass reads is written by the first pass, and
+  // the result by the final pass: nothing needs clearing
+  hipStream_t s = rt().stream;
+  k_courant<<<nb, kDiagThreads, 0, s>>>(g, du, dv, dw, ddx, ddy, ddz, *dt, ws + 8);
+  k_courant_final<<<1, 1024, 0, s>>>(ws + 8, nb, ws);
+  VK_LAUNCH_CHECK("check_courant_noalloc");
   return fetch(ws, courant, 1, s);
 }
 
