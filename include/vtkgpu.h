@@ -158,6 +158,62 @@ int vk_solve_t(double *u_t, const int *nx, const int *ny, const int *nz,
                const double *rhocp, const double *thc,
                const double *cellvalue, const double *eta);
 
+/* ------------------------------------------------- batched line solves
+   The reference's line-level solvers, nsys lines per call.  The reference
+   solves one line per call from inside its OpenMP sweeps; here each call
+   solves a batch, one lane per line, with the reference's elimination in
+   its operation order (bit for bit per line).  Every line array is a
+   Fortran array X(nsys, 0:n+1) -- the systems fastest -- unless stated;
+   per-line scalars of the reference (ft, fb, alpha, beta) are arrays of
+   nsys; the metrics (dx, dx12, ...) are shared, n+2 values.  With
+   nsys = 1 every array has the reference's own layout.  The fused sweeps
+   of the time step solve the same systems with the partition/PCR solver
+   instead; these serve callers of the line routines and tests. */
+
+/* CTRIDIAG(A,B,NN), tridiag.f:43-58: A(nsys,NN,3) real (overwritten as
+   there), B(nsys,NN) complex (re,im interleaved), solved in place */
+int vk_ctridiag_batched(double *a, double *b, const int *nn, const int *nsys);
+/* TDMA(a,b,c,r,u,n), periodic_implicit.f:47-64: rows 0..n+1 */
+int vk_tdma_batched(const double *a, const double *b, const double *c, const double *r,
+                    double *u, const int *n, const int *nsys);
+/* cyclic(a,b,c,alpha,beta,r,x,n), periodic_implicit.f:67-93 */
+int vk_cyclic_batched(const double *a, const double *b, const double *c, const double *alpha,
+                      const double *beta, const double *r, double *x, const int *n,
+                      const int *nsys);
+/* implicit_solver(SUX,nx,dx,dx12,Re,dt,ft,fb,eta,CUX), periodic_implicit.f:1-44:
+   SUX(nsys,0:nx+1) in/out, CUX(nsys,1:nx) */
+int vk_implicit_solver_batched(double *sux, const int *nx, const double *dx,
+                               const double *dx12, const double *re, const double *dt,
+                               const double *ft, const double *fb, const double *eta,
+                               const double *cux, const int *nsys);
+/* implicit_uy(S,ny,dy,dy12,Re,dt,ft,fb,eta,C), nse_dominik.f:1505-1577:
+   S(nsys,0:ny+1) in/out, C(nsys,1:ny) */
+int vk_implicit_uy_batched(double *s, const int *ny, const double *dy, const double *dy12,
+                           const double *re, const double *dt, const double *ft,
+                           const double *fb, const double *eta, const double *c,
+                           const int *nsys);
+/* implicit_vy(S,ny,dy,dy12,Re,dt,ft,fb,eta,C), nse_dominik.f:1583-1648:
+   S(nsys,0:ny) in/out, C(nsys,1:ny-1) */
+int vk_implicit_vy_batched(double *s, const int *ny, const double *dy, const double *dy12,
+                           const double *re, const double *dt, const double *ft,
+                           const double *fb, const double *eta, const double *c,
+                           const int *nsys);
+/* implicit_Tx(STX,nx,dx,dx12,Re,Sc,dt,llx,rrx,ft,fb,eta,CUX),
+   nse_dominik.f:1725-1781: STX, llx, rrx, CUX all (nsys,0:nx+1) */
+int vk_implicit_tx_batched(double *stx, const int *nx, const double *dx, const double *dx12,
+                           const double *re, const double *sc, const double *dt,
+                           const double *llx, const double *rrx, const double *ft,
+                           const double *fb, const double *eta, const double *cux,
+                           const int *nsys);
+/* implicit_Ty(STY,ny,dy,dy12,Re,Sc,dt,lly,rry,ft,fb,CUY,eta),
+   nse_dominik.f:1784-1870: STY, lly, rry, CUY all (nsys,0:ny+1); rows
+   1..ny of STY are written */
+int vk_implicit_ty_batched(double *sty, const int *ny, const double *dy, const double *dy12,
+                           const double *re, const double *sc, const double *dt,
+                           const double *lly, const double *rry, const double *ft,
+                           const double *fb, const double *cuy, const double *eta,
+                           const int *nsys);
+
 /* ------------------------------------------------- pressure projection */
 /* MKP, mkp.f:9-220 (ompnthreads is accepted and ignored, as there) */
 int vk_mkp(double *p, const double *u_temp, const double *v_temp,

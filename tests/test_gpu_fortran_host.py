@@ -151,3 +151,80 @@ def test_fortran_driver_adjoint(tmp_path):
     j_p, grad_p = prob.gradient(checkpoint_every=every)
     assert abs(j_f - j_p) <= 1e-14 * abs(j_p)
     assert nrel(grad_f, grad_p) <= 1e-14
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran host not built")
+def test_fortran_component_routines(tmp_path):
+    """Mode 4: the projection's and the adjoint's component routines through
+    the Fortran wrappers with the reference's names and argument lists
+    (P2S, S2P, SPN, SP0B, PHIBC, CTRIDIAG, set_adj_convect,
+    set_adj_convection_for_scalar, set_adj_NS_scalar), against the oracle
+    on the same inputs: the FFT-based results to 1e-12 normwise, the rest
+    bit for bit."""
+    from oracle_bind import P, oracle
+    cfg = G.small_config(16, 17, 12, seed=0x5EED00BE)
+    g = cfg.grid()
+    nx, ny, nz = g.nx, g.ny, g.nz
+    rng = np.random.default_rng(41)
+    st = {k: rng.normal(size=g.shape) for k in ("u", "v", "w", "t", "phi0")}
+    cv = np.zeros((4, nz, ny, nx))
+    o = oracle()
+    kxz = np.zeros((nz + 1, nx // 2 + 1))
+    ac = np.zeros((3, ny))
+    o.o_mkp_coefficients(nx, ny, nz, P(g.dx), P(g.dy), P(g.dz), P(g.dy12), g.xl, g.zl, P(kxz),
+                         P(ac))
+    atri = np.empty((3, ny))
+    atri[0], atri[2] = rng.uniform(-1, 0, size=ny), rng.uniform(-1, 0, size=ny)
+    atri[1] = -3.0 - rng.uniform(0, 1, size=ny)
+    btri = rng.normal(size=(ny, 2))
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_input(fin, cfg, g, st, cv, 0, 4)
+    with open(fin, "ab") as f:
+        for a in (ac, kxz, atri, btri):
+            np.ascontiguousarray(a).tofile(f)
+    print("vk_driver mode 4 16x17x12", flush=True)
+    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    raw = np.fromfile(fout, dtype=np.float64)
+    nf = int(np.prod(g.shape))
+    ns = 2 * (nx // 2 + 1) * (ny + 2) * (nz + 2)
+    sizes = [("fpt", ns), ("phi", nf), ("atri", 3 * ny), ("btri", 2 * ny), ("u_v", nf),
+             ("v_v", nf), ("w_v", nf), ("u_vt", nf), ("u_s", nf), ("v_s", nf), ("w_s", nf)]
+    got, q = {}, 0
+    for name, n in sizes:
+        got[name] = raw[q:q + n]
+        q += n
+    assert q == raw.size
+    # the oracle on the same inputs
+    fpt = np.zeros(ns)
+    o.o_p2s(P(st["u"]), P(fpt), nx, ny, nz)
+    assert nrel(got["fpt"], fpt) < 1e-12
+    fpt2 = np.zeros(ns)
+    o.o_p2s(P(st["v"]), P(fpt2), nx, ny, nz)
+    o.o_spn(nx, ny, nz, P(ac), P(kxz), P(fpt2))
+    o.o_sp0b(nx, ny, nz, P(ac), P(fpt2))
+    phi = np.zeros(g.shape)
+    o.o_s2p(P(fpt2), P(phi), nx, ny, nz)
+    o.o_phibc(nx, ny, nz, P(phi))
+    gphi = got["phi"].reshape(g.shape)
+    assert nrel(gphi[1:-1, 1:-1, 1:-1], phi[1:-1, 1:-1, 1:-1]) < 1e-12
+    a, b = atri.copy(), np.ascontiguousarray(btri).copy()
+    o.o_ctridiag(P(a), P(b), ny)
+    np.testing.assert_array_equal(got["atri"], a.ravel())
+    np.testing.assert_array_equal(got["btri"], b.ravel())
+    u, v, w, t, tf = (st[k] for k in ("u", "v", "w", "t", "phi0"))
+    rc = np.ones(g.shape)
+    met = [P(getattr(g, k)) for k in ("dx", "dy", "dz", "dx12", "dy12", "dz12")]
+    uv, vv, wv, uvt = (np.zeros(g.shape) for _ in range(4))
+    o.o_set_adj_convect(P(u), P(uv), P(v), P(vv), P(w), P(wv), cfg.re, cfg.dt, nx, ny, nz, *met,
+                        P(t), P(w), P(v))
+    o.o_set_adj_convection_for_scalar(P(u), P(v), P(w), cfg.re, cfg.dt, nx, ny, nz, *met, P(t),
+                                      P(uvt), P(rc), P(t), P(w), P(v))
+    us, vs, ws = (np.zeros(g.shape) for _ in range(3))
+    o.o_set_adj_ns_scalar(cfg.dt, nx, ny, nz, met[3], met[4], met[5], P(t), P(us), P(vs), P(ws),
+                          P(t), P(w), P(v), P(tf))
+    inner = (slice(1, -1),) * 3
+    for name, want in (("u_v", uv), ("v_v", vv), ("w_v", wv), ("u_vt", uvt), ("u_s", us),
+                       ("v_s", vs), ("w_s", ws)):
+        np.testing.assert_array_equal(got[name].reshape(g.shape)[inner], want[inner],
+                                      err_msg=name)

@@ -87,6 +87,57 @@ def solve_t(u_t, nx, ny, nz, dx, dy, dz, dx12, dy12, dz12, re, sc, dt, rhocp, th
                            D(sc), D(dt), rhocp, thc, cellvalue, D(eta)), "solve_t")
 
 
+# ------------------------------------------- batched line-level solvers
+# Line arrays are (n+2, nsys) C-order arrays on the device (Fortran
+# X(nsys, 0:n+1), the systems fastest); per-line scalars are (nsys,)
+# device arrays.
+
+def ctridiag_batched(a, b, nn, nsys):
+    """CTRIDIAG (tridiag.f:43): a (3, nn, nsys), b (nn, nsys) complex as
+    (nn, nsys, 2) doubles, in place."""
+    check(lib().vk_ctridiag_batched(a, b, I(nn), I(nsys)), "ctridiag_batched")
+
+
+def tdma_batched(a, b, c, r, u, n, nsys):
+    """TDMA (periodic_implicit.f:47)."""
+    check(lib().vk_tdma_batched(a, b, c, r, u, I(n), I(nsys)), "tdma_batched")
+
+
+def cyclic_batched(a, b, c, alpha, beta, r, x, n, nsys):
+    """cyclic (periodic_implicit.f:67)."""
+    check(lib().vk_cyclic_batched(a, b, c, alpha, beta, r, x, I(n), I(nsys)), "cyclic_batched")
+
+
+def implicit_solver_batched(sux, nx, dx, dx12, re, dt, ft, fb, eta, cux, nsys):
+    """implicit_solver (periodic_implicit.f:1)."""
+    check(lib().vk_implicit_solver_batched(sux, I(nx), dx, dx12, D(re), D(dt), ft, fb, D(eta),
+                                           cux, I(nsys)), "implicit_solver_batched")
+
+
+def implicit_uy_batched(s, ny, dy, dy12, re, dt, ft, fb, eta, c, nsys):
+    """implicit_uy (nse_dominik.f:1505)."""
+    check(lib().vk_implicit_uy_batched(s, I(ny), dy, dy12, D(re), D(dt), ft, fb, D(eta), c,
+                                       I(nsys)), "implicit_uy_batched")
+
+
+def implicit_vy_batched(s, ny, dy, dy12, re, dt, ft, fb, eta, c, nsys):
+    """implicit_vy (nse_dominik.f:1583)."""
+    check(lib().vk_implicit_vy_batched(s, I(ny), dy, dy12, D(re), D(dt), ft, fb, D(eta), c,
+                                       I(nsys)), "implicit_vy_batched")
+
+
+def implicit_tx_batched(stx, nx, dx, dx12, re, sc, dt, llx, rrx, ft, fb, eta, cux, nsys):
+    """implicit_Tx (nse_dominik.f:1725)."""
+    check(lib().vk_implicit_tx_batched(stx, I(nx), dx, dx12, D(re), D(sc), D(dt), llx, rrx, ft,
+                                       fb, D(eta), cux, I(nsys)), "implicit_tx_batched")
+
+
+def implicit_ty_batched(sty, ny, dy, dy12, re, sc, dt, lly, rry, ft, fb, cuy, eta, nsys):
+    """implicit_Ty (nse_dominik.f:1784)."""
+    check(lib().vk_implicit_ty_batched(sty, I(ny), dy, dy12, D(re), D(sc), D(dt), lly, rry, ft,
+                                       fb, cuy, D(eta), I(nsys)), "implicit_ty_batched")
+
+
 def mkp(p, u_temp, v_temp, w_temp, dt, dx, dy, dz, dx12, dy12, dz12, re, nx, ny, nz,
         ompnthreads, u_ta, v_ta, w_ta, u_ep2, v_ep2, w_ep2, divu, u, v, w, xl, zl):
     check(lib().vk_mkp(p, u_temp, v_temp, w_temp, D(dt), dx, dy, dz, dx12, dy12, dz12, D(re),

@@ -60,7 +60,7 @@ __device__ inline void mark_dirty_lines(const LineFlags& lf, const Geom& g, int 
 // the mask bits of a point from its cellvalue slots 1-4, rhoCp and thc
 template <typename T>
 __device__ inline unsigned dirty_bits(T cv1, T cv2, T cv3, T cv4, T rc, T thc) {
-  return ((cv1 != T(0)) | (rc != T(1)) | (thc != T(1)) ? 1u : 0u) | (cv2 != T(0) ? 2u : 0u) |
+  return (((cv1 != T(0)) | (rc != T(1)) | (thc != T(1))) ? 1u : 0u) | (cv2 != T(0) ? 2u : 0u) |
          (cv3 != T(0) ? 4u : 0u) | (cv4 != T(0) ? 8u : 0u);
 }
 

@@ -103,6 +103,7 @@ enum WorkSlot {
   kWorkDiag,
   kWorkPhi3,
   kWorkDivMax,
+  kWorkLineBatch,
   kWorkSlots
 };
 // `grown` (optional) reports whether this call allocated the slot anew.

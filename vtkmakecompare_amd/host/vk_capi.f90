@@ -238,9 +238,129 @@ module vk_capi
       real(c_double) :: u_ta(*), v_ta(*), w_ta(*), phi(*), divu(*)
       integer(c_int) :: nx, ny, nz
     end function
+    ! the pressure projection's components (p2s_fftw2d.f:5,116; mkp.f:223,263,293)
+    integer(c_int) function vk_p2s(fp, fpt, nx, ny, nz) bind(C, name="vk_p2s")
+      import :: c_int, c_double, c_double_complex
+      real(c_double) :: fp(*)
+      complex(c_double_complex) :: fpt(*)
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_s2p(fpt, fp, nx, ny, nz) bind(C, name="vk_s2p")
+      import :: c_int, c_double, c_double_complex
+      complex(c_double_complex) :: fpt(*)
+      real(c_double) :: fp(*)
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_spn(nx, ny, nz, ac, kxz, phit) bind(C, name="vk_spn")
+      import :: c_int, c_double, c_double_complex
+      integer(c_int) :: nx, ny, nz
+      real(c_double) :: ac(*), kxz(*)
+      complex(c_double_complex) :: phit(*)
+    end function
+    integer(c_int) function vk_sp0b(nx, ny, nz, ac, phit) bind(C, name="vk_sp0b")
+      import :: c_int, c_double, c_double_complex
+      integer(c_int) :: nx, ny, nz
+      real(c_double) :: ac(*)
+      complex(c_double_complex) :: phit(*)
+    end function
+    integer(c_int) function vk_phibc(nx, ny, nz, phi) bind(C, name="vk_phibc")
+      import :: c_int, c_double
+      integer(c_int) :: nx, ny, nz
+      real(c_double) :: phi(*)
+    end function
+    ! the batched line-level solvers (tridiag.f:43; periodic_implicit.f:1-93;
+    ! nse_dominik.f:1505-1870); line arrays X(nsys, 0:n+1)
+    integer(c_int) function vk_ctridiag_batched(a, b, nn, nsys) &
+        bind(C, name="vk_ctridiag_batched")
+      import :: c_int, c_double, c_double_complex
+      real(c_double) :: a(*)
+      complex(c_double_complex) :: b(*)
+      integer(c_int) :: nn, nsys
+    end function
+    integer(c_int) function vk_tdma_batched(a, b, c, r, u, n, nsys) &
+        bind(C, name="vk_tdma_batched")
+      import :: c_int, c_double
+      real(c_double) :: a(*), b(*), c(*), r(*), u(*)
+      integer(c_int) :: n, nsys
+    end function
+    integer(c_int) function vk_cyclic_batched(a, b, c, alpha, beta, r, x, n, nsys) &
+        bind(C, name="vk_cyclic_batched")
+      import :: c_int, c_double
+      real(c_double) :: a(*), b(*), c(*), alpha(*), beta(*), r(*), x(*)
+      integer(c_int) :: n, nsys
+    end function
+    integer(c_int) function vk_implicit_solver_batched(sux, nx, dx, dx12, re, dt, ft, fb, eta, &
+        cux, nsys) bind(C, name="vk_implicit_solver_batched")
+      import :: c_int, c_double
+      real(c_double) :: sux(*), dx(*), dx12(*), re, dt, ft(*), fb(*), eta, cux(*)
+      integer(c_int) :: nx, nsys
+    end function
+    integer(c_int) function vk_implicit_uy_batched(s, ny, dy, dy12, re, dt, ft, fb, eta, c, &
+        nsys) bind(C, name="vk_implicit_uy_batched")
+      import :: c_int, c_double
+      real(c_double) :: s(*), dy(*), dy12(*), re, dt, ft(*), fb(*), eta, c(*)
+      integer(c_int) :: ny, nsys
+    end function
+    integer(c_int) function vk_implicit_vy_batched(s, ny, dy, dy12, re, dt, ft, fb, eta, c, &
+        nsys) bind(C, name="vk_implicit_vy_batched")
+      import :: c_int, c_double
+      real(c_double) :: s(*), dy(*), dy12(*), re, dt, ft(*), fb(*), eta, c(*)
+      integer(c_int) :: ny, nsys
+    end function
+    integer(c_int) function vk_implicit_tx_batched(stx, nx, dx, dx12, re, sc, dt, llx, rrx, ft, &
+        fb, eta, cux, nsys) bind(C, name="vk_implicit_tx_batched")
+      import :: c_int, c_double
+      real(c_double) :: stx(*), dx(*), dx12(*), re, sc, dt, llx(*), rrx(*), ft(*), fb(*)
+      real(c_double) :: eta, cux(*)
+      integer(c_int) :: nx, nsys
+    end function
+    integer(c_int) function vk_implicit_ty_batched(sty, ny, dy, dy12, re, sc, dt, lly, rry, ft, &
+        fb, cuy, eta, nsys) bind(C, name="vk_implicit_ty_batched")
+      import :: c_int, c_double
+      real(c_double) :: sty(*), dy(*), dy12(*), re, sc, dt, lly(*), rry(*), ft(*), fb(*)
+      real(c_double) :: cuy(*), eta
+      integer(c_int) :: ny, nsys
+    end function
+    ! the adjoint operators (adjoint.f:1,128,185); T_final by address
+    integer(c_int) function vk_set_adj_convect(u, u_v, v, v_v, w, w_v, re, dt, nx, ny, nz, &
+        dx, dy, dz, dx12, dy12, dz12, u_final, v_final, w_final) &
+        bind(C, name="vk_set_adj_convect")
+      import :: c_int, c_double
+      real(c_double) :: u(*), u_v(*), v(*), v_v(*), w(*), w_v(*), re, dt
+      real(c_double) :: dx(*), dy(*), dz(*), dx12(*), dy12(*), dz12(*)
+      real(c_double) :: u_final(*), v_final(*), w_final(*)
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_set_adj_convection_for_scalar(u, v, w, re, dt, nx, ny, nz, &
+        dx, dy, dz, dx12, dy12, dz12, t, u_vt, rhocp, u_final, v_final, w_final) &
+        bind(C, name="vk_set_adj_convection_for_scalar")
+      import :: c_int, c_double
+      real(c_double) :: u(*), v(*), w(*), re, dt, dx(*), dy(*), dz(*), dx12(*), dy12(*)
+      real(c_double) :: dz12(*), t(*), u_vt(*), rhocp(*), u_final(*), v_final(*), w_final(*)
+      integer(c_int) :: nx, ny, nz
+    end function
+    integer(c_int) function vk_set_adj_ns_scalar(dt, nx, ny, nz, dx12, dy12, dz12, t, u_s, &
+        v_s, w_s, u_final, v_final, w_final, t_final) bind(C, name="vk_set_adj_ns_scalar")
+      import :: c_int, c_double, c_ptr
+      real(c_double) :: dt, dx12(*), dy12(*), dz12(*), t(*), u_s(*), v_s(*), w_s(*)
+      real(c_double) :: u_final(*), v_final(*), w_final(*)
+      type(c_ptr), value :: t_final
+      integer(c_int) :: nx, ny, nz
+    end function
   end interface
 
+  ! the field set_adj_NS_scalar reads as T_final, which the reference
+  ! leaves an uninitialised local (adjoint.f:190): named by the host with
+  ! vk_use_t_final before the first call
+  type(c_ptr), save :: vk_t_final = c_null_ptr
+
 contains
+
+  ! Names the (resident) halo field set_adj_NS_scalar reads as T_final.
+  subroutine vk_use_t_final(t_final)
+    real(c_double), target :: t_final(*)
+    vk_t_final = c_loc(t_final)
+  end subroutine
 
   ! Stops with the library's message on a non-zero status (the reference
   ! routines have no status argument, so a failure ends the run).
@@ -266,6 +386,14 @@ contains
     real(c_double) :: a(*)
     integer(c_size_t), intent(in) :: n
     call vk_check(vk_register(a, n * 8_c_size_t), 'vk_register')
+    call vk_check(vk_upload(a), 'vk_upload')
+  end subroutine
+
+  ! The same for an array of n double complex values.
+  subroutine vk_resident_c64(a, n)
+    complex(c_double_complex) :: a(*)
+    integer(c_size_t), intent(in) :: n
+    call vk_check(vk_register(a, n * 16_c_size_t), 'vk_register')
     call vk_check(vk_upload(a), 'vk_upload')
   end subroutine
 

@@ -21,9 +21,16 @@
 !   float64 dx, dy, dz, dx12, dy12, dz12, x, y, z   (n+2 each)
 !   float64 u, v, w, T, phi0                          (halo fields)
 !   float64 cellvalue                                 (nx*ny*nz*4)
+!   mode 4 only: float64 AC(ny,3), KXZ(0:nx/2,0:nz), A(ny,3), complex B(ny)
 ! output: u, v, w, p, T, u_ta, v_ta, w_ta, divu, u_temp, v_temp, w_temp
 !         (halo fields) and cellvalue; mode 3: the objective, then the
-!         gradient (nx*ny*nz)
+!         gradient (nx*ny*nz); mode 4 (the projection's and the adjoint's
+!         component routines by their reference names): P2S of u, the
+!         solution phi of SPN + SP0B on P2S of v then S2P and PHIBC,
+!         CTRIDIAG's A and B, set_adj_convect's u_v, v_v, w_v,
+!         set_adj_convection_for_scalar's u_vT and set_adj_NS_scalar's
+!         u_s, v_s, w_s (adjoint fields u_final = T, v_final = w,
+!         w_final = v, T_final = phi0)
 program vk_driver
   use vk_capi
   implicit none
@@ -41,6 +48,8 @@ program vk_driver
       v_taf, w_taf, rhocpf, thcf
   real(c_float), allocatable :: cellvaluef(:,:,:,:)
   double precision :: objective
+  double precision, allocatable :: ac(:,:), kxz(:,:), atri(:,:)
+  double complex, allocatable :: fpt(:,:,:), fpt2(:,:,:), btri(:)
   integer :: every
   character(len=512) :: fin, fout
   integer(c_size_t) :: nf, nc
@@ -70,6 +79,10 @@ program vk_driver
   read (10) dx, dy, dz, dx12, dy12, dz12, x, y, z
   read (10) u, v, w, t, phi0
   read (10) cellvalue
+  if (mode == 4) then
+    allocate (ac(1:ny,3), kxz(0:nx/2,0:nz), atri(1:ny,3), btri(1:ny))
+    read (10) ac, kxz, atri, btri
+  end if
   close (10)
 
   ! initialisation as program main does it (nse_dominik.f:104-191,246-257)
@@ -124,6 +137,43 @@ program vk_driver
     close (11)
     call vk_check(vk_finalize(), 'vk_finalize')
     write (6, '(a,i0,a,es23.15)') 'vk_driver: adjoint over ', nsteps, ' steps, J = ', objective
+    stop
+  end if
+
+  if (mode == 4) then
+    allocate (fpt(0:nx/2,0:ny+1,0:nz+1), fpt2(0:nx/2,0:ny+1,0:nz+1))
+    fpt = (0.0d0, 0.0d0)
+    fpt2 = (0.0d0, 0.0d0)
+    nc = int(nx / 2 + 1, c_size_t) * int(ny + 2, c_size_t) * int(nz + 2, c_size_t)
+    call vk_resident_c64(fpt, nc); call vk_resident_c64(fpt2, nc)
+    call vk_resident(ac, 3_c_size_t * int(ny, c_size_t))
+    call vk_resident(kxz, int(nx / 2 + 1, c_size_t) * int(nz + 1, c_size_t))
+    call vk_resident(atri, 3_c_size_t * int(ny, c_size_t))
+    call vk_resident_c64(btri, int(ny, c_size_t))
+    call p2s(u, fpt, nx, ny, nz)
+    call p2s(v, fpt2, nx, ny, nz)
+    call spn(nx, ny, nz, ac, kxz, fpt2)
+    call sp0b(nx, ny, nz, ac, fpt2)
+    call s2p(fpt2, phi, nx, ny, nz)
+    call phibc(nx, ny, nz, phi)
+    call ctridiag(atri, btri, ny)
+    call set_adj_convect(u, u_v, v, v_v, w, w_v, re, dt, nx, ny, nz, dx, dy, dz, dx12, dy12, &
+                         dz12, t, w, v)
+    call set_adj_convection_for_scalar(u, v, w, re, dt, nx, ny, nz, dx, dy, dz, dx12, dy12, &
+                                       dz12, t, u_vt, rhocp, t, w, v)
+    call vk_use_t_final(phi0)
+    call set_adj_ns_scalar(dt, nx, ny, nz, dx12, dy12, dz12, t, u_l, v_l, w_l, t, w, v)
+    call vk_check(vk_download(fpt), 'download'); call vk_check(vk_download(phi), 'download')
+    call vk_check(vk_download(atri), 'download'); call vk_check(vk_download(btri), 'download')
+    call vk_check(vk_download(u_v), 'download'); call vk_check(vk_download(v_v), 'download')
+    call vk_check(vk_download(w_v), 'download'); call vk_check(vk_download(u_vt), 'download')
+    call vk_check(vk_download(u_l), 'download'); call vk_check(vk_download(v_l), 'download')
+    call vk_check(vk_download(w_l), 'download')
+    open (11, file=trim(fout), access='stream', form='unformatted', status='replace')
+    write (11) fpt, phi, atri, btri, u_v, v_v, w_v, u_vt, u_l, v_l, w_l
+    close (11)
+    call vk_check(vk_finalize(), 'vk_finalize')
+    write (6, '(a)') 'vk_driver: component routines done'
     stop
   end if
 

@@ -15,7 +15,13 @@
 !   solve_u_temp                nse_dominik.f:1344
 !   solve_t                     nse_dominik.f:1465
 !   MKP                         mkp.f:9
+!   P2S, S2P                    p2s_fftw2d.f:5,116
+!   SPN, SP0B, PHIBC            mkp.f:223,263,293
+!   CTRIDIAG                    tridiag.f:43
 !   set_levelset                set_levelset.f:1
+!   set_adj_convect             adjoint.f:1
+!   set_adj_convection_for_scalar adjoint.f:128
+!   set_adj_NS_scalar           adjoint.f:185
 
 subroutine set_bc(u, v, w, p, nx, ny, nz, t)
   use vk_capi
@@ -165,6 +171,112 @@ subroutine mkp(p, u_temp, v_temp, w_temp, dt, dx, dy, dz, dx12, dy12, dz12, re, 
   call vk_check(vk_mkp(p, u_temp, v_temp, w_temp, dt, dx, dy, dz, dx12, dy12, dz12, re, &
                        nx, ny, nz, ompnthreads, u_ta, v_ta, w_ta, u_ep2, v_ep2, w_ep2, &
                        divu, u, v, w, xl, zl), 'MKP')
+end subroutine
+
+subroutine p2s(fp, fpt, nx, ny, nz)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: fp(0:nx+1,0:ny+1,0:nz+1)
+  double complex :: fpt(0:nx/2,0:ny+1,0:nz+1)
+  call vk_check(vk_p2s(fp, fpt, nx, ny, nz), 'P2S')
+end subroutine
+
+subroutine s2p(fpt, fp, nx, ny, nz)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: fp(0:nx+1,0:ny+1,0:nz+1)
+  double complex :: fpt(0:nx/2,0:ny+1,0:nz+1)
+  call vk_check(vk_s2p(fpt, fp, nx, ny, nz), 'S2P')
+end subroutine
+
+subroutine spn(nx, ny, nz, ac, kxz, phit)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: ac(1:ny,3), kxz(0:nx/2,0:nz)
+  double complex :: phit(0:nx/2,0:ny+1,0:nz+1)
+  call vk_check(vk_spn(nx, ny, nz, ac, kxz, phit), 'SPN')
+end subroutine
+
+subroutine sp0b(nx, ny, nz, ac, phit)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: ac(1:ny,3)
+  double complex :: phit(0:nx/2,0:ny+1,0:nz+1)
+  call vk_check(vk_sp0b(nx, ny, nz, ac, phit), 'SP0B')
+end subroutine
+
+subroutine phibc(nx, ny, nz, phi)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: phi(0:nx+1,0:ny+1,0:nz+1)
+  call vk_check(vk_phibc(nx, ny, nz, phi), 'PHIBC')
+end subroutine
+
+! one system, as the reference calls it: the batched solver with nsys = 1
+! has the reference's layout A(NN,3), B(NN)
+subroutine ctridiag(a, b, nn)
+  use vk_capi
+  implicit none
+  integer :: nn
+  double precision :: a(nn,3)
+  double complex :: b(nn)
+  call vk_check(vk_ctridiag_batched(a, b, nn, 1), 'CTRIDIAG')
+end subroutine
+
+subroutine set_adj_convect(u, u_v, v, v_v, w, w_v, re, dt, nx, ny, nz, dx, dy, dz, dx12, &
+                           dy12, dz12, u_final, v_final, w_final)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: re, dt
+  double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: u, u_v, v, v_v, w, w_v, u_final, &
+      v_final, w_final
+  double precision :: dx(0:nx+1), dy(0:ny+1), dz(0:nz+1)
+  double precision :: dx12(0:nx+1), dy12(0:ny+1), dz12(0:nz+1)
+  call vk_check(vk_set_adj_convect(u, u_v, v, v_v, w, w_v, re, dt, nx, ny, nz, dx, dy, dz, &
+                                   dx12, dy12, dz12, u_final, v_final, w_final), &
+                'set_adj_convect')
+end subroutine
+
+subroutine set_adj_convection_for_scalar(u, v, w, re, dt, nx, ny, nz, dx, dy, dz, dx12, dy12, &
+                                         dz12, t, u_vt, rhocp, u_final, v_final, w_final)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: re, dt
+  double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: u, v, w, t, u_vt, rhocp, u_final, &
+      v_final, w_final
+  double precision :: dx(0:nx+1), dy(0:ny+1), dz(0:nz+1)
+  double precision :: dx12(0:nx+1), dy12(0:ny+1), dz12(0:nz+1)
+  call vk_check(vk_set_adj_convection_for_scalar(u, v, w, re, dt, nx, ny, nz, dx, dy, dz, &
+                                                 dx12, dy12, dz12, t, u_vt, rhocp, u_final, &
+                                                 v_final, w_final), &
+                'set_adj_convection_for_scalar')
+end subroutine
+
+! The reference reads T_final from an uninitialised local (adjoint.f:190);
+! here it is the field the host named with vk_use_t_final.
+subroutine set_adj_ns_scalar(dt, nx, ny, nz, dx12, dy12, dz12, t, u_s, v_s, w_s, u_final, &
+                             v_final, w_final)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: dt
+  double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: t, u_s, v_s, w_s, u_final, v_final, &
+      w_final
+  double precision :: dx12(0:nx+1), dy12(0:ny+1), dz12(0:nz+1)
+  if (.not. c_associated(vk_t_final)) then
+    write (0, '(a)') 'set_adj_NS_scalar: name T_final with vk_use_t_final first'
+    error stop 2
+  end if
+  call vk_check(vk_set_adj_ns_scalar(dt, nx, ny, nz, dx12, dy12, dz12, t, u_s, v_s, w_s, &
+                                     u_final, v_final, w_final, vk_t_final), &
+                'set_adj_NS_scalar')
 end subroutine
 
 subroutine set_levelset(nx, ny, nz, dx, dx12, dy12, dz12, phi0, phi, x, y, z, cellvalue)
