@@ -502,6 +502,48 @@ int vk_output(const double *u, const double *v, const double *w,
               const double *v_ta, const double *w_ta, const double *phi,
               const double *divu);
 
+/* The legacy VTK files of the reference, byte for byte: levelset.vtk
+   (set_levelset.f:131-156: phi's interior, 'SCALARS phi float64'),
+   cellvalue1.vtk (set_levelset.f:171-198) and cellvalue.vtk
+   (nse_dominik.f:208-235: cellvalue(...,1), 'SCALARS PHI float').  ASCII
+   structured grids with the reference's Fortran edit descriptors:
+   'DIMENSIONS ',3(1x,i3) -- which prints *** for an axis of 1000 points or
+   more, as the reference does -- 'POINTS ',i9, the cell centres x(i),
+   y(j), z(k) as 3e15.7 and the values as e15.7, k, j, i order.  path is a
+   NUL-terminated file name (NULL: levelset.vtk / cellvalue.vtk).  Arrays
+   are host arrays (registered ones are downloaded first) or device
+   memory.  No reference routine of these names: the reference writes the
+   files inline. */
+int vk_vtk_levelset(const int *nx, const int *ny, const int *nz, const double *x,
+                    const double *y, const double *z, const double *phi, const char *path);
+int vk_vtk_cellvalue(const int *nx, const int *ny, const int *nz, const double *x,
+                     const double *y, const double *z, const double *cellvalue,
+                     const char *path);
+
+/* A restart that continues a run bit for bit.  No reference counterpart:
+   READFIELD (nse_dominik.f:2769-2793) restores u, v, w, p, T, phi, u_ta,
+   v_ta, w_ta but not the AB2 history u_pre, v_pre, w_pre, T_pre nor the
+   step counter l, so a run restarted from it takes an Euler step where
+   the uninterrupted run takes an AB2 step.  The record written here holds
+   everything vk_time_step reads: a header (magic 'VKRESTART 1', nx, ny,
+   nz, l), then u, v, w, p, T, u_pre, v_pre, w_pre, T_pre, u_ta, v_ta,
+   w_ta, rhoCp, thc (one halo field per record) and cellvalue's four
+   slots (one record each), Fortran sequential unformatted with 4-byte
+   markers (so each record stays below 2 GiB: grids up to 768x385x768 fit).
+   Arrays are host arrays (registered ones are downloaded first / uploaded
+   after) or device memory.  vk_read_restart refuses a record for another
+   grid or a short file and returns l; the next step is l + 1. */
+int vk_write_restart(const char *path, const int *nx, const int *ny, const int *nz,
+                     const int *l, const double *u, const double *v, const double *w,
+                     const double *p, const double *t, const double *u_pre,
+                     const double *v_pre, const double *w_pre, const double *t_pre,
+                     const double *u_ta, const double *v_ta, const double *w_ta,
+                     const double *rhocp, const double *thc, const double *cellvalue);
+int vk_read_restart(const char *path, const int *nx, const int *ny, const int *nz, int *l,
+                    double *u, double *v, double *w, double *p, double *t, double *u_pre,
+                    double *v_pre, double *w_pre, double *t_pre, double *u_ta, double *v_ta,
+                    double *w_ta, double *rhocp, double *thc, double *cellvalue);
+
 #ifdef __cplusplus
 }
 #endif

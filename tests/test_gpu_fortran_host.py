@@ -21,9 +21,10 @@ DRIVER = os.path.join(ROOT, "vtkmakecompare_amd", "host", "vk_driver")
 OUT = ("u", "v", "w", "p", "t", "u_ta", "v_ta", "w_ta", "divu", "u_temp", "v_temp", "w_temp")
 
 
-def write_input(path, cfg, g, st, cv, nsteps, mode, every=0):
+def write_input(path, cfg, g, st, cv, nsteps, mode, every=0, levelset=0, vtk=0):
     with open(path, "wb") as f:
-        np.array([g.nx, g.ny, g.nz, nsteps, mode, 0, every, 0], dtype=np.int32).tofile(f)
+        np.array([g.nx, g.ny, g.nz, nsteps, mode, levelset, every, vtk],
+                 dtype=np.int32).tofile(f)
         np.array([cfg.dt, cfg.re, cfg.sc, cfg.eta, g.xl, g.zl, g.yl, 0],
                  dtype=np.float64).tofile(f)
         for n in ("dx", "dy", "dz", "dx12", "dy12", "dz12", "x", "y", "z"):
@@ -228,3 +229,36 @@ def test_fortran_component_routines(tmp_path):
                        ("v_s", vs), ("w_s", ws)):
         np.testing.assert_array_equal(got[name].reshape(g.shape)[inner], want[inner],
                                       err_msg=name)
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran host not built")
+def test_fortran_set_levelset_writes_the_vtk_files(tmp_path):
+    """With the VTK flag, the set_levelset wrapper writes levelset.vtk and
+    cellvalue1.vtk and the driver cellvalue.vtk, as the reference's
+    set_levelset and program main do: cellvalue(...,1) byte for byte as the
+    library's writer gives it from the run's cellvalue (whose format is
+    pinned against the reference's own files on the CPU)."""
+    from vtkmakecompare_amd import api
+    cfg = G.small_config(16, 17, 12, seed=0x5EED00BF)
+    g = cfg.grid()
+    st = G.initial_state(cfg, g)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_input(fin, cfg, g, st, np.zeros((4, g.nz, g.ny, g.nx)), 1, 1, levelset=1, vtk=1)
+    r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120,
+                       cwd=tmp_path)
+    assert r.returncode == 0, r.stderr + r.stdout
+    got = read_output(fout, g)
+    (tmp_path / "py").mkdir()
+    cwd = os.getcwd()
+    os.chdir(tmp_path / "py")
+    try:
+        api.vtk_cellvalue(g, np.ascontiguousarray(got["cellvalue"]))
+    finally:
+        os.chdir(cwd)
+    want = (tmp_path / "py" / "cellvalue.vtk").read_bytes()
+    assert (tmp_path / "cellvalue1.vtk").read_bytes() == want
+    assert (tmp_path / "cellvalue.vtk").read_bytes() == want
+    lev = (tmp_path / "levelset.vtk").read_text().splitlines()
+    n = g.nx * g.ny * g.nz
+    assert lev[4] == f"DIMENSIONS  {g.nx:3d} {g.ny:3d} {g.nz:3d}"
+    assert lev[6 + n + 1] == "SCALARS phi float64" and len(lev) == 6 + n + 3 + n

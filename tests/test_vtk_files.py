@@ -1,0 +1,119 @@
+"""The legacy VTK files and the restart record, on the CPU (host arrays,
+no GPU touched).
+
+- levelset.vtk and cellvalue1.vtk byte for byte against the files the
+  reference's own compiled set_levelset writes (set_levelset.f:131-198),
+  from that call's phi and cellvalue; cellvalue.vtk (nse_dominik.f:208-235)
+  has cellvalue1.vtk's format and data, so it is the same bytes.  Grids
+  include an axis of 1000 points, where the reference's i3 DIMENSIONS
+  field overflows to ***.
+- the e15.7 formatter against the Fortran runtime itself (oracle's
+  fmt_probe) on values the reference's runs do not produce: NaN, Inf, -0,
+  three-digit exponents, subnormals, decimal rounding ties.
+- the restart record: a write/read round trip on host arrays bit for bit,
+  and the refusals (another grid, a short file).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle_bind import ORACLE_DIR, I, P, ref
+from vtkmakecompare_amd import api as A
+from vtkmakecompare_amd import grid as G
+
+PROBE = os.path.join(ORACLE_DIR, "_ref", "fmt_probe")
+
+
+class chdir:
+    def __init__(self, d):
+        self.d = d
+
+    def __enter__(self):
+        self.old = os.getcwd()
+        os.chdir(self.d)
+
+    def __exit__(self, *a):
+        os.chdir(self.old)
+
+
+@pytest.mark.skipif(ref() is None, reason="reference library not built")
+@pytest.mark.parametrize("n", [(6, 5, 4), (1000, 2, 2)])
+def test_vtk_files_match_reference(tmp_path, n):
+    g = G.channel_grid(*n)
+    spheres = [(0.5 * g.xl, 0.0, 0.5 * g.zl, 0.6)]
+    phi0 = G.sphere_levelset(g, spheres)
+    phi = np.zeros(g.shape)
+    cv = np.zeros((4, g.nz, g.ny, g.nx))
+    (tmp_path / "ref").mkdir()
+    (tmp_path / "ours").mkdir()
+    with chdir(tmp_path / "ref"):
+        ref().set_levelset_(I(g.nx), I(g.ny), I(g.nz), P(g.dx), P(g.dx12), P(g.dy12),
+                            P(g.dz12), P(phi0), P(phi), P(g.x), P(g.y), P(g.z), P(cv))
+    with chdir(tmp_path / "ours"):
+        A.vtk_levelset(g, phi)
+        A.vtk_cellvalue(g, cv, "cellvalue1.vtk")
+        A.vtk_cellvalue(g, cv)
+    for name in ("levelset.vtk", "cellvalue1.vtk"):
+        want = (tmp_path / "ref" / name).read_bytes()
+        assert (tmp_path / "ours" / name).read_bytes() == want, name
+    assert (tmp_path / "ours" / "cellvalue.vtk").read_bytes() == \
+        (tmp_path / "ref" / "cellvalue1.vtk").read_bytes()
+    head = (tmp_path / "ours" / "levelset.vtk").read_text().splitlines()[4]
+    if g.nx >= 1000:
+        assert head == "DIMENSIONS  ***   2   2"
+
+
+@pytest.mark.skipif(not os.path.exists(PROBE), reason="format probe not built")
+def test_e15_7_matches_fortran_runtime(tmp_path):
+    vals = np.array([1.5, -1.5, 0.0, -0.0, 0.12890625, 0.99609375, 0.3828125, 9.99999995,
+                     1e-100, -2.5e150, 4.9e-324, 2.2250738585072014e-308, 1.0 / 3.0,
+                     123456789.0, 1.7976931348623157e308, np.inf, -np.inf, np.nan,
+                     -9.9999999e-100, 0.5, 5e-1, 1e99, 9.9999999e99, 1e100])
+    raw = tmp_path / "v.bin"
+    vals.tofile(raw)
+    out = subprocess.run([PROBE, str(raw), str(vals.size)], capture_output=True, text=True,
+                         check=True).stdout.splitlines()
+    want = out[:vals.size]
+    g = G.set_c(vals.size, 1, 1, 1.0, 1.0, 1.0)
+    cv = np.zeros((4, 1, 1, vals.size))
+    cv[0, 0, 0, :] = vals
+    with chdir(tmp_path):
+        A.vtk_cellvalue(g, cv)
+    lines = (tmp_path / "cellvalue.vtk").read_text().splitlines()
+    got = lines[-vals.size:]
+    assert got == want
+    assert lines[4] == out[vals.size + 1].replace("  64 ***   7", f" {vals.size:3d}   1   1")
+
+
+def _restart_fields(g, seed):
+    rng = np.random.default_rng(seed)
+    f = {k: rng.normal(size=g.shape) for k in A.RESTART_FIELDS}
+    cv = rng.uniform(0, 1, size=(4, g.nz, g.ny, g.nx))
+    return f, cv
+
+
+def test_restart_round_trip_on_host(tmp_path):
+    g = G.channel_grid(12, 9, 8)
+    f, cv = _restart_fields(g, 5)
+    path = str(tmp_path / "run.restart")
+    A.write_restart(path, g, 37, f, cv)
+    f2 = {k: np.zeros(g.shape) for k in A.RESTART_FIELDS}
+    cv2 = np.zeros_like(cv)
+    assert A.read_restart(path, g, f2, cv2) == 37
+    for k in A.RESTART_FIELDS:
+        np.testing.assert_array_equal(f2[k], f[k], err_msg=k)
+    np.testing.assert_array_equal(cv2, cv)
+    # another grid, a truncated file and a file of another kind are refused
+    g2 = G.channel_grid(12, 9, 10)
+    f3 = {k: np.zeros(g2.shape) for k in A.RESTART_FIELDS}
+    with pytest.raises(A.VKError):
+        A.read_restart(path, g2, f3, np.zeros((4, g2.nz, g2.ny, g2.nx)))
+    data = open(path, "rb").read()
+    open(path, "wb").write(data[:len(data) // 2])
+    with pytest.raises(A.VKError):
+        A.read_restart(path, g, f2, cv2)
+    open(path, "wb").write(b"\x10\x00\x00\x00" + b"NOT A RESTART!!!" + b"\x10\x00\x00\x00")
+    with pytest.raises(A.VKError):
+        A.read_restart(path, g, f2, cv2)

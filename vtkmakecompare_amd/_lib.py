@@ -30,7 +30,8 @@ EXPORTS = [
     "vk_field_absmax", "vk_field_copy", "vk_check_courant", "vk_check_courant_noalloc",
     "vk_monitor",
     "vk_check_convergence", "vk_readgrid3d", "vk_outfield", "vk_readfield",
-    "vk_output",
+    "vk_output", "vk_vtk_levelset", "vk_vtk_cellvalue", "vk_write_restart",
+    "vk_read_restart",
     "vk_profile_begin", "vk_profile_end", "vk_profile_count", "vk_profile_entry",
 ]
 

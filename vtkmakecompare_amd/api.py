@@ -351,6 +351,66 @@ def readfield(g: G.Grid, fields: dict) -> dict:
     return fields
 
 
+def _ptr(a):
+    """A host numpy array's or a DeviceArray's address for the file routines."""
+    import ctypes
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise ValueError("file routines take C-contiguous float64 host arrays or DeviceArrays")
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _checked(a, shape, name):
+    if tuple(a.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(a.shape)}, need {tuple(shape)}")
+    if isinstance(a, DeviceArray) and a.dtype != np.float64:
+        raise ValueError(f"{name}: float64 arrays only")
+    return _ptr(a)
+
+
+def vtk_levelset(g: G.Grid, phi, path: str | None = None) -> None:
+    """levelset.vtk (set_levelset.f:131-156) from phi (halo shape, host or
+    device), byte for byte as the reference writes it."""
+    p = path.encode() if path else None
+    check(lib().vk_vtk_levelset(I(g.nx), I(g.ny), I(g.nz), *_coords(g),
+                                _checked(phi, g.shape, "phi"), p), "vtk_levelset")
+
+
+def vtk_cellvalue(g: G.Grid, cellvalue, path: str | None = None) -> None:
+    """cellvalue.vtk (nse_dominik.f:208-235; cellvalue1.vtk with that path,
+    set_levelset.f:171-198) from cellvalue (4, nz, ny, nx), host or device."""
+    p = path.encode() if path else None
+    check(lib().vk_vtk_cellvalue(I(g.nx), I(g.ny), I(g.nz), *_coords(g),
+                                 _checked(cellvalue, (4, g.nz, g.ny, g.nx), "cellvalue"), p),
+          "vtk_cellvalue")
+
+
+RESTART_FIELDS = ("u", "v", "w", "p", "t", "u_pre", "v_pre", "w_pre", "t_pre", "u_ta", "v_ta",
+                  "w_ta", "rhocp", "thc")
+
+
+def write_restart(path: str, g: G.Grid, l: int, fields: dict, cellvalue) -> None:
+    """The restart record (vk_write_restart): the fields of RESTART_FIELDS
+    (halo shape, float64, host or device), cellvalue and the step l."""
+    ptrs = [_checked(fields[k], g.shape, k) for k in RESTART_FIELDS]
+    check(lib().vk_write_restart(path.encode(), I(g.nx), I(g.ny), I(g.nz), I(l), *ptrs,
+                                 _checked(cellvalue, (4, g.nz, g.ny, g.nx), "cellvalue")),
+          "write_restart")
+
+
+def read_restart(path: str, g: G.Grid, fields: dict, cellvalue) -> int:
+    """Reads a restart record into the arrays (vk_read_restart) and returns
+    the step l it was written after."""
+    import ctypes
+    ptrs = [_checked(fields[k], g.shape, k) for k in RESTART_FIELDS]
+    l = ctypes.c_int(0)
+    check(lib().vk_read_restart(path.encode(), I(g.nx), I(g.ny), I(g.nz), ctypes.byref(l),
+                                *ptrs, _checked(cellvalue, (4, g.nz, g.ny, g.nx), "cellvalue")),
+          "read_restart")
+    return l.value
+
+
 def levelset_cellvalue(g: G.Grid, phi0: np.ndarray) -> np.ndarray:
     """cellvalue (4, nz, ny, nx) from set_levelset on the device (slot 1
     filled, slots 2-4 zero until a time step derives them)."""
@@ -501,6 +561,29 @@ class ChannelRun:
     def get(self, name: str) -> np.ndarray:
         """A field downloaded to the host, as float64 whatever its storage."""
         return self.d[name].download().astype(np.float64, copy=False)
+
+    def save(self, path: str) -> None:
+        """Writes the run's restart record (everything the next step reads,
+        and the step counter).  The mixed form's float fields are written
+        as doubles (exactly), so the record is the same format."""
+        names = RESTART_FIELDS + ("cellvalue",)
+        if self.precision == "fp64":
+            arrs = {k: self.d[k] for k in names}
+        else:
+            arrs = {k: self.d[k].download().astype(np.float64) for k in names}
+        write_restart(path, self.g, self.l, arrs, arrs["cellvalue"])
+
+    def restore(self, path: str) -> None:
+        """Continues from a restart record written by save() (or by any
+        caller of vk_write_restart) for the same grid."""
+        names = RESTART_FIELDS + ("cellvalue",)
+        if self.precision == "fp64":
+            self.l = read_restart(path, self.g, {k: self.d[k] for k in names}, self.d["cellvalue"])
+            return
+        host = {k: np.zeros(self.d[k].shape) for k in names}
+        self.l = read_restart(path, self.g, host, host["cellvalue"])
+        for k in names:
+            self.d[k].upload(host[k].astype(self.d[k].dtype))
 
     def free(self) -> None:
         """Releases every device array of the run now (rather than when

@@ -112,6 +112,22 @@ void* resolve(const void* p, size_t bytes, const char* what) {
   return nullptr;
 }
 
+bool is_registered(const void* p) {
+  return rt().initialised && registry().count(reinterpret_cast<uintptr_t>(p)) != 0;
+}
+
+bool is_device(const void* p) {
+  if (!rt().initialised || !p) return false;
+  hipPointerAttribute_t attr;
+  std::memset(&attr, 0, sizeof(attr));
+  const hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice;
+}
+
 bool device_extent(const void* p, size_t* bytes) {
   hipDeviceptr_t base = nullptr;
   size_t size = 0;

@@ -80,6 +80,10 @@ struct AuxFork {
 void* resolve(const void* p, size_t bytes, const char* what);
 // bytes from p to the end of its device allocation, when the runtime knows
 bool device_extent(const void* p, size_t* bytes);
+// whether p is the base of a registered host array (vk_register)
+bool is_registered(const void* p);
+// whether p points into device memory (false before vk_init)
+bool is_device(const void* p);
 
 // Registers a function that vk_finalize calls, after every stream has
 // drained and before the streams are destroyed, to free device memory a

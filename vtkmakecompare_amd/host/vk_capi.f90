@@ -347,7 +347,46 @@ module vk_capi
       type(c_ptr), value :: t_final
       integer(c_int) :: nx, ny, nz
     end function
+    ! the legacy VTK files (set_levelset.f:131-198, nse_dominik.f:208-235);
+    ! path: a NUL-terminated name or c_null_ptr for the reference's name
+    integer(c_int) function vk_vtk_levelset(nx, ny, nz, x, y, z, phi, path) &
+        bind(C, name="vk_vtk_levelset")
+      import :: c_int, c_double, c_ptr
+      integer(c_int) :: nx, ny, nz
+      real(c_double) :: x(*), y(*), z(*), phi(*)
+      type(c_ptr), value :: path
+    end function
+    integer(c_int) function vk_vtk_cellvalue(nx, ny, nz, x, y, z, cellvalue, path) &
+        bind(C, name="vk_vtk_cellvalue")
+      import :: c_int, c_double, c_ptr
+      integer(c_int) :: nx, ny, nz
+      real(c_double) :: x(*), y(*), z(*), cellvalue(*)
+      type(c_ptr), value :: path
+    end function
+    ! the restart record: the step's whole state and l
+    integer(c_int) function vk_write_restart(path, nx, ny, nz, l, u, v, w, p, t, u_pre, v_pre, &
+        w_pre, t_pre, u_ta, v_ta, w_ta, rhocp, thc, cellvalue) bind(C, name="vk_write_restart")
+      import :: c_int, c_double, c_char
+      character(kind=c_char) :: path(*)
+      integer(c_int) :: nx, ny, nz, l
+      real(c_double) :: u(*), v(*), w(*), p(*), t(*), u_pre(*), v_pre(*), w_pre(*), t_pre(*)
+      real(c_double) :: u_ta(*), v_ta(*), w_ta(*), rhocp(*), thc(*), cellvalue(*)
+    end function
+    integer(c_int) function vk_read_restart(path, nx, ny, nz, l, u, v, w, p, t, u_pre, v_pre, &
+        w_pre, t_pre, u_ta, v_ta, w_ta, rhocp, thc, cellvalue) bind(C, name="vk_read_restart")
+      import :: c_int, c_double, c_char
+      character(kind=c_char) :: path(*)
+      integer(c_int) :: nx, ny, nz, l
+      real(c_double) :: u(*), v(*), w(*), p(*), t(*), u_pre(*), v_pre(*), w_pre(*), t_pre(*)
+      real(c_double) :: u_ta(*), v_ta(*), w_ta(*), rhocp(*), thc(*), cellvalue(*)
+    end function
   end interface
+
+  ! whether set_levelset writes levelset.vtk and cellvalue1.vtk as the
+  ! reference's does (set_levelset.f:131-198)
+  logical, save :: vk_vtk_files = .true.
+  character(kind=c_char), target, save :: vk_cellvalue1_name(15) = &
+      ['c', 'e', 'l', 'l', 'v', 'a', 'l', 'u', 'e', '1', '.', 'v', 't', 'k', c_null_char]
 
   ! the field set_adj_NS_scalar reads as T_final, which the reference
   ! leaves an uninitialised local (adjoint.f:190): named by the host with

@@ -16,7 +16,9 @@
 !   int32   nx, ny, nz, nsteps, mode (0 = routine calls, 1 = fused step,
 !           2 = mixed-precision step, 3 = objective and adjoint gradient),
 !           levelset (1 = set_levelset from phi0, 0 = cellvalue given),
-!           checkpoint interval (mode 3), 0
+!           checkpoint interval (mode 3), VTK files (1: set_levelset writes
+!           levelset.vtk and cellvalue1.vtk and the driver cellvalue.vtk,
+!           as the reference does)
 !   float64 dt, Re, Sc, eta, xl, zl, yl, 0
 !   float64 dx, dy, dz, dx12, dy12, dz12, x, y, z   (n+2 each)
 !   float64 u, v, w, T, phi0                          (halo fields)
@@ -119,8 +121,12 @@ program vk_driver
   call vk_resident(phi, nf); call vk_resident(phi0, nf)
   call vk_resident(cellvalue, nc)
 
+  vk_vtk_files = hdr(8) == 1
   if (lsflag == 1) call set_levelset(nx, ny, nz, dx, dx12, dy12, dz12, phi0, phi, x, y, z, &
                                      cellvalue)
+  ! program main's dump of cellvalue(...,1) (nse_dominik.f:208-235)
+  if (vk_vtk_files) call vk_check(vk_vtk_cellvalue(nx, ny, nz, x, y, z, cellvalue, c_null_ptr), &
+                                  'cellvalue.vtk')
 
   if (mode == 3) then
     every = max(1, hdr(7))

@@ -289,6 +289,12 @@ subroutine set_levelset(nx, ny, nz, dx, dx12, dy12, dz12, phi0, phi, x, y, z, ce
   double precision :: cellvalue(1:nx,1:ny,1:nz,1:4)
   call vk_check(vk_set_levelset(nx, ny, nz, dx, dx12, dy12, dz12, phi0, phi, x, y, z, &
                                 cellvalue), 'set_levelset')
+  ! the reference's dumps of phi and cellvalue(...,1) (set_levelset.f:131-198)
+  if (vk_vtk_files) then
+    call vk_check(vk_vtk_levelset(nx, ny, nz, x, y, z, phi, c_null_ptr), 'levelset.vtk')
+    call vk_check(vk_vtk_cellvalue(nx, ny, nz, x, y, z, cellvalue, c_loc(vk_cellvalue1_name)), &
+                  'cellvalue1.vtk')
+  end if
 end subroutine
 
 ! CHECK_COURANT (nse_dominik.f:2519-2601): prints the largest Courant
