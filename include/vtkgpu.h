@@ -384,6 +384,33 @@ int vk_adjoint_gradient(const double *u, const double *v, const double *w,
                         const int *checkpoint_every, double *objective,
                         double *grad);
 
+/* The same gradient over a trajectory kept by binomial checkpointing, and
+   optionally about the mixed-precision trajectory.  No reference
+   counterpart (see vk_adjoint_gradient).
+   - snapshots: state slots besides the initial state (which is read from
+     the caller's arrays); a negative value takes as many as the free device
+     memory holds, keeping 4 GiB.  The schedule is the binomial one
+     (Griewank-Walther): the positions minimising the forward steps
+     recomputed, found exactly by dynamic programming, so nsteps steps
+     reverse in memory for `snapshots` states.  With mixed = 0 the
+     gradient is vk_adjoint_gradient's, bit for bit.
+   - mixed != 0: the forward trajectory is vk_time_step_mixed's (T, the
+     history, u_ta, cellvalue, rhoCp, thc in float; snapshots 3 double
+     and 8 float fields); J is that trajectory's, and the backward pass is
+     the FP64 adjoint linearised about it.
+   - stats (may be NULL): 4 doubles -- snapshots used, forward steps run
+     (the objective pass included), the schedule's recomputed steps, device
+     bytes the call held. */
+int vk_adjoint_gradient_revolve(
+    const double *u, const double *v, const double *w, const double *p, const double *t,
+    const double *u_pre, const double *v_pre, const double *w_pre, const double *t_pre,
+    const double *u_ta, const double *v_ta, const double *w_ta, const double *cellvalue,
+    const double *rhocp, const double *thc, const double *dx, const double *dy, const double *dz,
+    const double *dx12, const double *dy12, const double *dz12, const int *nx, const int *ny,
+    const int *nz, const double *dt, const double *re, const double *sc, const double *eta,
+    const double *xl, const double *zl, const int *nsteps, const int *snapshots,
+    const int *mixed, double *objective, double *grad, double *stats);
+
 /* dst := src over a whole halo field, or over its interior when
    *interior_only is non-zero: the inline array copies of program main
    (u_pre := u at nse_dominik.f:246-257, u_temp := u_a at :407-417,

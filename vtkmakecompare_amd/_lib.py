@@ -27,6 +27,7 @@ EXPORTS = [
     "vk_phibc", "vk_set_levelset", "vk_levelset_shapes", "vk_set_adj_convect",
     "vk_set_adj_convection_for_scalar", "vk_set_adj_ns_scalar",
     "vk_time_step", "vk_time_step_mixed", "vk_objective", "vk_adjoint_gradient",
+    "vk_adjoint_gradient_revolve",
     "vk_field_absmax", "vk_field_copy", "vk_check_courant", "vk_check_courant_noalloc",
     "vk_monitor",
     "vk_check_convergence", "vk_readgrid3d", "vk_outfield", "vk_readfield",

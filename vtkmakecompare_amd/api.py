@@ -440,7 +440,13 @@ class AdjointProblem:
                  nsteps: int):
         init()
         self.cfg, self.g, self.nsteps = cfg, g, nsteps
-        self.d = {n: DeviceArray.from_host(state[n]) for n in ADJ_FIELDS + ("rhocp", "thc")}
+        self.d = {}
+        for n in ADJ_FIELDS + ("rhocp", "thc"):
+            if state.get(n) is None:  # a lean state: the field starts at 0
+                self.d[n] = DeviceArray(g.shape)
+                self.d[n].fill_zero()
+            else:
+                self.d[n] = DeviceArray.from_host(state[n])
         for n in ("dx", "dy", "dz", "dx12", "dy12", "dz12"):
             self.d[n] = DeviceArray.from_host(getattr(g, n))
         self.cv = DeviceArray.from_host(np.asarray(cellvalue, dtype=np.float64))
@@ -462,6 +468,25 @@ class AdjointProblem:
         out = ctypes.c_double(0.0)
         check(lib().vk_objective(*self._args(cv), ctypes.byref(out)), "objective")
         return out.value
+
+    def gradient_revolve(self, snapshots: int = -1, mixed: bool = False):
+        """(J, dJ/dcellvalue(...,1), stats) by binomial checkpointing with
+        `snapshots` state slots (-1: as many as fit), about the FP64 or the
+        mixed-precision trajectory (vk_adjoint_gradient_revolve).  stats:
+        snapshots, forward steps run, recomputed steps, device bytes held."""
+        import ctypes
+        g = self.g
+        grad = DeviceArray((g.nz, g.ny, g.nx))
+        out = ctypes.c_double(0.0)
+        stats = (ctypes.c_double * 4)()
+        check(lib().vk_adjoint_gradient_revolve(*self._args(self.cv), I(snapshots),
+                                                I(1 if mixed else 0), ctypes.byref(out), grad,
+                                                stats), "adjoint_gradient_revolve")
+        g_host = grad.download()
+        grad.free()
+        st = {"snapshots": int(stats[0]), "forward_steps": int(stats[1]),
+              "recomputed_steps": int(stats[2]), "device_bytes": int(stats[3])}
+        return out.value, g_host, st
 
     def gradient(self, checkpoint_every: int = 20):
         """(J, dJ/dcellvalue(...,1)) with the gradient as an (nz, ny, nx) array."""

@@ -28,7 +28,11 @@
 //                                    modes and the transposed SP0B march
 //                                    on the zero mode
 //   cellvalue faces (step 1)         faces^T into slot 1
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "vk_adjoint.h"
@@ -264,12 +268,26 @@ __global__ void k_adj_explicit_weights(Geom g, StepIn in, AdjIn a, AdjOut out, W
 // J = dt sum_steps sum_(i,k) dx dz [2 T(i,1,k)/dy12(0) + 2 T(i,ny,k)/dy12(ny)]:
 // the wall heat flux through both walls (T's wall ghosts are the
 // reflections -T), integrated over the walls and over time.
-__global__ void k_objective_acc(Geom g, Metrics m, const double* t, double dt, double* acc) {
+template <typename TT>
+__global__ void k_objective_acc(Geom g, Metrics m, const TT* t, double dt, double* acc) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= g.nx * g.nz) return;
   const int i = 1 + q % g.nx, k = 1 + q / g.nx;
   const double w = dt * m.dx[i] * m.dz[k] * 2.0;
-  acc[q] += w * (t[g.at(i, 1, k)] / m.dy12[0] + t[g.at(i, g.ny, k)] / m.dy12[g.ny]);
+  acc[q] += w * ((double)t[g.at(i, 1, k)] / m.dy12[0] + (double)t[g.at(i, g.ny, k)] / m.dy12[g.ny]);
+}
+
+__global__ void k_copy_cells_f(long long n, const float* src, float* dst) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) dst[q] = src[q];
+}
+__global__ void k_to_double(long long n, const float* src, double* dst) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) dst[q] = (double)src[q];
+}
+__global__ void k_to_float(long long n, const double* src, float* dst) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) dst[q] = (float)src[q];
 }
 
 // its gradient with respect to T of one step, added to lambda_T
@@ -321,11 +339,18 @@ struct Dev {
   ~Dev() {
     for (void* p : ptrs) hipFree(p);
   }
-  double* alloc(size_t n) {
+  size_t bytes = 0;
+  double* alloc(size_t n) { return static_cast<double*>(raw(n * sizeof(double))); }
+  float* allocf(size_t n) { return static_cast<float*>(raw(n * sizeof(float))); }
+  void* raw(size_t b) {
     void* p = nullptr;
-    if (hipMalloc(&p, n * sizeof(double)) != hipSuccess) return nullptr;
+    if (hipMalloc(&p, b) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
     ptrs.push_back(p);
-    return static_cast<double*>(p);
+    bytes += b;
+    return p;
   }
 };
 
@@ -361,7 +386,7 @@ int forward_step(const Problem& P, Work& w, int l) {
 
 void objective_acc(const Problem& P, const Work& w) {
   const int n = P.g.nx * P.g.nz;
-  k_objective_acc<<<(n + 255) / 256, 256, 0, S()>>>(P.g, P.m, w.st[ST], P.dt, w.jacc);
+  k_objective_acc<double><<<(n + 255) / 256, 256, 0, S()>>>(P.g, P.m, w.st[ST], P.dt, w.jacc);
 }
 
 int sum_objective(const Geom& g, const double* jacc, double* out) {
@@ -613,6 +638,213 @@ int init_work(const Problem& P, const Inputs& in, Dev& dev, Work& w) {
   return rc;
 }
 
+// ------------------------------------------------- binomial checkpointing
+//
+// The backward pass needs the state before each step, in reverse order.
+// With s snapshot slots beyond the initial state, reversing n steps costs
+// cost(n, s) forward steps, the minimum over the first snapshot position m
+// of m + cost(n - m, s - 1) + cost(m, s), with cost(1, s) = 0 and
+// cost(n, 0) = n (n - 1) / 2 (every state recomputed from the range's
+// start).  The recursion is the binomial checkpointing of Griewank and
+// Walther (ACM TOMS 26, 2000): its optimum reverses C(s + r, s) steps with
+// r forward sweeps.  Solved exactly by dynamic programming on the host.
+struct Schedule {
+  int n = 0, s = 0;
+  std::vector<long long> cost;
+  std::vector<int> split;
+  long long& C(int len, int sl) { return cost[(size_t)len * (s + 1) + sl]; }
+  int& M(int len, int sl) { return split[(size_t)len * (s + 1) + sl]; }
+  void build(int n_, int s_) {
+    n = n_;
+    s = s_;
+    cost.assign((size_t)(n + 1) * (s + 1), 0);
+    split.assign((size_t)(n + 1) * (s + 1), 0);
+    for (int len = 1; len <= n; len++) {
+      C(len, 0) = (long long)len * (len - 1) / 2;
+      for (int sl = 1; sl <= s; sl++) {
+        if (len == 1) {
+          C(len, sl) = 0;
+          continue;
+        }
+        long long best = -1;
+        int bm = 1;
+        for (int m = 1; m < len; m++) {
+          const long long c = m + C(len - m, sl - 1) + C(m, sl);
+          if (best < 0 || c < best) {
+            best = c;
+            bm = m;
+          }
+        }
+        C(len, sl) = best;
+        M(len, sl) = bm;
+      }
+    }
+  }
+};
+
+// The forward trajectory the backward pass revisits.  FP64: the state of
+// vk_time_step.  Mixed: the state of vk_time_step_mixed -- u, v, w (and
+// p, u_temp.., divu) in FP64; T, the AB2 history and u_ta in float, with
+// float copies of cellvalue, rhoCp and thc -- about which the backward
+// pass linearises the FP64 step.  Snapshots hold the 11 state fields
+// (mixed: 3 double and 8 float); the initial state is read from the
+// caller's arrays, so it takes no slot.
+enum { MT, MUP, MVP, MWP, MTP, MUTA, MVTA, MWTA, kMixedF };
+
+struct Traj {
+  const Problem* P = nullptr;
+  const Inputs* in = nullptr;
+  bool mixed = false;
+  Work* w = nullptr;  // the FP64 state (the trajectory's own when FP64)
+  // the mixed trajectory's state
+  double *mu = nullptr, *mv = nullptr, *mw = nullptr, *mp = nullptr, *mut = nullptr,
+         *mvt = nullptr, *mwt = nullptr, *mdiv = nullptr;
+  float* mf[kMixedF] = {};
+  float *cvf = nullptr, *rcf = nullptr, *thcf = nullptr;
+  std::vector<void*> snap;  // slot q: kState consecutive pointers
+  long long steps_run = 0;
+
+  size_t nc() const { return P->g.cells(); }
+  void copyd(const double* a, double* b) const { copy(P->g, a, b); }
+  void copyf(const float* a, float* b) const {
+    const long long n = (long long)nc();
+    k_copy_cells_f<<<(unsigned)((n + 255) / 256), 256, 0, S()>>>(n, a, b);
+  }
+  void tod(const float* a, double* b, long long n) const {
+    k_to_double<<<(unsigned)((n + 255) / 256), 256, 0, S()>>>(n, a, b);
+  }
+  void tof(const double* a, float* b, long long n) const {
+    k_to_float<<<(unsigned)((n + 255) / 256), 256, 0, S()>>>(n, a, b);
+  }
+  // bytes of one snapshot
+  size_t snap_bytes() const {
+    return mixed ? nc() * (3 * sizeof(double) + kMixedF * sizeof(float))
+                 : nc() * kState * sizeof(double);
+  }
+  int alloc(Dev& dev, int slots) {
+    if (mixed) {
+      double** d[8] = {&mu, &mv, &mw, &mp, &mut, &mvt, &mwt, &mdiv};
+      for (auto q : d)
+        if (!(*q = dev.alloc(nc()))) return fail(VK_ERR_HIP, "adjoint: out of device memory");
+      for (auto& q : mf)
+        if (!(q = dev.allocf(nc()))) return fail(VK_ERR_HIP, "adjoint: out of device memory");
+      cvf = dev.allocf(P->g.interior() * 4);
+      rcf = dev.allocf(nc());
+      thcf = dev.allocf(nc());
+      if (!cvf || !rcf || !thcf) return fail(VK_ERR_HIP, "adjoint: out of device memory");
+    }
+    snap.assign((size_t)slots * kState, nullptr);
+    for (int q = 0; q < slots; q++)
+      for (int f = 0; f < kState; f++) {
+        const bool dbl = !mixed || f < 3;
+        void* p = dbl ? (void*)dev.alloc(nc()) : (void*)dev.allocf(nc());
+        if (!p) return fail(VK_ERR_HIP, "adjoint: out of device memory for the snapshots");
+        snap[(size_t)q * kState + f] = p;
+      }
+    return VK_OK;
+  }
+  // the trajectory state := the caller's initial state
+  void load_initial() {
+    const double* src[kState] = {in->u,  in->v,  in->w,   in->t,   in->up, in->vp,
+                                 in->wp, in->tp, in->uta, in->vta, in->wta};
+    if (!mixed) {
+      for (int f = 0; f < kState; f++) copyd(src[f], w->st[f]);
+      copyd(in->p, w->p);
+      return;
+    }
+    copyd(in->u, mu);
+    copyd(in->v, mv);
+    copyd(in->w, mw);
+    copyd(in->p, mp);
+    for (int f = 3; f < kState; f++) tof(src[f], mf[f - 3], (long long)nc());
+  }
+  // state arrays of the trajectory, field f of kState
+  void* field(int f) {
+    if (!mixed) return w->st[f];
+    if (f < 3) return f == 0 ? (void*)mu : f == 1 ? (void*)mv : (void*)mw;
+    return mf[f - 3];
+  }
+  void save(int slot) {
+    for (int f = 0; f < kState; f++) {
+      void* d = snap[(size_t)slot * kState + f];
+      if (!mixed || f < 3)
+        copyd(static_cast<double*>(field(f)), static_cast<double*>(d));
+      else
+        copyf(static_cast<float*>(field(f)), static_cast<float*>(d));
+    }
+  }
+  void load(int slot) {
+    if (slot < 0) return load_initial();
+    for (int f = 0; f < kState; f++) {
+      const void* d = snap[(size_t)slot * kState + f];
+      if (!mixed || f < 3)
+        copyd(static_cast<const double*>(d), static_cast<double*>(field(f)));
+      else
+        copyf(static_cast<const float*>(d), static_cast<float*>(field(f)));
+    }
+  }
+  // one step of the trajectory (state before step l -> after it)
+  int advance(int l) {
+    steps_run++;
+    if (!mixed) return forward_step(*P, *w, l);
+    const Problem& Q = *P;
+    const Geom& g = Q.g;
+    return vk_time_step_mixed(mu, mv, mw, mp, mf[MT], mf[MUP], mf[MVP], mf[MWP], mf[MTP],
+                              mf[MUTA], mf[MVTA], mf[MWTA], mut, mvt, mwt, mdiv, cvf, rcf, thcf,
+                              Q.m.dx, Q.m.dy, Q.m.dz, Q.m.dx12, Q.m.dy12, Q.m.dz12, &g.nx, &g.ny,
+                              &g.nz, &Q.dt, &Q.re, &Q.sc, &Q.eta, &Q.xl, &Q.zl, &l);
+  }
+  void objective_acc() {
+    const int n = P->g.nx * P->g.nz;
+    if (!mixed)
+      k_objective_acc<double><<<(n + 255) / 256, 256, 0, S()>>>(P->g, P->m, w->st[ST], P->dt,
+                                                                w->jacc);
+    else
+      k_objective_acc<float><<<(n + 255) / 256, 256, 0, S()>>>(P->g, P->m, mf[MT], P->dt,
+                                                               w->jacc);
+  }
+  // the FP64 work state := the trajectory state (the linearisation point
+  // of the taped step); the input u_ta kept in uta[3]
+  void to_work(double* const* uta) {
+    if (mixed) {
+      copyd(mu, w->st[SU]);
+      copyd(mv, w->st[SV]);
+      copyd(mw, w->st[SW]);
+      copyd(mp, w->p);
+      for (int f = 3; f < kState; f++) tod(mf[f - 3], w->st[f], (long long)nc());
+    }
+    for (int c = 0; c < 3; c++) copyd(w->st[SUTA + c], uta[c]);
+  }
+};
+
+// Runs the schedule: reverses steps a+1..a+n given the state after step a
+// in `slot` (-1: the initial state) and the free slots from `next` on.
+// `adjoint(l)` runs the backward step l from the trajectory state before it.
+template <typename ADJ>
+int reverse(Traj& T, Schedule& sc, int a, int n, int slot, int next, int free_slots, ADJ& adjoint) {
+  int rc;
+  if (n == 1) {
+    T.load(slot);
+    return adjoint(a + 1);
+  }
+  if (free_slots == 0) {
+    for (int i = n - 1; i >= 0; i--) {
+      T.load(slot);
+      for (int l = a + 1; l <= a + i; l++)
+        if ((rc = T.advance(l))) return rc;
+      if ((rc = adjoint(a + i + 1))) return rc;
+    }
+    return VK_OK;
+  }
+  const int m = sc.M(n, free_slots);
+  T.load(slot);
+  for (int l = a + 1; l <= a + m; l++)
+    if ((rc = T.advance(l))) return rc;
+  T.save(next);
+  if ((rc = reverse(T, sc, a + m, n - m, next, next + 1, free_slots - 1, adjoint))) return rc;
+  return reverse(T, sc, a, m, slot, next, free_slots, adjoint);
+}
+
 }  // namespace
 
 extern "C" {
@@ -774,6 +1006,174 @@ int vk_adjoint_gradient(const double* u, const double* v, const double* w, const
   rc = hip_check(hipStreamSynchronize(S()), "adjoint");
   if (rc) return rc;
   VK_LAUNCH_CHECK("adjoint_gradient");
+  return VK_OK;
+}
+
+int vk_adjoint_gradient_revolve(
+    const double* u, const double* v, const double* w, const double* p, const double* t,
+    const double* u_pre, const double* v_pre, const double* w_pre, const double* t_pre,
+    const double* u_ta, const double* v_ta, const double* w_ta, const double* cellvalue,
+    const double* rhocp, const double* thc, const double* dx, const double* dy, const double* dz,
+    const double* dx12, const double* dy12, const double* dz12, const int* nx, const int* ny,
+    const int* nz, const double* dt, const double* re, const double* sc, const double* eta,
+    const double* xl, const double* zl, const int* nsteps, const int* snapshots,
+    const int* mixed, double* objective, double* grad, double* stats) {
+  VK_REQUIRE_INIT();
+  const int N = *nsteps;
+  if (N < 1) return fail(VK_ERR_ARGUMENT, "adjoint: nsteps >= 1");
+  Problem P{Geom(*nx, *ny, *nz), Metrics{}, *dt, *re, *sc, *eta, *xl, *zl, nullptr, nullptr,
+            nullptr};
+  const Geom& g = P.g;
+  if (g.nx < 4 || g.ny < 4 || g.nz < 4) return fail(VK_ERR_BAD_SIZE, "adjoint: n >= 4");
+  if (!g.fits32()) return fail(VK_ERR_BAD_SIZE, "adjoint: grid too large");
+  if (int frc = fft_sizes_ok(g)) return frc;
+  Inputs in;
+  int rc = resolve_problem(P, in, u, v, w, p, t, u_pre, v_pre, w_pre, t_pre, u_ta, v_ta, w_ta,
+                           cellvalue, rhocp, thc, dx, dy, dz, dx12, dy12, dz12);
+  if (rc) return rc;
+  double* dgrad = static_cast<double*>(resolve(grad, g.interior() * sizeof(double), "grad"));
+  if (!dgrad) return last_status();
+  Dev dev;
+  const size_t nc = g.cells();
+  P.cv = dev.alloc(g.interior() * 4);
+  if (!P.cv) return fail(VK_ERR_HIP, "adjoint: out of device memory");
+  rc = hip_check(hipMemcpyAsync(P.cv, in.cv, g.interior() * 4 * sizeof(double),
+                                hipMemcpyDeviceToDevice, S()),
+                 "cellvalue copy");
+  if (rc) return rc;
+  Work wk;
+  rc = init_work(P, in, dev, wk);
+  if (rc) return rc;
+  // the backward step's buffers
+  double* tape[16];
+  double* in4[4];
+  double* uta[3];
+  for (auto& q : tape)
+    if (!(q = dev.alloc(nc))) return fail(VK_ERR_HIP, "adjoint: out of device memory");
+  for (auto& q : in4)
+    if (!(q = dev.alloc(nc))) return fail(VK_ERR_HIP, "adjoint: out of device memory");
+  for (auto& q : uta)
+    if (!(q = dev.alloc(nc))) return fail(VK_ERR_HIP, "adjoint: out of device memory");
+  AdjBufs B;
+  for (int f = 0; f < kState; f++) {
+    B.lam[f] = dev.alloc(nc);
+    B.nxt[f] = dev.alloc(nc);
+    if (!B.lam[f] || !B.nxt[f]) return fail(VK_ERR_HIP, "adjoint: out of device memory");
+  }
+  for (int c = 0; c < 3; c++) {
+    B.lut[c] = dev.alloc(nc);
+    B.lg[c] = dev.alloc(nc);
+  }
+  B.lphi = dev.alloc(nc);
+  B.ysol = dev.alloc(nc);
+  B.spec = dev.alloc(2 * g.spec_cells());
+  for (auto& q : B.w8) q = dev.alloc(nc);
+  B.coef = dev.alloc((size_t)(g.nx / 2 + 1) * (g.nz + 1) + 3 * (size_t)g.ny + 8);
+  B.gcv = dev.alloc(g.interior() * 4);
+  if (!B.lut[0] || !B.lut[1] || !B.lut[2] || !B.lg[0] || !B.lg[1] || !B.lg[2] || !B.lphi ||
+      !B.ysol || !B.spec || !B.w8[7] || !B.coef || !B.gcv)
+    return fail(VK_ERR_HIP, "adjoint: out of device memory");
+  Traj T;
+  T.P = &P;
+  T.in = &in;
+  T.mixed = *mixed != 0;
+  T.w = &wk;
+  rc = T.alloc(dev, 0);
+  if (rc) return rc;
+  if (T.mixed) {
+    T.tof(P.cv, T.cvf, (long long)g.interior() * 4);
+    T.tof(in.rc, T.rcf, (long long)nc);
+    T.tof(in.thc, T.thcf, (long long)nc);
+  }
+  const double* uta_in[3] = {uta[0], uta[1], uta[2]};
+  const int nJ = g.nx * g.nz;
+  auto zero = [&](double* a, size_t n) {
+    return hip_check(hipMemsetAsync(a, 0, n * sizeof(double), S()), "memset");
+  };
+  // with VK_ADJOINT_PROGRESS set in the environment, a line on stderr
+  // every twentieth of the backward pass (after the work so far has run)
+  const bool progress = std::getenv("VK_ADJOINT_PROGRESS") != nullptr;
+  const int every = std::max(1, N / 20);
+  int done = 0;
+  auto report = [&](const char* what, int k) -> int {
+    int r = hip_check(hipStreamSynchronize(S()), "adjoint progress");
+    if (r) return r;
+    std::fprintf(stderr, "[adjoint] %s %d of %d (forward steps run %lld)\n", what, k, N,
+                 T.steps_run);
+    std::fflush(stderr);
+    return VK_OK;
+  };
+  auto adjoint = [&](int l) -> int {
+    T.to_work(uta);
+    LineFlags lf{};
+    int r = taped_step(P, wk, l, tape, in4, &lf);
+    if (r) return r;
+    r = adjoint_step(P, B, l, tape, in4, uta_in, &lf);
+    if (r) return r;
+    if (l > 1) k_objective_src<<<(nJ + 255) / 256, 256, 0, S()>>>(g, P.m, P.dt, B.lam[ST]);
+    done++;
+    if (progress && done % every == 0) return report("backward step", done);
+    return VK_OK;
+  };
+  // one step of each kind first, so that every workspace the steps grow
+  // exists before the snapshots take what memory is left
+  T.load_initial();
+  if ((rc = T.advance(1))) return rc;
+  T.load_initial();
+  for (int f = 0; f < kState; f++)
+    if ((rc = zero(B.lam[f], nc))) return rc;
+  if ((rc = zero(B.gcv, g.interior() * 4))) return rc;
+  if ((rc = adjoint(1))) return rc;
+  T.steps_run = 0;
+  // the snapshots: as asked, or as many as the free memory holds (keeping
+  // 4 GiB), never more than the schedule can use
+  int s = *snapshots;
+  if (s < 0) {
+    size_t fr = 0, tot = 0;
+    if ((rc = hip_check(hipMemGetInfo(&fr, &tot), "hipMemGetInfo"))) return rc;
+    const size_t keep = (size_t)4 << 30;
+    s = fr > keep ? (int)std::min<size_t>((fr - keep) / T.snap_bytes(), 1u << 20) : 0;
+  }
+  s = std::min(s, std::max(N - 1, 0));
+  {  // the snapshot slots
+    T.snap.assign((size_t)s * kState, nullptr);
+    for (int q = 0; q < s; q++)
+      for (int f = 0; f < kState; f++) {
+        const bool dbl = !T.mixed || f < 3;
+        void* pp = dbl ? (void*)dev.alloc(nc) : (void*)dev.allocf(nc);
+        if (!pp)
+          return fail(VK_ERR_HIP, "adjoint: out of device memory for " + std::to_string(s) +
+                                      " snapshots");
+        T.snap[(size_t)q * kState + f] = pp;
+      }
+  }
+  Schedule sch;
+  sch.build(N, s);
+  // the objective over the whole trajectory
+  T.load_initial();
+  if ((rc = zero(wk.jacc, (size_t)nJ))) return rc;
+  for (int l = 1; l <= N; l++) {
+    if ((rc = T.advance(l))) return rc;
+    T.objective_acc();
+    if (progress && l % every == 0 && (rc = report("objective step", l))) return rc;
+  }
+  if ((rc = sum_objective(g, wk.jacc, objective))) return rc;
+  // backward
+  for (int f = 0; f < kState; f++)
+    if ((rc = zero(B.lam[f], nc))) return rc;
+  if ((rc = zero(B.gcv, g.interior() * 4))) return rc;
+  k_objective_src<<<(nJ + 255) / 256, 256, 0, S()>>>(g, P.m, P.dt, B.lam[ST]);
+  if ((rc = reverse(T, sch, 0, N, -1, 0, s, adjoint))) return rc;
+  k_adj_cv_faces<<<interior_grid(g), interior_block(), 0, S()>>>(g, B.gcv, dgrad);
+  rc = hip_check(hipStreamSynchronize(S()), "adjoint");
+  if (rc) return rc;
+  VK_LAUNCH_CHECK("adjoint_gradient_revolve");
+  if (stats) {
+    stats[0] = s;
+    stats[1] = (double)T.steps_run;
+    stats[2] = (double)sch.C(N, s);
+    stats[3] = (double)dev.bytes;
+  }
   return VK_OK;
 }
 
