@@ -214,6 +214,37 @@ int vk_implicit_ty_batched(double *sty, const int *ny, const double *dy, const d
                            const double *fb, const double *cuy, const double *eta,
                            const int *nsys);
 
+/* -------------------------------------------- iterative Poisson solver
+   The reference's alternative to MKP's spectral solve, which it keeps but
+   does not call.  Both keep the reference's arithmetic bit for bit:
+   line Gauss-Seidel in the reference's loop order (lines of one
+   anti-diagonal of the two outer loop indices solved together), each line
+   with its last row's metrics on every row, zero-gradient halos on all six
+   faces, p shifted so that p(1,1,1) = 0.  The reference declares its line
+   buffers fpy, fpz with nx+2 elements (undefined for ny or nz above nx+1);
+   here they hold whole lines. */
+
+/* linebyline(nx,ny,nz,u_temp,v_temp,w_temp,dx,dy,dz,dx12,dy12,dz12,dt,p),
+   nse_dominik.f:2136-2267 with implicit_px/py/pz (:2270-2457): one
+   iteration on p for L p = div(u_temp)/dt */
+int vk_linebyline(const int *nx, const int *ny, const int *nz, const double *u_temp,
+                  const double *v_temp, const double *w_temp, const double *dx, const double *dy,
+                  const double *dz, const double *dx12, const double *dy12, const double *dz12,
+                  const double *dt, double *p);
+/* solve_p(p,u_temp,v_temp,w_temp,dt,dx,dy,dz,dx12,dy12,dz12,nx,ny,nz,res,
+   res_max,res_int,res_var), nse_dominik.f:1955-2021, plus two trailing
+   arguments: linebyline until the largest residual |L p dt - div u_temp|
+   is below 1e-5 or max_iterations iterations (NULL: the reference's
+   100000); iterations (may be NULL) returns the count.  res is the last
+   interior point's residual, res_int and res_var the volume integrals of
+   the residual and its square (summed in a fixed tree order: equal to the
+   reference's serial sums to rounding); res_max is exact. */
+int vk_solve_p(double *p, const double *u_temp, const double *v_temp, const double *w_temp,
+               const double *dt, const double *dx, const double *dy, const double *dz,
+               const double *dx12, const double *dy12, const double *dz12, const int *nx,
+               const int *ny, const int *nz, double *res, double *res_max, double *res_int,
+               double *res_var, const int *max_iterations, int *iterations);
+
 /* ------------------------------------------------- pressure projection */
 /* MKP, mkp.f:9-220 (ompnthreads is accepted and ignored, as there) */
 int vk_mkp(double *p, const double *u_temp, const double *v_temp,

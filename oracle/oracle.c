@@ -1670,3 +1670,147 @@ void o_levelset_shapes(int nx, int ny, int nz, const double *x, const double *y,
         phi0[o] = best;
       }
 }
+
+/* ------------------------------------------- iterative Poisson (dead) */
+
+/* implicit_px/py/pz (nse_dominik.f:2270-2457): rows 0..n+1 of the dense
+   matrix the reference builds, kept as its three bands -- ghost rows
+   p0 - p1 = ft, p(n+1) - p(n) = fb, interior rows (blo, -bsum, bhi) --
+   eliminated as the reference eliminates them; f(1..n) in/out. */
+static void p_line(double *f, int n, double blo, double bhi, double bsum,
+                   double ft, double fb) {
+  int m = n + 1;
+  double *lo = calloc((size_t)4 * (m + 1), sizeof(double));
+  double *di = lo + (m + 1), *up = di + (m + 1), *u = up + (m + 1);
+  di[0] = 1.0;
+  up[0] = -1.0;
+  di[n + 1] = 1.0;
+  lo[n + 1] = -1.0;
+  for (int i = 1; i <= n; i++) {
+    lo[i] = blo;
+    di[i] = -(bsum);
+    up[i] = bhi;
+  }
+  for (int i = 1; i <= n; i++) u[i] = f[i];
+  u[0] = ft;
+  u[n + 1] = fb;
+  ul_eliminate(lo, di, up, u, m);
+  for (int i = 1; i <= n; i++) f[i] = u[i];
+  free(lo);
+}
+
+void o_linebyline(int nx, int ny, int nz, const double *u_temp,
+                  const double *v_temp, const double *w_temp, const double *dx,
+                  const double *dy, const double *dz, const double *dx12,
+                  const double *dy12, const double *dz12, double dt, double *p) {
+  GEOM;
+  const int nm = (nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz)) + 2;
+  double *q = calloc((size_t)(nx + 2) * (ny + 2) * (nz + 2), sizeof(double));
+  double *f = calloc((size_t)nm, sizeof(double));
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++)
+        FIELD(q, i, j, k) = ((FIELD(u_temp, i, j, k) - FIELD(u_temp, i - 1, j, k)) / dx[i] +
+                             (FIELD(v_temp, i, j, k) - FIELD(v_temp, i, j - 1, k)) / dy[j] +
+                             (FIELD(w_temp, i, j, k) - FIELD(w_temp, i, j, k - 1)) / dz[k]) /
+                            dt;
+  double bx1 = 0, bx2 = 0, by1 = 0, by2 = 0, bz1 = 0, bz2 = 0;
+#define PB(i, j, k)                        \
+  bx1 = 1.0 / dx12[i] / dx[i];             \
+  bx2 = 1.0 / dx12[(i) - 1] / dx[i];       \
+  by1 = 1.0 / dy12[j] / dy[j];             \
+  by2 = 1.0 / dy12[(j) - 1] / dy[j];       \
+  bz1 = 1.0 / dz12[k] / dz[k];             \
+  bz2 = 1.0 / dz12[(k) - 1] / dz[k]
+#define BSUM (bx1 + bx2 + by1 + by2 + bz1 + bz2)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++) {
+      for (int i = 1; i <= nx; i++) {
+        PB(i, j, k);
+        f[i] = -by1 * FIELD(p, i, j + 1, k) - by2 * FIELD(p, i, j - 1, k) -
+               bz1 * FIELD(p, i, j, k + 1) - bz2 * FIELD(p, i, j, k - 1) + FIELD(q, i, j, k);
+      }
+      p_line(f, nx, bx2, bx1, BSUM, 0.0, 0.0);
+      for (int i = 1; i <= nx; i++) FIELD(p, i, j, k) = f[i];
+    }
+  for (int k = 1; k <= nz; k++)
+    for (int i = 1; i <= nx; i++) {
+      for (int j = 1; j <= ny; j++) {
+        PB(i, j, k);
+        f[j] = -bx1 * FIELD(p, i + 1, j, k) - bx2 * FIELD(p, i - 1, j, k) -
+               bz1 * FIELD(p, i, j, k + 1) - bz2 * FIELD(p, i, j, k - 1) + FIELD(q, i, j, k);
+      }
+      p_line(f, ny, by2, by1, BSUM, 0.0, 0.0);
+      for (int j = 1; j <= ny; j++) FIELD(p, i, j, k) = f[j];
+    }
+  for (int j = 1; j <= ny; j++)
+    for (int i = 1; i <= nx; i++) {
+      for (int k = 1; k <= nz; k++) {
+        PB(i, j, k);
+        f[k] = -bx1 * FIELD(p, i + 1, j, k) - bx2 * FIELD(p, i - 1, j, k) -
+               by1 * FIELD(p, i, j + 1, k) - by2 * FIELD(p, i, j - 1, k) + FIELD(q, i, j, k);
+      }
+      p_line(f, nz, bz2, bz1, BSUM, 0.0, 0.0);
+      for (int k = 1; k <= nz; k++) FIELD(p, i, j, k) = f[k];
+    }
+#undef PB
+#undef BSUM
+  for (int j = 0; j <= ny + 1; j++)
+    for (int i = 0; i <= nx + 1; i++) {
+      FIELD(p, i, j, 0) = FIELD(p, i, j, 1);
+      FIELD(p, i, j, nz + 1) = FIELD(p, i, j, nz);
+    }
+  for (int k = 0; k <= nz + 1; k++)
+    for (int j = 0; j <= ny + 1; j++) {
+      FIELD(p, 0, j, k) = FIELD(p, 1, j, k);
+      FIELD(p, nx + 1, j, k) = FIELD(p, nx, j, k);
+    }
+  for (int k = 0; k <= nz + 1; k++)
+    for (int i = 0; i <= nx + 1; i++) {
+      FIELD(p, i, 0, k) = FIELD(p, i, 1, k);
+      FIELD(p, i, ny + 1, k) = FIELD(p, i, ny, k);
+    }
+  const double p_st = FIELD(p, 1, 1, 1);
+  for (int k = 0; k <= nz + 1; k++)
+    for (int j = 0; j <= ny + 1; j++)
+      for (int i = 0; i <= nx + 1; i++) FIELD(p, i, j, k) = FIELD(p, i, j, k) - p_st;
+  free(q);
+  free(f);
+}
+
+void o_solve_p(double *p, const double *u_temp, const double *v_temp,
+               const double *w_temp, double dt, const double *dx,
+               const double *dy, const double *dz, const double *dx12,
+               const double *dy12, const double *dz12, int nx, int ny, int nz,
+               double *res, double *res_max, double *res_int, double *res_var,
+               int max_iter, int *iters) {
+  GEOM;
+  *iters = 0;
+  for (int lll = 1; lll <= max_iter; lll++) {
+    o_linebyline(nx, ny, nz, u_temp, v_temp, w_temp, dx, dy, dz, dx12, dy12, dz12, dt, p);
+    *iters = lll;
+    *res = 0.0;
+    *res_max = 0.0;
+    *res_int = 0.0;
+    *res_var = 0.0;
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+          const double div = (FIELD(u_temp, i, j, k) - FIELD(u_temp, i - 1, j, k)) / dx[i] +
+                             (FIELD(v_temp, i, j, k) - FIELD(v_temp, i, j - 1, k)) / dy[j] +
+                             (FIELD(w_temp, i, j, k) - FIELD(w_temp, i, j, k - 1)) / dz[k];
+          const double pphi =
+              ((FIELD(p, i + 1, j, k) - FIELD(p, i, j, k)) / dx12[i] -
+               (FIELD(p, i, j, k) - FIELD(p, i - 1, j, k)) / dx12[i - 1]) / dx[i] +
+              ((FIELD(p, i, j + 1, k) - FIELD(p, i, j, k)) / dy12[j] -
+               (FIELD(p, i, j, k) - FIELD(p, i, j - 1, k)) / dy12[j - 1]) / dy[j] +
+              ((FIELD(p, i, j, k + 1) - FIELD(p, i, j, k)) / dz12[k] -
+               (FIELD(p, i, j, k) - FIELD(p, i, j, k - 1)) / dz12[k - 1]) / dz[k];
+          *res = fabs(pphi * dt - div);
+          if (*res >= *res_max) *res_max = *res;
+          *res_int = *res_int + (pphi * dt - div) * dx[i] * dy[j] * dz[k];
+          *res_var = *res_var + (*res * *res) * dx[i] * dy[j] * dz[k];
+        }
+    if (*res_max < 1.0e-005) break;
+  }
+}

@@ -44,6 +44,31 @@ void o_levelset_shapes(int nx, int ny, int nz, const double *x, const double *y,
                        const double *z, double xl, double zl, int nshapes,
                        const double *shapes, double *phi0);
 
+/* The iterative Poisson solver the reference keeps but does not call.
+ * nse_dominik.f:2136-2267 (linebyline) with implicit_px/py/pz
+ * (:2270-2457): one line Gauss-Seidel iteration on p for
+ * L p = div(u_temp)/dt -- x lines in (k, j) order, then y lines in (k, i)
+ * order, then z lines in (j, i) order, each line solved with its last
+ * row's metrics for every row -- then zero-gradient halos on all six faces
+ * and p shifted so that p(1,1,1) = 0.  The reference declares fpy and fpz
+ * with nx+2 elements but fills ny+2 and nz+2 of them (undefined past nx+1);
+ * here they are as long as they need to be. */
+void o_linebyline(int nx, int ny, int nz, const double *u_temp,
+                  const double *v_temp, const double *w_temp, const double *dx,
+                  const double *dy, const double *dz, const double *dx12,
+                  const double *dy12, const double *dz12, double dt, double *p);
+/* nse_dominik.f:1955-2021 (solve_p): linebyline until the largest residual
+ * |L p dt - div u_temp| is below 1e-5 or max_iter iterations (the
+ * reference's nnn = 100000); res is the last interior point's residual,
+ * res_int and res_var the volume integrals of the residual and of its
+ * square; *iters the iterations run. */
+void o_solve_p(double *p, const double *u_temp, const double *v_temp,
+               const double *w_temp, double dt, const double *dx,
+               const double *dy, const double *dz, const double *dx12,
+               const double *dy12, const double *dz12, int nx, int ny, int nz,
+               double *res, double *res_max, double *res_int, double *res_var,
+               int max_iter, int *iters);
+
 /* nse_dominik.f:553-607 */
 void o_set_c(int nx, int ny, int nz, double *x, double *x12, double *y,
              double *y12, double *z, double *z12, double *dx, double *dx12,

@@ -91,6 +91,8 @@ def _declare_oracle(lib):
         "o_fft": [vp, ci, ci, ci],
         "o_set_levelset": [ci, ci, ci] + [vp] * 10 + [ci],
         "o_levelset_shapes": [ci, ci, ci, vp, vp, vp, cd, cd, ci, vp, vp],
+        "o_linebyline": [ci, ci, ci] + [vp] * 9 + [cd, vp],
+        "o_solve_p": [vp] * 4 + [cd] + [vp] * 6 + [ci, ci, ci] + [vp] * 4 + [ci, vp],
         "o_set_adj_convect": [vp] * 6 + [cd, cd, ci, ci, ci] + [vp] * 9,
         "o_set_adj_convection_for_scalar": [vp, vp, vp, cd, cd, ci, ci, ci] + [vp] * 12,
         "o_set_adj_ns_scalar": [cd, ci, ci, ci] + [vp] * 11,

@@ -284,3 +284,60 @@ def test_adjoint_kernels():
                                              P(uf), P(vf), P(wf))
     for a, b in zip(outr, outo):
         exact(a, b)
+
+
+# ---------------------------------------------- iterative Poisson (dead)
+
+def _poisson_case(n, seed):
+    g = mk_grid(n)
+    rng = np.random.default_rng(seed)
+    ut, vt, wt = (rfield(rng, g) for _ in range(3))
+    p = rfield(rng, g, -0.1, 0.1)
+    return g, ut, vt, wt, p
+
+
+@pytest.mark.parametrize("n", [(8, 7, 6), (12, 9, 12)])
+def test_linebyline_bit_exact(n):
+    """nse_dominik.f:2136-2267: three iterations of the line Gauss-Seidel
+    sweep from the same p (grids with ny, nz <= nx + 1, where the
+    reference's fpy and fpz hold their lines)."""
+    g, ut, vt, wt, p = _poisson_case(n, 31)
+    a, b = p.copy(), p.copy()
+    dt = 1e-3
+    for _ in range(3):
+        oracle().o_linebyline(g.nx, g.ny, g.nz, P(ut), P(vt), P(wt), *grid_args(g), dt, P(a))
+        ref().linebyline_(I(g.nx), I(g.ny), I(g.nz), P(ut), P(vt), P(wt), *grid_args(g),
+                          D(dt), P(b))
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("uniform", [True, False])
+def test_solve_p_bit_exact(uniform):
+    """nse_dominik.f:1955-2021: linebyline until the 1e-5 residual; p and
+    the four residual statistics.  On the uniform grid the velocity has no
+    flux through the boundary (a compatible right-hand side for the
+    Neumann problem) and the iteration converges; on the stretched grid
+    every line solve uses its last row's metrics, the fixed point is not
+    the discrete solution and the reference runs all 100000 iterations."""
+    import ctypes
+    g, ut, vt, wt, p = _poisson_case((10, 8, 8), 37)
+    if uniform:
+        g = G.set_c(10, 8, 8, 2.0, 2.0, 2.0)
+        ut[:, :, 0] = ut[:, :, 10] = 0.0
+        vt[:, 0, :] = vt[:, 8, :] = 0.0
+        wt[0] = wt[8] = 0.0
+    ut *= 1e-3
+    vt *= 1e-3
+    wt *= 1e-3
+    dt = 1e-3
+    a, b = p.copy(), p.copy()
+    ro = [ctypes.c_double(0.0) for _ in range(4)]
+    rr = [ctypes.c_double(0.0) for _ in range(4)]
+    it = ctypes.c_int(0)
+    oracle().o_solve_p(P(a), P(ut), P(vt), P(wt), dt, *grid_args(g), g.nx, g.ny, g.nz,
+                       *[ctypes.byref(x) for x in ro], 100000, ctypes.byref(it))
+    ref().solve_p_(P(b), P(ut), P(vt), P(wt), D(dt), *grid_args(g), I(g.nx), I(g.ny),
+                   I(g.nz), *[ctypes.byref(x) for x in rr])
+    assert it.value >= 1 and (it.value < 100000) == uniform, it.value
+    np.testing.assert_array_equal(a, b)
+    assert [x.value for x in ro] == [x.value for x in rr]

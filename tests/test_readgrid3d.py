@@ -20,14 +20,17 @@ def fortran_d(v):
     return f"{v:.16E}".replace("E", "D")
 
 
-def write_structure(path, g, phi):
+def write_structure(path, g, phi, continued=True):
+    """structure.dat in every form the reader takes; with `continued`,
+    some items span two records (the serial reader's case), otherwise
+    every item is one record (the parallel reader's case)."""
     rng = np.random.default_rng(17)
     lines = []
     for k in range(1, g.nz + 1):
         for j in range(1, g.ny + 1):
             for i in range(1, g.nx + 1):
                 v = float(phi[k, j, i])
-                form = rng.integers(0, 5)
+                form = rng.integers(0, 5 if continued else 4)
                 a, b, c = float(g.x[i]), float(g.y[j]), float(g.z[k])
                 if form == 0:
                     lines.append(f"{a!r} {b!r} {c!r} {v!r}")
@@ -44,12 +47,13 @@ def write_structure(path, g, phi):
 
 
 @pytest.mark.skipif(ref() is None, reason="reference library not built")
-@pytest.mark.parametrize("n", [(6, 5, 4), (9, 7, 3)])
-def test_readgrid3d_matches_reference(tmp_path, n):
+@pytest.mark.parametrize("n,continued", [((6, 5, 4), True), ((9, 7, 3), True),
+                                         ((40, 30, 20), False), ((40, 30, 20), True)])
+def test_readgrid3d_matches_reference(tmp_path, n, continued):
     g = G.channel_grid(*n)
     phi = np.random.default_rng(3).normal(size=g.shape) * 10.0 ** np.random.default_rng(4).integers(
         -8, 8, size=g.shape)
-    write_structure(tmp_path / "structure.dat", g, phi)
+    write_structure(tmp_path / "structure.dat", g, phi, continued)
     got = np.full(g.shape, 7.0)
     want = np.full(g.shape, 7.0)
     cwd = os.getcwd()

@@ -23,7 +23,8 @@ EXPORTS = [
     "vk_set_explicit_for_tempreture", "vk_t_divide", "vk_solve_u_temp",
     "vk_solve_t", "vk_ctridiag_batched", "vk_tdma_batched", "vk_cyclic_batched",
     "vk_implicit_solver_batched", "vk_implicit_uy_batched", "vk_implicit_vy_batched",
-    "vk_implicit_tx_batched", "vk_implicit_ty_batched", "vk_mkp", "vk_p2s", "vk_s2p", "vk_spn", "vk_sp0b",
+    "vk_implicit_tx_batched", "vk_implicit_ty_batched", "vk_linebyline", "vk_solve_p",
+    "vk_mkp", "vk_p2s", "vk_s2p", "vk_spn", "vk_sp0b",
     "vk_phibc", "vk_set_levelset", "vk_levelset_shapes", "vk_set_adj_convect",
     "vk_set_adj_convection_for_scalar", "vk_set_adj_ns_scalar",
     "vk_time_step", "vk_time_step_mixed", "vk_objective", "vk_adjoint_gradient",

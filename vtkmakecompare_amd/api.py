@@ -138,6 +138,26 @@ def implicit_ty_batched(sty, ny, dy, dy12, re, sc, dt, lly, rry, ft, fb, cuy, et
                                        fb, cuy, D(eta), I(nsys)), "implicit_ty_batched")
 
 
+def linebyline(nx, ny, nz, u_temp, v_temp, w_temp, dx, dy, dz, dx12, dy12, dz12, dt, p):
+    """linebyline (nse_dominik.f:2136): one line Gauss-Seidel iteration."""
+    check(lib().vk_linebyline(I(nx), I(ny), I(nz), u_temp, v_temp, w_temp, dx, dy, dz, dx12,
+                              dy12, dz12, D(dt), p), "linebyline")
+
+
+def solve_p(p, u_temp, v_temp, w_temp, dt, dx, dy, dz, dx12, dy12, dz12, nx, ny, nz,
+            max_iterations=None):
+    """solve_p (nse_dominik.f:1955): returns (res, res_max, res_int, res_var,
+    iterations)."""
+    import ctypes
+    r = [ctypes.c_double(0.0) for _ in range(4)]
+    it = ctypes.c_int(0)
+    mi = None if max_iterations is None else I(max_iterations)
+    check(lib().vk_solve_p(p, u_temp, v_temp, w_temp, D(dt), dx, dy, dz, dx12, dy12, dz12,
+                           I(nx), I(ny), I(nz), *[ctypes.byref(x) for x in r], mi,
+                           ctypes.byref(it)), "solve_p")
+    return tuple(x.value for x in r) + (it.value,)
+
+
 def mkp(p, u_temp, v_temp, w_temp, dt, dx, dy, dz, dx12, dy12, dz12, re, nx, ny, nz,
         ompnthreads, u_ta, v_ta, w_ta, u_ep2, v_ep2, w_ep2, divu, u, v, w, xl, zl):
     check(lib().vk_mkp(p, u_temp, v_temp, w_temp, D(dt), dx, dy, dz, dx12, dy12, dz12, D(re),

@@ -12,8 +12,11 @@
 // consecutive addresses.  The fused sweeps of the time step solve the same
 // systems with the partition/PCR solver instead (vk_partition.h); these
 // entry points are the reference's own routines for callers and tests.
+#include <algorithm>
 #include <string>
+#include <type_traits>
 
+#include "vk_kernels.h"
 #include "vk_runtime.h"
 
 namespace vk {
@@ -286,6 +289,221 @@ __global__ void k_implicit_ty(double* sty, int ny, const double* dy, const doubl
   for (int i = 1; i <= ny; i++) sty[L(i)] = tt[L(i)];
 }
 
+// ------------------------------------------------ iterative Poisson
+// linebyline (nse_dominik.f:2136-2267): a lexicographic line Gauss-Seidel
+// sweep.  A line reads its neighbour lines: those before it in the
+// reference's loop order already updated, those after it not yet.  Lines
+// whose two loop indices sum to the same value read no line of their own
+// anti-diagonal, so each anti-diagonal is one launch (a wavefront), and the
+// result is the serial sweep's, bit for bit.
+
+// q = div(u_temp)/dt on the interior
+__global__ void k_lbl_q(Geom g, const double* u, const double* v, const double* w,
+                        const double* dx, const double* dy, const double* dz, double dt,
+                        double* q) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > g.nx || j > g.ny) return;
+  const long long o = g.at(i, j, k);
+  q[o] = ((u[o] - u[o - 1]) / dx[i] + (v[o] - v[o - g.sy]) / dy[j] +
+          (w[o] - w[o - g.sz]) / dz[k]) / dt;
+}
+
+struct PMetrics {
+  const double *dx, *dy, *dz, *dx12, *dy12, *dz12;
+};
+
+// the six coefficients of the Poisson row at (i, j, k)
+struct PB {
+  double bx1, bx2, by1, by2, bz1, bz2;
+  __device__ PB(const PMetrics& m, int i, int j, int k)
+      : bx1(1.0 / m.dx12[i] / m.dx[i]), bx2(1.0 / m.dx12[i - 1] / m.dx[i]),
+        by1(1.0 / m.dy12[j] / m.dy[j]), by2(1.0 / m.dy12[j - 1] / m.dy[j]),
+        bz1(1.0 / m.dz12[k] / m.dz[k]), bz2(1.0 / m.dz12[k - 1] / m.dz[k]) {}
+  __device__ double sum() const { return bx1 + bx2 + by1 + by2 + bz1 + bz2; }
+};
+
+// One anti-diagonal of the sweep along DIR (0 x lines (j,k), 1 y lines
+// (i,k), 2 z lines (i,j)): the line of lane t has loop indices (a, b) =
+// (d - b, bmin + t), a the inner loop's.  w: 4 band lines of n+2 values
+// per lane, lanes fastest (L lanes).
+template <int DIR>
+__global__ void k_lbl_sweep(Geom g, PMetrics m, double* p, const double* q, int d, int bmin,
+                            int nlines, double* w, int L) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlines) return;
+  const int b = bmin + t, a = d - b;
+  const int n = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
+  const Lane ln{t, L};
+  const long long ls = (long long)L * (n + 2);
+  double *lo = w, *di = w + ls, *up = w + 2 * ls, *u = w + 3 * ls;
+  int i = 0, j = 0, k = 0;
+  if (DIR == 0) { j = 1 + a; k = 1 + b; }
+  if (DIR == 1) { i = 1 + a; k = 1 + b; }
+  if (DIR == 2) { i = 1 + a; j = 1 + b; }
+  const long long st = DIR == 0 ? 1 : DIR == 1 ? g.sy : g.sz;
+  long long o = g.at(DIR == 0 ? 1 : i, DIR == 1 ? 1 : j, DIR == 2 ? 1 : k);
+  for (int e = 1; e <= n; e++, o += st) {
+    if (DIR == 0) i = e;
+    if (DIR == 1) j = e;
+    if (DIR == 2) k = e;
+    const PB c(m, i, j, k);
+    double f;
+    if (DIR == 0)
+      f = -c.by1 * p[o + g.sy] - c.by2 * p[o - g.sy] - c.bz1 * p[o + g.sz] - c.bz2 * p[o - g.sz] +
+          q[o];
+    else if (DIR == 1)
+      f = -c.bx1 * p[o + 1] - c.bx2 * p[o - 1] - c.bz1 * p[o + g.sz] - c.bz2 * p[o - g.sz] + q[o];
+    else
+      f = -c.bx1 * p[o + 1] - c.bx2 * p[o - 1] - c.by1 * p[o + g.sy] - c.by2 * p[o - g.sy] + q[o];
+    u[ln(e)] = f;
+  }
+  // the line's matrix: every row with the coefficients of its last row
+  const PB c(m, DIR == 0 ? g.nx : i, DIR == 1 ? g.ny : j, DIR == 2 ? g.nz : k);
+  const double blo = DIR == 0 ? c.bx2 : DIR == 1 ? c.by2 : c.bz2;
+  const double bhi = DIR == 0 ? c.bx1 : DIR == 1 ? c.by1 : c.bz1;
+  const double bsum = c.sum();
+  lo[ln(0)] = 0.0;
+  di[ln(0)] = 1.0;
+  up[ln(0)] = -1.0;
+  lo[ln(n + 1)] = -1.0;
+  di[ln(n + 1)] = 1.0;
+  up[ln(n + 1)] = 0.0;
+  for (int e = 1; e <= n; e++) {
+    lo[ln(e)] = blo;
+    di[ln(e)] = -(bsum);
+    up[ln(e)] = bhi;
+  }
+  u[ln(0)] = 0.0;
+  u[ln(n + 1)] = 0.0;
+  ul_eliminate(lo, di, up, u, ln, n + 1);
+  o = g.at(DIR == 0 ? 1 : i, DIR == 1 ? 1 : j, DIR == 2 ? 1 : k);
+  for (int e = 1; e <= n; e++, o += st) p[o] = u[ln(e)];
+}
+
+// zero-gradient halos in the reference's order: z faces, x faces, y faces
+// (each over the other two indices' full ranges)
+template <int FACE>
+__global__ void k_lbl_halo(Geom g, double* p) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (FACE == 0) {  // (i, j) over 0..nx+1, 0..ny+1
+    if (a > g.nx + 1) return;
+    p[g.at(a, b, 0)] = p[g.at(a, b, 1)];
+    p[g.at(a, b, g.nz + 1)] = p[g.at(a, b, g.nz)];
+  } else if (FACE == 1) {  // (j, k)
+    if (a > g.ny + 1) return;
+    p[g.at(0, a, b)] = p[g.at(1, a, b)];
+    p[g.at(g.nx + 1, a, b)] = p[g.at(g.nx, a, b)];
+  } else {  // (i, k)
+    if (a > g.nx + 1) return;
+    p[g.at(a, 0, b)] = p[g.at(a, 1, b)];
+    p[g.at(a, g.ny + 1, b)] = p[g.at(a, g.ny, b)];
+  }
+}
+
+__global__ void k_lbl_pst(Geom g, const double* p, double* pst) { *pst = p[g.at(1, 1, 1)]; }
+
+__global__ void k_lbl_shift(long long n, double* p, const double* pst) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) p[q] = p[q] - *pst;
+}
+
+// solve_p's residual statistics (nse_dominik.f:1982-2001), pass 1: per
+// block, over the interior in linear order, the largest |L p dt - div| and
+// the sums of (L p dt - div) dV and |.|^2 dV; block 0 also keeps the last
+// point's |.|.  Pass 2 combines the blocks in order.
+constexpr int kResBlocks = 1024;
+
+__global__ __launch_bounds__(256) void k_lbl_resid(Geom g, PMetrics m, const double* u,
+                                                   const double* v, const double* w,
+                                                   const double* p, double dt, double* part) {
+  __shared__ double sm[3][4];
+  const long long n = (long long)g.interior();
+  double rmax = 0.0, rint = 0.0, rvar = 0.0;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    const int i = 1 + (int)(q % g.nx);
+    const long long r = q / g.nx;
+    const int j = 1 + (int)(r % g.ny), k = 1 + (int)(r / g.ny);
+    const long long o = g.at(i, j, k);
+    const double div = (u[o] - u[o - 1]) / m.dx[i] + (v[o] - v[o - g.sy]) / m.dy[j] +
+                       (w[o] - w[o - g.sz]) / m.dz[k];
+    const double pphi =
+        ((p[o + 1] - p[o]) / m.dx12[i] - (p[o] - p[o - 1]) / m.dx12[i - 1]) / m.dx[i] +
+        ((p[o + g.sy] - p[o]) / m.dy12[j] - (p[o] - p[o - g.sy]) / m.dy12[j - 1]) / m.dy[j] +
+        ((p[o + g.sz] - p[o]) / m.dz12[k] - (p[o] - p[o - g.sz]) / m.dz12[k - 1]) / m.dz[k];
+    const double res = fabs(pphi * dt - div);
+    rmax = res >= rmax ? res : rmax;
+    rint = rint + (pphi * dt - div) * m.dx[i] * m.dy[j] * m.dz[k];
+    rvar = rvar + (res * res) * m.dx[i] * m.dy[j] * m.dz[k];
+    if (q == n - 1) part[4 * kResBlocks] = res;  // the last point's residual
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double a = __shfl_xor(rmax, off);
+    rmax = a > rmax ? a : rmax;
+    rint = rint + __shfl_xor(rint, off);
+    rvar = rvar + __shfl_xor(rvar, off);
+  }
+  if (threadIdx.x % 64 == 0) {
+    sm[0][threadIdx.x / 64] = rmax;
+    sm[1][threadIdx.x / 64] = rint;
+    sm[2][threadIdx.x / 64] = rvar;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < 4; q++) {
+      rmax = sm[0][q] > rmax ? sm[0][q] : rmax;
+      rint = rint + sm[1][q];
+      rvar = rvar + sm[2][q];
+    }
+    part[3 * blockIdx.x] = rmax;
+    part[3 * blockIdx.x + 1] = rint;
+    part[3 * blockIdx.x + 2] = rvar;
+  }
+}
+
+__global__ void k_lbl_resid_final(const double* part, int nb, double* out) {
+  if (threadIdx.x != 0) return;
+  double rmax = 0.0, rint = 0.0, rvar = 0.0;
+  for (int b = 0; b < nb; b++) {
+    rmax = part[3 * b] > rmax ? part[3 * b] : rmax;
+    rint = rint + part[3 * b + 1];
+    rvar = rvar + part[3 * b + 2];
+  }
+  out[0] = part[4 * kResBlocks];
+  out[1] = rmax;
+  out[2] = rint;
+  out[3] = rvar;
+}
+
+// one linebyline iteration on the stream
+static int linebyline(const Geom& g, const PMetrics& m, const double* u, const double* v,
+                      const double* w, double dt, double* p, double* q, double* band, int L,
+                      double* pst, hipStream_t s) {
+  k_lbl_q<<<interior_grid(g), interior_block(), 0, s>>>(g, u, v, w, m.dx, m.dy, m.dz, dt, q);
+  auto sweep = [&](auto dir, int na, int nb) {
+    constexpr int D = decltype(dir)::value;
+    for (int d = 0; d <= na + nb - 2; d++) {
+      const int bmin = d - (na - 1) > 0 ? d - (na - 1) : 0;
+      const int bmax = d < nb - 1 ? d : nb - 1;
+      const int nl = bmax - bmin + 1;
+      k_lbl_sweep<D><<<(unsigned)((nl + 63) / 64), 64, 0, s>>>(g, m, p, q, d, bmin, nl, band, L);
+    }
+  };
+  sweep(std::integral_constant<int, 0>(), g.ny, g.nz);
+  sweep(std::integral_constant<int, 1>(), g.nx, g.nz);
+  sweep(std::integral_constant<int, 2>(), g.nx, g.ny);
+  k_lbl_halo<0><<<dim3((unsigned)((g.nx + 2 + 63) / 64), (unsigned)(g.ny + 2)), 64, 0, s>>>(g, p);
+  k_lbl_halo<1><<<dim3((unsigned)((g.ny + 2 + 63) / 64), (unsigned)(g.nz + 2)), 64, 0, s>>>(g, p);
+  k_lbl_halo<2><<<dim3((unsigned)((g.nx + 2 + 63) / 64), (unsigned)(g.nz + 2)), 64, 0, s>>>(g, p);
+  k_lbl_pst<<<1, 1, 0, s>>>(g, p, pst);
+  const long long nc = (long long)g.cells();
+  k_lbl_shift<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(nc, p, pst);
+  return hip_check(hipGetLastError(), "linebyline");
+}
+
 constexpr int kLineThreads = 64;
 
 inline unsigned line_blocks(int nsys) { return (unsigned)((nsys + kLineThreads - 1) / kLineThreads); }
@@ -467,6 +685,87 @@ int vk_implicit_ty_batched(double* sty, const int* ny, const double* dy, const d
   k_implicit_ty<<<line_blocks(*nsys), kLineThreads, 0, rt().stream>>>(
       ds, *ny, ddy, ddy12, *re, *sc, *dt, dl, drr, dft, dfb, dc, *eta, w, *nsys);
   VK_LAUNCH_CHECK("implicit_ty_batched");
+  return VK_OK;
+}
+
+// the linebyline workspaces: q, the band scratch of the longest
+// anti-diagonal, p(1,1,1) and the residual partials
+struct LblWork {
+  double *q, *band, *pst;
+  int L;
+};
+
+static int lbl_work(const Geom& g, LblWork& lw) {
+  const int nmax = std::max(g.nx, std::max(g.ny, g.nz));
+  lw.L = std::max(std::min(g.ny, g.nz), std::max(std::min(g.nx, g.nz), std::min(g.nx, g.ny)));
+  lw.q = static_cast<double*>(workspace(kWorkLblQ, g.cells() * sizeof(double)));
+  lw.band = line_scratch(4, nmax, lw.L);
+  lw.pst = static_cast<double*>(
+      workspace(kWorkLblRes, (3 * (size_t)kResBlocks + 8 + 8) * sizeof(double)));
+  return (lw.q && lw.band && lw.pst) ? VK_OK : VK_ERR_HIP;
+}
+
+#define LBL_ARGS()                                                                \
+  Geom g(*nx, *ny, *nz);                                                          \
+  if (g.nx < 1 || g.ny < 1 || g.nz < 1)                                           \
+    return fail(VK_ERR_BAD_SIZE, "linebyline: nx, ny, nz >= 1");                  \
+  const size_t fb = g.cells() * sizeof(double);                                   \
+  VK_RESOLVE(du, const double, u_temp, fb);                                       \
+  VK_RESOLVE(dv, const double, v_temp, fb);                                       \
+  VK_RESOLVE(dw, const double, w_temp, fb);                                       \
+  VK_RESOLVE(dp, double, p, fb);                                                  \
+  VK_RESOLVE(ddx, const double, dx, (g.nx + 2) * sizeof(double));                 \
+  VK_RESOLVE(ddy, const double, dy, (g.ny + 2) * sizeof(double));                 \
+  VK_RESOLVE(ddz, const double, dz, (g.nz + 2) * sizeof(double));                 \
+  VK_RESOLVE(ddx12, const double, dx12, (g.nx + 2) * sizeof(double));             \
+  VK_RESOLVE(ddy12, const double, dy12, (g.ny + 2) * sizeof(double));             \
+  VK_RESOLVE(ddz12, const double, dz12, (g.nz + 2) * sizeof(double));             \
+  const PMetrics pm{ddx, ddy, ddz, ddx12, ddy12, ddz12};                          \
+  LblWork lw;                                                                     \
+  if (lbl_work(g, lw)) return VK_ERR_HIP
+
+int vk_linebyline(const int* nx, const int* ny, const int* nz, const double* u_temp,
+                  const double* v_temp, const double* w_temp, const double* dx, const double* dy,
+                  const double* dz, const double* dx12, const double* dy12, const double* dz12,
+                  const double* dt, double* p) {
+  VK_REQUIRE_INIT();
+  LBL_ARGS();
+  return linebyline(g, pm, du, dv, dw, *dt, dp, lw.q, lw.band, lw.L, lw.pst, rt().stream);
+}
+
+int vk_solve_p(double* p, const double* u_temp, const double* v_temp, const double* w_temp,
+               const double* dt, const double* dx, const double* dy, const double* dz,
+               const double* dx12, const double* dy12, const double* dz12, const int* nx,
+               const int* ny, const int* nz, double* res, double* res_max, double* res_int,
+               double* res_var, const int* max_iterations, int* iterations) {
+  VK_REQUIRE_INIT();
+  LBL_ARGS();
+  if (!res || !res_max || !res_int || !res_var)
+    return fail(VK_ERR_ARGUMENT, "solve_p: a residual output is null");
+  const int nnn = max_iterations ? *max_iterations : 100000;
+  hipStream_t s = rt().stream;
+  double* part = lw.pst + 8;
+  double* out = lw.pst + 4;
+  const long long nint = (long long)g.interior();
+  const int nb = (int)std::min<long long>(kResBlocks, (nint + 255) / 256);
+  int done = 0;
+  for (int lll = 1; lll <= nnn; lll++) {
+    int rc = linebyline(g, pm, du, dv, dw, *dt, dp, lw.q, lw.band, lw.L, lw.pst, s);
+    if (rc) return rc;
+    k_lbl_resid<<<nb, 256, 0, s>>>(g, pm, du, dv, dw, dp, *dt, part);
+    k_lbl_resid_final<<<1, 64, 0, s>>>(part, nb, out);
+    double r[4];
+    rc = hip_check(hipMemcpyAsync(r, out, sizeof(r), hipMemcpyDeviceToHost, s), "solve_p");
+    if (!rc) rc = hip_check(hipStreamSynchronize(s), "solve_p");
+    if (rc) return rc;
+    *res = r[0];
+    *res_max = r[1];
+    *res_int = r[2];
+    *res_var = r[3];
+    done = lll;
+    if (*res_max < 1.0e-005) break;
+  }
+  if (iterations) *iterations = done;
   return VK_OK;
 }
 

@@ -7,7 +7,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <map>
+#include <thread>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -454,6 +456,121 @@ int vk_memset(void* dst, const int* value, const size_t* bytes) {
 // null values and the slash terminator are refused.  phi0 is host memory,
 // as in the reference; when it is registered its device mirror is
 // refreshed.
+
+namespace {
+
+enum ItemStatus { kItemOk = 0, kItemEnd, kItemNull, kItemRepeat, kItemBad };
+
+// One item from p: its four values into val, p left at the start of the
+// next item's record.  `multi` reports an item that took more than one
+// record (or skipped a blank one).
+ItemStatus parse_item(const char*& p, const char* end, double val[4], bool* multi) {
+  char tok[128];
+  int nv = 0;
+  // a comma before the first value of an item is a null value, as is a
+  // comma after another comma (blanks and record ends between them do not
+  // count)
+  bool sep_comma = true;
+  *multi = false;
+  while (nv < 4) {
+    while (p < end && (*p == ' ' || *p == '\t' || *p == '\r' || *p == ',')) {
+      if (*p == ',') {
+        if (sep_comma) return kItemNull;
+        sep_comma = true;
+      }
+      p++;
+    }
+    if (p >= end) return kItemEnd;
+    if (*p == '\n') {  // the item continues on the next record
+      *multi = true;
+      p++;
+      continue;
+    }
+    const char* t0 = p;
+    while (p < end && *p != ' ' && *p != '\t' && *p != '\r' && *p != ',' && *p != '\n') p++;
+    const size_t n = (size_t)(p - t0);
+    if (n == 0 || n >= sizeof(tok)) return kItemBad;
+    for (size_t q = 0; q < n; q++) {
+      const char c = t0[q];
+      if (c == '*' || c == '/') return kItemRepeat;
+      tok[q] = (c == 'd' || c == 'D') ? 'e' : c;
+    }
+    tok[n] = '\0';
+    char* e = nullptr;
+    const double v = std::strtod(tok, &e);
+    if (e != tok + n) return kItemBad;
+    val[nv++] = v;
+    sep_comma = false;
+  }
+  while (p < end && *p != '\n') p++;  // the rest of the record
+  if (p < end) p++;
+  return kItemOk;
+}
+
+int item_fail(ItemStatus st) {
+  switch (st) {
+    case kItemEnd: return fail(VK_ERR_ARGUMENT, "readgrid3d: structure.dat ends early");
+    case kItemNull: return fail(VK_ERR_ARGUMENT, "readgrid3d: null value");
+    case kItemRepeat:
+      return fail(VK_ERR_ARGUMENT, "readgrid3d: repeat counts and '/' not supported");
+    default: return fail(VK_ERR_ARGUMENT, "readgrid3d: bad value in structure.dat");
+  }
+}
+
+// The file as one item per record, parsed in parallel: the text split on
+// record boundaries, each chunk's items parsed by parse_item.  Returns
+// false (the caller then reads serially) for anything else -- an item over
+// several records, a blank record, any refusal -- so that the serial
+// reader decides those cases.
+bool readgrid3d_parallel(const std::string& text, long long nitems, std::vector<double>& out) {
+  const char* const base = text.data();
+  const char* const end = base + text.size();
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = (int)std::min<long long>(std::min<unsigned>(hw, 16),
+                                          1 + (long long)text.size() / (1 << 20));
+  if (nt < 2) return false;
+  std::vector<const char*> cut((size_t)nt + 1);
+  cut[0] = base;
+  cut[nt] = end;
+  for (int t = 1; t < nt; t++) {
+    const char* c = base + text.size() * (size_t)t / (size_t)nt;
+    while (c < end && *c != '\n') c++;
+    cut[t] = c < end ? c + 1 : end;
+  }
+  std::vector<std::vector<double>> vals((size_t)nt);
+  std::vector<char> ok((size_t)nt, 1);
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++)
+    th.emplace_back([&, t]() {
+      const char* p = cut[t];
+      const char* e = cut[t + 1];
+      double v[4];
+      while (p < e) {
+        bool multi = false;
+        if (parse_item(p, e, v, &multi) != kItemOk || multi) {
+          ok[t] = 0;
+          return;
+        }
+        vals[t].push_back(v[3]);
+      }
+    });
+  for (auto& x : th) x.join();
+  long long total = 0;
+  for (int t = 0; t < nt; t++) {
+    if (!ok[t] && total < nitems) return false;  // irregular before the last item
+    total += (long long)vals[t].size();
+    if (!ok[t]) break;
+  }
+  if (total < nitems) return false;
+  out.resize((size_t)nitems);
+  long long q = 0;
+  for (int t = 0; t < nt && q < nitems; t++)
+    for (size_t r = 0; r < vals[t].size() && q < nitems; r++) out[(size_t)q++] = vals[t][r];
+  return true;
+}
+
+}  // namespace
+
 int vk_readgrid3d(const int* nx, const int* ny, const int* nz, double* phi0) {
   if (*nx < 1 || *ny < 1 || *nz < 1) return fail(VK_ERR_BAD_SIZE, "readgrid3d: nx, ny, nz >= 1");
   if (!phi0) return fail(VK_ERR_ARGUMENT, "readgrid3d: phi0 is null");
@@ -461,59 +578,38 @@ int vk_readgrid3d(const int* nx, const int* ny, const int* nz, double* phi0) {
   if (!f) return fail(VK_ERR_ARGUMENT, "readgrid3d: cannot open structure.dat");
   std::string text;
   {
+    // one read of the whole file where its size is known, else in pieces
+    struct stat sb;
+    if (fstat(fileno(f), &sb) == 0 && sb.st_size > 0) {
+      text.resize((size_t)sb.st_size);
+      text.resize(std::fread(&text[0], 1, text.size(), f));
+    }
     std::vector<char> buf(1 << 20);
     size_t got;
     while ((got = std::fread(buf.data(), 1, buf.size(), f)) > 0) text.append(buf.data(), got);
     std::fclose(f);
   }
-  const char* p = text.data();
-  const char* const end = p + text.size();
   const size_t sy = (size_t)*nx + 2, sz = sy * ((size_t)*ny + 2);
-  char tok[128];
-  for (int k = 1; k <= *nz; k++)
-    for (int j = 1; j <= *ny; j++)
-      for (int i = 1; i <= *nx; i++) {
-        double val[4];
-        int nv = 0;
-        // a comma before the first value of an item is a null value, as
-        // is a comma after another comma (blanks and record ends between
-        // them do not count)
-        bool sep_comma = true;
-        while (nv < 4) {
-          while (p < end && (*p == ' ' || *p == '\t' || *p == '\r' || *p == ',')) {
-            if (*p == ',') {
-              if (sep_comma) return fail(VK_ERR_ARGUMENT, "readgrid3d: null value");
-              sep_comma = true;
-            }
-            p++;
-          }
-          if (p >= end) return fail(VK_ERR_ARGUMENT, "readgrid3d: structure.dat ends early");
-          if (*p == '\n') {  // the item continues on the next record
-            p++;
-            continue;
-          }
-          const char* t0 = p;
-          while (p < end && *p != ' ' && *p != '\t' && *p != '\r' && *p != ',' && *p != '\n') p++;
-          const size_t n = (size_t)(p - t0);
-          if (n == 0 || n >= sizeof(tok))
-            return fail(VK_ERR_ARGUMENT, "readgrid3d: bad value in structure.dat");
-          for (size_t q = 0; q < n; q++) {
-            const char c = t0[q];
-            if (c == '*' || c == '/')
-              return fail(VK_ERR_ARGUMENT, "readgrid3d: repeat counts and '/' not supported");
-            tok[q] = (c == 'd' || c == 'D') ? 'e' : c;
-          }
-          tok[n] = '\0';
-          char* e = nullptr;
-          const double v = std::strtod(tok, &e);
-          if (e != tok + n) return fail(VK_ERR_ARGUMENT, "readgrid3d: bad value in structure.dat");
-          val[nv++] = v;
-          sep_comma = false;
+  const long long nitems = (long long)*nx * *ny * *nz;
+  std::vector<double> fast;
+  if (readgrid3d_parallel(text, nitems, fast)) {
+    long long q = 0;
+    for (int k = 1; k <= *nz; k++)
+      for (int j = 1; j <= *ny; j++)
+        for (int i = 1; i <= *nx; i++) phi0[(size_t)i + sy * (size_t)j + sz * (size_t)k] = fast[(size_t)q++];
+  } else {
+    const char* p = text.data();
+    const char* const end = p + text.size();
+    for (int k = 1; k <= *nz; k++)
+      for (int j = 1; j <= *ny; j++)
+        for (int i = 1; i <= *nx; i++) {
+          double val[4];
+          bool multi = false;
+          const ItemStatus st = parse_item(p, end, val, &multi);
+          if (st != kItemOk) return item_fail(st);
+          phi0[(size_t)i + sy * (size_t)j + sz * (size_t)k] = val[3];
         }
-        phi0[(size_t)i + sy * (size_t)j + sz * (size_t)k] = val[3];
-        while (p < end && *p != '\n') p++;  // the rest of the record
-        if (p < end) p++;
-      }
+  }
   if (rt().initialised &&
       registry().count(reinterpret_cast<uintptr_t>(static_cast<const void*>(phi0))))
     return vk_upload(phi0);

@@ -108,6 +108,8 @@ enum WorkSlot {
   kWorkPhi3,
   kWorkDivMax,
   kWorkLineBatch,
+  kWorkLblQ,
+  kWorkLblRes,
   kWorkSlots
 };
 // `grown` (optional) reports whether this call allocated the slot anew.

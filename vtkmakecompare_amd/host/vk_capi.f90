@@ -321,6 +321,23 @@ module vk_capi
       real(c_double) :: cuy(*), eta
       integer(c_int) :: ny, nsys
     end function
+    ! the iterative Poisson solver (nse_dominik.f:1955-2457)
+    integer(c_int) function vk_linebyline(nx, ny, nz, u_temp, v_temp, w_temp, dx, dy, dz, &
+        dx12, dy12, dz12, dt, p) bind(C, name="vk_linebyline")
+      import :: c_int, c_double
+      integer(c_int) :: nx, ny, nz
+      real(c_double) :: u_temp(*), v_temp(*), w_temp(*), dx(*), dy(*), dz(*), dx12(*)
+      real(c_double) :: dy12(*), dz12(*), dt, p(*)
+    end function
+    integer(c_int) function vk_solve_p(p, u_temp, v_temp, w_temp, dt, dx, dy, dz, dx12, &
+        dy12, dz12, nx, ny, nz, res, res_max, res_int, res_var, max_iterations, iterations) &
+        bind(C, name="vk_solve_p")
+      import :: c_int, c_double, c_ptr
+      real(c_double) :: p(*), u_temp(*), v_temp(*), w_temp(*), dt, dx(*), dy(*), dz(*)
+      real(c_double) :: dx12(*), dy12(*), dz12(*), res, res_max, res_int, res_var
+      integer(c_int) :: nx, ny, nz, iterations
+      type(c_ptr), value :: max_iterations
+    end function
     ! the adjoint operators (adjoint.f:1,128,185); T_final by address
     integer(c_int) function vk_set_adj_convect(u, u_v, v, v_v, w, w_v, re, dt, nx, ny, nz, &
         dx, dy, dz, dx12, dy12, dz12, u_final, v_final, w_final) &

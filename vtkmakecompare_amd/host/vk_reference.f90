@@ -22,6 +22,7 @@
 !   set_adj_convect             adjoint.f:1
 !   set_adj_convection_for_scalar adjoint.f:128
 !   set_adj_NS_scalar           adjoint.f:185
+!   linebyline, solve_p         nse_dominik.f:2136,1955
 
 subroutine set_bc(u, v, w, p, nx, ny, nz, t)
   use vk_capi
@@ -384,4 +385,37 @@ subroutine output(u, v, w, p, x, y, z, nx, ny, nz, t, cellvalue, u_ta, v_ta, w_t
   double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: u, v, w, p, t, u_ta, v_ta, w_ta, phi, divu
   call vk_check(vk_output(u, v, w, p, x, y, z, nx, ny, nz, t, cellvalue, u_ta, v_ta, w_ta, &
                           phi, divu), 'OUTPUT')
+end subroutine
+
+subroutine linebyline(nx, ny, nz, u_temp, v_temp, w_temp, dx, dy, dz, dx12, dy12, dz12, dt, p)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz
+  double precision :: dt
+  double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: u_temp, v_temp, w_temp, p
+  double precision :: dx(0:nx+1), dy(0:ny+1), dz(0:nz+1)
+  double precision :: dx12(0:nx+1), dy12(0:ny+1), dz12(0:nz+1)
+  call vk_check(vk_linebyline(nx, ny, nz, u_temp, v_temp, w_temp, dx, dy, dz, dx12, dy12, &
+                              dz12, dt, p), 'linebyline')
+end subroutine
+
+! solve_p (nse_dominik.f:1955-2021): the iteration on the device, the
+! reference's messages on unit 6
+subroutine solve_p(p, u_temp, v_temp, w_temp, dt, dx, dy, dz, dx12, dy12, dz12, nx, ny, nz, &
+                   res, res_max, res_int, res_var)
+  use vk_capi
+  implicit none
+  integer :: nx, ny, nz, iterations
+  double precision :: dt, res, res_max, res_int, res_var
+  double precision, dimension(0:nx+1,0:ny+1,0:nz+1) :: p, u_temp, v_temp, w_temp
+  double precision :: dx(0:nx+1), dy(0:ny+1), dz(0:nz+1)
+  double precision :: dx12(0:nx+1), dy12(0:ny+1), dz12(0:nz+1)
+  call vk_check(vk_solve_p(p, u_temp, v_temp, w_temp, dt, dx, dy, dz, dx12, dy12, dz12, nx, &
+                           ny, nz, res, res_max, res_int, res_var, c_null_ptr, iterations), &
+                'solve_p')
+  write (*, *) 'convergence of poisson =', res_max
+  write (*, *) 'res = ', res
+  write (*, *) 'res_maximum = ', res_max
+  write (*, *) 'res_integral = ', res_int
+  write (*, *) 'res_var = ', res_var
 end subroutine
