@@ -424,11 +424,169 @@ __global__ __launch_bounds__(256) void k_cfft_z(Geom g, R* phit, FftPlan<R> p, i
   }
 }
 
+// The z pass with one LDS buffer.  A thread takes W adjacent columns (W = 2
+// in float, so that every access of a thread moves 16 bytes as in double),
+// a block CC such column groups (CC W adjacent columns of the spectrum,
+// one 128-byte piece of a row per k where LDS allows it).  Row jj of
+// column group c sits at LDS slot (sw(jj) CC + c), sw(jj) = jj ^ ((jj >> 2)
+// & 3): the rows a wave reads or writes in one instruction -- consecutive
+// rows, rows Rd apart (the first pass's outputs), or groups of 4 rows 16
+// apart -- then fall on both halves of the banks evenly, so no access
+// conflicts.  Each pass loads all its inputs into registers, waits at a
+// barrier, computes and stores its outputs, and waits again; a thread takes
+// at most VPT rows per pass (butterflies jj = t, t + TPC, ...).  The first
+// pass reads the spectrum and the last writes it, as in k_cfft_z.
+__device__ __forceinline__ int zsw(int jj) { return jj ^ ((jj >> 2) & 3); }
+constexpr int kZ1Threads = 512;  // threads per block
+constexpr int kZ1Vpt = 6;        // rows per thread and pass, at most
+
+template <typename R, int W>
+struct ZCols {
+  typename Cx<R>::T c[W];
+};
+
+template <typename R, int W, int CC, int TPC, int Rd>
+__device__ __forceinline__ void cfft_z_pass(R* phit, ZCols<R, W>* sh, const FftPlan<R>& p,
+                                            int sign, int N, int ns, bool first, bool last,
+                                            int c, int t, const bool* okc, long long o0,
+                                            long long ks, R rsc) {
+  using C = typename Cx<R>::T;
+  constexpr int BPT = (kZ1Vpt + Rd - 1) / Rd;
+  const int nb = N / Rd;
+  const int twstep = N / (ns * Rd);
+  ZCols<R, W> v[BPT][4];
+  const C z = mkc<C>((R)0, (R)0);
+#pragma unroll
+  for (int b = 0; b < BPT; b++) {
+    const int jj = t + b * TPC;
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int w = 0; w < W; w++) v[b][m].c[w] = z;
+    if (jj < nb) {
+#pragma unroll
+      for (int m = 0; m < Rd; m++) {
+        const int row = jj + m * nb;
+        if (first) {
+          const long long off = o0 + ks * row;
+#pragma unroll
+          for (int w = 0; w < W; w++)
+            if (okc[w]) v[b][m].c[w] = mkc<C>(phit[off + 2 * w], phit[off + 2 * w + 1]);
+        } else {
+          v[b][m] = sh[zsw(row) * CC + c];
+        }
+      }
+    }
+  }
+  if (!first && !last) __syncthreads();  // every read of the buffer precedes a write
+#pragma unroll
+  for (int b = 0; b < BPT; b++) {
+    const int jj = t + b * TPC;
+    if (jj < nb) {
+      const int jq = jj / ns, kk = jj - jq * ns;
+#pragma unroll
+      for (int w = 0; w < W; w++)
+        stockham_bfly<R>(v[b][0].c[w], v[b][1].c[w], v[b][2].c[w], v[b][3].c[w], Rd, kk, twstep,
+                         p.tw, sign);
+      const int base = jq * ns * Rd + kk;
+#pragma unroll
+      for (int m = 0; m < Rd; m++) {
+        const int row = base + m * ns;
+        if (last) {
+          const long long off = o0 + ks * row;
+#pragma unroll
+          for (int w = 0; w < W; w++)
+            if (okc[w]) {
+              phit[off + 2 * w] = v[b][m].c[w].x * rsc;
+              phit[off + 2 * w + 1] = v[b][m].c[w].y * rsc;
+            }
+        } else {
+          sh[zsw(row) * CC + c] = v[b][m];
+        }
+      }
+    }
+  }
+  if (!last) __syncthreads();
+}
+
+template <typename R, int W, int CC>
+__global__ __launch_bounds__(kZ1Threads) void k_cfft_z1(Geom g, R* phit, FftPlan<R> p, int sign,
+                                                        int scale) {
+  constexpr int TPC = kZ1Threads / CC;
+  extern __shared__ unsigned char shraw[];
+  ZCols<R, W>* sh = reinterpret_cast<ZCols<R, W>*>(shraw);
+  const int N = p.n;  // nz
+  const int nh1 = g.nx / 2 + 1;
+  int bx, by, bz;
+  xcd_block(bx, by, bz);
+  const int c = threadIdx.x % CC, t = threadIdx.x / CC;
+  const int I = (bx * CC + c) * W, jrow = 1 + by;
+  bool okc[W];
+#pragma unroll
+  for (int w = 0; w < W; w++) okc[w] = I + w < nh1;
+  const long long o0 = 2 * g.spec_at(okc[0] ? I : 0, jrow, 0);
+  const long long ks = 2 * g.spec_at(0, 0, 1);
+  const R rsc = scale ? (R)(1.0 / ((double)g.nx * (double)g.nz)) : (R)1;
+  int ns = 1;
+  for (int s = 0; s < p.nst; s++) {
+    const bool first = s == 0, last = s == p.nst - 1;
+    switch (p.r[s]) {
+      case 4:
+        cfft_z_pass<R, W, CC, TPC, 4>(phit, sh, p, sign, N, ns, first, last, c, t, okc, o0, ks,
+                                      rsc);
+        break;
+      case 3:
+        cfft_z_pass<R, W, CC, TPC, 3>(phit, sh, p, sign, N, ns, first, last, c, t, okc, o0, ks,
+                                      rsc);
+        break;
+      default:
+        cfft_z_pass<R, W, CC, TPC, 2>(phit, sh, p, sign, N, ns, first, last, c, t, okc, o0, ks,
+                                      rsc);
+        break;
+    }
+    ns *= p.r[s];
+  }
+}
+
+// k_cfft_z1 with the widest group of at most 128 bytes per k whose buffer
+// (rows rounded up to 4 for the swizzle) fits in 64 KB and whose threads
+// take at most kZ1Vpt rows per pass; false when none does (lengths above
+// 3072 / 2 columns, or of one pass).
+template <typename R>
+static bool launch_cfft_z1(const Geom& g, R* phit, const FftPlan<R>& pz, int sign, int scale,
+                           hipStream_t s) {
+  constexpr int W = sizeof(R) == 4 ? 2 : 1;
+  using E = ZCols<R, W>;
+  const int N = g.nz;
+  const int nh1 = g.nx / 2 + 1;
+  const size_t rows = ((size_t)N + 3) / 4 * 4;
+  if (pz.nst < 2) return false;
+  for (int cc = (int)(128 / sizeof(E)); cc >= 2; cc /= 2) {
+    const size_t lds = rows * cc * sizeof(E);
+    const int tpc = kZ1Threads / cc;
+    if (lds > 65536 || (N + tpc - 1) / tpc > kZ1Vpt) continue;
+    const dim3 grid((unsigned)((nh1 + cc * W - 1) / (cc * W)), (unsigned)g.ny);
+    switch (cc) {
+      case 8:
+        k_cfft_z1<R, W, 8><<<grid, kZ1Threads, lds, s>>>(g, phit, pz, sign, scale);
+        return true;
+      case 4:
+        k_cfft_z1<R, W, 4><<<grid, kZ1Threads, lds, s>>>(g, phit, pz, sign, scale);
+        return true;
+      default:
+        k_cfft_z1<R, W, 2><<<grid, kZ1Threads, lds, s>>>(g, phit, pz, sign, scale);
+        return true;
+    }
+  }
+  return false;
+}
+
 // the z pass with the widest column group whose LDS fits in 64 KB
 template <typename R>
 static void launch_cfft_z(const Geom& g, R* phit, const FftPlan<R>& pz, int sign, int scale,
                           hipStream_t s) {
   using C = typename Cx<R>::T;
+  if (launch_cfft_z1<R>(g, phit, pz, sign, scale, s)) return;
   const size_t per_col = sizeof(C) * (2 * (size_t)g.nz + 1);
   const int nh1 = g.nx / 2 + 1;
   if (per_col * 8 <= 65536) {
