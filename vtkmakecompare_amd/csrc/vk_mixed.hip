@@ -109,20 +109,25 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
     const float cv2 = x.cv[g.cv(i, j, k, 2)], cv3 = x.cv[g.cv(i, j, k, 3)],
                 cv4 = x.cv[g.cv(i, j, k, 4)];
     const unsigned mk = x.lf.fx ? dirty_bits(0.0f, cv2, cv3, cv4, 1.0f, 1.0f) : 0u;
+    // u_ta read only where its term eta cv u_ta dt is not 0
+    float uta = 0.0f, vta = 0.0f, wta = 0.0f;
+    if (cv2 != 0.0f) uta = x.u_ta[o];
+    if (cv3 != 0.0f) vta = x.v_ta[o];
+    if (cv4 != 0.0f) wta = x.w_ta[o];
     // every value computed before the first store (the stores would fence
     // the later stencil loads: the outputs may alias the inputs as far as
     // the compiler knows)
     const float uv = conv_u_r<float>(U, V, W, f, f, i, j, k, dt);
     const float ud = cond_u_r<float>(U, f, i, j, k, x.c0);
-    const float ul = eta * cv2 * x.u_ta[o] * dt;
+    const float ul = eta * cv2 * uta * dt;
     const float ua = x.a1 * uv + x.a2 * x.u_pre[o] + x.b1 * ud + x.b1 * ul;
     const float vv = conv_v_r<float>(U, V, W, f, f, i, j, k, dt);
     const float vd = cond_v_r<float>(V, f, i, j, k, x.c0);
-    const float vl = eta * cv3 * x.v_ta[o] * dt;
+    const float vl = eta * cv3 * vta * dt;
     const float va = x.a1 * vv + x.a2 * x.v_pre[o] + x.b1 * vd + x.b1 * vl;
     const float wv = conv_w_r<float>(U, V, W, f, f, i, j, k, dt);
     const float wd = cond_w_r<float>(W, f, i, j, k, x.c0);
-    const float wl = eta * cv4 * x.w_ta[o] * dt;
+    const float wl = eta * cv4 * wta * dt;
     const float wa = x.a1 * wv + x.a2 * x.w_pre[o] + x.b1 * wd + x.b1 * wl;
     const double un = pen_div_m<double>(x.u[o] + (double)ua, x.etad, (double)cv2, x.dtd);
     const double vn = pen_div_m<double>(x.v[o] + (double)va, x.etad, (double)cv3, x.dtd);

@@ -47,6 +47,13 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
     const double cv2 = x.cv[g.cv(i, j, k, 2)], cv3 = x.cv[g.cv(i, j, k, 3)],
                  cv4 = x.cv[g.cv(i, j, k, 4)];
     const unsigned mk = x.lf.fx ? dirty_bits(0.0, cv2, cv3, cv4, 1.0, 1.0) : 0u;
+    // u_ta enters only through eta cv u_ta dt: where the face value is 0
+    // the term is 0 and u_ta is not read (in fluid, most of the grid, the
+    // three fields' traffic is saved)
+    double uta = 0.0, vta = 0.0, wta = 0.0;
+    if (cv2 != 0.0) uta = x.u_ta[o];
+    if (cv3 != 0.0) vta = x.v_ta[o];
+    if (cv4 != 0.0) wta = x.w_ta[o];
     // u, v, w: SET_CONVECT, SET_CONDUCTION, u_l, SET_EXPLICIT and
     // solve_u_temp's divide.  Every value is computed before the first
     // store, so the loads of all three stencils can be in flight together
@@ -54,15 +61,15 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
     // alias the inputs as far as the compiler knows).
     const double uv = conv_u_r(U, V, W, m, r, i, j, k, dt);
     const double ud = cond_u_r(U, r, i, j, k, x.c0);
-    const double ul = eta * cv2 * x.u_ta[o] * dt;
+    const double ul = eta * cv2 * uta * dt;
     const double ua = x.a1 * uv + x.a2 * x.u_pre[o] + x.b1 * ud + x.b1 * ul;
     const double vv = conv_v_r(U, V, W, m, r, i, j, k, dt);
     const double vd = cond_v_r(V, r, i, j, k, x.c0);
-    const double vl = eta * cv3 * x.v_ta[o] * dt;
+    const double vl = eta * cv3 * vta * dt;
     const double va = x.a1 * vv + x.a2 * x.v_pre[o] + x.b1 * vd + x.b1 * vl;
     const double wv = conv_w_r(U, V, W, m, r, i, j, k, dt);
     const double wd = cond_w_r(W, r, i, j, k, x.c0);
-    const double wl = eta * cv4 * x.w_ta[o] * dt;
+    const double wl = eta * cv4 * wta * dt;
     const double wa = x.a1 * wv + x.a2 * x.w_pre[o] + x.b1 * wd + x.b1 * wl;
     const double un = pen_div(x.u[o] + ua, eta, cv2, dt);
     const double vn = pen_div(x.v[o] + va, eta, cv3, dt);
