@@ -337,7 +337,7 @@ enum { SU, SV, SW, ST, SUP, SVP, SWP, STP, SUTA, SVTA, SWTA };
 struct Dev {
   std::vector<void*> ptrs;
   ~Dev() {
-    for (void* p : ptrs) hipFree(p);
+    for (void* p : ptrs) (void)hipFree(p);
   }
   size_t bytes = 0;
   double* alloc(size_t n) { return static_cast<double*>(raw(n * sizeof(double))); }
