@@ -440,8 +440,9 @@ __device__ __forceinline__ int zsw(int jj) { return jj ^ ((jj >> 2) & 3); }
 constexpr int kZ1Threads = 512;  // threads per block
 constexpr int kZ1Vpt = 6;        // rows per thread and pass, at most
 
+// (aligned to its size, so that a thread's LDS access is one 16-byte word)
 template <typename R, int W>
-struct ZCols {
+struct alignas(sizeof(typename Cx<R>::T) * W) ZCols {
   typename Cx<R>::T c[W];
 };
 
