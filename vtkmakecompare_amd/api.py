@@ -578,30 +578,41 @@ class ChannelRun:
             self.d["cellvalue"] = DeviceArray.from_host(cv)
         self.diagnostics = diagnostics and not mixed
         self.l = 0
+        self._call = None
+
+    def _step_call(self):
+        """The step's entry point and its arguments, converted once: the
+        arrays do not move while the run lives, and the step counter is a
+        c_int updated in place, so a step costs one call and no conversions
+        (the mixed step waits for its projection each step, and whatever the
+        host does between steps the GPU waits for)."""
+        if self._call is None:
+            import ctypes
+            d, g, c = self.d, self.g, self.cfg
+            fields = ["u", "v", "w", "p", "t", "u_pre", "v_pre", "w_pre", "t_pre", "u_ta",
+                      "v_ta", "w_ta", "u_temp", "v_temp", "w_temp", "divu", "cellvalue",
+                      "rhocp", "thc"]
+            ptrs = [d[k]._as_parameter_ for k in fields]
+            if self.precision == "fp64":
+                ptrs += [d[k]._as_parameter_ if self.diagnostics else None
+                         for k in ("u_ep2", "v_ep2", "w_ep2")]
+            ptrs += [d[k]._as_parameter_ for k in ("dx", "dy", "dz", "dx12", "dy12", "dz12")]
+            self._lc = ctypes.c_int(0)
+            args = tuple(ptrs) + (I(g.nx), I(g.ny), I(g.nz), D(c.dt), D(c.re), D(c.sc),
+                                  D(c.eta), D(g.xl), D(g.zl), ctypes.byref(self._lc))
+            fn = lib().vk_time_step_mixed if self.precision == "mixed" else lib().vk_time_step
+            self._call = (fn, args)
+        return self._call
 
     def step(self, n: int = 1) -> None:
-        d, g, c = self.d, self.g, self.cfg
-        if self.precision == "mixed":
-            for _ in range(n):
-                self.l += 1
-                check(lib().vk_time_step_mixed(
-                    d["u"], d["v"], d["w"], d["p"], d["t"], d["u_pre"], d["v_pre"],
-                    d["w_pre"], d["t_pre"], d["u_ta"], d["v_ta"], d["w_ta"], d["u_temp"],
-                    d["v_temp"], d["w_temp"], d["divu"], d["cellvalue"], d["rhocp"],
-                    d["thc"], d["dx"], d["dy"], d["dz"], d["dx12"], d["dy12"], d["dz12"],
-                    I(g.nx), I(g.ny), I(g.nz), D(c.dt), D(c.re), D(c.sc), D(c.eta), D(g.xl),
-                    D(g.zl), I(self.l)), f"mixed time step {self.l}")
-            return
-        ep = (d["u_ep2"], d["v_ep2"], d["w_ep2"]) if self.diagnostics else (None, None, None)
+        fn, args = self._step_call()
+        what = "mixed time step" if self.precision == "mixed" else "time step"
         for _ in range(n):
             self.l += 1
-            check(lib().vk_time_step(
-                d["u"], d["v"], d["w"], d["p"], d["t"], d["u_pre"], d["v_pre"], d["w_pre"],
-                d["t_pre"], d["u_ta"], d["v_ta"], d["w_ta"], d["u_temp"], d["v_temp"],
-                d["w_temp"], d["divu"], d["cellvalue"], d["rhocp"], d["thc"], ep[0], ep[1],
-                ep[2], d["dx"], d["dy"], d["dz"], d["dx12"], d["dy12"], d["dz12"], I(g.nx),
-                I(g.ny), I(g.nz), D(c.dt), D(c.re), D(c.sc), D(c.eta), D(g.xl), D(g.zl),
-                I(self.l)), f"time step {self.l}")
+            self._lc.value = self.l
+            rc = fn(*args)
+            if rc:
+                check(rc, f"{what} {self.l}")
 
     def get(self, name: str) -> np.ndarray:
         """A field downloaded to the host, as float64 whatever its storage."""
@@ -636,3 +647,4 @@ class ChannelRun:
         for a in self.d.values():
             a.free()
         self.d = {}
+        self._call = None
