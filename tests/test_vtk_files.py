@@ -112,8 +112,16 @@ def test_restart_round_trip_on_host(tmp_path):
         A.read_restart(path, g2, f3, np.zeros((4, g2.nz, g2.ny, g2.nx)))
     data = open(path, "rb").read()
     open(path, "wb").write(data[:len(data) // 2])
+    # a truncated record is refused before any array is written
+    f4 = {k: np.full(g.shape, 7.0) for k in A.RESTART_FIELDS}
+    cv4 = np.full_like(cv, 7.0)
     with pytest.raises(A.VKError):
-        A.read_restart(path, g, f2, cv2)
+        A.read_restart(path, g, f4, cv4)
+    assert all(np.all(f4[k] == 7.0) for k in A.RESTART_FIELDS) and np.all(cv4 == 7.0)
+    # so is one record too many (trailing data)
+    open(path, "wb").write(data + data[-64:])
+    with pytest.raises(A.VKError):
+        A.read_restart(path, g, f4, cv4)
     open(path, "wb").write(b"\x10\x00\x00\x00" + b"NOT A RESTART!!!" + b"\x10\x00\x00\x00")
     with pytest.raises(A.VKError):
         A.read_restart(path, g, f2, cv2)

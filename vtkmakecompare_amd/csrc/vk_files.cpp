@@ -103,6 +103,8 @@ struct HostIn {
 int host_in(const double* a, size_t n, HostIn& h, const char* what) {
   if (!a) return fail(VK_ERR_ARGUMENT, std::string(what) + " is null");
   if (is_registered(a)) {
+    // the registration must cover the n values read here
+    if (!resolve(a, n * sizeof(double), what)) return last_status();
     int rc = vk_download(const_cast<double*>(a));
     if (rc) return rc;
     h.p = a;
@@ -144,6 +146,7 @@ int host_out(double* a, size_t n, HostOut& h, const char* what) {
   h.dst = a;
   h.n = n;
   if (is_registered(a)) {
+    if (!resolve(a, n * sizeof(double), what)) return last_status();
     h.reg = true;
     h.p = a;
     return VK_OK;
@@ -262,7 +265,8 @@ constexpr int kRestartFields = 14;  // halo fields, then 4 cellvalue slots
 
 bool put_record(std::FILE* f, const void* data, size_t bytes) {
   const int32_t len = (int32_t)bytes;
-  return std::fwrite(&len, 4, 1, f) == 1 && (bytes == 0 || std::fwrite(data, 1, bytes, f) == bytes) &&
+  return std::fwrite(&len, 4, 1, f) == 1 &&
+         (bytes == 0 || std::fwrite(data, 1, bytes, f) == bytes) &&
          std::fwrite(&len, 4, 1, f) == 1;
 }
 
@@ -307,8 +311,9 @@ int vk_vtk_cellvalue(const int* nx, const int* ny, const int* nz, const double* 
   const double* c = hc.p;
   const size_t sy = (size_t)*nx, sz = sy * (size_t)*ny;
   return write_vtk(path ? path : "cellvalue.vtk", *nx, *ny, *nz, hx.p, hy.p, hz.p,
-                   "SCALARS PHI float",
-                   [&](int i, int j, int k) { return c[(size_t)(i - 1) + sy * (j - 1) + sz * (k - 1)]; });
+                   "SCALARS PHI float", [&](int i, int j, int k) {
+                     return c[(size_t)(i - 1) + sy * (j - 1) + sz * (k - 1)];
+                   });
 }
 
 int vk_write_restart(const char* path, const int* nx, const int* ny, const int* nz,
@@ -375,6 +380,17 @@ int vk_read_restart(const char* path, const int* nx, const int* ny, const int* n
     return fail(VK_ERR_ARGUMENT, "read_restart: the record is for a " + std::to_string(hdr[0]) +
                                      "x" + std::to_string(hdr[1]) + "x" +
                                      std::to_string(hdr[2]) + " grid");
+  }
+  // the whole file's length before anything is written: a short file
+  // leaves every caller array as it was
+  const long long want = (long long)(8 + sizeof(head)) +
+                         kRestartFields * (8 + (long long)(nf * sizeof(double))) +
+                         4 * (8 + (long long)(ni * sizeof(double)));
+  const long long here = std::ftell(f);
+  if (here < 0 || std::fseek(f, 0, SEEK_END) != 0 || std::ftell(f) != want ||
+      std::fseek(f, here, SEEK_SET) != 0) {
+    std::fclose(f);
+    return fail(VK_ERR_ARGUMENT, std::string("read_restart: short or damaged ") + path);
   }
   int rc = VK_OK;
   bool ok = true;
