@@ -82,3 +82,16 @@ def test_solve_p_matches_oracle():
     dp2 = A.DeviceArray.from_host(p)
     *_, it2 = A.solve_p(dp2, du, dv, dw, dt, *m, g.nx, g.ny, g.nz, max_iterations=3)
     assert it2 == 3
+
+
+def test_solve_p_refuses_no_iterations():
+    """An iteration limit below 1 is refused (no residual would be set)."""
+    from vtkmakecompare_amd import api as A
+    A.init()
+    g = G.set_c(8, 6, 6, 2.0, 2.0, 2.0)
+    ut, vt, wt, p = _case(g, 7)
+    m, (du, dv, dw, dp) = _dev(A, g, ut, vt, wt, p)
+    with pytest.raises(A.VKError) as e:
+        A.solve_p(dp, du, dv, dw, 1e-3, *m, g.nx, g.ny, g.nz, max_iterations=0)
+    assert e.value.status == 5  # VK_ERR_ARGUMENT
+    np.testing.assert_array_equal(dp.download(), p)

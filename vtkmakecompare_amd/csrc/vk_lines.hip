@@ -743,6 +743,7 @@ int vk_solve_p(double* p, const double* u_temp, const double* v_temp, const doub
   if (!res || !res_max || !res_int || !res_var)
     return fail(VK_ERR_ARGUMENT, "solve_p: a residual output is null");
   const int nnn = max_iterations ? *max_iterations : 100000;
+  if (nnn < 1) return fail(VK_ERR_ARGUMENT, "solve_p: max_iterations >= 1");
   hipStream_t s = rt().stream;
   double* part = lw.pst + 8;
   double* out = lw.pst + 4;
