@@ -965,11 +965,12 @@ int vk_adjoint_gradient(const double* u, const double* v, const double* w, const
   B.lphi = dev.alloc(nc);
   B.ysol = dev.alloc(nc);
   B.spec = dev.alloc(2 * g.spec_cells());
-  for (auto& q : B.w8) q = dev.alloc(nc);
+  bool w8ok = true;
+  for (auto& q : B.w8) w8ok = (q = dev.alloc(nc)) != nullptr && w8ok;
   B.coef = dev.alloc((size_t)(g.nx / 2 + 1) * (g.nz + 1) + 3 * (size_t)g.ny + 8);
   B.gcv = dev.alloc(g.interior() * 4);
   if (!B.lut[0] || !B.lut[1] || !B.lut[2] || !B.lg[0] || !B.lg[1] || !B.lg[2] || !B.lphi ||
-      !B.ysol || !B.spec || !B.w8[7] || !B.coef || !B.gcv)
+      !B.ysol || !B.spec || !w8ok || !B.coef || !B.gcv)
     return fail(VK_ERR_HIP, "adjoint: out of device memory");
   rc = hip_check(hipMemsetAsync(B.gcv, 0, g.interior() * 4 * sizeof(double), S()), "memset");
   if (rc) return rc;
@@ -1067,11 +1068,12 @@ int vk_adjoint_gradient_revolve(
   B.lphi = dev.alloc(nc);
   B.ysol = dev.alloc(nc);
   B.spec = dev.alloc(2 * g.spec_cells());
-  for (auto& q : B.w8) q = dev.alloc(nc);
+  bool w8ok = true;
+  for (auto& q : B.w8) w8ok = (q = dev.alloc(nc)) != nullptr && w8ok;
   B.coef = dev.alloc((size_t)(g.nx / 2 + 1) * (g.nz + 1) + 3 * (size_t)g.ny + 8);
   B.gcv = dev.alloc(g.interior() * 4);
   if (!B.lut[0] || !B.lut[1] || !B.lut[2] || !B.lg[0] || !B.lg[1] || !B.lg[2] || !B.lphi ||
-      !B.ysol || !B.spec || !B.w8[7] || !B.coef || !B.gcv)
+      !B.ysol || !B.spec || !w8ok || !B.coef || !B.gcv)
     return fail(VK_ERR_HIP, "adjoint: out of device memory");
   Traj T;
   T.P = &P;
