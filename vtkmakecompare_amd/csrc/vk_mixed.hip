@@ -101,13 +101,18 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
   const int j = 1 + by * blockDim.y + threadIdx.y;
   const int k = 1 + bz;
   if (i > g.nx || j > g.ny) return;
-  F3as<double, float> U{x.u, g}, V{x.v, g}, W{x.w, g};
-  F3<float> T{x.t, g}, RC{x.rc, g}, K{x.thc, g};
   const long long o = g.at(i, j, k);
   const float dt = x.dt, eta = x.eta;
   if (PART == 1) {
     const float cv2 = x.cv[g.cv(i, j, k, 2)], cv3 = x.cv[g.cv(i, j, k, 3)],
                 cv4 = x.cv[g.cv(i, j, k, 4)];
+    // the three stencils and the history, every load issued before any
+    // arithmetic; the schedule barrier keeps the compiler from spreading
+    // them over the computation
+    Nb27<double, float> U, V, W;
+    load_velocity_nbhd(U, V, W, x.u, x.v, x.w, g, i, j, k);
+    const float upre = x.u_pre[o], vpre = x.v_pre[o], wpre = x.w_pre[o];
+    __builtin_amdgcn_sched_barrier(0);
     const unsigned mk = x.lf.fx ? dirty_bits(0.0f, cv2, cv3, cv4, 1.0f, 1.0f) : 0u;
     // u_ta read only where its term eta cv u_ta dt is not 0
     float uta = 0.0f, vta = 0.0f, wta = 0.0f;
@@ -115,23 +120,23 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
     if (cv3 != 0.0f) vta = x.v_ta[o];
     if (cv4 != 0.0f) wta = x.w_ta[o];
     // every value computed before the first store (the stores would fence
-    // the later stencil loads: the outputs may alias the inputs as far as
-    // the compiler knows)
+    // the later loads: the outputs may alias the inputs as far as the
+    // compiler knows)
     const float uv = conv_u_r<float>(U, V, W, f, f, i, j, k, dt);
     const float ud = cond_u_r<float>(U, f, i, j, k, x.c0);
     const float ul = eta * cv2 * uta * dt;
-    const float ua = x.a1 * uv + x.a2 * x.u_pre[o] + x.b1 * ud + x.b1 * ul;
+    const float ua = x.a1 * uv + x.a2 * upre + x.b1 * ud + x.b1 * ul;
     const float vv = conv_v_r<float>(U, V, W, f, f, i, j, k, dt);
     const float vd = cond_v_r<float>(V, f, i, j, k, x.c0);
     const float vl = eta * cv3 * vta * dt;
-    const float va = x.a1 * vv + x.a2 * x.v_pre[o] + x.b1 * vd + x.b1 * vl;
+    const float va = x.a1 * vv + x.a2 * vpre + x.b1 * vd + x.b1 * vl;
     const float wv = conv_w_r<float>(U, V, W, f, f, i, j, k, dt);
     const float wd = cond_w_r<float>(W, f, i, j, k, x.c0);
     const float wl = eta * cv4 * wta * dt;
-    const float wa = x.a1 * wv + x.a2 * x.w_pre[o] + x.b1 * wd + x.b1 * wl;
-    const double un = pen_div_m<double>(x.u[o] + (double)ua, x.etad, (double)cv2, x.dtd);
-    const double vn = pen_div_m<double>(x.v[o] + (double)va, x.etad, (double)cv3, x.dtd);
-    const double wn = pen_div_m<double>(x.w[o] + (double)wa, x.etad, (double)cv4, x.dtd);
+    const float wa = x.a1 * wv + x.a2 * wpre + x.b1 * wd + x.b1 * wl;
+    const double un = pen_div_m<double>(U.v[1][1][1] + (double)ua, x.etad, (double)cv2, x.dtd);
+    const double vn = pen_div_m<double>(V.v[1][1][1] + (double)va, x.etad, (double)cv3, x.dtd);
+    const double wn = pen_div_m<double>(W.v[1][1][1] + (double)wa, x.etad, (double)cv4, x.dtd);
     x.u_pre[o] = uv;
     x.ua[o] = un;
     x.v_pre[o] = vv;
@@ -143,6 +148,8 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
     x.wa[o] = wn;
     if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
   } else {
+    F3as<double, float> U{x.u, g}, V{x.v, g}, W{x.w, g};
+    F3<float> T{x.t, g}, RC{x.rc, g}, K{x.thc, g};
     const float cv1 = x.cv[g.cv(i, j, k, 1)];
     const float rc0 = RC(i, j, k);
     const unsigned mk = x.lf.fx ? dirty_bits(cv1, 0.0f, 0.0f, 0.0f, rc0, K(i, j, k)) : 0u;

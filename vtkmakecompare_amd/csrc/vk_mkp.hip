@@ -186,8 +186,10 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
 // the corrected velocity, p and divu are FP64, u_ta is stored in float.
 // Same index ranges and formulas as project_point (mkp.f:141-217); the
 // u_ep2 diagnostic is not produced.  Returns |divu| at the point (0 on the
-// index-0 faces, where divu is not defined).
-template <bool INTERIOR>
+// index-0 faces, where divu is not defined).  PSI: the third solution o.psi
+// is added (known at compile time, so that no phi read waits on a branch
+// and every load of the point can be in flight together).
+template <bool INTERIOR, bool PSI>
 __device__ inline double project_mixed_point(const Geom& g, const RMetrics& r, const float* phi1,
                                              const float* dphi, const double* ut,
                                              const double* vt, const double* wt, double dt,
@@ -197,7 +199,7 @@ __device__ inline double project_mixed_point(const Geom& g, const RMetrics& r, c
   const float* psi = o.psi;
   auto phi = [&](long long q) {
     const double a = (double)phi1[q] + (double)dphi[q];
-    return psi ? a + (double)psi[q] : a;
+    return PSI ? a + (double)psi[q] : a;
   };
   const double ph = phi(c);
   const double gx = (phi(c + 1) - ph) * r.rdx12[i];
@@ -253,6 +255,7 @@ __device__ inline void block_divmax(double a, unsigned long long* slots) {
   }
 }
 
+template <bool PSI>
 __global__ void k_project_mixed(Geom g, RMetrics r, const float* phi1, const float* dphi,
                                 const double* ut, const double* vt, const double* wt, double dt,
                                 MkpOutMixed o) {
@@ -263,8 +266,8 @@ __global__ void k_project_mixed(Geom g, RMetrics r, const float* phi1, const flo
   // lanes past the interior take part in the block maximum with 0
   const double a = (i > g.nx || j > g.ny)
                        ? 0.0
-                       : project_mixed_point<true>(g, r, phi1, dphi, ut, vt, wt, dt, o, i, j,
-                                                   1 + bz);
+                       : project_mixed_point<true, PSI>(g, r, phi1, dphi, ut, vt, wt, dt, o, i,
+                                                        j, 1 + bz);
   block_divmax(a, o.divmax);
 }
 
@@ -272,12 +275,13 @@ __global__ void k_clear_slots(unsigned long long* p) {
   p[blockIdx.x * blockDim.x + threadIdx.x] = 0ull;
 }
 
+template <bool PSI>
 __global__ void k_project_mixed_faces(Geom g, RMetrics r, const float* phi1, const float* dphi,
                                       const double* ut, const double* vt, const double* wt,
                                       double dt, MkpOutMixed o) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   zero_faces(g, q, [&](int i, int j, int k) {
-    project_mixed_point<false>(g, r, phi1, dphi, ut, vt, wt, dt, o, i, j, k);
+    project_mixed_point<false, PSI>(g, r, phi1, dphi, ut, vt, wt, dt, o, i, j, k);
   });
 }
 
@@ -357,10 +361,18 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
   auto project = [&](const char* name) {
     ProfScope ps(name, s);
     k_clear_slots<<<kDivSlots / 256, 256, 0, s>>>(divmax);
-    k_project_mixed<<<interior_grid(g), interior_block(), 0, s>>>(g, r, phi1, dphi, u_temp,
-                                                                 v_temp, w_temp, dt, op);
-    k_project_mixed_faces<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(g, r, phi1, dphi, u_temp,
-                                                                       v_temp, w_temp, dt, op);
+    const unsigned nfb = (unsigned)((nf + 255) / 256);
+    if (op.psi) {
+      k_project_mixed<true><<<interior_grid(g), interior_block(), 0, s>>>(
+          g, r, phi1, dphi, u_temp, v_temp, w_temp, dt, op);
+      k_project_mixed_faces<true><<<nfb, 256, 0, s>>>(g, r, phi1, dphi, u_temp, v_temp, w_temp,
+                                                       dt, op);
+    } else {
+      k_project_mixed<false><<<interior_grid(g), interior_block(), 0, s>>>(
+          g, r, phi1, dphi, u_temp, v_temp, w_temp, dt, op);
+      k_project_mixed_faces<false><<<nfb, 256, 0, s>>>(g, r, phi1, dphi, u_temp, v_temp, w_temp,
+                                                        dt, op);
+    }
   };
   project("project_mixed");
   unsigned long long slots[kDivSlots];

@@ -177,6 +177,56 @@ struct F3as {
   __device__ R operator()(int i, int j, int k) const { return static_cast<R>(p[g.at(i, j, k)]); }
 };
 
+// A halo field's values at the points (i0 + di, j0 + dj, k0 + dk) with
+// |di|, |dj|, |dk| <= 1, loaded into registers before any arithmetic and
+// read as R by the stencil formulas through F3's (i, j, k) interface.
+// Loading them all first puts every load of a point in flight together;
+// left to itself the compiler issues a few, waits, computes and issues the
+// next few.  Only the points loaded may be read: after inlining every
+// index is a constant, so the array lives in registers.
+template <typename T, typename R>
+struct Nb27 {
+  T v[3][3][3];
+  int i0, j0, k0;
+  __device__ R operator()(int i, int j, int k) const {
+    return static_cast<R>(v[i - i0 + 1][j - j0 + 1][k - k0 + 1]);
+  }
+  __device__ void load(const T* __restrict__ p, const Geom& g, long long o, int di, int dj,
+                       int dk) {
+    v[di + 1][dj + 1][dk + 1] = p[o + di + g.sy * dj + g.sz * dk];
+  }
+};
+
+// the points of u, v and w that SET_CONVECT and SET_CONDUCTION read at one
+// point (nse_dominik.f:774-825, :903-939)
+template <typename T, typename R>
+__device__ inline void load_velocity_nbhd(Nb27<T, R>& U, Nb27<T, R>& V, Nb27<T, R>& W,
+                                          const T* u, const T* v, const T* w, const Geom& g,
+                                          int i, int j, int k) {
+  const long long o = g.at(i, j, k);
+  U.i0 = V.i0 = W.i0 = i;
+  U.j0 = V.j0 = W.j0 = j;
+  U.k0 = V.k0 = W.k0 = k;
+  Nb27<T, R>* f[3] = {&U, &V, &W};
+  const T* p[3] = {u, v, w};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    f[c]->load(p[c], g, o, 0, 0, 0);
+    f[c]->load(p[c], g, o, 1, 0, 0);
+    f[c]->load(p[c], g, o, -1, 0, 0);
+    f[c]->load(p[c], g, o, 0, 1, 0);
+    f[c]->load(p[c], g, o, 0, -1, 0);
+    f[c]->load(p[c], g, o, 0, 0, 1);
+    f[c]->load(p[c], g, o, 0, 0, -1);
+  }
+  U.load(u, g, o, -1, 1, 0);
+  U.load(u, g, o, -1, 0, 1);
+  V.load(v, g, o, 1, -1, 0);
+  V.load(v, g, o, 0, -1, 1);
+  W.load(w, g, o, 1, 0, -1);
+  W.load(w, g, o, 0, 1, -1);
+}
+
 // SET_CONVECT u (nse_dominik.f:774-785)
 template <typename R, typename VU, typename VV, typename VW, typename MT, typename RT>
 __device__ inline R conv_u_r(const VU& U, const VV& V, const VW& W, const MT& m, const RT& r,
