@@ -112,6 +112,8 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
     Nb27<double, float> U, V, W;
     load_velocity_nbhd(U, V, W, x.u, x.v, x.w, g, i, j, k);
     const float upre = x.u_pre[o], vpre = x.v_pre[o], wpre = x.w_pre[o];
+    PtMet<float> M;
+    M.load_f(f, i, j, k);
     __builtin_amdgcn_sched_barrier(0);
     const unsigned mk = x.lf.fx ? dirty_bits(0.0f, cv2, cv3, cv4, 1.0f, 1.0f) : 0u;
     // u_ta read only where its term eta cv u_ta dt is not 0
@@ -122,16 +124,16 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
     // every value computed before the first store (the stores would fence
     // the later loads: the outputs may alias the inputs as far as the
     // compiler knows)
-    const float uv = conv_u_r<float>(U, V, W, f, f, i, j, k, dt);
-    const float ud = cond_u_r<float>(U, f, i, j, k, x.c0);
+    const float uv = conv_u_r<float>(U, V, W, M, M, i, j, k, dt);
+    const float ud = cond_u_r<float>(U, M, i, j, k, x.c0);
     const float ul = eta * cv2 * uta * dt;
     const float ua = x.a1 * uv + x.a2 * upre + x.b1 * ud + x.b1 * ul;
-    const float vv = conv_v_r<float>(U, V, W, f, f, i, j, k, dt);
-    const float vd = cond_v_r<float>(V, f, i, j, k, x.c0);
+    const float vv = conv_v_r<float>(U, V, W, M, M, i, j, k, dt);
+    const float vd = cond_v_r<float>(V, M, i, j, k, x.c0);
     const float vl = eta * cv3 * vta * dt;
     const float va = x.a1 * vv + x.a2 * vpre + x.b1 * vd + x.b1 * vl;
-    const float wv = conv_w_r<float>(U, V, W, f, f, i, j, k, dt);
-    const float wd = cond_w_r<float>(W, f, i, j, k, x.c0);
+    const float wv = conv_w_r<float>(U, V, W, M, M, i, j, k, dt);
+    const float wd = cond_w_r<float>(W, M, i, j, k, x.c0);
     const float wl = eta * cv4 * wta * dt;
     const float wa = x.a1 * wv + x.a2 * wpre + x.b1 * wd + x.b1 * wl;
     const double un = pen_div_m<double>(U.v[1][1][1] + (double)ua, x.etad, (double)cv2, x.dtd);
@@ -148,17 +150,23 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
     x.wa[o] = wn;
     if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
   } else {
-    F3as<double, float> U{x.u, g}, V{x.v, g}, W{x.w, g};
-    F3<float> T{x.t, g}, RC{x.rc, g}, K{x.thc, g};
+    // every load issued before any arithmetic, as in PART 1
+    Nb27<float, float> T, RC, K;
+    Nb27<double, float> U, V, W;
     const float cv1 = x.cv[g.cv(i, j, k, 1)];
+    load_scalar_nbhd(T, RC, K, U, V, W, x.t, x.rc, x.thc, x.u, x.v, x.w, g, i, j, k);
+    const float tpre = x.t_pre[o];
+    PtMet<float> M;
+    M.load_f(f, i, j, k);
+    __builtin_amdgcn_sched_barrier(0);
     const float rc0 = RC(i, j, k);
     const unsigned mk = x.lf.fx ? dirty_bits(cv1, 0.0f, 0.0f, 0.0f, rc0, K(i, j, k)) : 0u;
     const float rrc = 1.0f / rc0;
-    const float vt = conv_scalar_r<float>(U, V, W, T, RC, f, f, i, j, k, dt, rrc);
-    const float dtt = diff_scalar_r<float>(T, K, f, f, i, j, k, x.cd, rrc);
-    const float ut = x.a1 * vt + x.a2 * x.t_pre[o] + x.b1 * dtt;
+    const float vt = conv_scalar_r<float>(U, V, W, T, RC, M, M, i, j, k, dt, rrc);
+    const float dtt = diff_scalar_r<float>(T, K, M, M, i, j, k, x.cd, rrc);
+    const float ut = x.a1 * vt + x.a2 * tpre + x.b1 * dtt;
     x.t_pre[o] = vt;
-    x.ut[o] = pen_div_m<float>(x.t[o] + ut, eta, cv1, dt);
+    x.ut[o] = pen_div_m<float>(T(i, j, k) + ut, eta, cv1, dt);
     if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
   }
 }

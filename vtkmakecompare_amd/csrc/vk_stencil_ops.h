@@ -197,6 +197,57 @@ struct Nb27 {
   }
 };
 
+// Entries c - 1, c, c + 1 of a metric array, loaded up front like Nb27's
+// points and read by absolute index.
+template <typename R>
+struct Met3 {
+  R v[3];
+  int c;
+  __device__ R operator[](int q) const { return v[q - c + 1]; }
+  template <typename P>
+  __device__ void load(const P* a, int at) {
+    c = at;
+    v[0] = a[at - 1];
+    v[1] = a[at];
+    v[2] = a[at + 1];
+  }
+};
+
+// The metric arrays and their reciprocals around one point (i, j, k), as
+// one set serving both metric arguments of the stencil formulas.  Entries
+// no formula reads are dead loads, which the compiler drops.
+template <typename R>
+struct PtMet {
+  Met3<R> dx, dy, dz, dx12, dy12, dz12, rdx, rdy, rdz, rdx12, rdy12, rdz12;
+  __device__ void load(const Metrics& m, const RMetrics& r, int i, int j, int k) {
+    dx.load(m.dx, i);
+    dy.load(m.dy, j);
+    dz.load(m.dz, k);
+    dx12.load(m.dx12, i);
+    dy12.load(m.dy12, j);
+    dz12.load(m.dz12, k);
+    rdx.load(r.rdx, i);
+    rdy.load(r.rdy, j);
+    rdz.load(r.rdz, k);
+    rdx12.load(r.rdx12, i);
+    rdy12.load(r.rdy12, j);
+    rdz12.load(r.rdz12, k);
+  }
+  // a metric set with float arrays d, 1/d and 1/d12 per axis (no d12)
+  template <typename FM>
+  __device__ void load_f(const FM& f, int i, int j, int k) {
+    dx.load(f.dx, i);
+    dy.load(f.dy, j);
+    dz.load(f.dz, k);
+    rdx.load(f.rdx, i);
+    rdy.load(f.rdy, j);
+    rdz.load(f.rdz, k);
+    rdx12.load(f.rdx12, i);
+    rdy12.load(f.rdy12, j);
+    rdz12.load(f.rdz12, k);
+  }
+};
+
 // the points of u, v and w that SET_CONVECT and SET_CONDUCTION read at one
 // point (nse_dominik.f:774-825, :903-939)
 template <typename T, typename R>
@@ -225,6 +276,40 @@ __device__ inline void load_velocity_nbhd(Nb27<T, R>& U, Nb27<T, R>& V, Nb27<T, 
   V.load(v, g, o, 0, -1, 1);
   W.load(w, g, o, 1, 0, -1);
   W.load(w, g, o, 0, 1, -1);
+}
+
+// the points SET_CONVECTION_FOR_SCALAR and SET_DIFFUSION_FOR_SCALAR read
+// at one point (nse_dominik.f:852-879, :969-999): the 7-point stencils of
+// T, rhoCp and thc, and each velocity component at the point and its
+// lower neighbour along the component's axis
+template <typename TS, typename TV, typename R>
+__device__ inline void load_scalar_nbhd(Nb27<TS, R>& T, Nb27<TS, R>& RC, Nb27<TS, R>& K,
+                                        Nb27<TV, R>& U, Nb27<TV, R>& V, Nb27<TV, R>& W,
+                                        const TS* t, const TS* rc, const TS* thc, const TV* u,
+                                        const TV* v, const TV* w, const Geom& g, int i, int j,
+                                        int k) {
+  const long long o = g.at(i, j, k);
+  T.i0 = RC.i0 = K.i0 = U.i0 = V.i0 = W.i0 = i;
+  T.j0 = RC.j0 = K.j0 = U.j0 = V.j0 = W.j0 = j;
+  T.k0 = RC.k0 = K.k0 = U.k0 = V.k0 = W.k0 = k;
+  Nb27<TS, R>* f[3] = {&T, &RC, &K};
+  const TS* p[3] = {t, rc, thc};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    f[c]->load(p[c], g, o, 0, 0, 0);
+    f[c]->load(p[c], g, o, 1, 0, 0);
+    f[c]->load(p[c], g, o, -1, 0, 0);
+    f[c]->load(p[c], g, o, 0, 1, 0);
+    f[c]->load(p[c], g, o, 0, -1, 0);
+    f[c]->load(p[c], g, o, 0, 0, 1);
+    f[c]->load(p[c], g, o, 0, 0, -1);
+  }
+  U.load(u, g, o, 0, 0, 0);
+  U.load(u, g, o, -1, 0, 0);
+  V.load(v, g, o, 0, 0, 0);
+  V.load(v, g, o, 0, -1, 0);
+  W.load(w, g, o, 0, 0, 0);
+  W.load(w, g, o, 0, 0, -1);
 }
 
 // SET_CONVECT u (nse_dominik.f:774-785)

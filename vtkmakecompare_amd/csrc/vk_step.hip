@@ -38,7 +38,6 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
   int j = 1 + by * blockDim.y + threadIdx.y;
   int k = 1 + bz;
   if (i > g.nx || j > g.ny) return;
-  F3<double> U{x.u, g}, V{x.v, g}, W{x.w, g}, T{x.t, g}, RC{x.rc, g}, K{x.thc, g};
   const long long o = g.at(i, j, k);
   const double dt = x.dt, eta = x.eta;
   if (PART == 1) {
@@ -46,6 +45,16 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
     // alias them as far as the compiler knows)
     const double cv2 = x.cv[g.cv(i, j, k, 2)], cv3 = x.cv[g.cv(i, j, k, 3)],
                  cv4 = x.cv[g.cv(i, j, k, 4)];
+    // the three stencils and the history, every load issued before any
+    // arithmetic (left to itself the compiler issues a few loads, waits,
+    // computes and issues the next few); the values and the operations on
+    // them are those of the F3 views
+    Nb27<double, double> U, V, W;
+    load_velocity_nbhd(U, V, W, x.u, x.v, x.w, g, i, j, k);
+    const double upre = x.u_pre[o], vpre = x.v_pre[o], wpre = x.w_pre[o];
+    PtMet<double> M;
+    M.load(m, r, i, j, k);
+    __builtin_amdgcn_sched_barrier(0);
     const unsigned mk = x.lf.fx ? dirty_bits(0.0, cv2, cv3, cv4, 1.0, 1.0) : 0u;
     // u_ta enters only through eta cv u_ta dt: where the face value is 0
     // the term is 0 and u_ta is not read (in fluid, most of the grid, the
@@ -59,21 +68,21 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
     // store, so the loads of all three stencils can be in flight together
     // (a store would otherwise fence the later loads: the outputs may
     // alias the inputs as far as the compiler knows).
-    const double uv = conv_u_r(U, V, W, m, r, i, j, k, dt);
-    const double ud = cond_u_r(U, r, i, j, k, x.c0);
+    const double uv = conv_u_r(U, V, W, M, M, i, j, k, dt);
+    const double ud = cond_u_r(U, M, i, j, k, x.c0);
     const double ul = eta * cv2 * uta * dt;
-    const double ua = x.a1 * uv + x.a2 * x.u_pre[o] + x.b1 * ud + x.b1 * ul;
-    const double vv = conv_v_r(U, V, W, m, r, i, j, k, dt);
-    const double vd = cond_v_r(V, r, i, j, k, x.c0);
+    const double ua = x.a1 * uv + x.a2 * upre + x.b1 * ud + x.b1 * ul;
+    const double vv = conv_v_r(U, V, W, M, M, i, j, k, dt);
+    const double vd = cond_v_r(V, M, i, j, k, x.c0);
     const double vl = eta * cv3 * vta * dt;
-    const double va = x.a1 * vv + x.a2 * x.v_pre[o] + x.b1 * vd + x.b1 * vl;
-    const double wv = conv_w_r(U, V, W, m, r, i, j, k, dt);
-    const double wd = cond_w_r(W, r, i, j, k, x.c0);
+    const double va = x.a1 * vv + x.a2 * vpre + x.b1 * vd + x.b1 * vl;
+    const double wv = conv_w_r(U, V, W, M, M, i, j, k, dt);
+    const double wd = cond_w_r(W, M, i, j, k, x.c0);
     const double wl = eta * cv4 * wta * dt;
-    const double wa = x.a1 * wv + x.a2 * x.w_pre[o] + x.b1 * wd + x.b1 * wl;
-    const double un = pen_div(x.u[o] + ua, eta, cv2, dt);
-    const double vn = pen_div(x.v[o] + va, eta, cv3, dt);
-    const double wn = pen_div(x.w[o] + wa, eta, cv4, dt);
+    const double wa = x.a1 * wv + x.a2 * wpre + x.b1 * wd + x.b1 * wl;
+    const double un = pen_div(U(i, j, k) + ua, eta, cv2, dt);
+    const double vn = pen_div(V(i, j, k) + va, eta, cv3, dt);
+    const double wn = pen_div(W(i, j, k) + wa, eta, cv4, dt);
     x.u_pre[o] = uv;
     x.ua[o] = un;
     x.v_pre[o] = vv;
@@ -85,16 +94,23 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
     x.wa[o] = wn;
     if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
   } else {
-    // T: convection, diffusion, T_l = 0, combine, history, main-loop divide
+    // T: convection, diffusion, T_l = 0, combine, history, main-loop divide;
+    // every load issued before any arithmetic, as in PART 1
+    Nb27<double, double> T, RC, K, U, V, W;
     const double cv1 = x.cv[g.cv(i, j, k, 1)];
+    load_scalar_nbhd(T, RC, K, U, V, W, x.t, x.rc, x.thc, x.u, x.v, x.w, g, i, j, k);
+    const double tpre = x.t_pre[o];
+    PtMet<double> M;
+    M.load(m, r, i, j, k);
+    __builtin_amdgcn_sched_barrier(0);
     const double rc0 = RC(i, j, k);
     const unsigned mk = x.lf.fx ? dirty_bits(cv1, 0.0, 0.0, 0.0, rc0, K(i, j, k)) : 0u;
     double rrc = 1.0 / rc0;
-    double vt = conv_scalar_r(U, V, W, T, RC, m, r, i, j, k, dt, rrc);
-    double dtt = diff_scalar_r(T, K, m, r, i, j, k, x.cd, rrc);
-    double ut = x.a1 * vt + x.a2 * x.t_pre[o] + x.b1 * dtt;
+    double vt = conv_scalar_r(U, V, W, T, RC, M, M, i, j, k, dt, rrc);
+    double dtt = diff_scalar_r(T, K, M, M, i, j, k, x.cd, rrc);
+    double ut = x.a1 * vt + x.a2 * tpre + x.b1 * dtt;
     x.t_pre[o] = vt;
-    x.ut[o] = pen_div(x.t[o] + ut, eta, cv1, dt);
+    x.ut[o] = pen_div(T(i, j, k) + ut, eta, cv1, dt);
     if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
   }
 }
