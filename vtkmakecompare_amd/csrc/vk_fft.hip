@@ -201,8 +201,8 @@ __device__ __forceinline__ void stockham_bfly(typename Cx<R>::T& v0, typename Cx
 // returns the buffer that holds the result.  A barrier follows each pass.
 template <typename R>
 __device__ typename Cx<R>::T* fft_lds(typename Cx<R>::T* a, typename Cx<R>::T* b, int t, int T,
-                                      const FftPlan<R>& p, int sign, int s_begin = 0,
-                                      int s_end = -1) {
+                                      const FftPlan<R>& p, const typename Cx<R>::T* tw, int sign,
+                                      int s_begin = 0, int s_end = -1) {
   using C = typename Cx<R>::T;
   const int n = p.n;
   if (s_end < 0) s_end = p.nst;
@@ -219,7 +219,7 @@ __device__ typename Cx<R>::T* fft_lds(typename Cx<R>::T* a, typename Cx<R>::T* b
       C v0 = src[j], v1 = src[j + nb], v2 = v0, v3 = v0;
       if (Rd > 2) v2 = src[j + 2 * nb];
       if (Rd > 3) v3 = src[j + 3 * nb];
-      stockham_bfly<R>(v0, v1, v2, v3, Rd, kk, twstep, p.tw, sign);
+      stockham_bfly<R>(v0, v1, v2, v3, Rd, kk, twstep, tw, sign);
       const int base = jq * ns * Rd + kk;
       dst[base] = v0;
       dst[base + ns] = v1;
@@ -239,16 +239,21 @@ __device__ typename Cx<R>::T* fft_lds(typename Cx<R>::T* a, typename Cx<R>::T* b
 // right-hand side div(u_temp)/dt computed on the fly (mkp.f:88-121, where
 // index 0 of x and z is the periodic image of nx and nz), or the residual
 // of a Poisson solution against that right-hand side.  Samples are
-// computed in FP64 whatever the transform's precision.
+// computed in FP64 whatever the transform's precision.  post() runs once
+// per interior sample after every load of its chunk has been issued (a
+// store among the loads would fence the ones after it).
 struct SampleField {
+  static constexpr int kChunk = 0;  // samples per chunk, 0: by precision
   const double* fp;
   __device__ double operator()(const Geom& g, int i, int j, int k) const {
     return fp[g.at(i, j, k)];
   }
+  __device__ void post(const Geom&, int, int, int, double) const {}
 };
 // (mkp.f:88-99 divides by the metrics and by dt; multiplying by their
 // reciprocals agrees to rounding)
 struct SampleDiv {
+  static constexpr int kChunk = 0;
   const double *u, *v, *w, *rdx, *rdy, *rdz;
   double rdt;
   __device__ double operator()(const Geom& g, int i, int j, int k) const {
@@ -258,16 +263,19 @@ struct SampleDiv {
             (w[o] - w[o - g.sz]) * rdz[kk]) *
            rdt;
   }
+  __device__ void post(const Geom&, int, int, int, double) const {}
 };
 // div(u_temp)/dt, also stored in FP64 at its interior point (every
 // interior point is sampled exactly once by the forward x pass)
 struct SampleDivStore {
+  static constexpr int kChunk = 0;
   SampleDiv div;
   double* out;
   __device__ double operator()(const Geom& g, int i, int j, int k) const {
-    const double v = div(g, i, j, k);
+    return div(g, i, j, k);
+  }
+  __device__ void post(const Geom& g, int i, int j, int k, double v) const {
     out[g.at((i == 0) ? g.nx : i, j, (k == 0) ? g.nz : k)] = v;
-    return v;
   }
 };
 
@@ -277,6 +285,7 @@ struct SampleDivStore {
 // 209-217): the divergence the projected velocity would have, over dt.
 // All FP64.
 struct SampleResid {
+  static constexpr int kChunk = 2;
   const double* rhs;
   const float* phi;
   const double *rdx12, *rdy12, *rdz12, *rdx, *rdy, *rdz;
@@ -292,25 +301,60 @@ struct SampleResid {
                        (c - (double)phi[o - g.sz]) * rdz12[kk - 1]) * rdz[kk];
     return rhs[o] - (lx + ly + lz);
   }
+  __device__ void post(const Geom&, int, int, int, double) const {}
 };
 
 // f(i,j,k) / dy(j) with x and z index 0 read as the periodic image n
 struct SampleWScaled {
+  static constexpr int kChunk = 0;
   const double *f, *rdy;
   __device__ double operator()(const Geom& g, int i, int j, int k) const {
     int ii = (i == 0) ? g.nx : i, kk = (k == 0) ? g.nz : k;
     return f[g.at(ii, j, kk)] * rdy[j];
   }
+  __device__ void post(const Geom&, int, int, int, double) const {}
 };
 
 constexpr int kRowsPerBlock = 4;
 constexpr int kRowThreads = 64;
+
+// Samples a row thread evaluates before it stores any, so that all their
+// loads are in flight together: a round of loads per chunk rather than
+// per sample (the samples are FP64 and take more registers in the FP64
+// transform).
+template <typename Sample, typename R>
+constexpr int sample_chunk() {
+  return Sample::kChunk > 0 ? Sample::kChunk : (sizeof(R) == 4 ? 6 : 3);
+}
+// The float transforms read their twiddle tables from LDS, staged once per
+// block with the first loads: a pass then waits on no global load.  (The
+// FP64 rows fill LDS to its occupancy limit already, and the FP64
+// transforms are bound by HBM bandwidth, not by these waits.)
+template <typename R>
+constexpr bool twiddles_in_lds() {
+  return sizeof(R) == 4;
+}
+// twiddle entries a thread of an x block stages at most (2 M + 1 entries,
+// M = nx / 2 <= 512)
+constexpr int kTwPerThread = (2 * 512 + 1 + 255) / 256;
+
+// LDS bytes of an x block: four rows of 2M complex, and with the twiddles
+// in LDS the M entries of the plan's table and the M + 1 of the split's
+template <typename R>
+inline size_t xrow_lds(int nx) {
+  const size_t m = (size_t)nx / 2;
+  return sizeof(typename Cx<R>::T) *
+         (2 * m * kRowsPerBlock + (twiddles_in_lds<R>() ? 2 * m + 1 : 0));
+}
 
 // forward x pass: rows (j, k) = (1 + blockIdx.x*4 + row, blockIdx.y)
 template <typename Sample, typename R>
 __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_rfft_x(
     Geom g, Sample smp, R* phit, FftPlan<R> p, const typename Cx<R>::T* twl) {
   using C = typename Cx<R>::T;
+  constexpr int NT = kRowsPerBlock * kRowThreads;
+  constexpr int CH = sample_chunk<Sample, R>();
+  constexpr bool TWL = twiddles_in_lds<R>();
   extern __shared__ unsigned char shraw[];
   C* sh = reinterpret_cast<C*>(shraw);
   const int M = p.n;  // nx/2
@@ -321,10 +365,49 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_rfft_x(
   C* a = sh + row * 2 * M;
   C* b = a + M;
   const bool ok = j <= g.ny;
+  const int js = ok ? j : g.ny;  // rows past ny sample row ny and keep zeros
   R* ar = reinterpret_cast<R*>(a);
-  for (int i = t; i < 2 * M; i += kRowThreads) ar[i] = ok ? (R)smp(g, i, j, k) : (R)0;
+  // twiddles: entries 0..M-1 the plan's table, M..2M the split's
+  C* tws = sh + kRowsPerBlock * 2 * M;
+  C tv[kTwPerThread];
+  if (TWL) {
+#pragma unroll
+    for (int u = 0; u < kTwPerThread; u++) {
+      const int q = (int)threadIdx.x + u * NT;
+      if (q <= 2 * M) tv[u] = q < M ? p.tw[q] : twl[q - M];
+    }
+  }
+  for (int i0 = 0; i0 < 2 * M; i0 += CH * kRowThreads) {
+    double v[CH];
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      const int i = i0 + c * kRowThreads + t;
+      v[c] = smp(g, i < 2 * M ? i : 2 * M - 1, js, k);
+    }
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      const int i = i0 + c * kRowThreads + t;
+      if (i < 2 * M) {
+        ar[i] = ok ? (R)v[c] : (R)0;
+        if (ok) smp.post(g, i, j, k, v[c]);
+      }
+    }
+  }
+  if (TWL) {
+#pragma unroll
+    for (int u = 0; u < kTwPerThread; u++) {
+      const int q = (int)threadIdx.x + u * NT;
+      if (q <= 2 * M) tws[q] = tv[u];
+    }
+  }
   __syncthreads();
-  C* z = fft_lds<R>(a, b, t, kRowThreads, p, -1);
+  const C* tw1 = p.tw;
+  const C* tw2 = twl;
+  if constexpr (TWL) {
+    tw1 = tws;
+    tw2 = tws + M;
+  }
+  C* z = fft_lds<R>(a, b, t, kRowThreads, p, tw1, -1);
   if (!ok) return;
   // X_k = E_k + e^{-2 pi i k/nx} O_k with E, O from Z_k and conj(Z_{M-k})
   for (int q = t; q <= M; q += kRowThreads) {
@@ -332,7 +415,7 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_rfft_x(
     C e = mkc<C>((R)0.5 * (zk.x + zc.x), (R)0.5 * (zk.y + zc.y));
     C dd = csub(zk, zc);
     C o = mkc<C>((R)0.5 * dd.y, (R)-0.5 * dd.x);  // -i/2 (zk - zc)
-    C x = cadd(e, cmul(twl[q], o));
+    C x = cadd(e, cmul(tw2[q], o));
     long long off = 2 * g.spec_at(q, j, k);
     phit[off] = x.x;
     phit[off + 1] = x.y;
@@ -439,6 +522,11 @@ __global__ __launch_bounds__(256) void k_cfft_z(Geom g, R* phit, FftPlan<R> p, i
 __device__ __forceinline__ int zsw(int jj) { return jj ^ ((jj >> 2) & 3); }
 constexpr int kZ1Threads = 512;  // threads per block
 constexpr int kZ1Vpt = 6;        // rows per thread and pass, at most
+// longest column the kernel takes (2 columns per block: kZ1Vpt rows for
+// each of 256 threads)
+constexpr int kZ1MaxN = kZ1Vpt * kZ1Threads / 2;
+// LDS rows of a column buffer (rounded up to 4 for the swizzle)
+inline __host__ __device__ size_t z1_rows(int n) { return ((size_t)n + 3) / 4 * 4; }
 
 // (aligned to its size, so that a thread's LDS access is one 16-byte word)
 template <typename R, int W>
@@ -448,7 +536,7 @@ struct alignas(sizeof(typename Cx<R>::T) * W) ZCols {
 
 template <typename R, int W, int CC, int TPC, int Rd>
 __device__ __forceinline__ void cfft_z_pass(R* phit, ZCols<R, W>* sh, const FftPlan<R>& p,
-                                            int sign, int N, int ns, bool first, bool last,
+                                            const typename Cx<R>::T* tw, int sign, int N, int ns, bool first, bool last,
                                             int c, int t, const bool* okc, long long o0,
                                             long long ks, R rsc) {
   using C = typename Cx<R>::T;
@@ -488,7 +576,7 @@ __device__ __forceinline__ void cfft_z_pass(R* phit, ZCols<R, W>* sh, const FftP
 #pragma unroll
       for (int w = 0; w < W; w++)
         stockham_bfly<R>(v[b][0].c[w], v[b][1].c[w], v[b][2].c[w], v[b][3].c[w], Rd, kk, twstep,
-                         p.tw, sign);
+                         tw, sign);
       const int base = jq * ns * Rd + kk;
 #pragma unroll
       for (int m = 0; m < Rd; m++) {
@@ -528,22 +616,49 @@ __global__ __launch_bounds__(kZ1Threads) void k_cfft_z1(Geom g, R* phit, FftPlan
   const long long o0 = 2 * g.spec_at(okc[0] ? I : 0, jrow, 0);
   const long long ks = 2 * g.spec_at(0, 0, 1);
   const R rsc = scale ? (R)(1.0 / ((double)g.nx * (double)g.nz)) : (R)1;
+  // float: the twiddle table (N entries) staged in LDS past the column
+  // buffer, loaded with the first pass's reads (which need no twiddle) and
+  // stored after it
+  using C = typename Cx<R>::T;
+  constexpr bool TWL = twiddles_in_lds<R>();
+  constexpr int TWP = kZ1MaxN / kZ1Threads;
+  C* tws = reinterpret_cast<C*>(shraw + z1_rows(N) * CC * sizeof(ZCols<R, W>));
+  C tv[TWP];
+  if (TWL) {
+#pragma unroll
+    for (int u = 0; u < TWP; u++) {
+      const int q = (int)threadIdx.x + u * kZ1Threads;
+      if (q < N) tv[u] = p.tw[q];
+    }
+  }
+  // the first pass reads no twiddle (ns = 1), so the plan may point at the
+  // LDS copy from the start
+  const C* tw1 = p.tw;
+  if constexpr (TWL) tw1 = tws;
   int ns = 1;
   for (int s = 0; s < p.nst; s++) {
     const bool first = s == 0, last = s == p.nst - 1;
     switch (p.r[s]) {
       case 4:
-        cfft_z_pass<R, W, CC, TPC, 4>(phit, sh, p, sign, N, ns, first, last, c, t, okc, o0, ks,
+        cfft_z_pass<R, W, CC, TPC, 4>(phit, sh, p, tw1, sign, N, ns, first, last, c, t, okc, o0, ks,
                                       rsc);
         break;
       case 3:
-        cfft_z_pass<R, W, CC, TPC, 3>(phit, sh, p, sign, N, ns, first, last, c, t, okc, o0, ks,
+        cfft_z_pass<R, W, CC, TPC, 3>(phit, sh, p, tw1, sign, N, ns, first, last, c, t, okc, o0, ks,
                                       rsc);
         break;
       default:
-        cfft_z_pass<R, W, CC, TPC, 2>(phit, sh, p, sign, N, ns, first, last, c, t, okc, o0, ks,
+        cfft_z_pass<R, W, CC, TPC, 2>(phit, sh, p, tw1, sign, N, ns, first, last, c, t, okc, o0, ks,
                                       rsc);
         break;
+    }
+    if (TWL && first) {
+#pragma unroll
+      for (int u = 0; u < TWP; u++) {
+        const int q = (int)threadIdx.x + u * kZ1Threads;
+        if (q < N) tws[q] = tv[u];
+      }
+      __syncthreads();
     }
     ns *= p.r[s];
   }
@@ -560,10 +675,11 @@ static bool launch_cfft_z1(const Geom& g, R* phit, const FftPlan<R>& pz, int sig
   using E = ZCols<R, W>;
   const int N = g.nz;
   const int nh1 = g.nx / 2 + 1;
-  const size_t rows = ((size_t)N + 3) / 4 * 4;
+  const size_t rows = z1_rows(N);
   if (pz.nst < 2) return false;
   for (int cc = (int)(128 / sizeof(E)); cc >= 2; cc /= 2) {
-    const size_t lds = rows * cc * sizeof(E);
+    const size_t lds =
+        rows * cc * sizeof(E) + (twiddles_in_lds<R>() ? N * sizeof(typename Cx<R>::T) : 0);
     const int tpc = kZ1Threads / cc;
     if (lds > 65536 || (N + tpc - 1) / tpc > kZ1Vpt) continue;
     const dim3 grid((unsigned)((nh1 + cc * W - 1) / (cc * W)), (unsigned)g.ny);
@@ -608,10 +724,26 @@ template <typename R>
 __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_irfft_x(
     Geom g, const R* phit, R* fp, FftPlan<R> p, const typename Cx<R>::T* twl) {
   using C = typename Cx<R>::T;
+  constexpr int NT = kRowsPerBlock * kRowThreads;
+  constexpr bool TWL = twiddles_in_lds<R>();
   extern __shared__ unsigned char shraw[];
   C* sh = reinterpret_cast<C*>(shraw);
   const int M = p.n;
   const int row = threadIdx.x / kRowThreads, t = threadIdx.x % kRowThreads;
+  // the plan's twiddles (M entries) staged in LDS in float, loaded here
+  // with the first pass's reads and stored before its barrier; the first
+  // pass itself needs none
+  C* tws = sh + kRowsPerBlock * 2 * M;
+  C tv[kTwPerThread];
+  if (TWL) {
+#pragma unroll
+    for (int u = 0; u < kTwPerThread; u++) {
+      const int q = (int)threadIdx.x + u * NT;
+      if (q < M) tv[u] = p.tw[q];
+    }
+  }
+  const C* tw1 = p.tw;
+  if constexpr (TWL) tw1 = tws;
   int bx, by, bz;
   xcd_block(bx, by, bz);
   const int j = 1 + bx * kRowsPerBlock + row, k = by;
@@ -621,13 +753,15 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_irfft_x(
   // Z_k = (X_k + conj X_{M-k}) + i e^{2 pi i k/nx} (X_k - conj X_{M-k}),
   // with the imaginary parts of X_0 and X_M dropped as a c2r transform does;
   // the first pass reads its inputs this way straight from the spectrum
+  // (the loads are unconditional, rows past ny reading row ny, so that the
+  // inputs of all a thread's butterflies are in flight together)
+  const int js = ok ? j : g.ny;
   auto zin = [&](int q) {
-    C xk = mkc<C>((R)0, (R)0), xc = mkc<C>((R)0, (R)0);
-    if (ok) {
-      long long o1 = 2 * g.spec_at(q, j, k), o2 = 2 * g.spec_at(M - q, j, k);
-      xk = mkc<C>(phit[o1], q == 0 ? (R)0 : phit[o1 + 1]);
-      xc = mkc<C>(phit[o2], (q == 0) ? (R)0 : -phit[o2 + 1]);
-    }
+    const long long o1 = 2 * g.spec_at(q, js, k), o2 = 2 * g.spec_at(M - q, js, k);
+    const R x1r = phit[o1], x1i = phit[o1 + 1], x2r = phit[o2], x2i = phit[o2 + 1];
+    const C zero = mkc<C>((R)0, (R)0);
+    const C xk = ok ? mkc<C>(x1r, q == 0 ? (R)0 : x1i) : zero;
+    const C xc = ok ? mkc<C>(x2r, q == 0 ? (R)0 : -x2i) : zero;
     C sum = cadd(xk, xc), dif = csub(xk, xc);
     C r = cmul(conjc(twl[q]), dif);
     return mkc<C>(sum.x - r.y, sum.y + r.x);
@@ -679,9 +813,16 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_irfft_x(
       if (Rd > 2) b[base + 2] = v2;
       if (Rd > 3) b[base + 3] = v3;
     }
+    if (TWL) {
+#pragma unroll
+      for (int u = 0; u < kTwPerThread; u++) {
+        const int q = (int)threadIdx.x + u * NT;
+        if (q < M) tws[q] = tv[u];
+      }
+    }
     __syncthreads();
   }
-  const C* z = fft_lds<R>(b, a, t, kRowThreads, p, +1, 1, last);
+  const C* z = fft_lds<R>(b, a, t, kRowThreads, p, tw1, +1, 1, last);
   {
     const int Rd = p.r[last], nb = M / Rd;
     const int twstep = M / (ns * Rd);
@@ -690,7 +831,7 @@ __global__ __launch_bounds__(kRowsPerBlock* kRowThreads) void k_irfft_x(
       C v0 = z[jj], v1 = z[jj + nb], v2 = v0, v3 = v0;
       if (Rd > 2) v2 = z[jj + 2 * nb];
       if (Rd > 3) v3 = z[jj + 3 * nb];
-      stockham_bfly<R>(v0, v1, v2, v3, Rd, kk, twstep, p.tw, +1);
+      stockham_bfly<R>(v0, v1, v2, v3, Rd, kk, twstep, tw1, +1);
       const int base = jq * ns * Rd + kk;
       if (ok) {
         out(base, v0);
@@ -744,7 +885,7 @@ static int launch_forward(const Geom& g, Sample smp, R* phit, const char* xname,
   const C* twl;
   int rc = plans_for<R>(g, px, pz, twl);
   if (rc) return rc;
-  size_t shx = sizeof(C) * 2 * (g.nx / 2) * kRowsPerBlock;
+  const size_t shx = xrow_lds<R>(g.nx);
   {
     ProfScope ps(xname, s);
     k_rfft_x<Sample, R><<<dim3((g.ny + kRowsPerBlock - 1) / kRowsPerBlock, g.nz),
@@ -807,7 +948,7 @@ int launch_s2p_inplace(const Geom& g, R* phit, R* fp, hipStream_t s, const char*
     ProfScope ps(zname, s);
     launch_cfft_z<R>(g, phit, pz, +1, 0, s);
   }
-  size_t shx = sizeof(C) * 2 * (g.nx / 2) * kRowsPerBlock;
+  const size_t shx = xrow_lds<R>(g.nx);
   ProfScope ps(xname, s);
   k_irfft_x<R><<<dim3((g.ny + kRowsPerBlock - 1) / kRowsPerBlock, g.nz),
                  kRowsPerBlock * kRowThreads, shx, s>>>(g, phit, fp, px, twl);
