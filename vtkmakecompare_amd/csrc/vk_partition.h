@@ -573,8 +573,9 @@ __global__ __launch_bounds__(NT) void k_line_general_q(Sys sys, LineQueue Q, uns
 }
 
 // The recorded-multiplier path.  The reference line's table (validated for
-// this call by k_line_factor) is staged in LDS from offset o_f on, once per
-// block and together with its first data loads; the block then takes GPB
+// this call by k_line_factor, which also keeps a copy in the LDS order of
+// LdsTab, so that staging it is one contiguous copy) is staged in LDS once
+// per block and together with its first data loads; the block then takes GPB
 // groups of LPB lines in turn.  A line whose rows all have the reference
 // inputs applies the recorded multipliers: its rows then have the
 // reference coefficients bit for bit, so it performs the same operations on
@@ -643,7 +644,7 @@ __global__ __launch_bounds__(NT) void k_line_fast(Sys sys, const double* __restr
 #pragma unroll
         for (int k = 0; k < KT; k++) {
           const int u = u0 + k * NT + (int)threadIdx.x;
-          v[k] = tab[T.src(u < T.total ? u : T.total - 1, n, L)];
+          v[k] = tab[u < T.total ? u : T.total - 1];
         }
 #pragma unroll
         for (int k = 0; k < KT; k++) {
@@ -717,7 +718,8 @@ __global__ __launch_bounds__(NT) void k_line_fast(Sys sys, const double* __restr
 // `fresh` is false; otherwise the line is factored again and every
 // multiplier applied to the right-hand side is recorded.
 template <int S, int MMAX, typename Sys>
-__global__ __launch_bounds__(S) void k_line_factor(Sys sys, double* tab, TabLayout L, int fresh) {
+__global__ __launch_bounds__(S) void k_line_factor(Sys sys, double* tab, TabLayout L, int fresh,
+                                                   double* tabs) {
   constexpr int P = 2 * S;
   __shared__ double s_a[P], s_b[P], s_c[P], s_d[P];
   const int seg = threadIdx.x;
@@ -766,6 +768,12 @@ __global__ __launch_bounds__(S) void k_line_factor(Sys sys, double* tab, TabLayo
   double x0, x1;
   pcr_pair<S, true, double>(s_a, s_b, s_c, LdsVec<double>{s_d, P}, sl, seg, Sys::kCyclic, x0, x1,
                             tab, L);
+  // the copy in LDS order (LdsTab::src), read back after every thread's
+  // table stores are visible to the block
+  __threadfence();
+  __syncthreads();
+  const LdsTab<S, MMAX> T(L.steps);
+  for (int u = seg; u < T.total; u += S) tabs[u] = tab[T.src(u, n, L)];
 }
 
 // The queue shared by the sweeps on the library stream, sized for at
@@ -789,9 +797,11 @@ inline int line_sweep(const Sys& sys, int n, int nla, int nlb, hipStream_t s) {
   TabLayout L = tab_layout(n, SF, Sys::kCyclic);
   const size_t lds = sizeof(double) * ((size_t)LdsTab<SF, MF>(L.steps).total +
                                        (size_t)LPB_F * 2 * SF * vwords<typename Sys::V>());
+  const long staged = LdsTab<SF, MF>(L.steps).total;
   if (Sys::kHasRef && lds <= 65536 && (n + SF - 1) / SF <= MF && n >= 2 * SF) {
     auto key = sys.ref_key(n, SF);
-    tab = line_table(&key, sizeof(key), L.total, &fresh, &validated);  // null: general only
+    // the table and its copy in LDS order; null: general only
+    tab = line_table(&key, sizeof(key), L.total + staged, &fresh, &validated);
   }
   if (tab) {
     const int nbx = (nla + LPB_F - 1) / LPB_F;
@@ -801,8 +811,10 @@ inline int line_sweep(const Sys& sys, int n, int nla, int nlb, hipStream_t s) {
     if (!Q.list) return -2;
     constexpr unsigned GPB = LANE_LINES ? 1u : 8u;  // groups per block, as in k_line_fast
     const unsigned ngb = (nb + GPB - 1) / GPB;
-    if (!validated) k_line_factor<SF, MF, Sys><<<1, SF, 0, s>>>(sys, tab, L, fresh ? 1 : 0);
-    k_line_fast<SF, MF, NT, LANE_LINES, LPB_G, Sys><<<ngb, NT, lds, s>>>(sys, tab, L, Q, nbx, nb);
+    double* tabs = tab + L.total;
+    if (!validated)
+      k_line_factor<SF, MF, Sys><<<1, SF, 0, s>>>(sys, tab, L, fresh ? 1 : 0, tabs);
+    k_line_fast<SF, MF, NT, LANE_LINES, LPB_G, Sys><<<ngb, NT, lds, s>>>(sys, tabs, L, Q, nbx, nb);
     const unsigned ngen = nb < 1024u ? nb : 1024u;
     k_line_general_q<SG, MG, NTG, LANE_LINES, Sys><<<ngen, NTG, 0, s>>>(sys, Q, nla_pad);
   } else {
