@@ -84,6 +84,42 @@ __device__ inline void xcd_block(int& bx, int& by, int& bz) {
   bz = (int)(q / (gridDim.x * gridDim.y));
 }
 
+// xcd_block in bricks, for stencil kernels over a 3D grid of blocks whose z
+// extent is a multiple of 8 (otherwise it is xcd_block).  Each XCD still
+// takes one slab of gridDim.z / 8 planes, but walks it a brick at a time:
+// RJ rows of tiles (all of x) through every plane of the slab, then the
+// next RJ rows.  The blocks in flight on an XCD then span a few planes of
+// one brick, so the planes a stencil reads again (k - 1, k + 1) and the
+// neighbouring rows are still in that XCD's L2; walking whole planes, the
+// planes in flight on an XCD hold more than its L2.
+template <unsigned RJ>
+__device__ inline void xcd_block_brick(int& bx, int& by, int& bz) {
+  const unsigned X = 8;
+  if (gridDim.z % X != 0) {
+    xcd_block(bx, by, bz);
+    return;
+  }
+  const unsigned p = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const unsigned T = gridDim.x * gridDim.y;  // tiles per plane
+  const unsigned kz = gridDim.z / X;         // planes per XCD
+  const unsigned C = gridDim.x * RJ;         // tiles per brick
+  const unsigned x = p % X, r = p / X;       // the XCD and its r-th block
+  const unsigned full = T / C * C;           // tiles in whole bricks
+  unsigned t, kl;
+  if (r < full * kz) {
+    const unsigned c = r / (C * kz), rr = r % (C * kz);
+    kl = rr / C;
+    t = c * C + rr % C;
+  } else {
+    const unsigned rr = r - full * kz, cl = T - full;
+    kl = rr / cl;
+    t = full + rr % cl;
+  }
+  bx = (int)(t % gridDim.x);
+  by = (int)(t / gridDim.x);
+  bz = (int)(x * kz + kl);
+}
+
 // decode halo index q into (i,j,k); returns false past the end.
 // segments: z faces (all i,j), y faces (k interior, all i), x faces
 // (j,k interior), and -- when `vplane` -- the interior of the j = ny plane.

@@ -91,7 +91,7 @@ template <bool EP2>
 __global__ void k_project(Geom g, RMetrics r, const double* phi, const double* ut,
                           const double* vt, const double* wt, double dt, MkpOut o) {
   int bx, by, bz;
-  xcd_block(bx, by, bz);
+  xcd_block_brick<7>(bx, by, bz);
   const int i = 1 + bx * blockDim.x + threadIdx.x;
   const int j = 1 + by * blockDim.y + threadIdx.y;
   if (i > g.nx || j > g.ny) return;
@@ -260,7 +260,7 @@ __global__ void k_project_mixed(Geom g, RMetrics r, const float* phi1, const flo
                                 const double* ut, const double* vt, const double* wt, double dt,
                                 MkpOutMixed o) {
   int bx, by, bz;
-  xcd_block(bx, by, bz);
+  xcd_block_brick<7>(bx, by, bz);
   const int i = 1 + bx * blockDim.x + threadIdx.x;
   const int j = 1 + by * blockDim.y + threadIdx.y;
   // lanes past the interior take part in the block maximum with 0
