@@ -275,6 +275,61 @@ __global__ void k_clear_slots(unsigned long long* p) {
   p[blockIdx.x * blockDim.x + threadIdx.x] = 0ull;
 }
 
+// The slots' maximum into *out, a pinned host word the host reads once the
+// stream has drained, and the slots back to 0 for the next projection (one
+// block of kDivSlots threads).
+__global__ __launch_bounds__(kDivSlots) void k_divmax_final(unsigned long long* slots,
+                                                            unsigned long long* out) {
+  __shared__ unsigned long long wm[kDivSlots / 64];
+  const int t = threadIdx.x;
+  unsigned long long b = slots[t];
+  slots[t] = 0ull;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(b, off);
+    b = o > b ? o : b;
+  }
+  if ((t & 63) == 0) wm[t >> 6] = b;
+  __syncthreads();
+  if (t == 0) {
+    for (int q = 1; q < kDivSlots / 64; q++) b = wm[q] > b ? wm[q] : b;
+    *out = b;
+  }
+}
+
+namespace {
+// one pinned, device-visible host word per process for the readback, and
+// its device address
+unsigned long long* g_divmax_host = nullptr;
+unsigned long long* g_divmax_dev = nullptr;
+void release_divmax_host() {
+  if (g_divmax_host) (void)hipHostFree(g_divmax_host);
+  g_divmax_host = nullptr;
+  g_divmax_dev = nullptr;
+}
+bool divmax_word(unsigned long long** host, unsigned long long** dev) {
+  on_finalize(release_divmax_host);
+  if (!g_divmax_host) {
+    if (hipHostMalloc((void**)&g_divmax_host, 64, hipHostMallocCoherent | hipHostMallocMapped) !=
+        hipSuccess) {
+      g_divmax_host = nullptr;
+      fail(VK_ERR_HIP, "mkp: pinned readback word");
+      return false;
+    }
+    if (hipHostGetDevicePointer((void**)&g_divmax_dev, g_divmax_host, 0) != hipSuccess) {
+      (void)hipHostFree(g_divmax_host);
+      g_divmax_host = nullptr;
+      g_divmax_dev = nullptr;
+      fail(VK_ERR_HIP, "mkp: device address of the readback word");
+      return false;
+    }
+  }
+  *host = g_divmax_host;
+  *dev = g_divmax_dev;
+  return true;
+}
+}  // namespace
+
 template <bool PSI>
 __global__ void k_project_mixed_faces(Geom g, RMetrics r, const float* phi1, const float* dphi,
                                       const double* ut, const double* vt, const double* wt,
@@ -351,16 +406,20 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
     rc = hip_check(hipStreamWaitEvent(s, before_project, 0), "mkp wait");
     if (rc) return rc;
   }
+  bool fresh_slots = false;
   unsigned long long* divmax = static_cast<unsigned long long*>(
-      workspace(kWorkDivMax, kDivSlots * sizeof(unsigned long long)));
-  if (!divmax) return VK_ERR_HIP;
+      workspace(kWorkDivMax, kDivSlots * sizeof(unsigned long long), &fresh_slots));
+  unsigned long long *hmax = nullptr, *dmaxp = nullptr;
+  if (!divmax || !divmax_word(&hmax, &dmaxp)) return VK_ERR_HIP;
+  // k_divmax_final leaves the slots at 0 for the next projection; a new
+  // buffer is cleared here
+  if (fresh_slots) k_clear_slots<<<kDivSlots / 256, 256, 0, s>>>(divmax);
   MkpOutMixed op = o;
   op.divmax = divmax;
   op.psi = nullptr;
   const long long nf = zero_face_count(g);
   auto project = [&](const char* name) {
     ProfScope ps(name, s);
-    k_clear_slots<<<kDivSlots / 256, 256, 0, s>>>(divmax);
     const unsigned nfb = (unsigned)((nf + 255) / 256);
     if (op.psi) {
       k_project_mixed<true><<<interior_grid(g), interior_block(), 0, s>>>(
@@ -373,15 +432,12 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
       k_project_mixed_faces<false><<<nfb, 256, 0, s>>>(g, r, phi1, dphi, u_temp, v_temp, w_temp,
                                                         dt, op);
     }
+    k_divmax_final<<<1, kDivSlots, 0, s>>>(divmax, dmaxp);
   };
   project("project_mixed");
-  unsigned long long slots[kDivSlots];
-  rc = hip_check(hipMemcpyAsync(slots, divmax, sizeof(slots), hipMemcpyDeviceToHost, s),
-                 "mkp divmax");
-  if (!rc) rc = hip_check(hipStreamSynchronize(s), "mkp divmax");
+  rc = hip_check(hipStreamSynchronize(s), "mkp divmax");
   if (rc) return rc;
-  unsigned long long bits = 0;
-  for (int q = 0; q < kDivSlots; q++) bits = slots[q] > bits ? slots[q] : bits;
+  const unsigned long long bits = *(volatile unsigned long long*)hmax;
   double dmax;
   std::memcpy(&dmax, &bits, sizeof(dmax));
   if (!(dmax > kMixedDivTol)) return VK_OK;
