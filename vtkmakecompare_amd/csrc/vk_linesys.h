@@ -381,6 +381,7 @@ struct SysSpectral {
   using V = dcx;
   using Pt = PtNone;
   static constexpr bool kCyclic = false, kHasRef = false, kNb = false;
+  static constexpr bool kWaves4 = true;  // k_line_general_w4
   Geom g;
   TS* phit;  // interleaved complex
   const double *ac, *kxz;

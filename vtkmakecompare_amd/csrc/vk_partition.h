@@ -23,6 +23,8 @@
 //     (x lines, which are contiguous in memory).
 #pragma once
 
+#include <type_traits>
+
 #include "vk_common.h"
 
 namespace vk {
@@ -527,7 +529,7 @@ __device__ inline void line_block_general(const Sys& sys, int la, int lb, bool o
 // la = bx * LPB + (line within block), lb = by of an nbx x nby grid of
 // blocks, in XCD-aware order.
 template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
-__global__ __launch_bounds__(NT) void k_line_general(Sys sys, int nbx, int nby) {
+__device__ inline void line_general_body(const Sys& sys, int nbx) {
   using V = typename Sys::V;
   using M = LineMap<S, NT, LANE_LINES>;
   constexpr int LPB = NT / S;
@@ -542,6 +544,24 @@ __global__ __launch_bounds__(NT) void k_line_general(Sys sys, int nbx, int nby) 
   line_block_general<S, MMAX, NT, LANE_LINES, false, Sys>(sys, ok ? la : bx * LPB, by, ok, s_a,
                                                           s_b, s_c, s_d, nullptr);
 }
+template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
+__global__ __launch_bounds__(NT) void k_line_general(Sys sys, int nbx, int nby) {
+  (void)nby;
+  line_general_body<S, MMAX, NT, LANE_LINES>(sys, nbx);
+}
+// The same with the registers held to 128 per lane (4 waves per SIMD), for
+// the systems that declare kWaves4: the spectral solve fits in 128 with no
+// spill and runs 8% faster than at its natural 130 (3 waves per SIMD).
+template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_line_general_w4(
+    Sys sys, int nbx, int nby) {
+  (void)nby;
+  line_general_body<S, MMAX, NT, LANE_LINES>(sys, nbx);
+}
+template <typename S, typename = void>
+struct waves4 : std::false_type {};
+template <typename S>
+struct waves4<S, std::void_t<decltype(S::kWaves4)>> : std::bool_constant<S::kWaves4> {};
 
 // The general solve of the line groups the fast kernel queued: each entry
 // is the first line (la0, lb) of LPB adjacent lines, which a block loads as
@@ -819,8 +839,12 @@ inline int line_sweep(const Sys& sys, int n, int nla, int nlb, hipStream_t s) {
     k_line_general_q<SG, MG, NTG, LANE_LINES, Sys><<<ngen, NTG, 0, s>>>(sys, Q, nla_pad);
   } else {
     const int nbx = (nla + LPB_G - 1) / LPB_G;
-    k_line_general<SG, MG, NTG, LANE_LINES, Sys><<<(unsigned)nbx * (unsigned)nlb, NTG, 0, s>>>(
-        sys, nbx, nlb);
+    if constexpr (waves4<Sys>::value && MG <= 8)  // longer segments would spill
+      k_line_general_w4<SG, MG, NTG, LANE_LINES, Sys>
+          <<<(unsigned)nbx * (unsigned)nlb, NTG, 0, s>>>(sys, nbx, nlb);
+    else
+      k_line_general<SG, MG, NTG, LANE_LINES, Sys><<<(unsigned)nbx * (unsigned)nlb, NTG, 0, s>>>(
+          sys, nbx, nlb);
   }
   return 0;
 }
