@@ -73,6 +73,57 @@ template <typename TS>
 int launch_spn(const Geom& g, TS* phit, const double* ac, const double* kxz, hipStream_t s);
 template <typename TS>
 void launch_sp0b(const Geom& g, TS* phit, const double* ac, hipStream_t s);
+// SPN and SP0B together (the zero mode's march in the solve's first block
+// where it fits, else beside it on an auxiliary stream)
+template <typename TS>
+int launch_spn_sp0b(const Geom& g, TS* phit, const double* ac, const double* kxz, hipStream_t s);
+
+// SP0B (mkp.f:263-290): the zero mode (I, K) = (0, 0) of the spectrum by
+// the forward march with phi(0) = phi(1) = 0.  Every thread of the calling
+// block takes part: the block loads the mode into smode (2 (ny + 2)
+// doubles) and the AC coefficients into sac (3 ny doubles) in parallel --
+// the march would otherwise wait on a global load at every row -- one lane
+// marches, and the block stores the mode.
+template <typename TS>
+__device__ inline void sp0b_march(const Geom& g, const double* ac, TS* phit, double* smode,
+                                  double* sac) {
+  const int ny = g.ny;
+  double* sh = smode;
+  for (int J = threadIdx.x; J <= ny + 1; J += blockDim.x) {
+    long long o = 2 * g.spec_at(0, J, 0);
+    sh[2 * J] = phit[o];
+    sh[2 * J + 1] = phit[o + 1];
+  }
+  for (int q = threadIdx.x; q < 3 * ny; q += blockDim.x) sac[q] = ac[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double prevr = 0.0, previ = 0.0, curr = 0.0, curi = 0.0;  // phi(J-1), phi(J)
+    for (int J = 1; J <= ny - 1; J++) {
+      double a1 = sac[J - 1], a2 = sac[J - 1 + ny], a3 = sac[J - 1 + 2 * ny];
+      double dr = sh[2 * J], di = sh[2 * J + 1];
+      double nr = (dr - a1 * prevr - a2 * curr) / a3;
+      double ni = (di - a1 * previ - a2 * curi) / a3;
+      sh[2 * J] = curr;  // reuse: slot J now holds phi(J)
+      sh[2 * J + 1] = curi;
+      prevr = curr;
+      previ = curi;
+      curr = nr;
+      curi = ni;
+    }
+    sh[2 * ny] = curr;
+    sh[2 * ny + 1] = curi;
+    sh[0] = 0.0;
+    sh[1] = 0.0;
+    sh[2 * (ny + 1)] = curr;
+    sh[2 * (ny + 1) + 1] = curi;
+  }
+  __syncthreads();
+  for (int J = threadIdx.x; J <= ny + 1; J += blockDim.x) {
+    long long o = 2 * g.spec_at(0, J, 0);
+    phit[o] = static_cast<TS>(sh[2 * J]);
+    phit[o + 1] = static_cast<TS>(sh[2 * J + 1]);
+  }
+}
 template <typename TC>
 int launch_adi_velocity(const Geom& g, const Metrics& m, const RMetrics& r, double* f,
                         const TC* cv, int comp, double beta, double dteta, hipStream_t s,

@@ -382,9 +382,19 @@ struct SysSpectral {
   using Pt = PtNone;
   static constexpr bool kCyclic = false, kHasRef = false, kNb = false;
   static constexpr bool kWaves4 = true;  // k_line_general_w4
+  static constexpr bool kBlock0 = true;  // SP0B's march, when block0 is set
   Geom g;
   TS* phit;  // interleaved complex
   const double *ac, *kxz;
+  int block0 = 0;
+  // SP0B (mkp.f:263-290) for the zero mode, which the other blocks leave
+  // alone (store() skips it): the march of k_sp0b, in the solve's first
+  // block.  The caller checks sp0b_fits_block0 for the scratch sizes.
+  __device__ void run_block0(double* s1, int n1, double* s2, int n2) const {
+    (void)n1;
+    (void)n2;
+    sp0b_march<TS>(g, ac, phit, s1, s2);
+  }
   __device__ int n() const { return g.ny; }
   __device__ bool valid(int la, int lb) const { return la <= g.nx / 2 && lb < g.nz; }
   __device__ dcx rhs(int la, int lb, int r) const {
@@ -437,6 +447,27 @@ static inline int line_status(int rc, const char* what) {
 template <typename TS>
 int launch_spn(const Geom& g, TS* phit, const double* ac, const double* kxz, hipStream_t s) {
   SysSpectral<TS> sys{g, phit, ac, kxz};
+  ProfScope ps("line_spectral", s);
+  return line_status(launch_line_solve<true>(sys, g.ny, g.nx / 2 + 1, g.nz, s), "spn");
+}
+
+// SPN with SP0B: the zero mode's march runs in the spectral solve's first
+// block where its mode and coefficients fit the block's LDS arrays (512
+// and 1024 doubles in every general-kernel shape of 256 threads), else
+// as its own one-block kernel on an auxiliary stream beside the solve.
+inline bool sp0b_fits_block0(const Geom& g) {
+  return 2 * (g.ny + 2) <= 512 && 3 * g.ny <= 1024;
+}
+template <typename TS>
+int launch_spn_sp0b(const Geom& g, TS* phit, const double* ac, const double* kxz, hipStream_t s) {
+  if (!sp0b_fits_block0(g)) {
+    AuxFork f0(s, 1u);
+    if (f0.err) return f0.err;
+    launch_sp0b<TS>(g, phit, ac, f0.aux(0));
+    return launch_spn<TS>(g, phit, ac, kxz, s);
+  }
+  SysSpectral<TS> sys{g, phit, ac, kxz};
+  sys.block0 = 1;
   ProfScope ps("line_spectral", s);
   return line_status(launch_line_solve<true>(sys, g.ny, g.nx / 2 + 1, g.nz, s), "spn");
 }

@@ -150,15 +150,10 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
   if (rc) return rc;
   rc = launch_p2s_div(g, r, u_temp, v_temp, w_temp, dt, phit, s);
   if (rc) return rc;
-  {
-    // the zero mode's serial march (one block) beside the other modes'
-    // solves, on an auxiliary stream joined back before the inverse pass
-    AuxFork f0(s, 1u);
-    if (f0.err) return f0.err;
-    launch_sp0b(g, phit, ac, f0.aux(0));
-    rc = launch_spn(g, phit, ac, kxz, s);
-    if (rc) return rc;
-  }
+  // the zero mode's serial march (SP0B) in the spectral solve's first
+  // block, beside the other modes' solves
+  rc = launch_spn_sp0b(g, phit, ac, kxz, s);
+  if (rc) return rc;
   rc = launch_s2p_inplace<double>(g, phit, phi, s, "cfft_z_inv", "irfft_x");
   if (rc) return rc;
   launch_halo_neumann(g, phi, s);
@@ -379,26 +374,16 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
   // pass 1
   rc = launch_p2s_div_store(g, r, u_temp, v_temp, w_temp, dt, rhs, phit, s);
   if (rc) return rc;
-  {
-    AuxFork f0(s, 1u);  // the zero mode beside the others, as in launch_mkp
-    if (f0.err) return f0.err;
-    launch_sp0b<float>(g, phit, ac, f0.aux(0));
-    rc = launch_spn<float>(g, phit, ac, kxz, s);
-    if (rc) return rc;
-  }
+  rc = launch_spn_sp0b<float>(g, phit, ac, kxz, s);
+  if (rc) return rc;
   rc = launch_s2p_inplace<float>(g, phit, phi1, s, "cfft_z_inv", "irfft_x");
   if (rc) return rc;
   launch_halo_neumann<float>(g, phi1, s);
   // pass 2: refinement on the FP64 residual
   rc = launch_p2s_resid(g, r, rhs, phi1, phit, s);
   if (rc) return rc;
-  {
-    AuxFork f0(s, 1u);  // the zero mode beside the others, as in launch_mkp
-    if (f0.err) return f0.err;
-    launch_sp0b<float>(g, phit, ac, f0.aux(0));
-    rc = launch_spn<float>(g, phit, ac, kxz, s);
-    if (rc) return rc;
-  }
+  rc = launch_spn_sp0b<float>(g, phit, ac, kxz, s);
+  if (rc) return rc;
   rc = launch_s2p_inplace<float>(g, phit, dphi, s, "cfft_z_inv2", "irfft_x2");
   if (rc) return rc;
   launch_halo_neumann<float>(g, dphi, s);
@@ -446,13 +431,8 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
   if (!psi) return VK_ERR_HIP;
   rc = launch_p2s_div_f32(g, r, o.u, o.v, o.w, dt, phit, s, "rfft_x_corr", "cfft_z_corr");
   if (rc) return rc;
-  {
-    AuxFork f0(s, 1u);
-    if (f0.err) return f0.err;
-    launch_sp0b<float>(g, phit, ac, f0.aux(0));
-    rc = launch_spn<float>(g, phit, ac, kxz, s);
-    if (rc) return rc;
-  }
+  rc = launch_spn_sp0b<float>(g, phit, ac, kxz, s);
+  if (rc) return rc;
   rc = launch_s2p_inplace<float>(g, phit, psi, s, "cfft_z_inv_corr", "irfft_x_corr");
   if (rc) return rc;
   launch_halo_neumann<float>(g, psi, s);

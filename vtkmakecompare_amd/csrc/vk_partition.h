@@ -528,6 +528,21 @@ __device__ inline void line_block_general(const Sys& sys, int la, int lb, bool o
 // The general solve of every line, block (bx, by) holding lines
 // la = bx * LPB + (line within block), lb = by of an nbx x nby grid of
 // blocks, in XCD-aware order.
+// Systems with a block-0 task (kBlock0, run when the system's `block0` is
+// set): the launch adds kBlock0Pad blocks in front, block 0 runs the task
+// with the block's LDS arrays as scratch and the other added blocks return,
+// so the line blocks keep their XCD placement.
+template <typename S, typename = void>
+struct block0_task : std::false_type {};
+template <typename S>
+struct block0_task<S, std::void_t<decltype(S::kBlock0)>> : std::bool_constant<S::kBlock0> {};
+constexpr unsigned kBlock0Pad = 8;
+template <typename Sys>
+inline unsigned block0_pad(const Sys& sys) {
+  if constexpr (block0_task<Sys>::value) return sys.block0 ? kBlock0Pad : 0u;
+  return 0u;
+}
+
 template <int S, int MMAX, int NT, bool LANE_LINES, typename Sys>
 __device__ inline void line_general_body(const Sys& sys, int nbx) {
   using V = typename Sys::V;
@@ -536,7 +551,18 @@ __device__ inline void line_general_body(const Sys& sys, int nbx) {
   constexpr int P = 2 * S;
   __shared__ double s_a[LPB * P], s_b[LPB * P], s_c[LPB * P];
   __shared__ double s_d[LPB * P * vwords<V>()];
-  const unsigned blk = xcd_linear(blockIdx.x, gridDim.x);
+  unsigned bid = blockIdx.x, nblk = gridDim.x;
+  if constexpr (block0_task<Sys>::value) {
+    if (sys.block0) {
+      if (bid < kBlock0Pad) {
+        if (bid == 0) sys.run_block0(s_a, LPB * P, s_d, LPB * P * vwords<V>());
+        return;
+      }
+      bid -= kBlock0Pad;
+      nblk -= kBlock0Pad;
+    }
+  }
+  const unsigned blk = xcd_linear(bid, nblk);
   const int bx = (int)(blk % (unsigned)nbx), by = (int)(blk / (unsigned)nbx);
   const M mp(threadIdx.x);
   const int la = bx * LPB + mp.line;
@@ -839,12 +865,11 @@ inline int line_sweep(const Sys& sys, int n, int nla, int nlb, hipStream_t s) {
     k_line_general_q<SG, MG, NTG, LANE_LINES, Sys><<<ngen, NTG, 0, s>>>(sys, Q, nla_pad);
   } else {
     const int nbx = (nla + LPB_G - 1) / LPB_G;
+    const unsigned nb = (unsigned)nbx * (unsigned)nlb + block0_pad(sys);
     if constexpr (waves4<Sys>::value && MG <= 8)  // longer segments would spill
-      k_line_general_w4<SG, MG, NTG, LANE_LINES, Sys>
-          <<<(unsigned)nbx * (unsigned)nlb, NTG, 0, s>>>(sys, nbx, nlb);
+      k_line_general_w4<SG, MG, NTG, LANE_LINES, Sys><<<nb, NTG, 0, s>>>(sys, nbx, nlb);
     else
-      k_line_general<SG, MG, NTG, LANE_LINES, Sys><<<(unsigned)nbx * (unsigned)nlb, NTG, 0, s>>>(
-          sys, nbx, nlb);
+      k_line_general<SG, MG, NTG, LANE_LINES, Sys><<<nb, NTG, 0, s>>>(sys, nbx, nlb);
   }
   return 0;
 }

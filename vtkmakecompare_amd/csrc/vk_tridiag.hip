@@ -90,49 +90,12 @@ double* line_table(const void* key, size_t keylen, long total, bool* fresh, bool
   return p;
 }
 
-// SP0B (mkp.f:263-290): the zero mode by the forward march with
-// phi(0) = phi(1) = 0; one block loads the mode and the AC coefficients
-// into LDS in parallel (the march would otherwise wait on a global load
-// at every row), one lane marches, the block stores.
+// SP0B (mkp.f:263-290): the zero mode by the forward march (sp0b_march)
+// in a one-block kernel.
 template <typename TS>
 __global__ void k_sp0b(Geom g, const double* ac, TS* phit) {
   extern __shared__ double sh[];  // 2*(ny+2) doubles of the mode, 3 ny of AC
-  int ny = g.ny;
-  double* sac = sh + 2 * (ny + 2);
-  for (int J = threadIdx.x; J <= ny + 1; J += blockDim.x) {
-    long long o = 2 * g.spec_at(0, J, 0);
-    sh[2 * J] = phit[o];
-    sh[2 * J + 1] = phit[o + 1];
-  }
-  for (int q = threadIdx.x; q < 3 * ny; q += blockDim.x) sac[q] = ac[q];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double d0r = 0.0, d0i = 0.0;
-    double prevr = 0.0, previ = 0.0, curr = 0.0, curi = 0.0;  // phi(J-1), phi(J)
-    (void)d0r; (void)d0i;
-    for (int J = 1; J <= ny - 1; J++) {
-      double a1 = sac[J - 1], a2 = sac[J - 1 + ny], a3 = sac[J - 1 + 2 * ny];
-      double dr = sh[2 * J], di = sh[2 * J + 1];
-      double nr = (dr - a1 * prevr - a2 * curr) / a3;
-      double ni = (di - a1 * previ - a2 * curi) / a3;
-      sh[2 * J] = curr;  // reuse: slot J now holds phi(J)
-      sh[2 * J + 1] = curi;
-      prevr = curr; previ = curi;
-      curr = nr; curi = ni;
-    }
-    sh[2 * ny] = curr;
-    sh[2 * ny + 1] = curi;
-    sh[0] = 0.0;
-    sh[1] = 0.0;
-    sh[2 * (ny + 1)] = curr;
-    sh[2 * (ny + 1) + 1] = curi;
-  }
-  __syncthreads();
-  for (int J = threadIdx.x; J <= ny + 1; J += blockDim.x) {
-    long long o = 2 * g.spec_at(0, J, 0);
-    phit[o] = static_cast<TS>(sh[2 * J]);
-    phit[o + 1] = static_cast<TS>(sh[2 * J + 1]);
-  }
+  sp0b_march<TS>(g, ac, phit, sh, sh + 2 * (g.ny + 2));
 }
 
 template <typename TS>
