@@ -353,7 +353,7 @@ int vk_time_step(double *u, double *v, double *w, double *p, double *t,
    solves in FP64 on the mixed storage; the projection adds one FP64
    iterative-refinement pass, so the post-projection divergence is at
    FP64 round-off.  When the divergence after that pass is still above
-   5e-11 (steps where the penalisation cuts the velocity sharply at the
+   9e-11 (steps where the penalisation cuts the velocity sharply at the
    solid's surface, such as the first steps from a non-solenoidal state),
    one more refinement pass on the post-projection divergence follows and
    the projection is redone; deciding this reads one word back to the

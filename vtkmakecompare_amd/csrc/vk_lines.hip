@@ -414,7 +414,17 @@ __global__ void k_lbl_shift(long long n, double* p, const double* pst) {
 // block, over the interior in linear order, the largest |L p dt - div| and
 // the sums of (L p dt - div) dV and |.|^2 dV; block 0 also keeps the last
 // point's |.|.  Pass 2 combines the blocks in order.
+//
+// The statistics share one workspace of kLblResWords doubles with the
+// mean-pressure word: [0] the pressure at (1,1,1), [4..7] the four
+// results, then from kLblPart the per-block triples (3 kResBlocks words)
+// and, after them, the last point's residual.
 constexpr int kResBlocks = 1024;
+constexpr int kLblOut = 4, kLblPart = 8;
+constexpr int kLblLast = 3 * kResBlocks;  // offset from kLblPart
+constexpr size_t kLblResWords = kLblPart + kLblLast + 8;
+static_assert(kLblOut + 4 <= kLblPart && kLblPart + kLblLast < (int)kLblResWords,
+              "solve_p's statistics overrun their workspace");
 
 __global__ __launch_bounds__(256) void k_lbl_resid(Geom g, PMetrics m, const double* u,
                                                    const double* v, const double* w,
@@ -438,7 +448,7 @@ __global__ __launch_bounds__(256) void k_lbl_resid(Geom g, PMetrics m, const dou
     rmax = res >= rmax ? res : rmax;
     rint = rint + (pphi * dt - div) * m.dx[i] * m.dy[j] * m.dz[k];
     rvar = rvar + (res * res) * m.dx[i] * m.dy[j] * m.dz[k];
-    if (q == n - 1) part[4 * kResBlocks] = res;  // the last point's residual
+    if (q == n - 1) part[kLblLast] = res;  // the last point's residual
   }
   for (int off = 32; off > 0; off >>= 1) {
     const double a = __shfl_xor(rmax, off);
@@ -472,7 +482,7 @@ __global__ void k_lbl_resid_final(const double* part, int nb, double* out) {
     rint = rint + part[3 * b + 1];
     rvar = rvar + part[3 * b + 2];
   }
-  out[0] = part[4 * kResBlocks];
+  out[0] = part[kLblLast];
   out[1] = rmax;
   out[2] = rint;
   out[3] = rvar;
@@ -701,7 +711,7 @@ static int lbl_work(const Geom& g, LblWork& lw) {
   lw.q = static_cast<double*>(workspace(kWorkLblQ, g.cells() * sizeof(double)));
   lw.band = line_scratch(4, nmax, lw.L);
   lw.pst = static_cast<double*>(
-      workspace(kWorkLblRes, (3 * (size_t)kResBlocks + 8 + 8) * sizeof(double)));
+      workspace(kWorkLblRes, kLblResWords * sizeof(double)));
   return (lw.q && lw.band && lw.pst) ? VK_OK : VK_ERR_HIP;
 }
 
@@ -745,8 +755,8 @@ int vk_solve_p(double* p, const double* u_temp, const double* v_temp, const doub
   const int nnn = max_iterations ? *max_iterations : 100000;
   if (nnn < 1) return fail(VK_ERR_ARGUMENT, "solve_p: max_iterations >= 1");
   hipStream_t s = rt().stream;
-  double* part = lw.pst + 8;
-  double* out = lw.pst + 4;
+  double* part = lw.pst + kLblPart;
+  double* out = lw.pst + kLblOut;
   const long long nint = (long long)g.interior();
   const int nb = (int)std::min<long long>(kResBlocks, (nint + 255) / 256);
   int done = 0;
