@@ -34,7 +34,8 @@ enum {
   VK_ERR_BAD_SIZE = 3,     /* grid size the kernels do not support */
   VK_ERR_HIP = 4,          /* HIP runtime error */
   VK_ERR_ARGUMENT = 5,     /* invalid argument */
-  VK_ERR_NONFINITE = 6     /* a field check found NaN or Inf */
+  VK_ERR_NONFINITE = 6,    /* a field check found NaN or Inf */
+  VK_ERR_GUARD = 7         /* vk_guard_check found a guard zone written */
 };
 
 /* ------------------------------------------------------------ runtime */
@@ -57,6 +58,17 @@ int vk_memset(void *dst, const int *value, const size_t *bytes);
    its arguments and refuses (VK_ERR_ARGUMENT) an array too small for the
    grid it is given. */
 int vk_extent(const void *p, size_t *bytes);
+/* Out-of-bounds check of the library's own device buffers.  No reference
+   counterpart: a check for test suites and debugging.  With VK_GUARD set
+   (and not "0") in the environment when vk_init runs, every device
+   allocation the library makes (vk_malloc, registered arrays' mirrors,
+   workspace, tables, checkpoints) lies between two 64 KiB zones of a byte
+   pattern.  vk_guard_check waits for the device, compares every zone with
+   the pattern, sets *damaged to the number of zones that changed (a kernel
+   wrote past an end of its buffer), refills them, and returns VK_ERR_GUARD
+   naming the allocations and offsets when there are any.  Without guard
+   zones it sets *damaged to -1 and returns VK_OK. */
+int vk_guard_check(int *damaged);
 
 /* Kernel timing on the library stream with HIP events, for benchmarks:
    between vk_profile_begin and vk_profile_end every kernel launch is

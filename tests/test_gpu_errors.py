@@ -19,6 +19,7 @@ VK_ERR_NOT_RESIDENT = 2
 VK_ERR_BAD_SIZE = 3
 VK_ERR_ARGUMENT = 5
 VK_ERR_NONFINITE = 6
+VK_ERR_GUARD = 7
 
 
 @pytest.fixture(scope="module")
@@ -132,3 +133,32 @@ def test_mkp_refuses_unsupported_fft_lengths(api, nx, nz):
                 g.nx, g.ny, g.nz, 1, f[4], f[5], f[6], f[7], f[8], f[9], f[10], f[11], f[12],
                 f[13], g.xl, g.zl)
     assert e.value.status == VK_ERR_BAD_SIZE
+
+
+def test_guard_zones_report_writes_outside_an_allocation(api):
+    """vk_guard_check names the allocation and the offsets of bytes written
+    in the zones around it.  The bytes are written from the host (vk_memset
+    on the zone's own memory, which the guarded allocation holds), so no
+    kernel goes out of bounds; a second check finds the zones refilled."""
+    from vtkmakecompare_amd._lib import lib
+    from vtkmakecompare_amd.device import DeviceArray
+    L = lib()
+    n = ctypes.c_int(-2)
+    assert L.vk_guard_check(ctypes.byref(n)) == 0
+    if n.value == -1:  # no zones: then the writes below would leave the allocation
+        pytest.skip("guard zones are off (VK_GUARD)")
+    assert n.value == 0
+    a = DeviceArray((125,))  # 1000 bytes
+    ext = ctypes.c_size_t(0)
+    assert L.vk_extent(a.ptr, ctypes.byref(ext)) == 0 and ext.value == 1000
+    base = a.ptr.value
+    zero = ctypes.byref(ctypes.c_int(0))
+    assert L.vk_memset(ctypes.c_void_p(base + 1000), zero, ctypes.byref(ctypes.c_size_t(8))) == 0
+    assert L.vk_memset(ctypes.c_void_p(base - 16), zero, ctypes.byref(ctypes.c_size_t(4))) == 0
+    rc = L.vk_guard_check(ctypes.byref(n))
+    msg = L.vk_last_error().decode()
+    assert rc == VK_ERR_GUARD and n.value == 2, msg
+    assert "vk_malloc (1000 bytes): 8 bytes written after its end, offsets 1000..1007" in msg
+    assert "vk_malloc (1000 bytes): 4 bytes written before its start, offsets -16..-13" in msg
+    assert L.vk_guard_check(ctypes.byref(n)) == 0 and n.value == 0
+    a.free()

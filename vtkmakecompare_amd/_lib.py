@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libvtkgpu.so")
 EXPORTS = [
     "vk_init", "vk_finalize", "vk_last_error", "vk_register", "vk_unregister",
     "vk_upload", "vk_download", "vk_synchronize", "vk_malloc", "vk_free",
-    "vk_memcpy_htod", "vk_memcpy_dtoh", "vk_memset", "vk_extent",
+    "vk_memcpy_htod", "vk_memcpy_dtoh", "vk_memset", "vk_extent", "vk_guard_check",
     "vk_set_bc", "vk_set_convect", "vk_set_convection_for_scalar",
     "vk_set_conduction", "vk_set_diffusion_for_scalar",
     "vk_set_immersed_boundary", "vk_set_explicit",

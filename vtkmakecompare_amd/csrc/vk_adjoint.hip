@@ -337,14 +337,14 @@ enum { SU, SV, SW, ST, SUP, SVP, SWP, STP, SUTA, SVTA, SWTA };
 struct Dev {
   std::vector<void*> ptrs;
   ~Dev() {
-    for (void* p : ptrs) (void)hipFree(p);
+    for (void* p : ptrs) (void)dev_free(p);
   }
   size_t bytes = 0;
   double* alloc(size_t n) { return static_cast<double*>(raw(n * sizeof(double))); }
   float* allocf(size_t n) { return static_cast<float*>(raw(n * sizeof(float))); }
   void* raw(size_t b) {
     void* p = nullptr;
-    if (hipMalloc(&p, b) != hipSuccess) {
+    if (dev_malloc(&p, b, "adjoint checkpoint or tape") != hipSuccess) {
       (void)hipGetLastError();
       return nullptr;
     }

@@ -53,8 +53,8 @@ std::map<int, const typename Cx<R>::T*>& tables() {
 // vk_finalize hook: the twiddle tables are device memory; the plans point
 // into them
 void release_fft_tables() {
-  for (auto& kv : tables<double>()) (void)hipFree(const_cast<double2*>(kv.second));
-  for (auto& kv : tables<float>()) (void)hipFree(const_cast<float2*>(kv.second));
+  for (auto& kv : tables<double>()) (void)dev_free(const_cast<double2*>(kv.second));
+  for (auto& kv : tables<float>()) (void)dev_free(const_cast<float2*>(kv.second));
   tables<double>().clear();
   tables<float>().clear();
   plans<double>().clear();
@@ -78,7 +78,8 @@ const typename Cx<R>::T* twiddle_table(int n) {
     h[m].y = (R)sinl(ang);
   }
   C* d = nullptr;
-  if (hipMalloc(&d, sizeof(C) * n) != hipSuccess) return nullptr;
+  if (dev_malloc(reinterpret_cast<void**>(&d), sizeof(C) * n, "twiddle table") != hipSuccess)
+    return nullptr;
   if (hipMemcpy(d, h.data(), sizeof(C) * n, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
   tables<R>()[n] = d;
   return d;

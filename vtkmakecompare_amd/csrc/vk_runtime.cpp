@@ -45,7 +45,50 @@ struct Work {
 };
 Work g_work[kWorkSlots];
 
+// Guard zones: with them on, dev_malloc places a zone of kGuardBytes
+// filled with kGuardByte before and after every allocation and keeps the
+// allocation here, under the address it returned.
+constexpr size_t kGuardBytes = 64 * 1024;
+constexpr unsigned char kGuardByte = 0xA5;
+struct Guarded {
+  unsigned char* base;  // start of the front zone
+  size_t bytes;         // the caller's bytes
+  std::string what;
+};
+bool g_guard = false;
+std::map<uintptr_t, Guarded>& guarded() {
+  static std::map<uintptr_t, Guarded> m;
+  return m;
+}
+
 }  // namespace
+
+hipError_t dev_malloc(void** p, size_t bytes, const char* what) {
+  if (!g_guard) return hipMalloc(p, bytes);
+  void* raw = nullptr;
+  hipError_t e = hipMalloc(&raw, bytes + 2 * kGuardBytes);
+  if (e != hipSuccess) return e;
+  unsigned char* b = static_cast<unsigned char*>(raw);
+  e = hipMemset(b, kGuardByte, kGuardBytes);
+  if (e == hipSuccess) e = hipMemset(b + kGuardBytes + bytes, kGuardByte, kGuardBytes);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    (void)hipFree(raw);
+    return e;
+  }
+  *p = b + kGuardBytes;
+  guarded()[reinterpret_cast<uintptr_t>(*p)] = Guarded{b, bytes, what};
+  return hipSuccess;
+}
+
+hipError_t dev_free(void* p) {
+  auto& m = guarded();
+  auto it = m.find(reinterpret_cast<uintptr_t>(p));
+  if (it == m.end()) return hipFree(p);
+  unsigned char* base = it->second.base;
+  m.erase(it);
+  return hipFree(base);
+}
 
 Runtime& rt() {
   static Runtime r;
@@ -141,6 +184,16 @@ bool device_extent(const void* p, size_t* bytes) {
   uintptr_t a = reinterpret_cast<uintptr_t>(p), b = reinterpret_cast<uintptr_t>(base);
   if (a < b || a - b > size) return false;
   *bytes = size - (a - b);
+  // a guarded allocation ends where its back zone starts
+  auto& m = guarded();
+  auto it = m.upper_bound(a);
+  if (it != m.begin()) {
+    --it;
+    const uintptr_t u = it->first, end = u + it->second.bytes;
+    if (reinterpret_cast<uintptr_t>(it->second.base) == b) {
+      *bytes = (a < u || a > end) ? 0 : end - a;  // 0 inside a zone
+    }
+  }
   return true;
 }
 
@@ -162,11 +215,12 @@ void* workspace(WorkSlot slot, size_t bytes, bool* grown) {
   if (w.ptr) {
     // every stream of the library may still read the old buffer
     (void)hipDeviceSynchronize();
-    (void)hipFree(w.ptr);
+    (void)dev_free(w.ptr);
     w.ptr = nullptr;
     w.bytes = 0;
   }
-  if (hipMalloc(&w.ptr, bytes) != hipSuccess) {
+  const std::string what = "workspace slot " + std::to_string((int)slot);
+  if (dev_malloc(&w.ptr, bytes, what.c_str()) != hipSuccess) {
     w.ptr = nullptr;
     fail(VK_ERR_HIP, "workspace allocation of " + std::to_string(bytes) + " bytes failed");
     return nullptr;
@@ -236,6 +290,10 @@ int vk_init(const int* device) {
                                      std::to_string(count));
   rc = hip_check(hipSetDevice(dev), "hipSetDevice");
   if (rc) return rc;
+  {
+    const char* gv = std::getenv("VK_GUARD");
+    g_guard = gv && *gv && std::strcmp(gv, "0") != 0;
+  }
   rc = hip_check(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking), "hipStreamCreate");
   if (rc) return rc;
   for (int q = 0; q < 3; q++) {
@@ -259,11 +317,11 @@ int vk_finalize(void) {
   if (!r.initialised) return VK_OK;
   (void)hipDeviceSynchronize();
   for (auto h : finalize_hooks()) h();
-  for (auto& kv : registry()) (void)hipFree(kv.second.dev);
+  for (auto& kv : registry()) (void)dev_free(kv.second.dev);
   registry().clear();
   for (auto& w : g_work)
     if (w.ptr) {
-      (void)hipFree(w.ptr);
+      (void)dev_free(w.ptr);
       w.ptr = nullptr;
       w.bytes = 0;
     }
@@ -297,7 +355,7 @@ int vk_register(const void* host, const size_t* bytes) {
     return fail(VK_ERR_ARGUMENT, "vk_register: address already registered with a smaller size");
   }
   void* d = nullptr;
-  int rc = hip_check(hipMalloc(&d, *bytes), "vk_register hipMalloc");
+  int rc = hip_check(dev_malloc(&d, *bytes, "vk_register"), "vk_register hipMalloc");
   if (rc) return rc;
   rc = hip_check(hipMemsetAsync(d, 0, *bytes, rt().stream), "vk_register memset");
   if (rc) return rc;
@@ -311,7 +369,7 @@ int vk_unregister(const void* host) {
   auto it = reg.find(reinterpret_cast<uintptr_t>(host));
   if (it == reg.end()) return fail(VK_ERR_ARGUMENT, "vk_unregister: not registered");
   (void)hipStreamSynchronize(rt().stream);
-  (void)hipFree(it->second.dev);
+  (void)dev_free(it->second.dev);
   reg.erase(it);
   return VK_OK;
 }
@@ -348,7 +406,7 @@ int vk_synchronize(void) {
 int vk_malloc(void** dptr, const size_t* bytes) {
   VK_REQUIRE_INIT();
   if (!dptr || !bytes) return fail(VK_ERR_ARGUMENT, "vk_malloc: null argument");
-  int rc = hip_check(hipMalloc(dptr, *bytes ? *bytes : 1), "vk_malloc");
+  int rc = hip_check(dev_malloc(dptr, *bytes ? *bytes : 1, "vk_malloc"), "vk_malloc");
   if (rc) return rc;
   return hip_check(hipMemsetAsync(*dptr, 0, *bytes ? *bytes : 1, rt().stream), "vk_malloc memset");
 }
@@ -356,7 +414,7 @@ int vk_malloc(void** dptr, const size_t* bytes) {
 int vk_free(void* dptr) {
   VK_REQUIRE_INIT();
   (void)hipStreamSynchronize(rt().stream);
-  return hip_check(hipFree(dptr), "vk_free");
+  return hip_check(dev_free(dptr), "vk_free");
 }
 
 int vk_memcpy_htod(void* dst, const void* src, const size_t* bytes) {
@@ -438,6 +496,47 @@ int vk_extent(const void* p, size_t* bytes) {
   }
   if (!device_extent(p, bytes))
     return fail(VK_ERR_ARGUMENT, "vk_extent: the runtime reports no allocation range here");
+  return VK_OK;
+}
+
+int vk_guard_check(int* damaged) {
+  VK_REQUIRE_INIT();
+  if (damaged) *damaged = g_guard ? 0 : -1;
+  if (!g_guard) return VK_OK;
+  int rc = hip_check(hipDeviceSynchronize(), "vk_guard_check");
+  if (rc) return rc;
+  std::vector<unsigned char> h(kGuardBytes);
+  std::string msg;
+  int n = 0;
+  for (auto& kv : guarded()) {
+    const Guarded& a = kv.second;
+    for (int side = 0; side < 2; side++) {
+      unsigned char* z = a.base + (side ? kGuardBytes + a.bytes : 0);
+      rc = hip_check(hipMemcpy(h.data(), z, kGuardBytes, hipMemcpyDeviceToHost), "vk_guard_check");
+      if (rc) return rc;
+      size_t first = kGuardBytes, last = 0, cnt = 0;
+      for (size_t q = 0; q < kGuardBytes; q++)
+        if (h[q] != kGuardByte) {
+          first = q < first ? q : first;
+          last = q;
+          cnt++;
+        }
+      if (!cnt) continue;
+      n++;
+      // offsets from the allocation's first byte (negative: before it)
+      const long long o0 = side ? (long long)(a.bytes + first) : (long long)first - (long long)kGuardBytes;
+      const long long o1 = side ? (long long)(a.bytes + last) : (long long)last - (long long)kGuardBytes;
+      msg += (msg.empty() ? "" : "; ") + a.what + " (" + std::to_string(a.bytes) + " bytes): " +
+             std::to_string(cnt) + " bytes written " + (side ? "after its end" : "before its start") +
+             ", offsets " + std::to_string(o0) + ".." + std::to_string(o1);
+      rc = hip_check(hipMemset(z, kGuardByte, kGuardBytes), "vk_guard_check");
+      if (rc) return rc;
+    }
+  }
+  rc = hip_check(hipDeviceSynchronize(), "vk_guard_check");
+  if (rc) return rc;
+  if (damaged) *damaged = n;
+  if (n) return fail(VK_ERR_GUARD, "guard zones written: " + msg);
   return VK_OK;
 }
 

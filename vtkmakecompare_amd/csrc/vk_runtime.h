@@ -91,6 +91,15 @@ bool is_device(const void* p);
 // per process and stays registered across finalize/init cycles.
 void on_finalize(void (*hook)());
 
+// Device memory for every buffer the library allocates or hands out
+// (vk_malloc, registered arrays' mirrors, workspace, tables, checkpoints).
+// With VK_GUARD set (and not "0") in the environment when vk_init runs,
+// each allocation lies between two zones of 64 KiB of a fixed byte
+// pattern; vk_guard_check reports any zone that a kernel has written.
+// dev_free also takes memory that dev_malloc did not allocate.
+hipError_t dev_malloc(void** p, size_t bytes, const char* what);
+hipError_t dev_free(void* p);
+
 // Grow-only device workspace, one per slot.
 enum WorkSlot {
   kWorkPhi = 0,

@@ -37,10 +37,10 @@ std::map<std::string, TableEntry> g_tables;
 // after the streams are destroyed
 void release_line_caches() {
   for (auto& kv : g_queues)
-    if (kv.second.p) (void)hipFree(kv.second.p);
+    if (kv.second.p) (void)dev_free(kv.second.p);
   g_queues.clear();
   for (auto& kv : g_tables)
-    if (kv.second.p) (void)hipFree(kv.second.p);
+    if (kv.second.p) (void)dev_free(kv.second.p);
   g_tables.clear();
 }
 }  // namespace
@@ -51,12 +51,13 @@ LineQueue line_queue(unsigned nlines, hipStream_t s) {
   if (q.cap < nlines || !q.p) {
     if (q.p) {
       (void)hipStreamSynchronize(s);
-      (void)hipFree(q.p);
+      (void)dev_free(q.p);
       q.p = nullptr;
       q.cap = 0;
     }
     void* p = nullptr;
-    if (hip_check(hipMalloc(&p, sizeof(unsigned) * (64 + (size_t)nlines)), "line queue")) {
+    if (hip_check(dev_malloc(&p, sizeof(unsigned) * (64 + (size_t)nlines), "line queue"),
+                  "line queue")) {
       return LineQueue{nullptr, nullptr, nullptr};
     }
     q.p = static_cast<unsigned*>(p);
@@ -83,7 +84,9 @@ double* line_table(const void* key, size_t keylen, long total, bool* fresh, bool
     return it->second.p;
   }
   double* p = nullptr;
-  if (hipMalloc(&p, sizeof(double) * (size_t)total) != hipSuccess) return nullptr;
+  if (dev_malloc(reinterpret_cast<void**>(&p), sizeof(double) * (size_t)total,
+                 "reference-line table") != hipSuccess)
+    return nullptr;
   cache[k] = TableEntry{p, now};
   *fresh = true;
   *validated = false;

@@ -41,6 +41,10 @@ module vk_capi
       type(*), dimension(*) :: host
       integer(c_size_t), intent(out) :: bytes
     end function
+    integer(c_int) function vk_guard_check(damaged) bind(C, name="vk_guard_check")
+      import :: c_int
+      integer(c_int), intent(out) :: damaged
+    end function
 
     integer(c_int) function vk_set_bc(u, v, w, p, nx, ny, nz, t) bind(C, name="vk_set_bc")
       import :: c_int, c_double
