@@ -84,42 +84,6 @@ __device__ inline void xcd_block(int& bx, int& by, int& bz) {
   bz = (int)(q / (gridDim.x * gridDim.y));
 }
 
-// xcd_block in bricks, for stencil kernels over a 3D grid of blocks whose z
-// extent is a multiple of 8 (otherwise it is xcd_block).  Each XCD still
-// takes one slab of gridDim.z / 8 planes, but walks it a brick at a time:
-// RJ rows of tiles (all of x) through every plane of the slab, then the
-// next RJ rows.  The blocks in flight on an XCD then span a few planes of
-// one brick, so the planes a stencil reads again (k - 1, k + 1) and the
-// neighbouring rows are still in that XCD's L2; walking whole planes, the
-// planes in flight on an XCD hold more than its L2.
-template <unsigned RJ>
-__device__ inline void xcd_block_brick(int& bx, int& by, int& bz) {
-  const unsigned X = 8;
-  if (gridDim.z % X != 0) {
-    xcd_block(bx, by, bz);
-    return;
-  }
-  const unsigned p = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  const unsigned T = gridDim.x * gridDim.y;  // tiles per plane
-  const unsigned kz = gridDim.z / X;         // planes per XCD
-  const unsigned C = gridDim.x * RJ;         // tiles per brick
-  const unsigned x = p % X, r = p / X;       // the XCD and its r-th block
-  const unsigned full = T / C * C;           // tiles in whole bricks
-  unsigned t, kl;
-  if (r < full * kz) {
-    const unsigned c = r / (C * kz), rr = r % (C * kz);
-    kl = rr / C;
-    t = c * C + rr % C;
-  } else {
-    const unsigned rr = r - full * kz, cl = T - full;
-    kl = rr / cl;
-    t = full + rr % cl;
-  }
-  bx = (int)(t % gridDim.x);
-  by = (int)(t / gridDim.x);
-  bz = (int)(x * kz + kl);
-}
-
 // decode halo index q into (i,j,k); returns false past the end.
 // segments: z faces (all i,j), y faces (k interior, all i), x faces
 // (j,k interior), and -- when `vplane` -- the interior of the j = ny plane.
@@ -165,6 +129,38 @@ __device__ inline bool halo_decode(const Geom& g, long long q, bool vplane, int&
 __host__ __device__ inline long long halo_count(const Geom& g, bool vplane) {
   long long n = 2LL * (g.nx + 2) * (g.ny + 2) + 2LL * (g.nx + 2) * g.nz + 2LL * g.ny * g.nz;
   return vplane ? n + (long long)g.nx * g.nz : n;
+}
+
+// A launch over the interior in blocks of kFlatThreads consecutive cells of
+// the halo array (linear index c = i + sy j + sz k, for grids that fit32()),
+// each block's first cell a multiple of kFlatThreads, in the dispatcher's
+// own block order.  Every 128-byte line of a field is then written by one
+// block: with 2D tiles the ends of a tile's row segments (rows start 8
+// bytes into a line) share lines with the tiles beside them, which run on
+// other XCDs, so those lines are written back in pieces from two L2s.
+// Lanes on halo cells take no point.
+constexpr int kFlatThreads = 256;
+struct FlatRange {
+  unsigned c0;  // first cell of block 0
+  unsigned nb;  // blocks
+};
+inline FlatRange flat_interior(const Geom& g) {
+  const unsigned first = g.at32(1, 1, 1), last = g.at32(g.nx, g.ny, g.nz);
+  const unsigned c0 = first / kFlatThreads * kFlatThreads;
+  return FlatRange{c0, (last - c0) / kFlatThreads + 1};
+}
+// cell c of a flat launch as (i, j, k); false on a halo cell
+__device__ inline bool flat_point(const Geom& g, unsigned c, int& i, int& j, int& k) {
+  const unsigned sz = (unsigned)g.sz, sy = (unsigned)g.sy;
+  const unsigned kk = c / sz, r = c - kk * sz;
+  const unsigned jj = r / sy, ii = r - jj * sy;
+  i = (int)ii;
+  j = (int)jj;
+  k = (int)kk;
+  return ii - 1u < (unsigned)g.nx && jj - 1u < (unsigned)g.ny && kk - 1u < (unsigned)g.nz;
+}
+__device__ inline unsigned flat_cell(const FlatRange& fr) {
+  return fr.c0 + blockIdx.x * (unsigned)kFlatThreads + threadIdx.x;
 }
 
 // A launch over the interior points (i,j,k) in 1..n, 64 x 4 threads per

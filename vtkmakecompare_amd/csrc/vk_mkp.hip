@@ -86,16 +86,15 @@ __device__ inline void project_point(const Geom& g, const RMetrics& r, const dou
   o.w_ta[c] = dt * gz;
 }
 
-// the interior points, in full 64 x 4 tiles along (i, j)
+// the interior points, a flat launch (flat_interior)
 template <bool EP2>
-__global__ void k_project(Geom g, RMetrics r, const double* phi, const double* ut,
-                          const double* vt, const double* wt, double dt, MkpOut o) {
-  int bx, by, bz;
-  xcd_block_brick<7>(bx, by, bz);
-  const int i = 1 + bx * blockDim.x + threadIdx.x;
-  const int j = 1 + by * blockDim.y + threadIdx.y;
-  if (i > g.nx || j > g.ny) return;
-  project_point<true, EP2>(g, r, phi, ut, vt, wt, dt, o, i, j, 1 + bz);
+__global__ __launch_bounds__(kFlatThreads) void k_project(Geom g, FlatRange fr, RMetrics r,
+                                                          const double* phi, const double* ut,
+                                                          const double* vt, const double* wt,
+                                                          double dt, MkpOut o) {
+  int i, j, k;
+  if (!flat_point(g, flat_cell(fr), i, j, k)) return;
+  project_point<true, EP2>(g, r, phi, ut, vt, wt, dt, o, i, j, k);
 }
 
 // the points of 0..n with an index 0: the faces i = 0, j = 0, k = 0
@@ -164,13 +163,12 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
   ProfScope ps("project", s);
   const long long nf = zero_face_count(g);
   const unsigned nfb = (unsigned)((nf + 255) / 256);
+  const FlatRange fr = flat_interior(g);
   if (o.u_ep2) {
-    k_project<true><<<interior_grid(g), interior_block(), 0, s>>>(g, r, phi, u_temp, v_temp,
-                                                                  w_temp, dt, o);
+    k_project<true><<<fr.nb, kFlatThreads, 0, s>>>(g, fr, r, phi, u_temp, v_temp, w_temp, dt, o);
     k_project_faces<true><<<nfb, 256, 0, s>>>(g, r, phi, u_temp, v_temp, w_temp, dt, o);
   } else {
-    k_project<false><<<interior_grid(g), interior_block(), 0, s>>>(g, r, phi, u_temp, v_temp,
-                                                                   w_temp, dt, o);
+    k_project<false><<<fr.nb, kFlatThreads, 0, s>>>(g, fr, r, phi, u_temp, v_temp, w_temp, dt, o);
     k_project_faces<false><<<nfb, 256, 0, s>>>(g, r, phi, u_temp, v_temp, w_temp, dt, o);
   }
   return VK_OK;
@@ -232,6 +230,7 @@ constexpr int kDivSlots = 1024;
 // the block's largest |divu| into its slot: non-negative doubles order as
 // their bit patterns do, so an integer maximum over the bits is the
 // floating-point maximum.  Every lane of the block must call it.
+static_assert(kFlatThreads == kBX * kBY, "block_divmax folds blocks of kBX x kBY lanes");
 __device__ inline void block_divmax(double a, unsigned long long* slots) {
   __shared__ unsigned long long wmax[kBX * kBY / 64];
   unsigned long long b = (unsigned long long)__double_as_longlong(a);
@@ -250,19 +249,21 @@ __device__ inline void block_divmax(double a, unsigned long long* slots) {
   }
 }
 
+// the interior points, a flat launch (flat_interior)
 template <bool PSI>
-__global__ void k_project_mixed(Geom g, RMetrics r, const float* phi1, const float* dphi,
-                                const double* ut, const double* vt, const double* wt, double dt,
-                                MkpOutMixed o) {
-  int bx, by, bz;
-  xcd_block_brick<7>(bx, by, bz);
-  const int i = 1 + bx * blockDim.x + threadIdx.x;
-  const int j = 1 + by * blockDim.y + threadIdx.y;
-  // lanes past the interior take part in the block maximum with 0
-  const double a = (i > g.nx || j > g.ny)
-                       ? 0.0
-                       : project_mixed_point<true, PSI>(g, r, phi1, dphi, ut, vt, wt, dt, o, i,
-                                                        j, 1 + bz);
+__global__ __launch_bounds__(kFlatThreads) void k_project_mixed(Geom g, FlatRange fr, RMetrics r,
+                                                                const float* phi1,
+                                                                const float* dphi,
+                                                                const double* ut,
+                                                                const double* vt,
+                                                                const double* wt, double dt,
+                                                                MkpOutMixed o) {
+  int i, j, k;
+  // lanes on halo cells take part in the block maximum with 0
+  const double a = flat_point(g, flat_cell(fr), i, j, k)
+                       ? project_mixed_point<true, PSI>(g, r, phi1, dphi, ut, vt, wt, dt, o, i,
+                                                        j, k)
+                       : 0.0;
   block_divmax(a, o.divmax);
 }
 
@@ -403,17 +404,18 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
   op.divmax = divmax;
   op.psi = nullptr;
   const long long nf = zero_face_count(g);
+  const FlatRange fr = flat_interior(g);
   auto project = [&](const char* name) {
     ProfScope ps(name, s);
     const unsigned nfb = (unsigned)((nf + 255) / 256);
     if (op.psi) {
-      k_project_mixed<true><<<interior_grid(g), interior_block(), 0, s>>>(
-          g, r, phi1, dphi, u_temp, v_temp, w_temp, dt, op);
+      k_project_mixed<true><<<fr.nb, kFlatThreads, 0, s>>>(g, fr, r, phi1, dphi, u_temp, v_temp,
+                                                           w_temp, dt, op);
       k_project_mixed_faces<true><<<nfb, 256, 0, s>>>(g, r, phi1, dphi, u_temp, v_temp, w_temp,
                                                        dt, op);
     } else {
-      k_project_mixed<false><<<interior_grid(g), interior_block(), 0, s>>>(
-          g, r, phi1, dphi, u_temp, v_temp, w_temp, dt, op);
+      k_project_mixed<false><<<fr.nb, kFlatThreads, 0, s>>>(g, fr, r, phi1, dphi, u_temp, v_temp,
+                                                            w_temp, dt, op);
       k_project_mixed_faces<false><<<nfb, 256, 0, s>>>(g, r, phi1, dphi, u_temp, v_temp, w_temp,
                                                         dt, op);
     }
