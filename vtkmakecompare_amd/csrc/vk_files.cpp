@@ -94,7 +94,10 @@ void fortran_i(long long v, int w, std::string& out) {
 
 // A caller's array of n doubles on the host: a registered host array is
 // brought up to date from its device mirror, device memory is copied into
-// a buffer, any other pointer is host memory.
+// a buffer, any other pointer is host memory.  Device copies go on the
+// library stream, after the work queued there (every entry point joins its
+// streams into it; a copy on the null stream would not wait for them, as
+// the library's streams do not block on it).
 struct HostIn {
   const double* p = nullptr;
   std::vector<double> buf;
@@ -115,8 +118,10 @@ int host_in(const double* a, size_t n, HostIn& h, const char* what) {
     if (device_extent(a, &ext) && ext < n * sizeof(double))
       return fail(VK_ERR_ARGUMENT, std::string(what) + ": device allocation too small");
     h.buf.resize(n);
-    int rc = hip_check(hipMemcpy(h.buf.data(), a, n * sizeof(double), hipMemcpyDeviceToHost),
+    int rc = hip_check(hipMemcpyAsync(h.buf.data(), a, n * sizeof(double),
+                                      hipMemcpyDeviceToHost, rt().stream),
                        what);
+    if (!rc) rc = hip_check(hipStreamSynchronize(rt().stream), what);
     if (rc) return rc;
     h.p = h.buf.data();
     return VK_OK;
@@ -135,7 +140,13 @@ struct HostOut {
   bool dev = false, reg = false;
   size_t n = 0;
   int finish(const char* what) {
-    if (dev) return hip_check(hipMemcpy(dst, p, n * sizeof(double), hipMemcpyHostToDevice), what);
+    if (dev) {
+      // on the library stream, after the work queued there, like host_in
+      int rc = hip_check(
+          hipMemcpyAsync(dst, p, n * sizeof(double), hipMemcpyHostToDevice, rt().stream), what);
+      if (!rc) rc = hip_check(hipStreamSynchronize(rt().stream), what);
+      return rc;
+    }
     if (reg) return vk_upload(dst);
     return VK_OK;
   }
