@@ -38,10 +38,8 @@ HBM_PEAK_GBS = 8000.0
 # general kernel for blocks next to the solid -- and are reported by their
 # sweep name only)
 KERNEL_SYMBOL = {
-    "explicit_vel": "vk::k_explicit_fused<1>",
-    "explicit_t": "vk::k_explicit_fused<2>",
-    "explicit_mixed": "vk::k_explicit_mixed<1>",
-    "explicit_mixed_t": "vk::k_explicit_mixed<2>",
+    "explicit_fused": "vk::k_explicit_fused",
+    "explicit_mixed": "vk::k_explicit_mixed",
     "project": "vk::k_project<false>",
     "project_mixed": "vk::k_project_mixed",
     "rfft_x_div": "vk::k_rfft_x<vk::SampleDiv,double>",
@@ -63,10 +61,6 @@ def kernel_bytes(name: str, g, with_ep2: bool = False, mixed: bool = False) -> f
         table = {
             # u,v,w,T,rhoCp,thc, 4 history, 3 u_ta, 4 cellvalue in; 4 rhs, 4 history out
             "explicit_fused": 25 * f,
-            # the same in two kernels: u,v,w, 3 history, 3 u_ta, cellvalue 2-4 in,
-            # 3 rhs and 3 history out; u,v,w, T, rhoCp, thc, history, cellvalue 1
-            # in, T work and history out
-            "explicit_vel": 18 * f, "explicit_t": 10 * f,
             "line_y_face": 3 * f, "line_y_v": 3 * f, "line_z_face": 3 * f, "line_x_face": 3 * f,
             # T lines also read rhoCp and thc
             "line_y_t": 5 * f, "line_x_t": 5 * f, "line_z_t": 5 * f,
@@ -79,11 +73,9 @@ def kernel_bytes(name: str, g, with_ep2: bool = False, mixed: bool = False) -> f
     h = 4.0 * N  # one float field
     S = (g.nx // 2 + 1) * g.ny * g.nz * 8.0
     table = {
-        # velocity part -- in: u,v,w (double), 3 history, 3 u_ta, cellvalue 2-4
-        # (float); out: u_temp, v_temp, w_temp (double), 3 history (float).
-        # temperature part -- in: u,v,w (double), T, rhoCp, thc, history,
-        # cellvalue 1 (float); out: T work, history (float)
-        "explicit_mixed": 6 * f + 12 * h, "explicit_mixed_t": 3 * f + 7 * h,
+        # in: u,v,w (double); 4 history, 3 u_ta, 4 cellvalue, T, rhoCp, thc
+        # (float); out: u_temp, v_temp, w_temp (double), T work, 4 history (float)
+        "explicit_mixed": 6 * f + 19 * h,
         # velocity lines: double rhs and solution, float cellvalue
         "line_y_face": 2 * f + h, "line_y_v": 2 * f + h, "line_z_face": 2 * f + h,
         "line_x_face": 2 * f + h,
@@ -107,10 +99,7 @@ def kernel_bytes(name: str, g, with_ep2: bool = False, mixed: bool = False) -> f
 # form: u, v, w in FP64, every other array (history, u_ta, cellvalue, the
 # ADI right-hand sides, u_temp, the spectra, the Poisson solutions) in FP32.
 #   A  explicit: u,v,w,T, 4 history, 3 u_ta, cv1 in; u*,v*,w*,T*, 4 history
-#      out (20 values).  Velocity kernel: u,v,w, 3 history, 3 u_ta, cv1 in,
-#      6 out (16 values, FP64 128 B, mixed 3*8 + 13*4 = 76 B); temperature
-#      kernel: T, T history in, T*, T history out (4 values; u, v, w and cv1
-#      are counted with the velocity kernel).
+#      out (20 values: FP64 160 B, mixed 3*8 + 17*4 = 92 B), one kernel.
 #   B, C, D  the y, z, x ADI sweeps: 4 right-hand sides and cv1 in, 4 out
 #      per phase; each component's sweep 2 values plus a quarter of cv1.
 #   E+F  divergence and x real FFT: u_temp, v_temp, w_temp in, the half
@@ -125,12 +114,12 @@ def kernel_bytes(name: str, g, with_ep2: bool = False, mixed: bool = False) -> f
 #      3 u_ta, divu out (11 values; mixed 72 B).
 # Totals: FP64 552 B per point per step, mixed 372 B.
 PHASE_BYTES_PER_POINT = {
-    False: {"explicit_vel": 128.0, "explicit_t": 32.0,
+    False: {"explicit_fused": 160.0,
             "line_y_face": 18.0, "line_y_v": 18.0, "line_z_face": 18.0, "line_x_face": 18.0,
             "line_y_t": 18.0, "line_x_t": 18.0, "line_z_t": 18.0,
             "rfft_x_div": 32.0, "cfft_z_fwd": 40.0 / 3, "line_spectral": 40.0 / 3,
             "cfft_z_inv": 40.0 / 3, "irfft_x": 16.0, "project": 88.0},
-    True: {"explicit_mixed": 76.0, "explicit_mixed_t": 16.0,
+    True: {"explicit_mixed": 92.0,
            "line_y_face": 9.0, "line_y_v": 9.0, "line_z_face": 9.0, "line_x_face": 9.0,
            "line_y_t": 9.0, "line_x_t": 9.0, "line_z_t": 9.0,
            "rfft_x_div": 16.0, "cfft_z_fwd": 20.0 / 3, "line_spectral": 20.0 / 3,
@@ -168,7 +157,7 @@ def line_unknowns(name: str, g) -> int:
     return g.nx * g.ny * g.nz
 
 
-STEP_LAUNCHES = {"explicit_vel": 1, "explicit_t": 1, "explicit_mixed": 1, "explicit_mixed_t": 1, "line_y_face": 2, "line_y_v": 1,
+STEP_LAUNCHES = {"explicit_fused": 1, "explicit_mixed": 1, "line_y_face": 2, "line_y_v": 1,
                  "line_z_face": 3, "line_x_face": 3, "line_y_t": 1, "line_x_t": 1,
                  "line_z_t": 1, "rfft_x_div": 1, "rfft_x_resid": 1, "cfft_z_fwd": 1,
                  "cfft_z_fwd2": 1, "cfft_z_inv": 1, "cfft_z_inv2": 1, "irfft_x": 1,

@@ -182,10 +182,9 @@ int fft_sizes_ok(const Geom& g);
 struct MkpOut {
   double *p, *u_ta, *v_ta, *w_ta, *u_ep2, *v_ep2, *w_ep2, *divu, *u, *v, *w;
 };
-// before_project: an event the projection (which writes u, v, w) waits for
 int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const double* v_temp,
                const double* w_temp, double dt, double xl, double zl, const MkpOut& o,
-               hipStream_t s, hipEvent_t before_project = nullptr);
+               hipStream_t s);
 
 // the mixed-precision projection: FP64 velocity, p and divu, float u_ta
 struct MkpOutMixed {
@@ -206,7 +205,7 @@ struct MkpOutMixed {
 constexpr double kMixedDivTol = 9e-11;
 int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
                      const double* v_temp, const double* w_temp, double dt, double xl, double zl,
-                     const MkpOutMixed& o, hipStream_t s, hipEvent_t before_project = nullptr);
+                     const MkpOutMixed& o, hipStream_t s);
 
 // explicit phase of the FP64 fused step (vk_step.hip): all explicit terms,
 // penalisation, AB2 and both divides; writes u_temp.. (ua, va, wa), the
@@ -220,9 +219,9 @@ struct ExplicitArgs {
 };
 // the scalar part of ExplicitArgs for step l (AB2 weights, CN factors)
 ExplicitArgs explicit_args(double dt, double re, double sc, double eta, int l);
-// part 1: the velocity components, part 2: the temperature, 0: both
+// the explicit phase of u, v, w and T (one kernel, the velocity read once)
 void launch_explicit_fused(const Geom& g, const Metrics& m, const RMetrics& r,
-                           const ExplicitArgs& x, hipStream_t s, int part = 0);
+                           const ExplicitArgs& x, hipStream_t s);
 // the spectrum (double) of f(i,j,k) / dy(j), index 0 of x and z read as
 // the periodic image n (for the transposed Poisson solve)
 int launch_p2s_wscaled(const Geom& g, const double* f, const double* rdy, double* phit,

@@ -91,9 +91,9 @@ __device__ inline R pen_div_m(R x, R eta, R cv, R dt) {
 
 // The explicit phase of one step: the same terms as k_explicit_fused
 // (vk_step.hip), from the FP64 velocity read as float.
-// PART 1: u, v, w; PART 2: T (disjoint outputs, so the two may run
-// concurrently)
-template <int PART>
+// u, v, w, then T, in one pass: T takes the velocity at the point and its
+// lower neighbours from the stencils loaded for u, v, w, and T's loads are
+// issued before the velocity's stores
 __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, ExplicitMixedArgs x) {
   int bx, by, bz;
   xcd_block(bx, by, bz);
@@ -103,9 +103,10 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
   if (i > g.nx || j > g.ny) return;
   const long long o = g.at(i, j, k);
   const float dt = x.dt, eta = x.eta;
-  if (PART == 1) {
+  {
     const float cv2 = x.cv[g.cv(i, j, k, 2)], cv3 = x.cv[g.cv(i, j, k, 3)],
                 cv4 = x.cv[g.cv(i, j, k, 4)];
+    const float cv1 = x.cv[g.cv(i, j, k, 1)];
     // the three stencils and the history, every load issued before any
     // arithmetic; the schedule barrier keeps the compiler from spreading
     // them over the computation
@@ -115,7 +116,7 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
     PtMet<float> M;
     M.load_f(f, i, j, k);
     __builtin_amdgcn_sched_barrier(0);
-    const unsigned mk = x.lf.fx ? dirty_bits(0.0f, cv2, cv3, cv4, 1.0f, 1.0f) : 0u;
+    unsigned mk = x.lf.fx ? dirty_bits(0.0f, cv2, cv3, cv4, 1.0f, 1.0f) : 0u;
     // u_ta read only where its term eta cv u_ta dt is not 0
     float uta = 0.0f, vta = 0.0f, wta = 0.0f;
     if (cv2 != 0.0f) uta = x.u_ta[o];
@@ -139,6 +140,10 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
     const double un = pen_div_m<double>(U.v[1][1][1] + (double)ua, x.etad, (double)cv2, x.dtd);
     const double vn = pen_div_m<double>(V.v[1][1][1] + (double)va, x.etad, (double)cv3, x.dtd);
     const double wn = pen_div_m<double>(W.v[1][1][1] + (double)wa, x.etad, (double)cv4, x.dtd);
+    // T's 7-point stencils of T, rhoCp and thc
+    Nb27<float, float> T, RC, K;
+    load_scalar_fields(T, RC, K, x.t, x.rc, x.thc, g, i, j, k);
+    const float tpre = x.t_pre[o];
     x.u_pre[o] = uv;
     x.ua[o] = un;
     x.v_pre[o] = vv;
@@ -148,19 +153,10 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
     if (j == g.ny) x.va[g.at(i, g.ny + 1, k)] = -(double)va;
     x.w_pre[o] = wv;
     x.wa[o] = wn;
-    if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
-  } else {
-    // every load issued before any arithmetic, as in PART 1
-    Nb27<float, float> T, RC, K;
-    Nb27<double, float> U, V, W;
-    const float cv1 = x.cv[g.cv(i, j, k, 1)];
-    load_scalar_nbhd(T, RC, K, U, V, W, x.t, x.rc, x.thc, x.u, x.v, x.w, g, i, j, k);
-    const float tpre = x.t_pre[o];
-    PtMet<float> M;
-    M.load_f(f, i, j, k);
+    // T, with the velocity values of U, V, W
     __builtin_amdgcn_sched_barrier(0);
     const float rc0 = RC(i, j, k);
-    const unsigned mk = x.lf.fx ? dirty_bits(cv1, 0.0f, 0.0f, 0.0f, rc0, K(i, j, k)) : 0u;
+    if (x.lf.fx) mk |= dirty_bits(cv1, 0.0f, 0.0f, 0.0f, rc0, K(i, j, k));
     const float rrc = 1.0f / rc0;
     const float vt = conv_scalar_r<float>(U, V, W, T, RC, M, M, i, j, k, dt, rrc);
     const float dtt = diff_scalar_r<float>(T, K, M, M, i, j, k, x.cd, rrc);
@@ -170,6 +166,7 @@ __global__ __launch_bounds__(256) void k_explicit_mixed(Geom g, FMet f, Explicit
     if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
   }
 }
+
 
 }  // namespace vk
 
@@ -247,20 +244,15 @@ extern "C" int vk_time_step_mixed(double* u, double* v, double* w, double* p, fl
   x.cd = (float)(*dt / (2.0 * *re * *sc));
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
   {
+    // the explicit phase of u, v, w and T in one kernel (the velocity read
+    // once); every later kernel that writes u, v, w runs after it
     ProfScope ps("explicit_mixed", s);
-    k_explicit_mixed<1><<<interior_grid(g), interior_block(), 0, s>>>(g, f, x);
+    k_explicit_mixed<<<interior_grid(g), interior_block(), 0, s>>>(g, f, x);
   }
-  // T's explicit part, sweeps and halo on auxiliary stream 2 beside the
-  // velocity sweeps and MKP, whose projection waits until the T part has
-  // read u, v, w; the y sweep of T writes T's interior
+  // T's sweeps and halo on auxiliary stream 2 beside the velocity sweeps
+  // and MKP; the y sweep of T writes T's interior
   AuxFork ft(s, 4u);
   if (ft.err) return ft.err;
-  {
-    ProfScope ps("explicit_mixed_t", ft.aux(2));
-    k_explicit_mixed<2><<<interior_grid(g), interior_block(), 0, ft.aux(2)>>>(g, f, x);
-  }
-  rc = hip_check(hipEventRecord(rt().ev_mid, ft.aux(2)), "event");
-  if (rc) return rc;
   rc = launch_adi_temperature<float>(g, m, r, twork, dcv, drc, dk, beta_t, dteta, ft.aux(2),
                                      dtt, &lf);
   if (rc) return rc;
@@ -279,8 +271,7 @@ extern "C" int vk_time_step_mixed(double* u, double* v, double* w, double* p, fl
   }
   launch_set_bc<float>(g, dut, dvt, dwt, dp, nullptr, s);
   MkpOutMixed o{dp, duta, dvta, dwta, ddiv, du, dv, dw};
-  rc = launch_mkp_mixed(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s,
-                        rt().serial ? nullptr : rt().ev_mid);
+  rc = launch_mkp_mixed(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s);
   if (rc) return rc;
   rc = ft.join(2);
   if (rc) return rc;

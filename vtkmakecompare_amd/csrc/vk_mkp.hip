@@ -129,7 +129,7 @@ __global__ void k_project_faces(Geom g, RMetrics r, const double* phi, const dou
 
 int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const double* v_temp,
                const double* w_temp, double dt, double xl, double zl, const MkpOut& o,
-               hipStream_t s, hipEvent_t before_project) {
+               hipStream_t s) {
   if (int frc = fft_sizes_ok(g)) return frc;
   const int nh = g.nx / 2;
   size_t nk = (size_t)(nh + 1) * (g.nz + 1) + 3 * (size_t)g.ny;
@@ -156,10 +156,6 @@ int launch_mkp(const Geom& g, const Metrics& m, const double* u_temp, const doub
   rc = launch_s2p_inplace<double>(g, phit, phi, s, "cfft_z_inv", "irfft_x");
   if (rc) return rc;
   launch_halo_neumann(g, phi, s);
-  if (before_project) {
-    rc = hip_check(hipStreamWaitEvent(s, before_project, 0), "mkp wait");
-    if (rc) return rc;
-  }
   ProfScope ps("project", s);
   const long long nf = zero_face_count(g);
   const unsigned nfb = (unsigned)((nf + 255) / 256);
@@ -353,7 +349,7 @@ __global__ void k_project_mixed_faces(Geom g, RMetrics r, const float* phi1, con
 // waits for the stream here.
 int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
                      const double* v_temp, const double* w_temp, double dt, double xl, double zl,
-                     const MkpOutMixed& o, hipStream_t s, hipEvent_t before_project) {
+                     const MkpOutMixed& o, hipStream_t s) {
   if (int frc = fft_sizes_ok(g)) return frc;
   const int nh = g.nx / 2;
   size_t nk = (size_t)(nh + 1) * (g.nz + 1) + 3 * (size_t)g.ny;
@@ -388,10 +384,6 @@ int launch_mkp_mixed(const Geom& g, const Metrics& m, const double* u_temp,
   rc = launch_s2p_inplace<float>(g, phit, dphi, s, "cfft_z_inv2", "irfft_x2");
   if (rc) return rc;
   launch_halo_neumann<float>(g, dphi, s);
-  if (before_project) {
-    rc = hip_check(hipStreamWaitEvent(s, before_project, 0), "mkp wait");
-    if (rc) return rc;
-  }
   bool fresh_slots = false;
   unsigned long long* divmax = static_cast<unsigned long long*>(
       workspace(kWorkDivMax, kDivSlots * sizeof(unsigned long long), &fresh_slots));

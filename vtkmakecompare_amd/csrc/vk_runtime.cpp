@@ -305,8 +305,6 @@ int vk_init(const int* device) {
   }
   rc = hip_check(hipEventCreateWithFlags(&r.ev_fork, hipEventDisableTiming), "hipEventCreate");
   if (rc) return rc;
-  rc = hip_check(hipEventCreateWithFlags(&r.ev_mid, hipEventDisableTiming), "hipEventCreate");
-  if (rc) return rc;
   r.device = dev;
   r.initialised = true;
   return VK_OK;
@@ -333,9 +331,7 @@ int vk_finalize(void) {
     r.ev_join[q] = nullptr;
   }
   (void)hipEventDestroy(r.ev_fork);
-  (void)hipEventDestroy(r.ev_mid);
   r.ev_fork = nullptr;
-  r.ev_mid = nullptr;
   (void)hipStreamDestroy(r.stream);
   r.stream = nullptr;
   r.initialised = false;

@@ -23,7 +23,6 @@ struct Runtime {
   hipStream_t aux[3] = {nullptr, nullptr, nullptr};
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join[3] = {nullptr, nullptr, nullptr};
-  hipEvent_t ev_mid = nullptr;  // a point within auxiliary stream 2's work
   // set while kernels are being timed: the work meant for the auxiliary
   // streams is issued on the forking stream instead, one kernel after
   // another, so that each kernel's events time it alone

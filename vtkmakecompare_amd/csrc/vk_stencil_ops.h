@@ -278,20 +278,19 @@ __device__ inline void load_velocity_nbhd(Nb27<T, R>& U, Nb27<T, R>& V, Nb27<T, 
   W.load(w, g, o, 0, 1, -1);
 }
 
-// the points SET_CONVECTION_FOR_SCALAR and SET_DIFFUSION_FOR_SCALAR read
-// at one point (nse_dominik.f:852-879, :969-999): the 7-point stencils of
-// T, rhoCp and thc, and each velocity component at the point and its
-// lower neighbour along the component's axis
-template <typename TS, typename TV, typename R>
-__device__ inline void load_scalar_nbhd(Nb27<TS, R>& T, Nb27<TS, R>& RC, Nb27<TS, R>& K,
-                                        Nb27<TV, R>& U, Nb27<TV, R>& V, Nb27<TV, R>& W,
-                                        const TS* t, const TS* rc, const TS* thc, const TV* u,
-                                        const TV* v, const TV* w, const Geom& g, int i, int j,
-                                        int k) {
+// the points of T, rhoCp and thc that SET_CONVECTION_FOR_SCALAR and
+// SET_DIFFUSION_FOR_SCALAR read at one point (nse_dominik.f:852-879,
+// :969-999): their 7-point stencils at (i, j, k).  The two routines also
+// read each velocity component at the point and its lower neighbour along
+// the component's axis, which load_velocity_nbhd's stencils hold.
+template <typename TS, typename R>
+__device__ inline void load_scalar_fields(Nb27<TS, R>& T, Nb27<TS, R>& RC, Nb27<TS, R>& K,
+                                          const TS* t, const TS* rc, const TS* thc,
+                                          const Geom& g, int i, int j, int k) {
   const long long o = g.at(i, j, k);
-  T.i0 = RC.i0 = K.i0 = U.i0 = V.i0 = W.i0 = i;
-  T.j0 = RC.j0 = K.j0 = U.j0 = V.j0 = W.j0 = j;
-  T.k0 = RC.k0 = K.k0 = U.k0 = V.k0 = W.k0 = k;
+  T.i0 = RC.i0 = K.i0 = i;
+  T.j0 = RC.j0 = K.j0 = j;
+  T.k0 = RC.k0 = K.k0 = k;
   Nb27<TS, R>* f[3] = {&T, &RC, &K};
   const TS* p[3] = {t, rc, thc};
 #pragma unroll
@@ -304,12 +303,6 @@ __device__ inline void load_scalar_nbhd(Nb27<TS, R>& T, Nb27<TS, R>& RC, Nb27<TS
     f[c]->load(p[c], g, o, 0, 0, 1);
     f[c]->load(p[c], g, o, 0, 0, -1);
   }
-  U.load(u, g, o, 0, 0, 0);
-  U.load(u, g, o, -1, 0, 0);
-  V.load(v, g, o, 0, 0, 0);
-  V.load(v, g, o, 0, -1, 0);
-  W.load(w, g, o, 0, 0, 0);
-  W.load(w, g, o, 0, 0, -1);
 }
 
 // SET_CONVECT u (nse_dominik.f:774-785)

@@ -27,9 +27,10 @@ __device__ inline double pen_div(double x, double eta, double cv, double dt) {
   return x / (1.0 + eta * cv * dt);
 }
 
-// PART 1: u, v, w; PART 2: T.  The two parts read the same velocity and
-// write disjoint arrays, so they may run concurrently.
-template <int PART>
+// u, v, w, then T, in one pass: T takes the velocity at the point and its
+// lower neighbours from the stencils loaded for u, v, w, so the velocity is
+// read once, and T's loads are issued before the velocity's stores, so
+// they are in flight while those go out.
 __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetrics r,
                                                         ExplicitArgs x) {
   int bx, by, bz;
@@ -40,11 +41,12 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
   if (i > g.nx || j > g.ny) return;
   const long long o = g.at(i, j, k);
   const double dt = x.dt, eta = x.eta;
-  if (PART == 1) {
+  {
     // the point's cellvalue slots, read before any store (the outputs may
     // alias them as far as the compiler knows)
     const double cv2 = x.cv[g.cv(i, j, k, 2)], cv3 = x.cv[g.cv(i, j, k, 3)],
                  cv4 = x.cv[g.cv(i, j, k, 4)];
+    const double cv1 = x.cv[g.cv(i, j, k, 1)];
     // the three stencils and the history, every load issued before any
     // arithmetic (left to itself the compiler issues a few loads, waits,
     // computes and issues the next few); the values and the operations on
@@ -55,7 +57,7 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
     PtMet<double> M;
     M.load(m, r, i, j, k);
     __builtin_amdgcn_sched_barrier(0);
-    const unsigned mk = x.lf.fx ? dirty_bits(0.0, cv2, cv3, cv4, 1.0, 1.0) : 0u;
+    unsigned mk = x.lf.fx ? dirty_bits(0.0, cv2, cv3, cv4, 1.0, 1.0) : 0u;
     // u_ta enters only through eta cv u_ta dt: where the face value is 0
     // the term is 0 and u_ta is not read (in fluid, most of the grid, the
     // three fields' traffic is saved)
@@ -83,6 +85,10 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
     const double un = pen_div(U(i, j, k) + ua, eta, cv2, dt);
     const double vn = pen_div(V(i, j, k) + va, eta, cv3, dt);
     const double wn = pen_div(W(i, j, k) + wa, eta, cv4, dt);
+    // T's 7-point stencils of T, rhoCp and thc
+    Nb27<double, double> T, RC, K;
+    load_scalar_fields(T, RC, K, x.t, x.rc, x.thc, g, i, j, k);
+    const double tpre = x.t_pre[o];
     x.u_pre[o] = uv;
     x.ua[o] = un;
     x.v_pre[o] = vv;
@@ -92,19 +98,11 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
     if (j == g.ny) x.va[g.at(i, g.ny + 1, k)] = -1.0 * va;
     x.w_pre[o] = wv;
     x.wa[o] = wn;
-    if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
-  } else {
-    // T: convection, diffusion, T_l = 0, combine, history, main-loop divide;
-    // every load issued before any arithmetic, as in PART 1
-    Nb27<double, double> T, RC, K, U, V, W;
-    const double cv1 = x.cv[g.cv(i, j, k, 1)];
-    load_scalar_nbhd(T, RC, K, U, V, W, x.t, x.rc, x.thc, x.u, x.v, x.w, g, i, j, k);
-    const double tpre = x.t_pre[o];
-    PtMet<double> M;
-    M.load(m, r, i, j, k);
+    // T: convection, diffusion, T_l = 0, combine, history, main-loop
+    // divide, with the velocity values of U, V, W
     __builtin_amdgcn_sched_barrier(0);
     const double rc0 = RC(i, j, k);
-    const unsigned mk = x.lf.fx ? dirty_bits(cv1, 0.0, 0.0, 0.0, rc0, K(i, j, k)) : 0u;
+    if (x.lf.fx) mk |= dirty_bits(cv1, 0.0, 0.0, 0.0, rc0, K(i, j, k));
     double rrc = 1.0 / rc0;
     double vt = conv_scalar_r(U, V, W, T, RC, M, M, i, j, k, dt, rrc);
     double dtt = diff_scalar_r(T, K, M, M, i, j, k, x.cd, rrc);
@@ -114,6 +112,7 @@ __global__ __launch_bounds__(256) void k_explicit_fused(Geom g, Metrics m, RMetr
     if (x.lf.fx) mark_dirty_lines(x.lf, g, i, j, k, mk);
   }
 }
+
 
 __global__ void k_fill_u32(unsigned* p, long long n) {
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
@@ -136,15 +135,9 @@ int line_flags_clear(const Geom& g, LineFlags& lf, hipStream_t s) {
 }
 
 void launch_explicit_fused(const Geom& g, const Metrics& m, const RMetrics& r,
-                           const ExplicitArgs& x, hipStream_t s, int part) {
-  if (part != 2) {
-    ProfScope ps("explicit_vel", s);
-    k_explicit_fused<1><<<interior_grid(g), interior_block(), 0, s>>>(g, m, r, x);
-  }
-  if (part != 1) {
-    ProfScope ps("explicit_t", s);
-    k_explicit_fused<2><<<interior_grid(g), interior_block(), 0, s>>>(g, m, r, x);
-  }
+                           const ExplicitArgs& x, hipStream_t s) {
+  ProfScope ps("explicit_fused", s);
+  k_explicit_fused<<<interior_grid(g), interior_block(), 0, s>>>(g, m, r, x);
 }
 
 ExplicitArgs explicit_args(double dt, double re, double sc, double eta, int l) {
@@ -235,17 +228,15 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
   if (rrc) return rrc;
   x.lf = lf;
   const double beta_u = *dt / *re / 2.0, beta_t = *dt / *re / *sc / 2.0, dteta = *dt * *eta;
-  launch_explicit_fused(g, m, r, x, s, 1);
-  // The temperature's explicit part, sweeps and halo run on auxiliary
-  // stream 2 beside the velocity sweeps and MKP, whose projection (which
-  // writes u, v, w) waits only until the T part has read them.  T = T work
-  // after the sweeps (nse_dominik.f:429-437): the y sweep writes T's
-  // interior and the x and z sweeps run there.
+  // the explicit phase of u, v, w and T in one kernel (the velocity read
+  // once); every later kernel that writes u, v, w runs after it
+  launch_explicit_fused(g, m, r, x, s);
+  // The temperature's sweeps and halo run on auxiliary stream 2 beside the
+  // velocity sweeps and MKP.  T = T work after the sweeps
+  // (nse_dominik.f:429-437): the y sweep writes T's interior and the x and
+  // z sweeps run there.
   AuxFork ft(s, 4u);
   if (ft.err) return ft.err;
-  launch_explicit_fused(g, m, r, x, ft.aux(2), 2);
-  rrc = hip_check(hipEventRecord(rt().ev_mid, ft.aux(2)), "event");
-  if (rrc) return rrc;
   rrc = launch_adi_temperature(g, m, r, twork, dcv, drc, dk, beta_t, dteta, ft.aux(2), dtt, &lf);
   if (rrc) return rrc;
   launch_set_bc<double>(g, nullptr, nullptr, nullptr, nullptr, dtt, ft.aux(2));
@@ -264,7 +255,7 @@ extern "C" int vk_time_step(double* u, double* v, double* w, double* p, double* 
   }
   launch_set_bc<double>(g, dut, dvt, dwt, dp, nullptr, s);
   MkpOut o{dp, duta, dvta, dwta, de1, de2, de3, ddiv, du, dv, dw};
-  int rc = launch_mkp(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s, rt().serial ? nullptr : rt().ev_mid);
+  int rc = launch_mkp(g, m, dut, dvt, dwt, *dt, *xl, *zl, o, s);
   if (rc) return rc;
   rc = ft.join(2);
   if (rc) return rc;
