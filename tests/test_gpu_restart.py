@@ -70,3 +70,30 @@ def test_readfield_restart_is_not_exact(tmp_path):
     second.step(3)
     d = float(np.max(np.abs(second.get("u") - whole.get("u"))))
     assert d > 1e-10, d
+
+
+def test_restart_written_behind_queued_steps(tmp_path):
+    """A record written from device arrays while steps are still queued
+    holds those steps' results: the FP64 step returns with its kernels in
+    flight, and the writer's copies wait for the library's streams (a copy
+    on the null stream would not, the library's streams being
+    non-blocking)."""
+    from vtkmakecompare_amd import api as A
+    A.init()
+    cfg = G.small_config(96, 65, 64, seed=0x5EED0404)
+    g = cfg.grid()
+    st = G.initial_state(cfg, g)
+    _, cv = oracle_levelset(g, st["phi0"], 100)
+    run = A.ChannelRun(cfg, g, st, cellvalue=cv)
+    run.step(1)
+    run.get("u")  # the first step's one-off set-up done, the queue drained
+    path = str(tmp_path / "queued.restart")
+    run.step(6)   # queued: vk_time_step returns before its kernels finish
+    run.save(path)
+    host = {k: np.zeros(g.shape) for k in A.RESTART_FIELDS}
+    hcv = np.zeros((4, g.nz, g.ny, g.nx))
+    assert A.read_restart(path, g, host, hcv) == 7
+    for k in A.RESTART_FIELDS:
+        np.testing.assert_array_equal(host[k], run.get(k), err_msg=k)
+    np.testing.assert_array_equal(hcv, run.get("cellvalue"))
+    run.free()
