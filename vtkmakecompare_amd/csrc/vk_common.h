@@ -149,18 +149,22 @@ inline FlatRange flat_interior(const Geom& g) {
   const unsigned c0 = first / kFlatThreads * kFlatThreads;
   return FlatRange{c0, (last - c0) / kFlatThreads + 1};
 }
-// cell c of a flat launch as (i, j, k); false on a halo cell
-__device__ inline bool flat_point(const Geom& g, unsigned c, int& i, int& j, int& k) {
+// cell c of a flat launch as (i, j, k); false on a halo cell or past the
+// array (the last block's lanes run past it; c is 64-bit, so that they do
+// not wrap round to cells of the array's start)
+__device__ inline bool flat_point(const Geom& g, unsigned long long c, int& i, int& j, int& k) {
+  if (c >= (unsigned long long)g.cells()) return false;
+  const unsigned c32 = (unsigned)c;  // below cells(), so 32 bits hold it
   const unsigned sz = (unsigned)g.sz, sy = (unsigned)g.sy;
-  const unsigned kk = c / sz, r = c - kk * sz;
+  const unsigned kk = c32 / sz, r = c32 - kk * sz;
   const unsigned jj = r / sy, ii = r - jj * sy;
   i = (int)ii;
   j = (int)jj;
   k = (int)kk;
   return ii - 1u < (unsigned)g.nx && jj - 1u < (unsigned)g.ny && kk - 1u < (unsigned)g.nz;
 }
-__device__ inline unsigned flat_cell(const FlatRange& fr) {
-  return fr.c0 + blockIdx.x * (unsigned)kFlatThreads + threadIdx.x;
+__device__ inline unsigned long long flat_cell(const FlatRange& fr) {
+  return (unsigned long long)fr.c0 + (unsigned long long)blockIdx.x * kFlatThreads + threadIdx.x;
 }
 
 // A launch over the interior points (i,j,k) in 1..n, 64 x 4 threads per
