@@ -400,7 +400,12 @@ int vk_time_step_mixed(double *u, double *v, double *w, double *p, float *t,
    dJ/dcellvalue(:,:,:,1) (nx*ny*nz doubles, x fastest): the exact
    discrete adjoint of the forward step, run backward over the trajectory
    with a checkpoint every `checkpoint_every` steps held in device memory
-   and each segment recomputed from its checkpoint. */
+   and each segment recomputed from its checkpoint.  Where a segment's
+   tapes fit in device memory (23 fields per step, with 4 GiB to spare),
+   every step of the segment is taped as it is recomputed; otherwise each
+   step is rerun in the backward pass with one step's tape.  The two give
+   the same gradient bit for bit; VK_ADJOINT_SEGMENT_TAPES=0 in the
+   environment asks for the second. */
 int vk_objective(const double *u, const double *v, const double *w,
                  const double *p, const double *t, const double *u_pre,
                  const double *v_pre, const double *w_pre, const double *t_pre,

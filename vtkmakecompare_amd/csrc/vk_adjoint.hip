@@ -940,16 +940,47 @@ int vk_adjoint_gradient(const double* u, const double* v, const double* w, const
   rc = sum_objective(g, wk.jacc, objective);
   if (rc) return rc;
 
-  // backward buffers
-  std::vector<double*> seg((size_t)C * kState);
-  for (auto& q : seg)
-    if (!(q = dev.alloc(nc))) return fail(VK_ERR_HIP, "adjoint: out of device memory");
-  double* tape[16];
-  double* in4[4];
-  for (auto& q : tape)
-    if (!(q = dev.alloc(nc))) return fail(VK_ERR_HIP, "adjoint: out of device memory");
-  for (auto& q : in4)
-    if (!(q = dev.alloc(nc))) return fail(VK_ERR_HIP, "adjoint: out of device memory");
+  // backward buffers.  Where a segment's tapes fit in device memory with
+  // 4 GiB to spare (per step the 16 tape fields, the 4 inputs, the 3 u_ta
+  // and the dirty-line masks), every step of a segment is taped once, as
+  // the segment is recomputed from its checkpoint, and the backward pass
+  // runs on those tapes.  Otherwise the recompute keeps each step's state
+  // and the backward pass reruns each step with one step's tape.
+  const size_t nxl = (size_t)g.ny * g.nz, nyl = (size_t)g.nx * g.nz;
+  const size_t nflags = nxl + nyl + (size_t)g.nx * g.ny;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    free_b = 0;
+  }
+  const size_t keep_bytes = (size_t)C * (23 * nc * sizeof(double) + nflags * sizeof(unsigned));
+  // VK_ADJOINT_SEGMENT_TAPES=0 in the environment asks for the per-step
+  // path whatever the memory (the two give the same gradient bit for bit)
+  const char* seg_env = std::getenv("VK_ADJOINT_SEGMENT_TAPES");
+  const bool keep_tapes = keep_bytes + ((size_t)4 << 30) < free_b &&
+                          !(seg_env && std::strcmp(seg_env, "0") == 0);
+  std::vector<double*> seg, tseg, iseg, useg;
+  unsigned* fseg = nullptr;
+  double* tape[16] = {};
+  double* in4[4] = {};
+  if (keep_tapes) {
+    tseg.resize((size_t)C * 16);
+    iseg.resize((size_t)C * 4);
+    useg.resize((size_t)C * 3);
+    for (auto* v : {&tseg, &iseg, &useg})
+      for (auto& q : *v)
+        if (!(q = dev.alloc(nc))) return fail(VK_ERR_HIP, "adjoint: out of device memory");
+    fseg = static_cast<unsigned*>(dev.raw((size_t)C * nflags * sizeof(unsigned)));
+    if (!fseg) return fail(VK_ERR_HIP, "adjoint: out of device memory");
+  } else {
+    seg.resize((size_t)C * kState);
+    for (auto& q : seg)
+      if (!(q = dev.alloc(nc))) return fail(VK_ERR_HIP, "adjoint: out of device memory");
+    for (auto& q : tape)
+      if (!(q = dev.alloc(nc))) return fail(VK_ERR_HIP, "adjoint: out of device memory");
+    for (auto& q : in4)
+      if (!(q = dev.alloc(nc))) return fail(VK_ERR_HIP, "adjoint: out of device memory");
+  }
   AdjBufs B;
   for (int f = 0; f < kState; f++) {
     B.lam[f] = dev.alloc(nc);
@@ -980,8 +1011,33 @@ int vk_adjoint_gradient(const double* u, const double* v, const double* w, const
 
   for (int sidx = ncp - 1; sidx >= 0; sidx--) {
     const int l0 = sidx * C + 1, l1 = std::min(N, (sidx + 1) * C);
-    // recompute the segment from its checkpoint, keeping each step's input
     for (int f = 0; f < kState; f++) copy(g, cp[(size_t)sidx * kState + f], wk.st[f]);
+    if (keep_tapes) {
+      // every step of the segment taped from its checkpoint, with its
+      // input u_ta and a copy of its dirty-line masks
+      for (int l = l0; l <= l1; l++) {
+        const size_t q = (size_t)(l - l0);
+        for (int c = 0; c < 3; c++) copy(g, wk.st[SUTA + c], useg[q * 3 + c]);
+        LineFlags lf{};
+        rc = taped_step(P, wk, l, tseg.data() + q * 16, iseg.data() + q * 4, &lf);
+        if (rc) return rc;
+        rc = hip_check(hipMemcpyAsync(fseg + q * nflags, lf.fx, nflags * sizeof(unsigned),
+                                      hipMemcpyDeviceToDevice, S()),
+                       "adjoint masks");
+        if (rc) return rc;
+      }
+      for (int l = l1; l >= l0; l--) {
+        const size_t q = (size_t)(l - l0);
+        const double* uta_in[3] = {useg[q * 3], useg[q * 3 + 1], useg[q * 3 + 2]};
+        unsigned* fq = fseg + q * nflags;
+        const LineFlags lf{fq, fq + nxl, fq + nxl + nyl};
+        rc = adjoint_step(P, B, l, tseg.data() + q * 16, iseg.data() + q * 4, uta_in, &lf);
+        if (rc) return rc;
+        if (l > 1) k_objective_src<<<(nJ + 255) / 256, 256, 0, S()>>>(g, P.m, P.dt, B.lam[ST]);
+      }
+      continue;
+    }
+    // recompute the segment from its checkpoint, keeping each step's input
     for (int l = l0; l <= l1; l++) {
       for (int f = 0; f < kState; f++) copy(g, wk.st[f], seg[(size_t)(l - l0) * kState + f]);
       if (l < l1) {
