@@ -399,8 +399,9 @@ def main():
                     help="also compute MKP's u_ep2 splitting-error fields (FP64 form)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU y-slab design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this library runs on one GPU; --gpus {args.gpus} is not supported "
+              "(a multi-GPU form, y slabs with halo exchange, is designed but not built)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
